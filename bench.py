@@ -1,0 +1,256 @@
+"""LBVH build + closest-hit benchmark on one MI355X.
+
+Workload: the skewed single-scene case — 1,048,576 FP32 triangles whose
+centres come from 64 Gaussian clusters (seed 2, sigma 0.02, clamped to [0,1],
+edge offsets U[-0.002, 0.002]^3), 63-bit Morton codes, primitive-id leaves;
+1920x1080 pinhole primary rays from (0.5, 0.5, -1) looking +z (60 degree
+vertical FOV, tmin 1e-4, tmax 1e30) plus one cosine-weighted diffuse bounce
+ray per primary hit.  Inputs are generated once on the host and are resident
+in HBM before the timed region.
+
+One step = build the BVH (bounds, encode, sort, tree, fit) + closest-hit of
+the primary rays + closest-hit of the bounce rays, all on one stream.
+
+Prints one JSON line.  `value` is the build throughput in Mprims/s; the line
+also carries closest-hit Mrays/s, the per-phase split, the roofline of the
+dominant kernel, and the reference CPU path timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N_PRIMS = 1 << 20
+WIDTH, HEIGHT = 1920, 1080
+# Algorithmic bytes per primitive of the build with 63-bit codes (u64 key +
+# u32 id): bounds 36, encode 48, sort 8 + 8 passes x 24, tree 16, fit 92.
+BUILD_BYTES_PER_PRIM_63 = 36 + 48 + (8 + 8 * 24) + 16 + 92
+SORT_PASS_BYTES_PER_PRIM_63 = 24  # read + write of an 8-byte key and a 4-byte id
+RAY_COMPULSORY_BYTES = 48  # 32-byte ray in, 16-byte hit out
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget of the CPU-baseline sample")
+    return ap.parse_args()
+
+
+def cpu_baseline(tri, rays, bounce, budget_s):
+    """The real lbvh.h (oracle/_ref, built from the reference sources) timed
+    on this host: builds of the same scene, then the reference traverser on
+    a sample of the same rays; falls back to the oracle restatement."""
+    import numpy as np
+    import oracle
+    kind = "reference"
+    fast = oracle.ref_available(fast=True)
+    if not (fast or oracle.ref_available()):
+        kind = "port"
+    threads = os.cpu_count() or 1
+    if kind == "reference":
+        threads = oracle.ref(fast).ref_hardware_concurrency() or threads
+    t_end = time.time() + budget_s * 0.6
+    times = []
+    nodes = None
+    while time.time() < t_end or len(times) < 2:
+        t0 = time.perf_counter()
+        if kind == "reference":
+            nodes, _, _ = oracle.ref_build(tri, 63, 0, threads=threads, fast=fast)
+        else:
+            nodes, _, _, _ = oracle.build(tri, 63, 0)
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 6:
+            break
+    build_s = sorted(times)[len(times) // 2]
+    # traversal sample on the primitive-id tree (the reference-order tree is
+    # near brute force for this scene: hours of CPU)
+    if kind == "reference":
+        pid_nodes, _, _ = oracle.ref_build(tri, 63, 1, threads=threads, fast=fast)
+    else:
+        pid_nodes, _, _, _ = oracle.build(tri, 63, 1)
+    sample = np.concatenate([rays[::16], bounce[::16]])
+    t0 = time.perf_counter()
+    if kind == "reference":
+        oracle.ref_trace(pid_nodes, tri, sample, 0, threads=threads)
+    else:
+        oracle.trace(pid_nodes, tri, sample, 0, threads=threads)
+    trace_s = time.perf_counter() - t0
+    return {
+        "value": round(N_PRIMS / build_s / 1e6, 3),
+        "unit": "Mprims/s",
+        "cores": int(threads),
+        "kind": kind,
+        "flags": ("-O3 -ffast-math" if (kind == "reference" and fast) else "-O2 -ffp-contract=off"),
+        "sample": "%d builds of the 1,048,576-triangle scene (63-bit codes, median %.1f ms); "
+                  "closest hit of %d rays (every 16th primary and bounce ray) on the "
+                  "primitive-id tree in %.2f s" % (len(times), build_s * 1e3, len(sample), trace_s),
+        "build_ms": round(build_s * 1e3, 2),
+        "closest_hit_mrays_s": round(len(sample) / trace_s / 1e6, 4),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
+              "(scenes shard across GPUs only by assigning whole scenes)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import lbvh_amd
+    from lbvh_amd import workload as W
+
+    log("bench: generating the 1,048,576-triangle clustered scene (seed 2) and %dx%d rays"
+        % (WIDTH, HEIGHT))
+    _, tri = W.scene_clustered(2, N_PRIMS)
+    rays = W.primary_rays(WIDTH, HEIGHT)
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    ctx = lbvh_amd.Context(0)
+    ctx.set_stream(stream)
+    d_tri = torch.from_numpy(tri).to(dev)
+    d_rays = torch.from_numpy(rays).to(dev)
+
+    # bounce rays from the GPU's own primary hits (identical to the oracle's)
+    bvh = ctx.build_triangles(d_tri, code_bits=63, leaf_order=lbvh_amd.LEAF_PRIMITIVE_ID)
+    p, t, _ = ctx.trace(bvh, d_rays, lbvh_amd.ISECT_REFERENCE)
+    ctx.synchronize()
+    bvh.destroy()
+    hp = p.cpu().numpy().view(np.uint32)
+    ht = t.cpu().numpy()
+    bounce, _ = W.bounce_rays(5, tri, rays, hp, ht)
+    d_bounce = torch.from_numpy(bounce).to(dev)
+    n_primary, n_bounce = rays.shape[0], bounce.shape[0]
+    log("bench: %d primary rays, %d hits -> %d bounce rays" % (n_primary, int((hp != 0xFFFFFFFF).sum()),
+                                                             n_bounce))
+    out_p = (torch.empty(n_primary, dtype=torch.int32, device=dev),
+             torch.empty(n_primary, dtype=torch.float32, device=dev),
+             torch.empty((n_primary, 2), dtype=torch.float32, device=dev))
+    out_b = (torch.empty(n_bounce, dtype=torch.int32, device=dev),
+             torch.empty(n_bounce, dtype=torch.float32, device=dev),
+             torch.empty((n_bounce, 2), dtype=torch.float32, device=dev))
+
+    def step():
+        b = ctx.build_triangles(d_tri, code_bits=63, leaf_order=lbvh_amd.LEAF_PRIMITIVE_ID)
+        ctx.trace(b, d_rays, lbvh_amd.ISECT_REFERENCE, out=out_p)
+        ctx.trace(b, d_bounce, lbvh_amd.ISECT_REFERENCE, out=out_b)
+        return b
+
+    prev = None
+    for i in range(args.warmup):
+        b = step()
+        if prev is not None:
+            prev.destroy()
+        prev = b
+    ctx.synchronize()
+    log("bench: warmup done (%d steps)" % args.warmup)
+
+    # timed region: K steps back to back, per-phase HIP events on the stream
+    # the kernels run on (the context's stream is torch's current stream)
+    trees = []
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        trees.append(step())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.synchronize()
+    build_ph, nb = ctx.phase_stats(1)
+    trace_ph, nt = ctx.phase_stats(2)
+    ctx.set_timing(False)
+    for b in trees:
+        b.destroy()
+    if prev is not None:
+        prev.destroy()
+    log("bench: timed %d steps in %.3f s" % (args.steps, wall))
+
+    K = args.steps
+    build_ms = sum(build_ph) / K
+    phases = [x / K for x in build_ph]
+    trace_ms = sum(trace_ph) / K  # primary + bounce per step
+    ms_per_step = wall * 1e3 / K
+    mprims = N_PRIMS / (build_ms * 1e-3) / 1e6
+    mrays = (n_primary + n_bounce) / (trace_ms * 1e-3) / 1e6
+
+    sort_passes = 8  # 63-bit codes (60 bits used), 8-bit digits
+    sort_pass_ms = phases[1] / sort_passes
+    sort_pass_bytes = N_PRIMS * SORT_PASS_BYTES_PER_PRIM_63
+    kernels = {
+        "trace (primary+bounce)": trace_ms,
+        "sort pass x%d" % sort_passes: phases[1],
+        "bounds+encode": phases[0],
+        "tree": phases[2],
+        "fit": phases[3],
+    }
+    dominant = max(kernels, key=kernels.get)
+    build_bytes = N_PRIMS * BUILD_BYTES_PER_PRIM_63
+    build_frac = build_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    if dominant.startswith("sort"):
+        ach = sort_pass_bytes / (sort_pass_ms * 1e-3) / 1e9
+        roof = {"kernel": "k_onesweep (one digit pass, u64 key + u32 id)", "bound": "hbm",
+                "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": sort_pass_bytes, "avg_launch_ms": round(sort_pass_ms, 5)}
+    else:
+        # traversal: compulsory ray/hit bytes only (a lower bound on its traffic)
+        tb = (n_primary + n_bounce) * RAY_COMPULSORY_BYTES
+        ach = tb / (trace_ms * 1e-3) / 1e9
+        roof = {"kernel": "k_trace (closest hit, reference intersector)", "bound": "hbm",
+                "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                "bytes_per_launch": tb / 2, "avg_launch_ms": round(trace_ms / 2, 5)}
+
+    res = {
+        "metric": "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain",
+        "value": round(mprims, 3),
+        "unit": "Mprims/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "clustered scene, 1,048,576 FP32 triangles (64 Gaussian clusters, "
+                               "seed 2), 63-bit Morton codes, primitive-id leaves; 1920x1080 primary "
+                               "rays + one diffuse bounce per hit",
+                   "n_prims": N_PRIMS, "code_bits": 63, "primary_rays": n_primary,
+                   "bounce_rays": n_bounce},
+        "build_ms": round(build_ms, 4),
+        "build_phases_ms": {"bounds+encode": round(phases[0], 4), "sort": round(phases[1], 4),
+                            "tree": round(phases[2], 4), "fit": round(phases[3], 4)},
+        "build_roofline_frac": round(build_frac, 4),
+        "build_algorithmic_bytes_per_prim": BUILD_BYTES_PER_PRIM_63,
+        "closest_hit_mrays_s": round(mrays, 3),
+        "trace_ms": round(trace_ms, 4),
+        "dominant_kernel": dominant,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        log("bench: timing the reference CPU path (bounded sample)")
+        res["cpu_baseline"] = cpu_baseline(tri, rays, bounce, args.cpu_seconds)
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

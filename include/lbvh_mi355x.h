@@ -1,0 +1,179 @@
+/*
+ * lbvh_mi355x.h — C ABI of the MI355X (gfx950) LBVH engine.
+ *
+ * Plain C: opaque handles, POD option structs, integer status codes, no
+ * exceptions across the boundary.  Each entry point names the tay10r/lbvh
+ * interface it replaces (file:line in the reference repository).
+ *
+ * Data layouts at the boundary are the reference's own:
+ *   triangle  48 B  lbvh::triangle<float> {p0, e0 = a - b, e1 = c - a, normal}
+ *                   (lbvh.h:1959-1983)
+ *   aabb      24 B  lbvh::aabb<float> {min xyz, max xyz} (lbvh.h:205-212)
+ *   node      32 B  lbvh::node<float> {aabb box; u32 left; u32 right}, bit 31
+ *                   of left/right marks a leaf, root = node 0 (lbvh.h:220-255)
+ *   ray       32 B  lbvh::ray<float> {pos, dir, tmin, tmax} (lbvh.h:387-401)
+ *
+ * Threading: a context is used by one host thread at a time; separate
+ * contexts may be used concurrently.  Work is queued on the context's stream
+ * (its own, or one installed with lbvh_ctx_set_stream).  Entry points taking
+ * host pointers copy in and out and return after the work has finished;
+ * the *_device variants take device pointers, borrow them, and return after
+ * queuing (call lbvh_ctx_synchronize before reading results).
+ */
+#ifndef LBVH_MI355X_H
+#define LBVH_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lbvh_status
+{
+  LBVH_OK = 0,
+  LBVH_E_INVALID = 1,  /* bad argument: n < 2, null pointer, bad option */
+  LBVH_E_OOM = 2,      /* device allocation failed or memory cap exceeded */
+  LBVH_E_HIP = 3,      /* a HIP runtime call failed */
+  LBVH_E_BUSY = 4,     /* no free slot / request rejected for now */
+  LBVH_E_NODEVICE = 5, /* no usable gfx950 device */
+  LBVH_E_TIMEOUT = 6   /* a kernel's bounded inter-workgroup wait expired */
+} lbvh_status;
+
+typedef struct lbvh_ctx lbvh_ctx;
+typedef struct lbvh_bvh lbvh_bvh;
+
+/* Leaf numbering.  REFERENCE reproduces lbvh.h byte for byte: a leaf index
+ * is the primitive's position in Morton order, and boxes are fitted from
+ * primitives[position] (lbvh.h:1728-1729, 1855, 1862).  PRIMITIVE_ID stores
+ * the id of the primitive at that position (same topology). */
+enum
+{
+  LBVH_LEAF_REFERENCE = 0,
+  LBVH_LEAF_PRIMITIVE_ID = 1
+};
+
+/* Closest-hit intersector.  REFERENCE is lbvh::triangle_intersector
+ * (lbvh.h:2055-2075) with the traverser's first-hit-wins tie rule
+ * (lbvh.h:1909, 1920); REFERENCE_LOWID breaks equal-distance ties toward
+ * the lowest primitive id; WATERTIGHT uses the watertight test of Woop,
+ * Benthin and Wald (JCGT 2013) with lowest-id ties. */
+enum
+{
+  LBVH_ISECT_REFERENCE = 0,
+  LBVH_ISECT_REFERENCE_LOWID = 1,
+  LBVH_ISECT_WATERTIGHT = 2
+};
+
+typedef struct lbvh_build_opts
+{
+  uint32_t code_bits;       /* 30 (u32 codes, as builder<float>) or 63 (u64) */
+  uint32_t leaf_order;      /* LBVH_LEAF_* */
+  uint32_t sort_digit_bits; /* 0 = tuned default; else 6..10 */
+  uint32_t sort_tile;       /* 0 = tuned default; else keys per workgroup */
+  uint32_t keep_codes;      /* keep sorted codes / ids for download */
+  uint32_t reserved[3];
+} lbvh_build_opts;
+
+typedef struct lbvh_bvh_info
+{
+  uint64_t n_prims;
+  uint64_t n_nodes; /* n_prims - 1 */
+  uint32_t code_bits;
+  uint32_t leaf_order;
+  const void* d_nodes;   /* device lbvh::node<float>[n_nodes] */
+  const void* d_prims;   /* device triangles (NULL for AABB builds) */
+  uint64_t device_bytes; /* device memory held by the handle */
+} lbvh_bvh_info;
+
+/* -- status / options ---------------------------------------------------- */
+
+const char* lbvh_status_string(lbvh_status s);
+void lbvh_build_opts_init(lbvh_build_opts* o); /* 30-bit, REFERENCE leaves */
+
+/* -- context ---------------------------------------------------------------
+ * Replaces the scheduler plugin of lbvh.h (single_thread_scheduler /
+ * naive_thread_scheduler, lbvh.h:87-165): work runs on a HIP stream. */
+lbvh_status lbvh_ctx_create(int device, lbvh_ctx** out);
+void lbvh_ctx_destroy(lbvh_ctx* ctx);
+lbvh_status lbvh_ctx_set_stream(lbvh_ctx* ctx, void* hip_stream);
+lbvh_status lbvh_ctx_synchronize(lbvh_ctx* ctx);
+/* Per-phase device timing of the next builds/traces (HIP events). */
+lbvh_status lbvh_ctx_set_timing(lbvh_ctx* ctx, int enable);
+/* Milliseconds of the phases of the last timed call: for a build
+ * [bounds+encode, sort, tree, fit], for a trace [trace], for a refit [fit]. */
+lbvh_status lbvh_ctx_phase_ms(lbvh_ctx* ctx, float* ms, uint32_t cap,
+                              uint32_t* count);
+/* Phase milliseconds summed over every call of one kind (1 build, 2 trace,
+ * 3 refit) since timing was last enabled; *calls receives the call count.
+ * Events are recorded on the context stream without synchronizing, so a
+ * timed loop runs back to back. */
+lbvh_status lbvh_ctx_phase_stats(lbvh_ctx* ctx, int kind, float* sum_ms,
+                                 uint32_t cap, uint32_t* count,
+                                 uint32_t* calls);
+const char* lbvh_ctx_last_error(const lbvh_ctx* ctx);
+/* Device bytes held by the context's scratch workspace. */
+uint64_t lbvh_ctx_workspace_bytes(const lbvh_ctx* ctx);
+
+/* -- build -----------------------------------------------------------------
+ * Replaces lbvh::builder<float>::operator()(primitives, count, converter)
+ * (lbvh.h:342-345, 1793-1822) for P = triangle<float> with
+ * C = triangle_aabb_converter<float> (lbvh.h:1988-2013). */
+lbvh_status lbvh_build_triangles(lbvh_ctx* ctx, const float* tri48,
+                                 uint64_t n, const lbvh_build_opts* opts,
+                                 lbvh_bvh** out);
+lbvh_status lbvh_build_triangles_device(lbvh_ctx* ctx, const float* d_tri48,
+                                        uint64_t n,
+                                        const lbvh_build_opts* opts,
+                                        lbvh_bvh** out);
+/* Same builder for any primitive whose converter was evaluated by the
+ * caller: boxes24[i] = converter(primitives[i]) (lbvh.h:1289, 1453, 1855). */
+lbvh_status lbvh_build_aabbs(lbvh_ctx* ctx, const float* boxes24, uint64_t n,
+                             const lbvh_build_opts* opts, lbvh_bvh** out);
+lbvh_status lbvh_build_aabbs_device(lbvh_ctx* ctx, const float* d_boxes24,
+                                    uint64_t n, const lbvh_build_opts* opts,
+                                    lbvh_bvh** out);
+
+/* Refit: re-runs the box fit (lbvh.h:1827-1866) over new primitive data on
+ * the unchanged topology.  The reference has no refit entry point (the fit
+ * is protected, lbvh.h:347-352); the result equals a fresh fit. */
+lbvh_status lbvh_refit_triangles(lbvh_ctx* ctx, lbvh_bvh* bvh,
+                                 const float* tri48);
+lbvh_status lbvh_refit_triangles_device(lbvh_ctx* ctx, lbvh_bvh* bvh,
+                                        const float* d_tri48);
+
+/* -- the built tree --------------------------------------------------------
+ * Replaces lbvh::bvh<float> (lbvh.h:259-299): nodes stay resident on the
+ * device; download yields the reference's node array. */
+lbvh_status lbvh_bvh_get_info(const lbvh_bvh* bvh, lbvh_bvh_info* out);
+lbvh_status lbvh_bvh_download_nodes(lbvh_ctx* ctx, const lbvh_bvh* bvh,
+                                    void* dst, uint64_t cap_bytes);
+/* Sorted Morton codes (u32 for 30-bit, u64 for 63-bit), the primitive ids in
+ * sorted order (space_filling_curve entries after sort, lbvh.h:1104-1134) and
+ * the centroid bounds (6 floats).  Needs opts.keep_codes; any pointer may be
+ * NULL. */
+lbvh_status lbvh_bvh_download_codes(lbvh_ctx* ctx, const lbvh_bvh* bvh,
+                                    void* codes, uint32_t* ids,
+                                    float* bounds6);
+void lbvh_bvh_destroy(lbvh_bvh* bvh);
+
+/* -- traversal -------------------------------------------------------------
+ * Replaces lbvh::traverser<float, triangle<float>,
+ * triangle_intersection<float>>::operator()(ray, triangle_intersector)
+ * (lbvh.h:531-533, 1869-1942) applied to a batch of rays.  Outputs per ray:
+ * prim (UINT32_MAX = miss, lbvh.h:366), t (+inf on a miss), uv (2 floats,
+ * 0 on a miss).  uv may be NULL. */
+lbvh_status lbvh_trace_closest(lbvh_ctx* ctx, const lbvh_bvh* bvh,
+                               const float* rays32, uint64_t nrays, int mode,
+                               uint32_t* prim, float* t, float* uv);
+lbvh_status lbvh_trace_closest_device(lbvh_ctx* ctx, const lbvh_bvh* bvh,
+                                      const float* d_rays32, uint64_t nrays,
+                                      int mode, uint32_t* d_prim, float* d_t,
+                                      float* d_uv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBVH_MI355X_H */

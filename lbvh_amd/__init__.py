@@ -1,0 +1,310 @@
+"""lbvh_amd — Python access to the MI355X LBVH engine through its C ABI.
+
+The engine is the native library lbvh_amd/lib/liblbvh_mi355x.so (HIP kernels
+for gfx950 behind include/lbvh_mi355x.h).  This module only marshals
+arguments: numpy arrays go through the host-pointer entry points (copy in,
+copy out); torch CUDA tensors go through the *_device entry points and stay
+resident.  There is no CPU fallback: if the library is missing or no gfx950
+device is present, calls raise.
+
+Host layouts (the reference's own, lbvh.h):
+  triangles float32[n, 12]  lbvh::triangle<float> {p0, e0, e1, normal}
+  boxes     float32[n, 6]   lbvh::aabb<float>
+  rays      float32[m, 8]   lbvh::ray<float> {pos, dir, tmin, tmax}
+  nodes     NODE_DTYPE[n-1] lbvh::node<float> {box[6], left, right}
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+LEAF_REFERENCE = 0
+LEAF_PRIMITIVE_ID = 1
+ISECT_REFERENCE = 0
+ISECT_REFERENCE_LOWID = 1
+ISECT_WATERTIGHT = 2
+LEAF_BIT = 0x80000000
+MISS = 0xFFFFFFFF
+
+NODE_DTYPE = np.dtype([("box", np.float32, (6,)), ("left", np.uint32), ("right", np.uint32)])
+
+STATUS = {0: "LBVH_OK", 1: "LBVH_E_INVALID", 2: "LBVH_E_OOM", 3: "LBVH_E_HIP", 4: "LBVH_E_BUSY",
+          5: "LBVH_E_NODEVICE", 6: "LBVH_E_TIMEOUT"}
+
+
+class LbvhError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS.get(status, status), msg))
+        self.status = status
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [("code_bits", ctypes.c_uint32), ("leaf_order", ctypes.c_uint32),
+                ("sort_digit_bits", ctypes.c_uint32), ("sort_tile", ctypes.c_uint32),
+                ("keep_codes", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class BvhInfo(ctypes.Structure):
+    _fields_ = [("n_prims", ctypes.c_uint64), ("n_nodes", ctypes.c_uint64),
+                ("code_bits", ctypes.c_uint32), ("leaf_order", ctypes.c_uint32),
+                ("d_nodes", ctypes.c_void_p), ("d_prims", ctypes.c_void_p),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+VP = ctypes.c_void_p
+U64 = ctypes.c_uint64
+F32P = ctypes.POINTER(ctypes.c_float)
+
+# Every symbol include/lbvh_mi355x.h declares, with its ctypes signature.
+SIGNATURES = {
+    "lbvh_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "lbvh_build_opts_init": (None, [ctypes.POINTER(BuildOpts)]),
+    "lbvh_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
+    "lbvh_ctx_destroy": (None, [VP]),
+    "lbvh_ctx_set_stream": (ctypes.c_int, [VP, VP]),
+    "lbvh_ctx_synchronize": (ctypes.c_int, [VP]),
+    "lbvh_ctx_set_timing": (ctypes.c_int, [VP, ctypes.c_int]),
+    "lbvh_ctx_phase_ms": (ctypes.c_int, [VP, F32P, ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_uint32)]),
+    "lbvh_ctx_phase_stats": (ctypes.c_int, [VP, ctypes.c_int, F32P, ctypes.c_uint32,
+                                            ctypes.POINTER(ctypes.c_uint32),
+                                            ctypes.POINTER(ctypes.c_uint32)]),
+    "lbvh_ctx_last_error": (ctypes.c_char_p, [VP]),
+    "lbvh_ctx_workspace_bytes": (ctypes.c_uint64, [VP]),
+    "lbvh_build_triangles": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), ctypes.POINTER(VP)]),
+    "lbvh_build_triangles_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts),
+                                                   ctypes.POINTER(VP)]),
+    "lbvh_build_aabbs": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), ctypes.POINTER(VP)]),
+    "lbvh_build_aabbs_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts),
+                                               ctypes.POINTER(VP)]),
+    "lbvh_refit_triangles": (ctypes.c_int, [VP, VP, VP]),
+    "lbvh_refit_triangles_device": (ctypes.c_int, [VP, VP, VP]),
+    "lbvh_bvh_get_info": (ctypes.c_int, [VP, ctypes.POINTER(BvhInfo)]),
+    "lbvh_bvh_download_nodes": (ctypes.c_int, [VP, VP, VP, U64]),
+    "lbvh_bvh_download_codes": (ctypes.c_int, [VP, VP, VP, VP, VP]),
+    "lbvh_bvh_destroy": (None, [VP]),
+    "lbvh_trace_closest": (ctypes.c_int, [VP, VP, VP, U64, ctypes.c_int, VP, VP, VP]),
+    "lbvh_trace_closest_device": (ctypes.c_int, [VP, VP, VP, U64, ctypes.c_int, VP, VP, VP]),
+}
+
+_lib = None
+
+
+def library_path():
+    return _build.ENGINE_SO
+
+
+def lib():
+    """Loads the engine library; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError("the HIP engine library is not built (%s); run __graft_entry__.build()"
+                               % path)
+        L = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _dptr(x):
+    return ctypes.c_void_p(int(x.data_ptr()))
+
+
+def _hptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _make_opts(code_bits, leaf_order, keep_codes, sort_digit_bits=0, sort_tile=0):
+    o = BuildOpts()
+    lib().lbvh_build_opts_init(ctypes.byref(o))
+    o.code_bits = code_bits
+    o.leaf_order = leaf_order
+    o.keep_codes = 1 if keep_codes else 0
+    o.sort_digit_bits = sort_digit_bits
+    o.sort_tile = sort_tile
+    return o
+
+
+class Bvh:
+    """A device-resident tree (lbvh::bvh<float> equivalent)."""
+
+    def __init__(self, ctx, handle, n, code_bits, leaf_order, keep=None):
+        self.ctx = ctx
+        self.handle = handle
+        self.n = n
+        self.code_bits = code_bits
+        self.leaf_order = leaf_order
+        self._keep = keep  # borrowed device tensors that must outlive the tree
+
+    def info(self):
+        inf = BvhInfo()
+        self.ctx._check(lib().lbvh_bvh_get_info(self.handle, ctypes.byref(inf)))
+        return inf
+
+    def nodes(self):
+        out = np.empty(self.n - 1, dtype=NODE_DTYPE)
+        self.ctx._check(lib().lbvh_bvh_download_nodes(self.ctx.handle, self.handle, _hptr(out),
+                                                      out.nbytes))
+        return out
+
+    def codes(self):
+        """(sorted codes, sorted ids, centroid bounds[6]); needs keep_codes."""
+        codes = np.empty(self.n, dtype=np.uint32 if self.code_bits == 30 else np.uint64)
+        ids = np.empty(self.n, dtype=np.uint32)
+        bounds = np.empty(6, dtype=np.float32)
+        self.ctx._check(lib().lbvh_bvh_download_codes(self.ctx.handle, self.handle, _hptr(codes),
+                                                      _hptr(ids), _hptr(bounds)))
+        return codes, ids, bounds
+
+    def bounds(self):
+        bounds = np.empty(6, dtype=np.float32)
+        self.ctx._check(lib().lbvh_bvh_download_codes(self.ctx.handle, self.handle, None, None,
+                                                      _hptr(bounds)))
+        return bounds
+
+    def destroy(self):
+        if self.handle:
+            lib().lbvh_bvh_destroy(self.handle)
+            self.handle = None
+            self._keep = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Context:
+    """An engine context on one gfx950 device (lbvh_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        st = lib().lbvh_ctx_create(device, ctypes.byref(h))
+        if st != 0:
+            raise LbvhError(st, lib().lbvh_status_string(st).decode())
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().lbvh_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            msg = lib().lbvh_ctx_last_error(self.handle).decode() if self.handle else ""
+            raise LbvhError(st, msg)
+
+    def set_stream(self, stream):
+        """stream: a torch.cuda.Stream, a raw hipStream_t int, or None (own stream)."""
+        raw = None
+        if stream is not None:
+            raw = ctypes.c_void_p(int(getattr(stream, "cuda_stream", stream)))
+        self._check(lib().lbvh_ctx_set_stream(self.handle, raw))
+
+    def synchronize(self):
+        self._check(lib().lbvh_ctx_synchronize(self.handle))
+
+    def set_timing(self, enable=True):
+        self._check(lib().lbvh_ctx_set_timing(self.handle, 1 if enable else 0))
+
+    def phase_ms(self):
+        buf = (ctypes.c_float * 8)()
+        cnt = ctypes.c_uint32()
+        self._check(lib().lbvh_ctx_phase_ms(self.handle, buf, 8, ctypes.byref(cnt)))
+        return [buf[i] for i in range(cnt.value)]
+
+    def phase_stats(self, kind):
+        """(summed phase ms, number of calls) over timed calls of a kind:
+        1 build [bounds+encode, sort, tree, fit], 2 trace, 3 refit."""
+        buf = (ctypes.c_float * 8)()
+        cnt = ctypes.c_uint32()
+        calls = ctypes.c_uint32()
+        self._check(lib().lbvh_ctx_phase_stats(self.handle, kind, buf, 8, ctypes.byref(cnt),
+                                               ctypes.byref(calls)))
+        return [buf[i] for i in range(cnt.value)], calls.value
+
+    def workspace_bytes(self):
+        return int(lib().lbvh_ctx_workspace_bytes(self.handle))
+
+    # -- build --------------------------------------------------------------
+    def build_triangles(self, tri, code_bits=30, leaf_order=LEAF_REFERENCE, keep_codes=False,
+                        sort_digit_bits=0, sort_tile=0):
+        return self._build(tri, 12, "lbvh_build_triangles", code_bits, leaf_order, keep_codes,
+                           sort_digit_bits, sort_tile)
+
+    def build_aabbs(self, boxes, code_bits=30, leaf_order=LEAF_REFERENCE, keep_codes=False,
+                    sort_digit_bits=0, sort_tile=0):
+        return self._build(boxes, 6, "lbvh_build_aabbs", code_bits, leaf_order, keep_codes,
+                           sort_digit_bits, sort_tile)
+
+    def _build(self, prims, width, fn, code_bits, leaf_order, keep_codes, sdb, stile):
+        opts = _make_opts(code_bits, leaf_order, keep_codes, sdb, stile)
+        out = ctypes.c_void_p()
+        if _is_torch(prims):
+            if not prims.is_cuda or not prims.is_contiguous():
+                raise ValueError("device primitives must be a contiguous CUDA tensor")
+            n = prims.numel() // width
+            self._check(getattr(lib(), fn + "_device")(self.handle, _dptr(prims), n,
+                                                       ctypes.byref(opts), ctypes.byref(out)))
+            return Bvh(self, out, n, code_bits, leaf_order, keep=prims)
+        a = np.ascontiguousarray(prims, dtype=np.float32).reshape(-1, width)
+        n = a.shape[0]
+        self._check(getattr(lib(), fn)(self.handle, _hptr(a), n, ctypes.byref(opts),
+                                       ctypes.byref(out)))
+        return Bvh(self, out, n, code_bits, leaf_order)
+
+    def refit_triangles(self, bvh, tri):
+        if _is_torch(tri):
+            self._check(lib().lbvh_refit_triangles_device(self.handle, bvh.handle, _dptr(tri)))
+            bvh._keep = tri
+            return
+        a = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 12)
+        if a.shape[0] != bvh.n:
+            raise ValueError("refit needs the same primitive count")
+        self._check(lib().lbvh_refit_triangles(self.handle, bvh.handle, _hptr(a)))
+
+    # -- traversal ---------------------------------------------------------
+    def trace(self, bvh, rays, mode=ISECT_REFERENCE, out=None, with_uv=True):
+        """Closest hit per ray.  numpy rays -> numpy (prim, t, uv);
+        torch rays -> torch tensors (asynchronous on the context stream)."""
+        if _is_torch(rays):
+            import torch
+            m = rays.numel() // 8
+            if out is None:
+                prim = torch.empty(m, dtype=torch.int32, device=rays.device)
+                t = torch.empty(m, dtype=torch.float32, device=rays.device)
+                uv = torch.empty((m, 2), dtype=torch.float32, device=rays.device) if with_uv else None
+            else:
+                prim, t, uv = out
+            self._check(lib().lbvh_trace_closest_device(self.handle, bvh.handle, _dptr(rays), m,
+                                                        mode, _dptr(prim), _dptr(t),
+                                                        _dptr(uv) if uv is not None else None))
+            return prim, t, uv
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        m = r.shape[0]
+        prim = np.empty(m, dtype=np.uint32)
+        t = np.empty(m, dtype=np.float32)
+        uv = np.empty((m, 2), dtype=np.float32) if with_uv else None
+        self._check(lib().lbvh_trace_closest(self.handle, bvh.handle, _hptr(r), m, mode,
+                                             _hptr(prim), _hptr(t),
+                                             _hptr(uv) if uv is not None else None))
+        return prim, t, uv

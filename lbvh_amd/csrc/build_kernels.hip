@@ -1,0 +1,496 @@
+// gfx950 build kernels: centroid bounds, Morton encode, Karras radix tree and
+// the bottom-up box fit.  Each kernel cites the reference code whose results
+// it reproduces bit for bit.
+#include "device_common.hpp"
+#include "engine.hpp"
+
+namespace lbvh_dev {
+
+constexpr int kBlock = 256;
+constexpr int kCentroidMaxBlocks = 2048;
+
+// ---- centroid bounds --------------------------------------------------------
+// Reproduces centroid_bounds_kernel + the serial union of per-thread boxes
+// (lbvh.h:1282-1293, 1539-1543): the union of (min + max) * 0.5 over every
+// primitive box (center_of, lbvh.h:1160-1167).  min/max are exact, so the
+// reduction order cannot change the result for finite inputs.  Centres are
+// also written out (SoA) for the encode pass.
+
+__device__ __forceinline__ void
+wave_reduce_box(float lo[3], float hi[3])
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      float ol = __shfl_down(lo[k], off, kWave);
+      float oh = __shfl_down(hi[k], off, kWave);
+      lo[k] = std_min(lo[k], ol);
+      hi[k] = std_max(hi[k], oh);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void
+k_centroid_bounds(const float* __restrict__ prims, int kind, uint64_t n,
+                  float* __restrict__ cx, float* __restrict__ cy,
+                  float* __restrict__ cz, float* __restrict__ partials,
+                  uint32_t* __restrict__ ticket, float* __restrict__ bounds,
+                  uint32_t* __restrict__ zero_ptr, uint32_t zero_words)
+{
+  __shared__ float s_box[kBlock / kWave][6];
+  __shared__ int s_last;
+  const uint64_t tid = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+
+  for (uint64_t j = tid; j < zero_words; j += stride)
+    zero_ptr[j] = 0u;
+
+  float lo[3] = { INFINITY, INFINITY, INFINITY };
+  float hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+  for (uint64_t i = tid; i < n; i += stride) {
+    Box b = load_prim_box(prims, kind, i);
+    float c[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      float s = b.lo[k] + b.hi[k];
+      c[k] = s * 0.5f;
+      lo[k] = std_min(lo[k], c[k]);
+      hi[k] = std_max(hi[k], c[k]);
+    }
+    cx[i] = c[0];
+    cy[i] = c[1];
+    cz[i] = c[2];
+  }
+  wave_reduce_box(lo, hi);
+  const int w = threadIdx.x / kWave;
+  if (__lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      s_box[w][k] = lo[k];
+      s_box[w][3 + k] = hi[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float blo[3], bhi[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      blo[k] = s_box[0][k];
+      bhi[k] = s_box[0][3 + k];
+    }
+    for (int ww = 1; ww < kBlock / kWave; ww++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        blo[k] = std_min(blo[k], s_box[ww][k]);
+        bhi[k] = std_max(bhi[k], s_box[ww][3 + k]);
+      }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      coherent_store_f32(&partials[6 * blockIdx.x + k], blo[k]);
+      coherent_store_f32(&partials[6 * blockIdx.x + 3 + k], bhi[k]);
+    }
+    vmem_drain();
+    uint32_t t = atomic_add_agent(ticket, 1u);
+    s_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last)
+    return;
+  // Last workgroup: union of the per-workgroup boxes in workgroup order.
+  if (threadIdx.x == 0) {
+    float blo[3] = { INFINITY, INFINITY, INFINITY };
+    float bhi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (uint32_t g = 0; g < gridDim.x; g++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        blo[k] = std_min(blo[k], coherent_load_f32(&partials[6 * g + k]));
+        bhi[k] = std_max(bhi[k], coherent_load_f32(&partials[6 * g + 3 + k]));
+      }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      bounds[k] = blo[k];
+      bounds[3 + k] = bhi[k];
+    }
+    *ticket = 0u;
+  }
+}
+
+// ---- Morton encode ----------------------------------------------------------
+// morton_curve_kernel (lbvh.h:1432-1473): scale = D / (extent + 1) with
+// D = 1024 (morton_domain<4>) or 2^20 (morton_domain<8>); q = trunc((c - min)
+// * scale); code = expand(qx) << 2 | expand(qy) << 1 | expand(qz) with the
+// magic-mask expansions of morton_encoder<4>/<8> (lbvh.h:1352-1358,
+// 1377-1385).  The 63-bit variant applies morton_encoder<8> to FP32 centres.
+// Also accumulates the radix sort's digit histograms for every pass.
+
+__device__ __forceinline__ uint32_t
+expand10(uint32_t n)
+{
+  n = (n | (n << 16)) & 0x030000ffu;
+  n = (n | (n << 8)) & 0x0300f00fu;
+  n = (n | (n << 4)) & 0x030c30c3u;
+  n = (n | (n << 2)) & 0x09249249u;
+  return n;
+}
+
+__device__ __forceinline__ uint64_t
+expand20(uint64_t n)
+{
+  n = (n | n << 32) & 0x001f00000000ffffull;
+  n = (n | n << 16) & 0x001f0000ff0000ffull;
+  n = (n | n << 8) & 0x100f00f00f00f00full;
+  n = (n | n << 4) & 0x10c30c30c30c30c3ull;
+  n = (n | n << 2) & 0x1249249249249249ull;
+  return n;
+}
+
+template<typename KeyT>
+__device__ __forceinline__ KeyT
+encode(float qx, float qy, float qz);
+
+template<>
+__device__ __forceinline__ uint32_t
+encode<uint32_t>(float qx, float qy, float qz)
+{
+  uint32_t x = uint32_t(qx), y = uint32_t(qy), z = uint32_t(qz);
+  return (expand10(x) << 2) | (expand10(y) << 1) | expand10(z);
+}
+
+template<>
+__device__ __forceinline__ uint64_t
+encode<uint64_t>(float qx, float qy, float qz)
+{
+  uint64_t x = uint64_t(qx), y = uint64_t(qy), z = uint64_t(qz);
+  return (expand20(x) << 2) | (expand20(y) << 1) | expand20(z);
+}
+
+constexpr int kHistMaxWords = 8192; // passes * radix, 32 KiB of LDS
+
+template<typename KeyT>
+__global__ __launch_bounds__(kBlock) void
+k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
+         const float* __restrict__ cz, uint64_t n,
+         const float* __restrict__ bounds, KeyT* __restrict__ codes,
+         uint32_t* __restrict__ hist, int digit_bits, int passes)
+{
+  extern __shared__ uint32_t s_hist[];
+  const int radix = 1 << digit_bits;
+  const int words = radix * passes;
+  for (int j = threadIdx.x; j < words; j += kBlock)
+    s_hist[j] = 0u;
+  const float dom = sizeof(KeyT) == 4 ? 1024.0f : 1048576.0f;
+  float bmin[3], scale[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    bmin[k] = bounds[k];
+    float ext = bounds[3 + k] - bmin[k];
+    float den = ext + 1.0f;
+    scale[k] = dom / den;
+  }
+  __syncthreads();
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  const KeyT mask = KeyT(radix - 1);
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+    float dx = cx[i] - bmin[0];
+    float dy = cy[i] - bmin[1];
+    float dz = cz[i] - bmin[2];
+    float qx = dx * scale[0];
+    float qy = dy * scale[1];
+    float qz = dz * scale[2];
+    KeyT code = encode<KeyT>(qx, qy, qz);
+    codes[i] = code;
+    for (int p = 0; p < passes; p++) {
+      uint32_t d = uint32_t((code >> (p * digit_bits)) & mask);
+      atomicAdd(&s_hist[p * radix + d], 1u);
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < words; j += kBlock) {
+    uint32_t v = s_hist[j];
+    if (v)
+      atomicAdd(&hist[j], v);
+  }
+}
+
+// ---- Karras radix tree -------------------------------------------------------
+// divide_node + builder_kernel (lbvh.h:1588-1688, 1713-1731), one internal
+// node per thread:
+//   delta(j, k) = -1 if k is outside [0, n); clz(code_j ^ code_k) if the codes
+//   differ; clz(j ^ k) + bits(code) if they are equal;
+//   d = sign(delta(i, i+1) - delta(i, i-1)); delta_min = delta(i, i-d);
+//   l_max starts at 128 and grows x4 while delta(i, i + l_max d) > delta_min
+//   (the loop exits on k >= n only); binary search for l; split search with
+//   ceil_div; gamma = i + s d + min(d, 0).
+// left = gamma | leaf if min(i, j) == gamma; right = gamma + 1 | leaf if
+// max(i, j) == gamma + 1.  With PRIMITIVE_ID leaves, a leaf stores the id of
+// the primitive at that sorted position.
+
+template<typename KeyT>
+struct KeyTraits;
+template<>
+struct KeyTraits<uint32_t>
+{
+  static constexpr int bits = 32;
+  __device__ static int clz(uint32_t x) { return __clz(int(x)); }
+};
+template<>
+struct KeyTraits<uint64_t>
+{
+  static constexpr int bits = 64;
+  __device__ static int clz(uint64_t x) { return __clzll((long long)x); }
+};
+
+template<typename KeyT>
+__device__ __forceinline__ int
+calc_delta(const KeyT* __restrict__ codes, int64_t n, int64_t j, KeyT cj, int64_t k)
+{
+  if (k < 0 || k >= n)
+    return -1;
+  KeyT ck = codes[k];
+  if (cj == ck)
+    return KeyTraits<KeyT>::clz(KeyT(uint64_t(j ^ k))) + KeyTraits<KeyT>::bits;
+  return KeyTraits<KeyT>::clz(cj ^ ck);
+}
+
+template<typename KeyT>
+__global__ __launch_bounds__(kBlock) void
+k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64_t n,
+         Node* __restrict__ nodes, uint32_t* __restrict__ parent,
+         uint32_t* __restrict__ flags)
+{
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n - 1)
+    return;
+  const KeyT ci = codes[i];
+  const int dd = calc_delta(codes, n, i, ci, i + 1) - calc_delta(codes, n, i, ci, i - 1);
+  const int64_t d = (dd < 0) ? -1 : 1;
+  const int delta_min = calc_delta(codes, n, i, ci, i - d);
+  int64_t l_max = 128;
+  for (;;) {
+    int64_t k = i + l_max * d;
+    if (k >= n)
+      break;
+    if (!(calc_delta(codes, n, i, ci, k) > delta_min))
+      break;
+    l_max *= 4;
+  }
+  int64_t l = 0;
+  for (int64_t div = 2;; div *= 2) {
+    int64_t t = l_max / div;
+    if (calc_delta(codes, n, i, ci, i + (l + t) * d) > delta_min)
+      l += t;
+    if (t == 1)
+      break;
+  }
+  const int64_t j = i + l * d;
+  const int delta_node = calc_delta(codes, n, i, ci, j);
+  int64_t s = 0;
+  for (int64_t div = 2;; div *= 2) {
+    int64_t t = (l + (div - 1)) / div;
+    if (calc_delta(codes, n, i, ci, i + (s + t) * d) > delta_node)
+      s += t;
+    if (div >= l)
+      break;
+  }
+  const int64_t gamma = i + s * d + (d < 0 ? d : 0);
+  if (gamma < 0 || gamma + 1 > n - 1)
+    return; // unreachable for sorted codes; keeps every index below in range
+  const int64_t mn = (i < j) ? i : j;
+  const int64_t mx = (j > i) ? j : i;
+  const bool lleaf = (mn == gamma);
+  const bool rleaf = (mx == gamma + 1);
+  uint32_t left, right;
+  if (lleaf) {
+    uint32_t v = ids ? ids[gamma] : uint32_t(gamma);
+    left = v | kLeafBit;
+    if (v < uint64_t(n))
+      parent[(n - 1) + v] = uint32_t(i);
+  } else {
+    left = uint32_t(gamma);
+    parent[gamma] = uint32_t(i);
+  }
+  if (rleaf) {
+    uint32_t v = ids ? ids[gamma + 1] : uint32_t(gamma + 1);
+    right = v | kLeafBit;
+    if (v < uint64_t(n))
+      parent[(n - 1) + v] = uint32_t(i);
+  } else {
+    right = uint32_t(gamma + 1);
+    parent[gamma + 1] = uint32_t(i);
+  }
+  *reinterpret_cast<uint2*>(&nodes[i].left) = make_uint2(left, right);
+  flags[i] = 0u;
+}
+
+// ---- bottom-up fit -------------------------------------------------------------
+// builder::fit_boxes (lbvh.h:1827-1866) computes every internal box as
+// union(left box, right box), left operand first, where a leaf's box is the
+// converter applied to the primitive its index names.  Here one thread per
+// leaf climbs toward the root; the second thread to reach a node computes its
+// box (the sibling's box is re-derived from the primitive for a leaf child or
+// read coherently for an internal child), publishes it, and continues.  The
+// second arrival also clears the node's flag, so flags are zero again when
+// the kernel ends (ready for a refit).
+
+__device__ __forceinline__ Box
+load_node_box_coherent(const Node* nd)
+{
+  Box b;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    b.lo[k] = coherent_load_f32(&nd->bmin[k]);
+    b.hi[k] = coherent_load_f32(&nd->bmax[k]);
+  }
+  return b;
+}
+
+__device__ __forceinline__ void
+store_node_box_coherent(Node* nd, const Box& b)
+{
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    coherent_store_f32(&nd->bmin[k], b.lo[k]);
+    coherent_store_f32(&nd->bmax[k], b.hi[k]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void
+k_fit(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
+      const uint32_t* __restrict__ parent, uint32_t* __restrict__ flags)
+{
+  const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (v >= n)
+    return;
+  Box mine = load_prim_box(prims, kind, uint64_t(v));
+  uint32_t my_ref = uint32_t(v) | kLeafBit;
+  uint32_t p = parent[(n - 1) + v];
+  for (;;) {
+    if (int64_t(p) >= n - 1)
+      return; // parent links come from the tree kernel; never taken for a valid tree
+    uint32_t old = atomic_add_agent(&flags[p], 1u);
+    if (old == 0u)
+      return; // first arrival: the sibling's thread completes p
+    const uint2 lr = *reinterpret_cast<const uint2*>(&nodes[p].left);
+    const bool i_am_left = (lr.x == my_ref);
+    const uint32_t other = i_am_left ? lr.y : lr.x;
+    Box ob;
+    if (other & kLeafBit) {
+      if (int64_t(other & ~kLeafBit) >= n)
+        return;
+      ob = load_prim_box(prims, kind, uint64_t(other & ~kLeafBit));
+    } else {
+      if (int64_t(other) >= n - 1)
+        return;
+      ob = load_node_box_coherent(&nodes[other]);
+    }
+    Box b = i_am_left ? box_union(mine, ob) : box_union(ob, mine);
+    store_node_box_coherent(&nodes[p], b);
+    coherent_store_u32(&flags[p], 0u);
+    if (p == 0u)
+      return;
+    vmem_drain();
+    mine = b;
+    my_ref = p;
+    p = parent[p];
+  }
+}
+
+__global__ void
+k_fill_u32(uint32_t* __restrict__ p, uint64_t words, uint32_t value)
+{
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride)
+    p[i] = value;
+}
+
+} // namespace lbvh_dev
+
+namespace lbvh_eng {
+
+using namespace lbvh_dev;
+
+static inline uint32_t
+grid_for(uint64_t n, uint32_t cap)
+{
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1)
+    g = 1;
+  if (g > cap)
+    g = cap;
+  return uint32_t(g);
+}
+
+uint32_t
+centroid_partials_needed(uint64_t n)
+{
+  return grid_for(n, kCentroidMaxBlocks);
+}
+
+hipError_t
+launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx, float* cy,
+                       float* cz, float* partials, uint32_t* ticket, float* bounds,
+                       uint32_t* zero_ptr, uint32_t zero_words, hipStream_t s)
+{
+  uint32_t g = grid_for(n, kCentroidMaxBlocks);
+  hipLaunchKernelGGL(k_centroid_bounds, dim3(g), dim3(kBlock), 0, s, prims, kind, n, cx, cy,
+                     cz, partials, ticket, bounds, zero_ptr, zero_words);
+  return hipGetLastError();
+}
+
+hipError_t
+launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, uint64_t n,
+              const float* bounds, void* codes, uint32_t* hist, int digit_bits, int passes,
+              hipStream_t s)
+{
+  if ((1 << digit_bits) * passes > kHistMaxWords)
+    return hipErrorInvalidValue;
+  size_t lds = size_t(1 << digit_bits) * passes * sizeof(uint32_t);
+  uint32_t g = grid_for(n, 2048);
+  if (code_bits == 30)
+    hipLaunchKernelGGL(k_encode<uint32_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
+                       bounds, static_cast<uint32_t*>(codes), hist, digit_bits, passes);
+  else
+    hipLaunchKernelGGL(k_encode<uint64_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
+                       bounds, static_cast<uint64_t*>(codes), hist, digit_bits, passes);
+  return hipGetLastError();
+}
+
+hipError_t
+launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n, void* nodes,
+              uint32_t* parent, uint32_t* flags, hipStream_t s)
+{
+  uint64_t m = n - 1;
+  uint32_t g = uint32_t((m + kBlock - 1) / kBlock);
+  if (code_bits == 30)
+    hipLaunchKernelGGL(k_karras<uint32_t>, dim3(g), dim3(kBlock), 0, s,
+                       static_cast<const uint32_t*>(sorted), ids, int64_t(n),
+                       static_cast<Node*>(nodes), parent, flags);
+  else
+    hipLaunchKernelGGL(k_karras<uint64_t>, dim3(g), dim3(kBlock), 0, s,
+                       static_cast<const uint64_t*>(sorted), ids, int64_t(n),
+                       static_cast<Node*>(nodes), parent, flags);
+  return hipGetLastError();
+}
+
+hipError_t
+launch_fit(const float* prims, int kind, uint64_t n, void* nodes, const uint32_t* parent,
+           uint32_t* flags, hipStream_t s)
+{
+  uint32_t g = uint32_t((n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_fit, dim3(g), dim3(kBlock), 0, s, prims, kind, int64_t(n),
+                     static_cast<Node*>(nodes), parent, flags);
+  return hipGetLastError();
+}
+
+hipError_t
+launch_fill_u32(uint32_t* p, uint64_t words, uint32_t value, hipStream_t s)
+{
+  if (words == 0)
+    return hipSuccess;
+  uint32_t g = grid_for(words, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, words, value);
+  return hipGetLastError();
+}
+
+} // namespace lbvh_eng

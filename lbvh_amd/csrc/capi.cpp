@@ -1,0 +1,825 @@
+// Host runtime and C ABI of the gfx950 LBVH engine (include/lbvh_mi355x.h).
+//
+// A context owns a HIP stream (or uses one the caller installs), a grow-only
+// device workspace reused across builds, and optional per-phase HIP-event
+// timing.  A bvh handle owns its node array, parent links, fit flags and
+// centroid bounds, and either owns a device copy of its primitives (host
+// builds) or borrows the caller's device pointer (*_device builds).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "lbvh_mi355x.h"
+
+using namespace lbvh_eng;
+
+namespace {
+
+struct DevBuf
+{
+  void* p = nullptr;
+  uint64_t cap = 0;
+};
+
+} // namespace
+
+// One timed call: its kind and the events bracketing its phases.
+struct EventSet
+{
+  int kind = 0; // 1 build, 2 trace, 3 refit
+  int recorded = 0;
+  hipEvent_t ev[8] = {};
+};
+
+struct lbvh_ctx
+{
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  std::vector<EventSet> sets; // pool; sets[0..used) belong to timed calls
+  size_t used = 0;
+  std::string last_error;
+  uint32_t epoch_next = 1;
+  // workspace
+  DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, small, status, io_in, io_out;
+  bool status_zeroed_words = 0;
+};
+
+struct lbvh_bvh
+{
+  int device = 0;
+  uint64_t n = 0;
+  uint32_t code_bits = 30;
+  uint32_t leaf_order = 0;
+  int prim_kind = 0; // 0 triangles, 1 aabbs
+  void* d_nodes = nullptr;
+  uint32_t* d_parent = nullptr;
+  uint32_t* d_flags = nullptr;
+  float* d_bounds = nullptr;
+  float* d_prims = nullptr;
+  bool owns_prims = false;
+  void* d_codes = nullptr;
+  uint32_t* d_ids = nullptr;
+  uint64_t bytes = 0;
+};
+
+namespace {
+
+// small workspace region layout (32-bit words)
+constexpr uint32_t kSmallBoundsTicket = 0; // self-resetting
+constexpr uint32_t kSmallErr = 1;          // cleared by the host after reading
+constexpr uint32_t kSmallTickets = 2;      // kMaxSortPasses words
+constexpr uint32_t kSmallHist = kSmallTickets + kMaxSortPasses;
+
+lbvh_status
+fail(lbvh_ctx* c, lbvh_status s, const std::string& msg)
+{
+  if (c)
+    c->last_error = msg;
+  return s;
+}
+
+lbvh_status
+hip_fail(lbvh_ctx* c, hipError_t e, const char* what)
+{
+  return fail(c, LBVH_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                                     \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return hip_fail((ctx), e_, #expr);                                       \
+  } while (0)
+
+lbvh_status
+ensure(lbvh_ctx* c, DevBuf& b, uint64_t bytes, bool zero_new = false)
+{
+  if (bytes == 0)
+    bytes = 256;
+  if (b.cap >= bytes)
+    return LBVH_OK;
+  if (b.p) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess)
+      return hip_fail(c, e, "hipStreamSynchronize");
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  uint64_t cap = (bytes + 4095) & ~uint64_t(4095);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, cap);
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return fail(c, LBVH_E_OOM, "device allocation of " + std::to_string(cap) + " bytes failed");
+  }
+  b.p = p;
+  b.cap = cap;
+  if (zero_new) {
+    hipError_t z = launch_fill_u32(static_cast<uint32_t*>(p), cap / 4, 0u, c->stream);
+    if (z != hipSuccess)
+      return hip_fail(c, z, "fill");
+  }
+  return LBVH_OK;
+}
+
+void
+dev_free(void* p)
+{
+  if (p)
+    (void)hipFree(p);
+}
+
+lbvh_status
+set_device(lbvh_ctx* c)
+{
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess)
+    return hip_fail(c, e, "hipSetDevice");
+  return LBVH_OK;
+}
+
+// Starts the event set of a timed call (no-op when timing is off).
+void
+begin_timed(lbvh_ctx* c, int kind)
+{
+  if (!c->timing)
+    return;
+  if (c->used == c->sets.size()) {
+    EventSet es;
+    for (auto& e : es.ev)
+      (void)hipEventCreate(&e);
+    c->sets.push_back(es);
+  }
+  EventSet& es = c->sets[c->used++];
+  es.kind = kind;
+  es.recorded = 0;
+}
+
+void
+record(lbvh_ctx* c, int idx)
+{
+  if (c->timing && c->used > 0 && idx < 8) {
+    EventSet& es = c->sets[c->used - 1];
+    (void)hipEventRecord(es.ev[idx], c->stream);
+    es.recorded = idx + 1;
+  }
+}
+
+// Sort tuning: digit width and tile by size and key width (see
+// tune_sort_config callers; overridable through the build options).
+SortConfig
+tune_sort(uint64_t n, uint32_t code_bits, const lbvh_build_opts* o)
+{
+  SortConfig c;
+  c.digit_bits = 8;
+  c.items = (n <= (uint64_t(1) << 21)) ? 8 : 16;
+  (void)code_bits;
+  if (o && o->sort_digit_bits)
+    c.digit_bits = int(o->sort_digit_bits);
+  if (o && o->sort_tile)
+    c.items = int(o->sort_tile / 256u);
+  return c;
+}
+
+bool
+valid_sort_config(const SortConfig& c)
+{
+  if (c.digit_bits == 8)
+    return c.items == 4 || c.items == 8 || c.items == 16;
+  if (c.digit_bits == 10)
+    return c.items == 8 || c.items == 16;
+  return false;
+}
+
+lbvh_status
+alloc_bvh(lbvh_ctx* c, lbvh_bvh* b, bool keep_codes)
+{
+  const uint64_t n = b->n;
+  const uint64_t m = n - 1;
+  size_t cw = b->code_bits == 30 ? 4 : 8;
+  struct
+  {
+    void** p;
+    uint64_t bytes;
+  } req[] = {
+    { &b->d_nodes, m * 32 },
+    { reinterpret_cast<void**>(&b->d_parent), (2 * n - 1) * 4 },
+    { reinterpret_cast<void**>(&b->d_flags), m * 4 },
+    { reinterpret_cast<void**>(&b->d_bounds), 32 },
+    { &b->d_codes, keep_codes ? n * cw : 0 },
+    { reinterpret_cast<void**>(&b->d_ids), keep_codes ? n * 4 : 0 },
+  };
+  for (auto& r : req) {
+    if (!r.bytes)
+      continue;
+    hipError_t e = hipMalloc(r.p, r.bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, LBVH_E_OOM, "device allocation for the tree failed");
+    }
+    b->bytes += r.bytes;
+  }
+  return LBVH_OK;
+}
+
+void
+free_bvh(lbvh_bvh* b)
+{
+  if (!b)
+    return;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(b->device);
+  dev_free(b->d_nodes);
+  dev_free(b->d_parent);
+  dev_free(b->d_flags);
+  dev_free(b->d_bounds);
+  dev_free(b->d_codes);
+  dev_free(b->d_ids);
+  if (b->owns_prims)
+    dev_free(b->d_prims);
+  (void)hipSetDevice(cur);
+  delete b;
+}
+
+lbvh_status
+build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
+             const lbvh_build_opts* opts_in, bool owns, lbvh_bvh** out)
+{
+  lbvh_build_opts opts;
+  lbvh_build_opts_init(&opts);
+  if (opts_in)
+    opts = *opts_in;
+  if (opts.code_bits != 30 && opts.code_bits != 63)
+    return fail(c, LBVH_E_INVALID, "code_bits must be 30 or 63");
+  if (opts.leaf_order > 1)
+    return fail(c, LBVH_E_INVALID, "unknown leaf order");
+  if (n < 2)
+    return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
+  if (n >= (uint64_t(1) << 30))
+    return fail(c, LBVH_E_INVALID, "at most 2^30 - 1 primitives per tree");
+  SortConfig cfg = tune_sort(n, opts.code_bits, &opts);
+  if (!valid_sort_config(cfg))
+    return fail(c, LBVH_E_INVALID, "unsupported sort digit width / tile");
+  const int key_bits = opts.code_bits == 30 ? 30 : 60;
+  const int passes = sort_passes(key_bits, cfg.digit_bits);
+  const uint32_t radix = 1u << cfg.digit_bits;
+  const size_t kw = opts.code_bits == 30 ? 4 : 8;
+  const bool with_ids = opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID || opts.keep_codes;
+
+  lbvh_status st;
+  if ((st = ensure(c, c->cent, 12 * n)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->keys_a, kw * n)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->keys_b, kw * n)) != LBVH_OK)
+    return st;
+  if (with_ids) {
+    if ((st = ensure(c, c->ids_a, 4 * n)) != LBVH_OK)
+      return st;
+    if ((st = ensure(c, c->ids_b, 4 * n)) != LBVH_OK)
+      return st;
+  }
+  if ((st = ensure(c, c->partials, 24ull * centroid_partials_needed(n))) != LBVH_OK)
+    return st;
+  const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
+  if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
+    return st;
+  const uint64_t sw = sort_status_words(n, cfg);
+  if (c->status.cap < sw * 8) {
+    if ((st = ensure(c, c->status, sw * 8, true)) != LBVH_OK)
+      return st;
+  }
+  if (c->epoch_next + kMaxSortPasses >= (1u << 24)) {
+    HIP_TRY(c, launch_fill_u32(static_cast<uint32_t*>(c->status.p), c->status.cap / 4, 0u,
+                               c->stream));
+    c->epoch_next = 1;
+  }
+  const uint32_t epoch = c->epoch_next;
+  c->epoch_next += kMaxSortPasses;
+
+  auto* b = new (std::nothrow) lbvh_bvh;
+  if (!b)
+    return fail(c, LBVH_E_OOM, "host allocation failed");
+  b->device = c->device;
+  b->n = n;
+  b->code_bits = opts.code_bits;
+  b->leaf_order = opts.leaf_order;
+  b->prim_kind = kind;
+  b->d_prims = const_cast<float*>(d_prims);
+  b->owns_prims = owns;
+  if ((st = alloc_bvh(c, b, opts.keep_codes != 0)) != LBVH_OK) {
+    b->owns_prims = false;
+    free_bvh(b);
+    return st;
+  }
+  if (owns)
+    b->bytes += n * (kind == 0 ? 48 : 24);
+
+  uint32_t* small = static_cast<uint32_t*>(c->small.p);
+  float* cent = static_cast<float*>(c->cent.p);
+  SortWorkspace ws;
+  ws.status = static_cast<uint64_t*>(c->status.p);
+  ws.status_words = sw;
+  ws.tickets = small + kSmallTickets;
+  ws.hist = small + kSmallHist;
+  ws.err = small + kSmallErr;
+
+  begin_timed(c, 1);
+  record(c, 0);
+  hipError_t e = launch_centroid_bounds(d_prims, kind, n, cent, cent + n, cent + 2 * n,
+                                        static_cast<float*>(c->partials.p),
+                                        small + kSmallBoundsTicket, b->d_bounds,
+                                        small + kSmallTickets, small_words - kSmallTickets,
+                                        c->stream);
+  if (e == hipSuccess)
+    e = launch_encode(opts.code_bits, cent, cent + n, cent + 2 * n, n, b->d_bounds,
+                      c->keys_a.p, ws.hist, cfg.digit_bits, passes, c->stream);
+  record(c, 1);
+  bool in_b = false;
+  if (e == hipSuccess)
+    e = launch_sort(key_bits, n, c->keys_a.p, c->keys_b.p,
+                    with_ids ? static_cast<uint32_t*>(c->ids_a.p) : nullptr,
+                    with_ids ? static_cast<uint32_t*>(c->ids_b.p) : nullptr, cfg, ws, epoch,
+                    &in_b, c->stream);
+  record(c, 2);
+  void* sorted = in_b ? c->keys_b.p : c->keys_a.p;
+  uint32_t* sorted_ids = with_ids ? static_cast<uint32_t*>(in_b ? c->ids_b.p : c->ids_a.p)
+                                  : nullptr;
+  if (e == hipSuccess)
+    e = launch_karras(opts.code_bits, sorted,
+                      opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr, n,
+                      b->d_nodes, b->d_parent, b->d_flags, c->stream);
+  record(c, 3);
+  if (e == hipSuccess)
+    e = launch_fit(d_prims, kind, n, b->d_nodes, b->d_parent, b->d_flags, c->stream);
+  record(c, 4);
+  if (e == hipSuccess && opts.keep_codes) {
+    e = hipMemcpyAsync(b->d_codes, sorted, kw * n, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b->d_ids, sorted_ids, 4 * n, hipMemcpyDeviceToDevice, c->stream);
+  }
+  if (e != hipSuccess) {
+    b->owns_prims = false;
+    free_bvh(b);
+    return hip_fail(c, e, "build launch");
+  }
+  *out = b;
+  return LBVH_OK;
+}
+
+// Reads and clears the bounded-wait error flag; synchronizes the stream.
+lbvh_status
+sync_and_check(lbvh_ctx* c)
+{
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (c->small.p) {
+    uint32_t err = 0;
+    HIP_TRY(c, hipMemcpy(&err, static_cast<uint32_t*>(c->small.p) + kSmallErr, 4,
+                         hipMemcpyDeviceToHost));
+    if (err) {
+      HIP_TRY(c, launch_fill_u32(static_cast<uint32_t*>(c->small.p) + kSmallErr, 1, 0u,
+                                 c->stream));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      return fail(c, LBVH_E_TIMEOUT, "a radix-sort look-back wait exceeded its spin limit");
+    }
+  }
+  return LBVH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char*
+lbvh_status_string(lbvh_status s)
+{
+  switch (s) {
+    case LBVH_OK:
+      return "ok";
+    case LBVH_E_INVALID:
+      return "invalid argument";
+    case LBVH_E_OOM:
+      return "out of device memory";
+    case LBVH_E_HIP:
+      return "HIP runtime error";
+    case LBVH_E_BUSY:
+      return "busy";
+    case LBVH_E_NODEVICE:
+      return "no usable gfx950 device";
+    case LBVH_E_TIMEOUT:
+      return "bounded wait expired";
+  }
+  return "unknown status";
+}
+
+void
+lbvh_build_opts_init(lbvh_build_opts* o)
+{
+  if (!o)
+    return;
+  std::memset(o, 0, sizeof(*o));
+  o->code_bits = 30;
+  o->leaf_order = LBVH_LEAF_REFERENCE;
+}
+
+lbvh_status
+lbvh_ctx_create(int device, lbvh_ctx** out)
+{
+  if (!out)
+    return LBVH_E_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    (void)hipGetLastError();
+    return LBVH_E_NODEVICE;
+  }
+  if (device < 0 || device >= count)
+    return LBVH_E_INVALID;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+    return LBVH_E_NODEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return LBVH_E_NODEVICE;
+  auto* c = new (std::nothrow) lbvh_ctx;
+  if (!c)
+    return LBVH_E_OOM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->own_stream,
+                                                                      hipStreamNonBlocking) !=
+                                               hipSuccess) {
+    delete c;
+    return LBVH_E_HIP;
+  }
+  c->stream = c->own_stream;
+  *out = c;
+  return LBVH_OK;
+}
+
+void
+lbvh_ctx_destroy(lbvh_ctx* c)
+{
+  if (!c)
+    return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = { &c->cent, &c->keys_a, &c->keys_b, &c->ids_a, &c->ids_b, &c->partials,
+                     &c->small, &c->status, &c->io_in, &c->io_out };
+  for (auto* b : bufs)
+    dev_free(b->p);
+  for (auto& es : c->sets)
+    for (auto& e : es.ev)
+      if (e)
+        (void)hipEventDestroy(e);
+  if (c->own_stream)
+    (void)hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+lbvh_status
+lbvh_ctx_set_stream(lbvh_ctx* c, void* stream)
+{
+  if (!c)
+    return LBVH_E_INVALID;
+  c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+  return LBVH_OK;
+}
+
+lbvh_status
+lbvh_ctx_synchronize(lbvh_ctx* c)
+{
+  if (!c)
+    return LBVH_E_INVALID;
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  return sync_and_check(c);
+}
+
+lbvh_status
+lbvh_ctx_set_timing(lbvh_ctx* c, int enable)
+{
+  if (!c)
+    return LBVH_E_INVALID;
+  c->timing = enable != 0;
+  c->used = 0;
+  return LBVH_OK;
+}
+
+// Sums the phase times of timed calls of `kind` (0 = the last call only).
+static lbvh_status
+phase_sum(lbvh_ctx* c, int kind, float* ms, uint32_t cap, uint32_t* count, uint32_t* calls)
+{
+  if (!c)
+    return LBVH_E_INVALID;
+  if (count)
+    *count = 0;
+  if (calls)
+    *calls = 0;
+  if (c->used == 0)
+    return LBVH_OK;
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  size_t first = kind == 0 ? c->used - 1 : 0;
+  uint32_t nph = 0, ncalls = 0;
+  for (uint32_t i = 0; i < cap; i++)
+    ms[i] = 0.0f;
+  for (size_t s = first; s < c->used; s++) {
+    EventSet& es = c->sets[s];
+    if (kind != 0 && es.kind != kind)
+      continue;
+    uint32_t k = es.recorded > 1 ? uint32_t(es.recorded - 1) : 0u;
+    if (k == 0)
+      continue;
+    HIP_TRY(c, hipEventSynchronize(es.ev[k]));
+    for (uint32_t i = 0; i < k && i < cap; i++) {
+      float v = 0.0f;
+      HIP_TRY(c, hipEventElapsedTime(&v, es.ev[i], es.ev[i + 1]));
+      ms[i] += v;
+    }
+    if (k > nph)
+      nph = k;
+    ncalls++;
+  }
+  if (count)
+    *count = nph;
+  if (calls)
+    *calls = ncalls;
+  return LBVH_OK;
+}
+
+lbvh_status
+lbvh_ctx_phase_ms(lbvh_ctx* c, float* ms, uint32_t cap, uint32_t* count)
+{
+  return phase_sum(c, 0, ms, cap, count, nullptr);
+}
+
+lbvh_status
+lbvh_ctx_phase_stats(lbvh_ctx* c, int kind, float* sum_ms, uint32_t cap, uint32_t* count,
+                     uint32_t* calls)
+{
+  if (kind < 1 || kind > 3)
+    return fail(c, LBVH_E_INVALID, "kind is 1 (build), 2 (trace) or 3 (refit)");
+  return phase_sum(c, kind, sum_ms, cap, count, calls);
+}
+
+const char*
+lbvh_ctx_last_error(const lbvh_ctx* c)
+{
+  return c ? c->last_error.c_str() : "null context";
+}
+
+uint64_t
+lbvh_ctx_workspace_bytes(const lbvh_ctx* c)
+{
+  if (!c)
+    return 0;
+  return c->cent.cap + c->keys_a.cap + c->keys_b.cap + c->ids_a.cap + c->ids_b.cap +
+         c->partials.cap + c->small.cap + c->status.cap + c->io_in.cap + c->io_out.cap;
+}
+
+lbvh_status
+lbvh_build_triangles_device(lbvh_ctx* c, const float* d_tri, uint64_t n,
+                            const lbvh_build_opts* opts, lbvh_bvh** out)
+{
+  if (!c || !out || (!d_tri && n))
+    return fail(c, LBVH_E_INVALID, "null argument");
+  *out = nullptr;
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  return build_common(c, d_tri, 0, n, opts, false, out);
+}
+
+lbvh_status
+lbvh_build_aabbs_device(lbvh_ctx* c, const float* d_boxes, uint64_t n,
+                        const lbvh_build_opts* opts, lbvh_bvh** out)
+{
+  if (!c || !out || (!d_boxes && n))
+    return fail(c, LBVH_E_INVALID, "null argument");
+  *out = nullptr;
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  return build_common(c, d_boxes, 1, n, opts, false, out);
+}
+
+static lbvh_status
+build_host(lbvh_ctx* c, const float* host, int kind, uint64_t n, const lbvh_build_opts* opts,
+           lbvh_bvh** out)
+{
+  if (!c || !out || (!host && n))
+    return fail(c, LBVH_E_INVALID, "null argument");
+  *out = nullptr;
+  if (n < 2)
+    return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  const uint64_t bytes = n * (kind == 0 ? 48 : 24);
+  float* d = nullptr;
+  if (hipMalloc(&d, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, LBVH_E_OOM, "device allocation for primitives failed");
+  }
+  hipError_t e = hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) {
+    dev_free(d);
+    return hip_fail(c, e, "hipMemcpyAsync");
+  }
+  st = build_common(c, d, kind, n, opts, true, out);
+  if (st != LBVH_OK) {
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(d);
+    return st;
+  }
+  st = sync_and_check(c);
+  if (st != LBVH_OK) {
+    free_bvh(*out);
+    *out = nullptr;
+  }
+  return st;
+}
+
+lbvh_status
+lbvh_build_triangles(lbvh_ctx* c, const float* tri, uint64_t n, const lbvh_build_opts* opts,
+                     lbvh_bvh** out)
+{
+  return build_host(c, tri, 0, n, opts, out);
+}
+
+lbvh_status
+lbvh_build_aabbs(lbvh_ctx* c, const float* boxes, uint64_t n, const lbvh_build_opts* opts,
+                 lbvh_bvh** out)
+{
+  return build_host(c, boxes, 1, n, opts, out);
+}
+
+lbvh_status
+lbvh_refit_triangles_device(lbvh_ctx* c, lbvh_bvh* b, const float* d_tri)
+{
+  if (!c || !b || !d_tri)
+    return fail(c, LBVH_E_INVALID, "null argument");
+  if (b->prim_kind != 0)
+    return fail(c, LBVH_E_INVALID, "refit_triangles on a tree built from boxes");
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  if (b->owns_prims) {
+    HIP_TRY(c, hipMemcpyAsync(b->d_prims, d_tri, b->n * 48, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    b->d_prims = const_cast<float*>(d_tri);
+  }
+  begin_timed(c, 3);
+  record(c, 0);
+  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_flags, c->stream));
+  record(c, 1);
+  return LBVH_OK;
+}
+
+lbvh_status
+lbvh_refit_triangles(lbvh_ctx* c, lbvh_bvh* b, const float* tri)
+{
+  if (!c || !b || !tri)
+    return fail(c, LBVH_E_INVALID, "null argument");
+  if (b->prim_kind != 0)
+    return fail(c, LBVH_E_INVALID, "refit_triangles on a tree built from boxes");
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  if (!b->owns_prims) {
+    float* d = nullptr;
+    if (hipMalloc(&d, b->n * 48) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, LBVH_E_OOM, "device allocation for primitives failed");
+    }
+    b->d_prims = d;
+    b->owns_prims = true;
+    b->bytes += b->n * 48;
+  }
+  HIP_TRY(c, hipMemcpyAsync(b->d_prims, tri, b->n * 48, hipMemcpyHostToDevice, c->stream));
+  begin_timed(c, 3);
+  record(c, 0);
+  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_flags, c->stream));
+  record(c, 1);
+  return sync_and_check(c);
+}
+
+lbvh_status
+lbvh_bvh_get_info(const lbvh_bvh* b, lbvh_bvh_info* out)
+{
+  if (!b || !out)
+    return LBVH_E_INVALID;
+  out->n_prims = b->n;
+  out->n_nodes = b->n - 1;
+  out->code_bits = b->code_bits;
+  out->leaf_order = b->leaf_order;
+  out->d_nodes = b->d_nodes;
+  out->d_prims = b->prim_kind == 0 ? b->d_prims : nullptr;
+  out->device_bytes = b->bytes;
+  return LBVH_OK;
+}
+
+lbvh_status
+lbvh_bvh_download_nodes(lbvh_ctx* c, const lbvh_bvh* b, void* dst, uint64_t cap)
+{
+  if (!c || !b || !dst)
+    return fail(c, LBVH_E_INVALID, "null argument");
+  const uint64_t bytes = (b->n - 1) * 32;
+  if (cap < bytes)
+    return fail(c, LBVH_E_INVALID, "destination too small for the node array");
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  HIP_TRY(c, hipMemcpyAsync(dst, b->d_nodes, bytes, hipMemcpyDeviceToHost, c->stream));
+  return sync_and_check(c);
+}
+
+lbvh_status
+lbvh_bvh_download_codes(lbvh_ctx* c, const lbvh_bvh* b, void* codes, uint32_t* ids,
+                        float* bounds6)
+{
+  if (!c || !b)
+    return fail(c, LBVH_E_INVALID, "null argument");
+  if ((codes || ids) && !b->d_codes)
+    return fail(c, LBVH_E_INVALID, "tree was built without keep_codes");
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  if (codes)
+    HIP_TRY(c, hipMemcpyAsync(codes, b->d_codes, b->n * (b->code_bits == 30 ? 4 : 8),
+                              hipMemcpyDeviceToHost, c->stream));
+  if (ids)
+    HIP_TRY(c, hipMemcpyAsync(ids, b->d_ids, b->n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (bounds6)
+    HIP_TRY(c, hipMemcpyAsync(bounds6, b->d_bounds, 24, hipMemcpyDeviceToHost, c->stream));
+  return sync_and_check(c);
+}
+
+void
+lbvh_bvh_destroy(lbvh_bvh* b)
+{
+  free_bvh(b);
+}
+
+lbvh_status
+lbvh_trace_closest_device(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, uint64_t nrays,
+                          int mode, uint32_t* d_prim, float* d_t, float* d_uv)
+{
+  if (!c || !b || (nrays && (!d_rays || !d_prim || !d_t)))
+    return fail(c, LBVH_E_INVALID, "null argument");
+  if (b->prim_kind != 0)
+    return fail(c, LBVH_E_INVALID, "device traversal needs a triangle tree");
+  if (mode < 0 || mode > 2)
+    return fail(c, LBVH_E_INVALID, "unknown intersector mode");
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  begin_timed(c, 2);
+  record(c, 0);
+  HIP_TRY(c, launch_trace(mode, b->d_nodes, b->d_prims, b->n, d_rays, nrays, d_prim, d_t, d_uv,
+                          c->stream));
+  record(c, 1);
+  return LBVH_OK;
+}
+
+lbvh_status
+lbvh_trace_closest(lbvh_ctx* c, const lbvh_bvh* b, const float* rays, uint64_t nrays, int mode,
+                   uint32_t* prim, float* t, float* uv)
+{
+  if (!c || !b || (nrays && (!rays || !prim || !t)))
+    return fail(c, LBVH_E_INVALID, "null argument");
+  if (nrays == 0)
+    return LBVH_OK;
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->io_in, nrays * 32)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->io_out, nrays * 16)) != LBVH_OK)
+    return st;
+  float* d_rays = static_cast<float*>(c->io_in.p);
+  uint32_t* d_prim = static_cast<uint32_t*>(c->io_out.p);
+  float* d_t = reinterpret_cast<float*>(d_prim + nrays);
+  float* d_uv = d_t + nrays;
+  HIP_TRY(c, hipMemcpyAsync(d_rays, rays, nrays * 32, hipMemcpyHostToDevice, c->stream));
+  st = lbvh_trace_closest_device(c, b, d_rays, nrays, mode, d_prim, d_t, uv ? d_uv : nullptr);
+  if (st != LBVH_OK)
+    return st;
+  HIP_TRY(c, hipMemcpyAsync(prim, d_prim, nrays * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(t, d_t, nrays * 4, hipMemcpyDeviceToHost, c->stream));
+  if (uv)
+    HIP_TRY(c, hipMemcpyAsync(uv, d_uv, nrays * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync_and_check(c);
+}
+
+} // extern "C"

@@ -1,0 +1,74 @@
+// Host-side launch interface of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbvh_eng {
+
+constexpr uint32_t kMaxSortPasses = 16;
+
+// Radix-sort launch configuration chosen by the tuner.
+struct SortConfig
+{
+  int digit_bits = 8; // 8 or 10 (radix 256 / 1024)
+  int items = 8;      // keys per thread; tile = 256 * items
+};
+
+inline int
+sort_passes(int key_bits, int digit_bits)
+{
+  return (key_bits + digit_bits - 1) / digit_bits;
+}
+
+// Scratch needed by the sort for n keys.
+struct SortWorkspace
+{
+  uint64_t* status = nullptr;  // [nblocks * radix] look-back words
+  uint64_t status_words = 0;
+  uint32_t* tickets = nullptr; // [kMaxSortPasses] virtual block counters
+  uint32_t* hist = nullptr;    // [passes * radix] global digit histograms
+  uint32_t* err = nullptr;     // bounded-wait failure flag
+};
+
+// centroid of every primitive box (SoA) and the centroid bounds; also zeroes
+// `zero_words` 32-bit words at `zero_ptr` (histograms, tickets, error flag).
+hipError_t launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx,
+                                  float* cy, float* cz, float* partials, uint32_t* ticket,
+                                  float* bounds, uint32_t* zero_ptr, uint32_t zero_words,
+                                  hipStream_t s);
+uint32_t centroid_partials_needed(uint64_t n);
+
+// Morton codes (u32 for 30-bit, u64 for 63-bit) and per-pass digit histograms.
+hipError_t launch_encode(int code_bits, const float* cx, const float* cy, const float* cz,
+                         uint64_t n, const float* bounds, void* codes, uint32_t* hist,
+                         int digit_bits, int passes, hipStream_t s);
+
+// LSD onesweep radix sort of (key, id) pairs in place across ping-pong
+// buffers; ids_in == nullptr means ids = 0..n-1; ids_a == nullptr sorts keys
+// only.  Returns in *result_in_b whether the sorted data ended in the B
+// buffers.  `epoch` must change between sorts sharing `ws.status`.
+hipError_t launch_sort(int key_bits, uint64_t n, void* keys_a, void* keys_b, uint32_t* ids_a,
+                       uint32_t* ids_b, const SortConfig& cfg, const SortWorkspace& ws,
+                       uint32_t epoch, bool* result_in_b, hipStream_t s);
+uint64_t sort_status_words(uint64_t n, const SortConfig& cfg);
+
+// Karras radix tree: left/right of every internal node, parent links
+// (internal node c at c, leaf value v at n-1+v), fit flags cleared.
+hipError_t launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
+                         void* nodes, uint32_t* parent, uint32_t* flags, hipStream_t s);
+
+// Bottom-up box fit with atomic visit flags (flags must be zero; they are
+// left zero again).
+hipError_t launch_fit(const float* prims, int kind, uint64_t n, void* nodes,
+                      const uint32_t* parent, uint32_t* flags, hipStream_t s);
+
+// Closest-hit traversal.
+hipError_t launch_trace(int mode, const void* nodes, const float* tris, uint64_t n,
+                        const float* rays, uint64_t nrays, uint32_t* prim, float* t, float* uv,
+                        hipStream_t s);
+
+// Fill `words` 32-bit words with `value`.
+hipError_t launch_fill_u32(uint32_t* p, uint64_t words, uint32_t value, hipStream_t s);
+
+} // namespace lbvh_eng

@@ -1,0 +1,357 @@
+// Onesweep-style LSD radix sort of Morton codes (and primitive ids) for gfx950.
+//
+// The reference orders its curve with an unstable std::sort on the code
+// (lbvh.h:1126-1134).  This sort is stable and keyed on the code only, so
+// ids stay in ascending order within equal codes: the "(code, id)" order,
+// one legal output of the reference's sort.
+//
+// One kernel per digit pass; the digit histograms of all passes come from
+// the encode kernel.  Per workgroup (4 waves, a tile of 256 * ITEMS keys):
+//   1. take a virtual tile id from an atomic ticket (tiles are numbered in
+//      the order workgroups start, so every tile a workgroup waits on is
+//      already running);
+//   2. rank keys stably inside each wave with ballot-based digit matching
+//      (wave-striped tile order: item j of lane l is key j * 64 + l);
+//   3. publish the tile's digit counts, then resolve each digit's global
+//      offset by decoupled look-back over earlier tiles' epoch-tagged status
+//      words (windowed: several predecessors are read per step);
+//   4. stage the tile in LDS in digit order and write it out so that
+//      consecutive threads write consecutive addresses.
+// Every wait is bounded: past a spin limit the kernel raises an error flag
+// and drains, and the host reports a timeout instead of hanging.
+#include "device_common.hpp"
+#include "engine.hpp"
+
+namespace lbvh_dev {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortWaves = kSortBlock / kWave;
+constexpr int kLookbackWindow = 8;
+constexpr uint32_t kSpinLimit = 1u << 20;
+
+constexpr uint64_t kFlagAgg = 1;
+constexpr uint64_t kFlagInc = 2;
+
+__device__ __forceinline__ uint64_t
+pack_status(uint32_t epoch, uint64_t flag, uint32_t count)
+{
+  return (uint64_t(epoch & 0xffffffu) << 40) | (flag << 32) | uint64_t(count);
+}
+__device__ __forceinline__ bool
+status_ready(uint64_t s, uint32_t epoch)
+{
+  return uint32_t(s >> 40) == (epoch & 0xffffffu) && ((s >> 32) & 0xff) != 0;
+}
+
+// Exclusive scan of one value per thread across the workgroup.
+__device__ __forceinline__ uint32_t
+block_exclusive_scan(uint32_t x, uint32_t* s_tmp, uint32_t* total)
+{
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+  uint32_t incl = x;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t y = __shfl_up(incl, off, kWave);
+    if (lane >= off)
+      incl += y;
+  }
+  if (lane == kWave - 1)
+    s_tmp[w] = incl;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kSortWaves; i++) {
+    uint32_t v = s_tmp[i];
+    woff += (i < w) ? v : 0u;
+    tot += v;
+  }
+  __syncthreads();
+  *total = tot;
+  return incl - x + woff;
+}
+
+template<typename KeyT, bool VALS, int RBITS, int ITEMS>
+__global__ __launch_bounds__(kSortBlock) void
+k_onesweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
+           const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout, uint32_t n,
+           int shift, const uint32_t* __restrict__ ghist, uint64_t* __restrict__ status,
+           uint32_t* __restrict__ ticket, uint32_t epoch, uint32_t* __restrict__ err)
+{
+  constexpr int RADIX = 1 << RBITS;
+  constexpr int TILE = kSortBlock * ITEMS;
+  constexpr int DPT = RADIX / kSortBlock; // digits owned per thread
+  static_assert(DPT >= 1, "radix must be >= workgroup size");
+
+  __shared__ KeyT s_keys[TILE];
+  __shared__ uint32_t s_vals[VALS ? TILE : 1];
+  __shared__ uint32_t s_wcnt[kSortWaves][RADIX];
+  __shared__ uint32_t s_bstart[RADIX];
+  __shared__ uint32_t s_goff[RADIX];
+  __shared__ uint32_t s_tmp[kSortWaves];
+  __shared__ uint32_t s_bid;
+
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int w = tid / kWave;
+  const KeyT dmask = KeyT(RADIX - 1);
+
+  if (tid == 0)
+    s_bid = atomic_add_agent(ticket, 1u);
+  for (int j = tid; j < kSortWaves * RADIX; j += kSortBlock)
+    (&s_wcnt[0][0])[j] = 0u;
+  __syncthreads();
+  const uint32_t bid = s_bid;
+
+  // ---- load (wave-striped) ----
+  const uint32_t base = bid * uint32_t(TILE) + uint32_t(w) * (ITEMS * kWave) + uint32_t(lane);
+  KeyT key[ITEMS];
+  uint32_t val[ITEMS];
+  uint32_t rank[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    uint32_t idx = base + uint32_t(j) * kWave;
+    bool valid = idx < n;
+    key[j] = valid ? kin[idx] : KeyT(0);
+    if (VALS)
+      val[j] = valid ? (vin ? vin[idx] : idx) : 0u;
+  }
+
+  // ---- stable ranking within the wave ----
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    const uint32_t idx = base + uint32_t(j) * kWave;
+    const bool valid = idx < n;
+    const uint32_t d = uint32_t((key[j] >> shift) & dmask);
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < RBITS; b++) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = ballot(valid && bit);
+      peers &= bit ? m : ~m;
+    }
+    uint32_t r = 0;
+    if (valid) {
+      const uint32_t below = uint32_t(popc64(peers & lt));
+      const uint32_t cnt = uint32_t(popc64(peers));
+      const uint32_t wbase = s_wcnt[w][d];
+      r = wbase + below;
+      if (below == 0)
+        s_wcnt[w][d] = wbase + cnt;
+    }
+    rank[j] = r;
+  }
+  __syncthreads();
+
+  // ---- tile digit counts, wave prefixes, early publication ----
+  uint32_t tot[DPT];
+  uint32_t my_sum = 0;
+#pragma unroll
+  for (int dd = 0; dd < DPT; dd++) {
+    const int d = tid * DPT + dd;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int ww = 0; ww < kSortWaves; ww++) {
+      uint32_t c = s_wcnt[ww][d];
+      s_wcnt[ww][d] = acc;
+      acc += c;
+    }
+    tot[dd] = acc;
+    my_sum += acc;
+    coherent_store_u64(&status[uint64_t(bid) * RADIX + d],
+                       pack_status(epoch, bid == 0 ? kFlagInc : kFlagAgg, acc));
+  }
+  // block-exclusive digit starts
+  uint32_t tile_total;
+  uint32_t run = block_exclusive_scan(my_sum, s_tmp, &tile_total);
+#pragma unroll
+  for (int dd = 0; dd < DPT; dd++) {
+    s_bstart[tid * DPT + dd] = run;
+    run += tot[dd];
+  }
+  // global digit starts of this pass
+  uint32_t gh[DPT];
+  uint32_t gsum = 0;
+#pragma unroll
+  for (int dd = 0; dd < DPT; dd++) {
+    gh[dd] = ghist[tid * DPT + dd];
+    gsum += gh[dd];
+  }
+  uint32_t gtotal;
+  uint32_t grun = block_exclusive_scan(gsum, s_tmp, &gtotal);
+  __syncthreads(); // s_bstart and the wave prefixes are visible
+
+  // ---- stage the tile in LDS in digit order ----
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    const uint32_t idx = base + uint32_t(j) * kWave;
+    if (idx < n) {
+      const uint32_t d = uint32_t((key[j] >> shift) & dmask);
+      const uint32_t pos = s_bstart[d] + s_wcnt[w][d] + rank[j];
+      s_keys[pos] = key[j];
+      if (VALS)
+        s_vals[pos] = val[j];
+    }
+  }
+
+  // ---- decoupled look-back per owned digit ----
+#pragma unroll
+  for (int dd = 0; dd < DPT; dd++) {
+    const int d = tid * DPT + dd;
+    uint32_t excl = 0;
+    if (bid > 0) {
+      int64_t p = int64_t(bid) - 1;
+      uint32_t spins = 0;
+      bool done = false;
+      while (!done) {
+        uint64_t sv[kLookbackWindow];
+#pragma unroll
+        for (int q = 0; q < kLookbackWindow; q++) {
+          int64_t pq = p - q;
+          sv[q] = (pq >= 0) ? coherent_load_u64(&status[uint64_t(pq) * RADIX + d])
+                            : pack_status(epoch, kFlagInc, 0);
+        }
+        int consumed = 0;
+#pragma unroll
+        for (int q = 0; q < kLookbackWindow; q++) {
+          if (done || consumed != q)
+            continue;
+          const uint64_t s = sv[q];
+          if (!status_ready(s, epoch))
+            continue;
+          excl += uint32_t(s);
+          consumed = q + 1;
+          if (((s >> 32) & 0xff) == kFlagInc)
+            done = true;
+        }
+        if (done)
+          break;
+        p -= consumed;
+        if (consumed == 0) {
+          if (++spins > kSpinLimit || coherent_load_u32(err) != 0u) {
+            atomic_add_agent(err, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+    }
+    coherent_store_u64(&status[uint64_t(bid) * RADIX + d],
+                       pack_status(epoch, kFlagInc, excl + tot[dd]));
+    s_goff[d] = grun + excl - s_bstart[d];
+    grun += gh[dd];
+  }
+  __syncthreads();
+
+  // ---- write the tile out ----
+  const uint32_t tile_begin = bid * uint32_t(TILE);
+  const uint32_t tile_n = (n - tile_begin < uint32_t(TILE)) ? (n - tile_begin) : uint32_t(TILE);
+  for (uint32_t i = tid; i < tile_n; i += kSortBlock) {
+    const KeyT k = s_keys[i];
+    const uint32_t d = uint32_t((k >> shift) & dmask);
+    const uint32_t dst = s_goff[d] + i;
+    if (dst < n) { // always true unless a bounded wait expired (then err is set)
+      kout[dst] = k;
+      if (VALS)
+        vout[dst] = s_vals[i];
+    }
+  }
+}
+
+} // namespace lbvh_dev
+
+namespace lbvh_eng {
+
+using namespace lbvh_dev;
+
+uint64_t
+sort_status_words(uint64_t n, const SortConfig& cfg)
+{
+  const uint64_t tile = uint64_t(kSortBlock) * cfg.items;
+  const uint64_t blocks = (n + tile - 1) / tile;
+  return blocks * (uint64_t(1) << cfg.digit_bits);
+}
+
+template<typename KeyT, bool VALS, int RBITS, int ITEMS>
+static hipError_t
+run_passes(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb,
+           const SortWorkspace& ws, uint32_t epoch, bool* in_b, hipStream_t s)
+{
+  constexpr int RADIX = 1 << RBITS;
+  constexpr uint32_t TILE = kSortBlock * ITEMS;
+  const int passes = sort_passes(key_bits, RBITS);
+  const uint32_t blocks = uint32_t((n + TILE - 1) / TILE);
+  KeyT* src = ka;
+  KeyT* dst = kb;
+  uint32_t* vsrc = nullptr; // iota on the first pass
+  uint32_t* vdst = vb;
+  bool cur_b = false;
+  for (int p = 0; p < passes; p++) {
+    hipLaunchKernelGGL((k_onesweep<KeyT, VALS, RBITS, ITEMS>), dim3(blocks), dim3(kSortBlock),
+                       0, s, src, dst, vsrc, vdst, uint32_t(n), p * RBITS,
+                       ws.hist + size_t(p) * RADIX, ws.status, ws.tickets + p, epoch + uint32_t(p),
+                       ws.err);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
+    cur_b = !cur_b;
+    KeyT* t = src;
+    src = dst;
+    dst = t;
+    if (VALS) {
+      vsrc = vdst;
+      vdst = (vdst == vb) ? va : vb;
+    }
+  }
+  *in_b = cur_b;
+  return hipSuccess;
+}
+
+template<typename KeyT, bool VALS>
+static hipError_t
+dispatch(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb,
+         const SortConfig& cfg, const SortWorkspace& ws, uint32_t epoch, bool* in_b,
+         hipStream_t s)
+{
+  if (cfg.digit_bits == 8) {
+    if (cfg.items == 8)
+      return run_passes<KeyT, VALS, 8, 8>(key_bits, n, ka, kb, va, vb, ws, epoch, in_b, s);
+    if (cfg.items == 16)
+      return run_passes<KeyT, VALS, 8, 16>(key_bits, n, ka, kb, va, vb, ws, epoch, in_b, s);
+    if (cfg.items == 4)
+      return run_passes<KeyT, VALS, 8, 4>(key_bits, n, ka, kb, va, vb, ws, epoch, in_b, s);
+  } else if (cfg.digit_bits == 10) {
+    if (cfg.items == 8)
+      return run_passes<KeyT, VALS, 10, 8>(key_bits, n, ka, kb, va, vb, ws, epoch, in_b, s);
+    if (cfg.items == 16)
+      return run_passes<KeyT, VALS, 10, 16>(key_bits, n, ka, kb, va, vb, ws, epoch, in_b, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t
+launch_sort(int key_bits, uint64_t n, void* keys_a, void* keys_b, uint32_t* ids_a,
+            uint32_t* ids_b, const SortConfig& cfg, const SortWorkspace& ws, uint32_t epoch,
+            bool* result_in_b, hipStream_t s)
+{
+  if (n == 0 || n >= (uint64_t(1) << 31))
+    return hipErrorInvalidValue;
+  const bool vals = ids_a != nullptr;
+  if (key_bits <= 32) {
+    auto* a = static_cast<uint32_t*>(keys_a);
+    auto* b = static_cast<uint32_t*>(keys_b);
+    return vals ? dispatch<uint32_t, true>(key_bits, n, a, b, ids_a, ids_b, cfg, ws, epoch,
+                                           result_in_b, s)
+                : dispatch<uint32_t, false>(key_bits, n, a, b, nullptr, nullptr, cfg, ws, epoch,
+                                            result_in_b, s);
+  }
+  auto* a = static_cast<uint64_t*>(keys_a);
+  auto* b = static_cast<uint64_t*>(keys_b);
+  return vals ? dispatch<uint64_t, true>(key_bits, n, a, b, ids_a, ids_b, cfg, ws, epoch,
+                                         result_in_b, s)
+              : dispatch<uint64_t, false>(key_bits, n, a, b, nullptr, nullptr, cfg, ws, epoch,
+                                          result_in_b, s);
+}
+
+} // namespace lbvh_eng

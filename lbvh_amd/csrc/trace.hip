@@ -1,0 +1,340 @@
+// Closest-hit traversal on gfx950.
+//
+// Reproduces traverser::operator() (lbvh.h:1874-1942) exactly: the root is
+// pushed with tmin = +inf; a popped entry is skipped when the closest hit so
+// far is nearer than its tmin; leaf children are intersected at once, left
+// then right; internal children get the octant slab test (make_accel_ray,
+// lbvh.h:991-1010; detail::intersect, lbvh.h:1071-1091; hit iff
+// tmax >= max(0, tmin), lbvh.h:1023-1026) and are pushed far first, near
+// last; the stack holds 128 entries and drops pushes when full
+// (traversal_stack<float,128>, lbvh.h:1746-1789).
+#include "device_common.hpp"
+#include "engine.hpp"
+
+namespace lbvh_dev {
+
+constexpr int kTraceBlock = 256;
+constexpr int kStack = 128;
+
+struct Ray
+{
+  float o[3];
+  float d[3];
+  float tmin, tmax;
+};
+
+// lbvh::math::dot (lbvh.h:621-630): accumulation starts from 0.
+__device__ __forceinline__ float
+dot3(const float* a, const float* b)
+{
+  float out = 0.0f;
+  float px = a[0] * b[0];
+  out = out + px;
+  float py = a[1] * b[1];
+  out = out + py;
+  float pz = a[2] * b[2];
+  out = out + pz;
+  return out;
+}
+
+// lbvh::math::cross (lbvh.h:609-616)
+__device__ __forceinline__ void
+cross3(const float* a, const float* b, float* o)
+{
+  float ay_bz = a[1] * b[2], az_by = a[2] * b[1];
+  float az_bx = a[2] * b[0], ax_bz = a[0] * b[2];
+  float ax_by = a[0] * b[1], ay_bx = a[1] * b[0];
+  o[0] = ay_bz - az_by;
+  o[1] = az_bx - ax_bz;
+  o[2] = ax_by - ay_bx;
+}
+
+struct Tri
+{
+  float p0[3], e0[3], e1[3], nrm[3];
+};
+
+__device__ __forceinline__ Tri
+load_tri(const float* __restrict__ tris, uint32_t i)
+{
+  const float4* p = reinterpret_cast<const float4*>(tris + 12 * uint64_t(i));
+  float4 a = p[0], b = p[1], c = p[2];
+  Tri t;
+  t.p0[0] = a.x;
+  t.p0[1] = a.y;
+  t.p0[2] = a.z;
+  t.e0[0] = a.w;
+  t.e0[1] = b.x;
+  t.e0[2] = b.y;
+  t.e1[0] = b.z;
+  t.e1[1] = b.w;
+  t.e1[2] = c.x;
+  t.nrm[0] = c.y;
+  t.nrm[1] = c.z;
+  t.nrm[2] = c.w;
+  return t;
+}
+
+// triangle_intersector (lbvh.h:2055-2075): two-sided, not watertight.
+__device__ __forceinline__ bool
+isect_reference(const Tri& tr, const Ray& r, float& t_out, float& u_out, float& v_out)
+{
+  float c[3] = { tr.p0[0] - r.o[0], tr.p0[1] - r.o[1], tr.p0[2] - r.o[2] };
+  float rr[3];
+  cross3(r.d, c, rr);
+  float inv_det = 1.0f / dot3(tr.nrm, r.d);
+  float u = dot3(rr, tr.e1) * inv_det;
+  float v = dot3(rr, tr.e0) * inv_det;
+  float w = (1.0f - u) - v;
+  if ((u >= 0.0f) && (v >= 0.0f) && (w >= 0.0f)) {
+    float t = dot3(tr.nrm, c) * inv_det;
+    if ((t >= r.tmin) && (t <= r.tmax)) {
+      t_out = t;
+      u_out = u;
+      v_out = v;
+      return true;
+    }
+  }
+  return false;
+}
+
+// Watertight test (Woop, Benthin, Wald, JCGT 2(1) 2013) on A = p0,
+// B = p0 - e0, C = p0 + e1; u, v are the weights of B and C.
+__device__ __forceinline__ bool
+isect_watertight(const Tri& tr, const Ray& r, float& t_out, float& u_out, float& v_out)
+{
+  float ax = fabsf(r.d[0]), ay = fabsf(r.d[1]), az = fabsf(r.d[2]);
+  int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+  int kx = (kz == 2) ? 0 : kz + 1;
+  int ky = (kx == 2) ? 0 : kx + 1;
+  float dkz = r.d[0], dkx = r.d[0], dky = r.d[0];
+  // component selects without dynamic register indexing
+  auto comp = [](const float* v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); };
+  dkz = comp(r.d, kz);
+  if (dkz < 0.0f) {
+    int tmp = kx;
+    kx = ky;
+    ky = tmp;
+  }
+  dkx = comp(r.d, kx);
+  dky = comp(r.d, ky);
+  float Sx = dkx / dkz;
+  float Sy = dky / dkz;
+  float Sz = 1.0f / dkz;
+  float va[3], vb[3], vc[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float b = tr.p0[k] - tr.e0[k];
+    float c = tr.p0[k] + tr.e1[k];
+    va[k] = tr.p0[k] - r.o[k];
+    vb[k] = b - r.o[k];
+    vc[k] = c - r.o[k];
+  }
+  float az_ = comp(va, kz), bz_ = comp(vb, kz), cz_ = comp(vc, kz);
+  float sxa = Sx * az_, sya = Sy * az_;
+  float sxb = Sx * bz_, syb = Sy * bz_;
+  float sxc = Sx * cz_, syc = Sy * cz_;
+  float Ax = comp(va, kx) - sxa, Ay = comp(va, ky) - sya;
+  float Bx = comp(vb, kx) - sxb, By = comp(vb, ky) - syb;
+  float Cx = comp(vc, kx) - sxc, Cy = comp(vc, ky) - syc;
+  float cb = Cx * By, cyb = Cy * Bx;
+  float U = cb - cyb;
+  float acy = Ax * Cy, ayc = Ay * Cx;
+  float V = acy - ayc;
+  float bay = Bx * Ay, bya = By * Ax;
+  float W = bay - bya;
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+    return false;
+  float det = U + V;
+  det = det + W;
+  if (det == 0.0f)
+    return false;
+  float Az = Sz * az_, Bz = Sz * bz_, Cz = Sz * cz_;
+  float ua = U * Az, vb2 = V * Bz, wc = W * Cz;
+  float T = ua + vb2;
+  T = T + wc;
+  float rcp = 1.0f / det;
+  float t = T * rcp;
+  if (!(t >= r.tmin && t <= r.tmax))
+    return false;
+  t_out = t;
+  u_out = V * rcp;
+  v_out = W * rcp;
+  return true;
+}
+
+template<int MODE>
+__device__ __forceinline__ bool
+better(float t, uint32_t id, float best_t, uint32_t best_id)
+{
+  if (MODE == 0)
+    return t < best_t;
+  return (t < best_t) || (t == best_t && id < best_id && t != INFINITY);
+}
+
+template<int MODE>
+__global__ __launch_bounds__(kTraceBlock) void
+k_trace(const Node* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
+        const float* __restrict__ rays, uint64_t nrays, uint32_t* __restrict__ prim_out,
+        float* __restrict__ t_out, float* __restrict__ uv_out)
+{
+  const uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+  if (i >= nrays)
+    return;
+  const float4* rp = reinterpret_cast<const float4*>(rays + 8 * i);
+  const float4 r0 = rp[0], r1 = rp[1];
+  Ray r;
+  r.o[0] = r0.x;
+  r.o[1] = r0.y;
+  r.o[2] = r0.z;
+  r.d[0] = r0.w;
+  r.d[1] = r1.x;
+  r.d[2] = r1.y;
+  r.tmin = r1.z;
+  r.tmax = r1.w;
+
+  // make_accel_ray
+  float rcp[3], ipos[3];
+  bool pos_dir[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    rcp[k] = 1.0f / r.d[k];
+    float np = -r.o[k];
+    ipos[k] = np * rcp[k];
+    pos_dir[k] = r.d[k] > 0.0f;
+  }
+
+  uint32_t st_node[kStack];
+  float st_t[kStack];
+  int sp = 0;
+  st_node[0] = 0u;
+  st_t[0] = INFINITY;
+  sp = 1;
+
+  uint32_t best = 0xffffffffu;
+  float best_t = INFINITY, best_u = 0.0f, best_v = 0.0f;
+
+  while (sp > 0) {
+    sp--;
+    const uint32_t ni = st_node[sp];
+    const float etmin = st_t[sp];
+    if (best_t < etmin)
+      continue;
+    const uint2 lr = *reinterpret_cast<const uint2*>(&nodes[ni].left);
+    bool lh = false, rh = false;
+    float lt = 0.0f, rt = 0.0f;
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const uint32_t ch = side ? lr.y : lr.x;
+      if ((ch & kLeafBit) ? ((ch & ~kLeafBit) >= nprims) : (ch + 1 >= nprims))
+        continue; // out-of-range child: only possible for a corrupt node array
+      if (ch & kLeafBit) {
+        const uint32_t p = ch & ~kLeafBit;
+        const Tri tr = load_tri(tris, p);
+        float t, u, v;
+        bool hit = (MODE == 2) ? isect_watertight(tr, r, t, u, v) : isect_reference(tr, r, t, u, v);
+        if (hit && better<MODE>(t, p, best_t, best)) {
+          best = p;
+          best_t = t;
+          best_u = u;
+          best_v = v;
+        }
+      } else {
+        const float4 b0 = *reinterpret_cast<const float4*>(&nodes[ch].bmin[0]);
+        const float2 b1 = *reinterpret_cast<const float2*>(&nodes[ch].bmax[1]);
+        const float lo[3] = { b0.x, b0.y, b0.z };
+        const float hi[3] = { b0.w, b1.x, b1.y };
+        float tn[3], tf[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          float nb = pos_dir[k] ? lo[k] : hi[k];
+          float fb = pos_dir[k] ? hi[k] : lo[k];
+          float pn = nb * rcp[k];
+          tn[k] = pn + ipos[k];
+          float pf = fb * rcp[k];
+          tf[k] = pf + ipos[k];
+        }
+        float tmin = m_max(m_max(tn[0], tn[1]), tn[2]);
+        float tmax = m_min(m_min(tf[0], tf[1]), tf[2]);
+        bool h = tmax >= m_max(0.0f, tmin);
+        if (side) {
+          rh = h;
+          rt = tmin;
+        } else {
+          lh = h;
+          lt = tmin;
+        }
+      }
+    }
+    if (lh && rh) {
+      uint32_t first_n, second_n;
+      float first_t, second_t;
+      if (lt < rt) {
+        first_n = lr.y;
+        first_t = rt;
+        second_n = lr.x;
+        second_t = lt;
+      } else {
+        first_n = lr.x;
+        first_t = lt;
+        second_n = lr.y;
+        second_t = rt;
+      }
+      if (sp < kStack) {
+        st_node[sp] = first_n;
+        st_t[sp] = first_t;
+        sp++;
+      }
+      if (sp < kStack) {
+        st_node[sp] = second_n;
+        st_t[sp] = second_t;
+        sp++;
+      }
+    } else if (lh || rh) {
+      if (sp < kStack) {
+        st_node[sp] = lh ? lr.x : lr.y;
+        st_t[sp] = lh ? lt : rt;
+        sp++;
+      }
+    }
+  }
+  prim_out[i] = best;
+  t_out[i] = best_t;
+  if (uv_out)
+    *reinterpret_cast<float2*>(uv_out + 2 * i) = make_float2(best_u, best_v);
+}
+
+} // namespace lbvh_dev
+
+namespace lbvh_eng {
+
+using namespace lbvh_dev;
+
+hipError_t
+launch_trace(int mode, const void* nodes, const float* tris, uint64_t n, const float* rays,
+             uint64_t nrays, uint32_t* prim, float* t, float* uv, hipStream_t s)
+{
+  if (nrays == 0)
+    return hipSuccess;
+  uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
+  const Node* nd = static_cast<const Node*>(nodes);
+  switch (mode) {
+    case 0:
+      hipLaunchKernelGGL(k_trace<0>, dim3(g), dim3(kTraceBlock), 0, s, nd, tris, uint32_t(n), rays,
+                         nrays, prim, t, uv);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_trace<1>, dim3(g), dim3(kTraceBlock), 0, s, nd, tris, uint32_t(n), rays,
+                         nrays, prim, t, uv);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_trace<2>, dim3(g), dim3(kTraceBlock), 0, s, nd, tris, uint32_t(n), rays,
+                         nrays, prim, t, uv);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+} // namespace lbvh_eng

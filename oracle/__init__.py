@@ -59,6 +59,8 @@ def orc():
         L.orc_brute_closest.argtypes = [F32P, ctypes.c_uint64, F32P, ctypes.c_uint64, ctypes.c_int,
                                         ctypes.c_int, U32P, F32P]
         L.orc_t_fp64.argtypes = [F32P, F32P, U32P, ctypes.c_uint64, F64P]
+        L.orc_fnv1a64.argtypes = [VP, ctypes.c_uint64]
+        L.orc_fnv1a64.restype = ctypes.c_uint64
         _orc = L
     return _orc
 
@@ -201,6 +203,12 @@ def t_fp64(tri, rays, prim):
     out = np.empty(rays.shape[0], dtype=np.float64)
     orc().orc_t_fp64(_p(tri, F32P), _p(rays, F32P), _p(prim, U32P), rays.shape[0], _p(out, F64P))
     return out
+
+
+def fnv1a64(a):
+    """FNV-1a 64-bit hash of an array's raw bytes (as a hex string)."""
+    a = np.ascontiguousarray(a)
+    return "%016x" % orc().orc_fnv1a64(a.ctypes.data, a.nbytes)
 
 
 # ---- the real reference (compiled lbvh.h) ---------------------------------

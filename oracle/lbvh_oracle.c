@@ -809,3 +809,16 @@ orc_t_fp64(const float* tris, const float* rays, const uint32_t* prim,
     t64[i] = num / den;
   }
 }
+
+/* FNV-1a 64-bit over raw bytes: fixture hashes of large arrays. */
+uint64_t
+orc_fnv1a64(const void* data, uint64_t len)
+{
+  const unsigned char* p = (const unsigned char*)data;
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (uint64_t i = 0; i < len; i++) {
+    h ^= p[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}

@@ -84,6 +84,7 @@ void orc_trace(const orc_node* nodes, uint64_t n, const float* tris,
 void orc_brute_closest(const float* tris, uint64_t n, const float* rays,
                        uint64_t nrays, int mode, int nthreads, uint32_t* prim,
                        float* t);
+uint64_t orc_fnv1a64(const void* data, uint64_t len);
 void orc_t_fp64(const float* tris, const float* rays, const uint32_t* prim,
                 uint64_t nrays, double* t64);
 

@@ -1,0 +1,237 @@
+"""GPU parity: the HIP engine, called through its C ABI, against the golden
+vectors written by the real tay10r/lbvh and against the oracle.
+
+Bar: bit-exact Morton codes, sorted order, topology and boxes; identical
+hit/miss, primitive ids, t bits and uv bits for the reference intersector.
+The watertight intersector is compared with the oracle's restatement of the
+same test with a tolerance of 2 ulp on t (ids identical)."""
+import numpy as np
+import pytest
+
+import lbvh_amd
+import oracle
+from lbvh_amd import workload as W
+from conftest import golden_cases, nodes_u32
+
+pytestmark = pytest.mark.gpu
+
+CASES = golden_cases()
+IDS = [c[0] for c in CASES]
+VARIANTS = [(30, 0), (30, 1), (63, 0), (63, 1)]
+VIDS = ["b30_ref", "b30_pid", "b63_ref", "b63_pid"]
+
+
+def ulp_diff(a, b):
+    ai = a.view(np.int32).astype(np.int64)
+    bi = b.view(np.int32).astype(np.int64)
+    ai = np.where(ai < 0, -(ai & 0x7FFFFFFF), ai)
+    bi = np.where(bi < 0, -(bi & 0x7FFFFFFF), bi)
+    return np.abs(ai - bi)
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("bits,leaf", VARIANTS, ids=VIDS)
+def test_build_matches_golden(ctx, case, bits, leaf):
+    name, rec = case
+    tri = rec["tri"]
+    bvh = ctx.build_triangles(tri, code_bits=bits, leaf_order=leaf, keep_codes=True)
+    key = "b%d_l%d" % (bits, leaf)
+    nodes = bvh.nodes()
+    assert np.array_equal(nodes_u32(nodes), rec[key + "_nodes"]), "node array differs"
+    sc, ids, bounds = bvh.codes()
+    assert np.array_equal(sc, rec[key + "_sorted"]), "sorted codes differ"
+    assert np.array_equal(bounds.view(np.uint32), rec["bounds"].view(np.uint32))
+    # stable (code, id) order
+    order = np.argsort(rec["codes%d" % bits], kind="stable").astype(np.uint32)
+    assert np.array_equal(ids, order)
+    bvh.destroy()
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+@pytest.mark.parametrize("mode", [0, 1], ids=["first", "lowid"])
+def test_trace_matches_golden(ctx, case, leaf, mode):
+    name, rec = case
+    bvh = ctx.build_triangles(rec["tri"], code_bits=30, leaf_order=leaf)
+    p, t, uv = ctx.trace(bvh, rec["rays"], mode)
+    key = "b30_l%d_m%d" % (leaf, mode)
+    assert np.array_equal(p, rec[key + "_prim"])
+    assert np.array_equal(t.view(np.uint32), rec[key + "_t"].view(np.uint32))
+    hit = p != 0xFFFFFFFF
+    assert np.array_equal(uv[hit].view(np.uint32), rec[key + "_uv"][hit].view(np.uint32))
+    assert np.all(uv[~hit] == 0)
+    bvh.destroy()
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 33, 4097, 70001])
+@pytest.mark.parametrize("bits,leaf", VARIANTS, ids=VIDS)
+def test_build_matches_oracle_ragged(ctx, n, bits, leaf):
+    v = W.cluster_vertices(1000 + n, n)
+    tri = W.make_triangles(v)
+    bvh = ctx.build_triangles(tri, code_bits=bits, leaf_order=leaf, keep_codes=True)
+    on, ob, osc, oids = oracle.build(tri, bits, leaf)
+    assert bvh.nodes().tobytes() == on.tobytes()
+    sc, ids, bounds = bvh.codes()
+    assert np.array_equal(sc, osc)
+    assert np.array_equal(ids, oids)
+
+
+@pytest.mark.parametrize("digit_bits,tile", [(8, 1024), (8, 2048), (8, 4096), (10, 2048), (10, 4096)])
+@pytest.mark.parametrize("bits", [30, 63])
+def test_sort_configurations_agree(ctx, digit_bits, tile, bits):
+    v = W.cluster_vertices(77, 300000)
+    tri = W.make_triangles(v)
+    bvh = ctx.build_triangles(tri, code_bits=bits, leaf_order=1, keep_codes=True,
+                              sort_digit_bits=digit_bits, sort_tile=tile)
+    on, _, osc, oids = oracle.build(tri, bits, 1)
+    sc, ids, _ = bvh.codes()
+    assert np.array_equal(sc, osc)
+    assert np.array_equal(ids, oids)
+    assert bvh.nodes().tobytes() == on.tobytes()
+
+
+def test_rejects_bad_arguments(ctx):
+    tri = W.make_triangles(W.uniform_vertices(1, 10, 0.01))
+    with pytest.raises(lbvh_amd.LbvhError) as e:
+        ctx.build_triangles(tri[:1])
+    assert e.value.status == 1
+    with pytest.raises(lbvh_amd.LbvhError):
+        ctx.build_triangles(tri, code_bits=31)
+    with pytest.raises(lbvh_amd.LbvhError):
+        ctx.build_triangles(tri, sort_digit_bits=7)
+
+
+def test_uniform65536_hashes(ctx, hashes):
+    e = hashes["uniform65536_seed1"]
+    v, tri = W.scene_uniform(1, 65536, 0.01)
+    assert oracle.fnv1a64(tri) == e["tri"]
+    rays = W.primary_rays(256, 256)
+    for leaf in (0, 1):
+        bvh = ctx.build_triangles(tri, code_bits=30, leaf_order=leaf, keep_codes=True)
+        assert oracle.fnv1a64(bvh.nodes()) == e["b30_l%d_nodes" % leaf]
+        sc, _, _ = bvh.codes()
+        assert oracle.fnv1a64(sc) == e["b30_l%d_sorted" % leaf]
+        for mode in (0, 1):
+            p, t, _ = ctx.trace(bvh, rays, mode)
+            assert oracle.fnv1a64(np.concatenate([p, t.view(np.uint32)])) == \
+                e["b30_l%d_m%d_hits" % (leaf, mode)]
+        bvh.destroy()
+
+
+def test_clustered1m_hashes(ctx, hashes):
+    e = hashes["clustered1m_seed2"]
+    v, tri = W.scene_clustered(2, 1 << 20)
+    if oracle.fnv1a64(tri) != e["tri"]:
+        pytest.skip("libm differs from the fixture host: compared in-run only")
+    for bits in (30, 63):
+        for leaf in (0, 1):
+            bvh = ctx.build_triangles(tri, code_bits=bits, leaf_order=leaf, keep_codes=True)
+            assert oracle.fnv1a64(bvh.nodes()) == e["b%d_l%d_nodes" % (bits, leaf)]
+            sc, _, _ = bvh.codes()
+            assert oracle.fnv1a64(sc) == e["b%d_l%d_sorted" % (bits, leaf)]
+            bvh.destroy()
+
+
+def test_uniform16m_hashes(ctx, hashes):
+    e = hashes["uniform16m_seed3"]
+    v, tri = W.scene_uniform(3, 1 << 24, 5e-4)
+    del v
+    assert oracle.fnv1a64(tri) == e["tri"]
+    for bits in (30, 63):
+        for leaf in (0, 1):
+            bvh = ctx.build_triangles(tri, code_bits=bits, leaf_order=leaf, keep_codes=True)
+            assert oracle.fnv1a64(bvh.nodes()) == e["b%d_l%d_nodes" % (bits, leaf)]
+            sc, _, _ = bvh.codes()
+            assert oracle.fnv1a64(sc) == e["b%d_l%d_sorted" % (bits, leaf)]
+            bvh.destroy()
+
+
+def test_clustered1m_primary_and_bounce_rays(ctx):
+    """The 1,048,576-triangle clustered scene with 63-bit codes: primary rays
+    and one diffuse bounce per hit, against the oracle traversal."""
+    v, tri = W.scene_clustered(2, 1 << 20)
+    bvh = ctx.build_triangles(tri, code_bits=63, leaf_order=1)
+    nodes = bvh.nodes()
+    rays = W.primary_rays(480, 270)
+    for mode in (0, 1):
+        p, t, uv = ctx.trace(bvh, rays, mode)
+        op, ot, ouv = oracle.trace(nodes, tri, rays, mode)
+        assert np.array_equal(p, op)
+        assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+        assert np.array_equal(uv.view(np.uint32), ouv.view(np.uint32))
+    b, src = W.bounce_rays(5, tri, rays, p, t)
+    assert len(b) > 1000
+    for mode in (0, 1):
+        p2, t2, _ = ctx.trace(bvh, b, mode)
+        op2, ot2, _ = oracle.trace(nodes, tri, b, mode)
+        assert np.array_equal(p2, op2)
+        assert np.array_equal(t2.view(np.uint32), ot2.view(np.uint32))
+
+
+def test_watertight_matches_oracle(ctx):
+    v, tri = W.scene_uniform(1, 65536, 0.01)
+    bvh = ctx.build_triangles(tri, code_bits=30, leaf_order=1)
+    nodes = bvh.nodes()
+    rays = W.primary_rays(256, 256)
+    p, t, uv = ctx.trace(bvh, rays, 2)
+    op, ot, ouv = oracle.trace(nodes, tri, rays, 2)
+    assert np.array_equal(p, op)
+    hit = p != 0xFFFFFFFF
+    assert ulp_diff(t[hit], ot[hit]).max() <= 2  # tolerance: 2 ulp on t
+    t64 = oracle.t_fp64(tri, rays, p)
+    rel = np.abs(t[hit] - t64[hit]) / np.abs(t64[hit])
+    print("watertight t: max rel err vs FP64 %.3e over %d hits" % (rel.max(), hit.sum()))
+    assert rel.max() < 1e-5
+
+
+@pytest.mark.parametrize("leaf", [0, 1])
+def test_refit_equals_oracle_fit(ctx, leaf):
+    v, tri = W.scene_uniform(3, 200000, 5e-4)
+    bvh = ctx.build_triangles(tri, code_bits=30, leaf_order=leaf)
+    before = bvh.nodes()
+    v2 = W.perturb(33, v, 1e-4)
+    tri2 = W.make_triangles(v2)
+    ctx.refit_triangles(bvh, tri2)
+    after = bvh.nodes()
+    assert np.array_equal(after["left"], before["left"])
+    assert np.array_equal(after["right"], before["right"])
+    expect = oracle.fit(before, tri2)
+    assert after.tobytes() == expect.tobytes()
+    # a second refit back to the original data restores the original boxes
+    ctx.refit_triangles(bvh, tri)
+    assert bvh.nodes().tobytes() == before.tobytes()
+
+
+def test_device_path_equals_host_path(ctx):
+    import torch
+    v, tri = W.scene_uniform(1, 65536, 0.01)
+    rays = W.primary_rays(128, 128)
+    host = ctx.build_triangles(tri, code_bits=63, leaf_order=1)
+    hp, ht, huv = ctx.trace(host, rays, 1)
+    dtri = torch.from_numpy(tri).cuda()
+    drays = torch.from_numpy(rays).cuda()
+    ctx.set_stream(torch.cuda.current_stream())
+    dev = ctx.build_triangles(dtri, code_bits=63, leaf_order=1)
+    dp, dt, duv = ctx.trace(dev, drays, 1)
+    ctx.synchronize()
+    ctx.set_stream(None)
+    assert dev.nodes().tobytes() == host.nodes().tobytes()
+    assert np.array_equal(dp.cpu().numpy().view(np.uint32), hp)
+    assert np.array_equal(dt.cpu().numpy().view(np.uint32), ht.view(np.uint32))
+
+
+def test_aabb_build_equals_triangle_build(ctx):
+    v, tri = W.scene_uniform(4, 50000, 0.01)
+    boxes = oracle.triangle_aabb(tri)
+    a = ctx.build_triangles(tri, code_bits=30, leaf_order=1)
+    b = ctx.build_aabbs(boxes, code_bits=30, leaf_order=1)
+    assert a.nodes().tobytes() == b.nodes().tobytes()
+
+
+def test_phase_timing_reports_four_phases(ctx):
+    v, tri = W.scene_uniform(1, 65536, 0.01)
+    ctx.set_timing(True)
+    bvh = ctx.build_triangles(tri, code_bits=30)
+    ms = ctx.phase_ms()
+    ctx.set_timing(False)
+    assert len(ms) == 4 and all(x >= 0 for x in ms)

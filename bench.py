@@ -110,9 +110,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
-    import torch
 
     import lbvh_amd
+    from lbvh_amd import hip
     from lbvh_amd import workload as W
 
     log("bench: generating the 1,048,576-triangle clustered scene (seed 2) and %dx%d rays"
@@ -120,31 +120,26 @@ def main():
     _, tri = W.scene_clustered(2, N_PRIMS)
     rays = W.primary_rays(WIDTH, HEIGHT)
 
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
     ctx = lbvh_amd.Context(0)
-    ctx.set_stream(stream)
-    d_tri = torch.from_numpy(tri).to(dev)
-    d_rays = torch.from_numpy(rays).to(dev)
+    d_tri = hip.DeviceArray.from_host(tri)
+    d_rays = hip.DeviceArray.from_host(rays)
 
     # bounce rays from the GPU's own primary hits (identical to the oracle's)
     bvh = ctx.build_triangles(d_tri, code_bits=63, leaf_order=lbvh_amd.LEAF_PRIMITIVE_ID)
     p, t, _ = ctx.trace(bvh, d_rays, lbvh_amd.ISECT_REFERENCE)
     ctx.synchronize()
     bvh.destroy()
-    hp = p.cpu().numpy().view(np.uint32)
-    ht = t.cpu().numpy()
+    hp = p.numpy()
+    ht = t.numpy()
     bounce, _ = W.bounce_rays(5, tri, rays, hp, ht)
-    d_bounce = torch.from_numpy(bounce).to(dev)
+    d_bounce = hip.DeviceArray.from_host(bounce)
     n_primary, n_bounce = rays.shape[0], bounce.shape[0]
     log("bench: %d primary rays, %d hits -> %d bounce rays" % (n_primary, int((hp != 0xFFFFFFFF).sum()),
                                                              n_bounce))
-    out_p = (torch.empty(n_primary, dtype=torch.int32, device=dev),
-             torch.empty(n_primary, dtype=torch.float32, device=dev),
-             torch.empty((n_primary, 2), dtype=torch.float32, device=dev))
-    out_b = (torch.empty(n_bounce, dtype=torch.int32, device=dev),
-             torch.empty(n_bounce, dtype=torch.float32, device=dev),
-             torch.empty((n_bounce, 2), dtype=torch.float32, device=dev))
+    out_p = (hip.DeviceArray(n_primary, np.uint32), hip.DeviceArray(n_primary, np.float32),
+             hip.DeviceArray((n_primary, 2), np.float32))
+    out_b = (hip.DeviceArray(n_bounce, np.uint32), hip.DeviceArray(n_bounce, np.float32),
+             hip.DeviceArray((n_bounce, 2), np.float32))
 
     def step():
         b = ctx.build_triangles(d_tri, code_bits=63, leaf_order=lbvh_amd.LEAF_PRIMITIVE_ID)
@@ -161,15 +156,15 @@ def main():
     ctx.synchronize()
     log("bench: warmup done (%d steps)" % args.warmup)
 
-    # timed region: K steps back to back, per-phase HIP events on the stream
-    # the kernels run on (the context's stream is torch's current stream)
+    # timed region: K steps back to back; per-phase HIP events are recorded
+    # by the engine on the stream its kernels run on (the context's stream)
     trees = []
     ctx.set_timing(True)
-    torch.cuda.synchronize()
+    hip.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         trees.append(step())
-    torch.cuda.synchronize()
+    hip.synchronize()
     wall = time.perf_counter() - t0
     ctx.synchronize()
     build_ph, nb = ctx.phase_stats(1)

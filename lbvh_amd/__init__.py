@@ -117,6 +117,36 @@ def _is_torch(x):
     return type(x).__module__.startswith("torch")
 
 
+def _single_runtime():
+    """True when the process maps one copy of the HIP runtime."""
+    try:
+        with open("/proc/self/maps") as f:
+            paths = set(line.split()[-1] for line in f if "libamdhip64" in line)
+    except OSError:
+        return True
+    return len(paths) <= 1
+
+
+def _is_device(x):
+    from .hip import DeviceArray
+    if isinstance(x, DeviceArray):
+        return True
+    if _is_torch(x):
+        if not x.is_cuda:
+            return False
+        if not _single_runtime():
+            raise RuntimeError("PyTorch and the engine use different HIP runtimes in this "
+                               "process; import torch before the engine is first used")
+        if not x.is_contiguous():
+            raise ValueError("device arrays must be contiguous")
+        return True
+    return False
+
+
+def _numel(x):
+    return int(x.numel())
+
+
 def _dptr(x):
     return ctypes.c_void_p(int(x.data_ptr()))
 
@@ -259,10 +289,8 @@ class Context:
     def _build(self, prims, width, fn, code_bits, leaf_order, keep_codes, sdb, stile):
         opts = _make_opts(code_bits, leaf_order, keep_codes, sdb, stile)
         out = ctypes.c_void_p()
-        if _is_torch(prims):
-            if not prims.is_cuda or not prims.is_contiguous():
-                raise ValueError("device primitives must be a contiguous CUDA tensor")
-            n = prims.numel() // width
+        if _is_device(prims):
+            n = _numel(prims) // width
             self._check(getattr(lib(), fn + "_device")(self.handle, _dptr(prims), n,
                                                        ctypes.byref(opts), ctypes.byref(out)))
             return Bvh(self, out, n, code_bits, leaf_order, keep=prims)
@@ -273,7 +301,7 @@ class Context:
         return Bvh(self, out, n, code_bits, leaf_order)
 
     def refit_triangles(self, bvh, tri):
-        if _is_torch(tri):
+        if _is_device(tri):
             self._check(lib().lbvh_refit_triangles_device(self.handle, bvh.handle, _dptr(tri)))
             bvh._keep = tri
             return
@@ -284,15 +312,17 @@ class Context:
 
     # -- traversal ---------------------------------------------------------
     def trace(self, bvh, rays, mode=ISECT_REFERENCE, out=None, with_uv=True):
-        """Closest hit per ray.  numpy rays -> numpy (prim, t, uv);
-        torch rays -> torch tensors (asynchronous on the context stream)."""
-        if _is_torch(rays):
-            import torch
-            m = rays.numel() // 8
+        """Closest hit per ray.  numpy rays -> numpy (prim, t, uv) after the
+        work has finished; device rays (DeviceArray or a torch CUDA tensor) ->
+        device outputs, queued on the context stream (synchronize before
+        reading)."""
+        if _is_device(rays):
+            m = _numel(rays) // 8
             if out is None:
-                prim = torch.empty(m, dtype=torch.int32, device=rays.device)
-                t = torch.empty(m, dtype=torch.float32, device=rays.device)
-                uv = torch.empty((m, 2), dtype=torch.float32, device=rays.device) if with_uv else None
+                from .hip import DeviceArray
+                prim = DeviceArray(m, np.uint32)
+                t = DeviceArray(m, np.float32)
+                uv = DeviceArray((m, 2), np.float32) if with_uv else None
             else:
                 prim, t, uv = out
             self._check(lib().lbvh_trace_closest_device(self.handle, bvh.handle, _dptr(rays), m,

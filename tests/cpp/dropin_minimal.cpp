@@ -1,0 +1,147 @@
+// Compile-and-run conformance check of the drop-in header: the call shapes
+// of the reference's own users (a lambda converter over a user primitive,
+// the triangle plugin, the traverser with a user intersection type) compile
+// unchanged against lbvh_mi355x.hpp and run on the GPU engine.
+//
+// Usage: dropin_minimal <out.bin>
+// Writes: u64 n, n triangles (48 B each), (n-1) nodes (32 B each),
+//         u64 m, m rays (32 B), m u32 prim, m f32 t (host traverser),
+//         then the 2 nodes of the 3-sphere tree.
+// Exits non-zero if the device-tree closest hits differ from the host ones.
+#include <lbvh_mi355x.hpp>
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+struct sphere
+{
+  float radius;
+  float pos[3];
+};
+
+struct sphere_hit
+{
+  float tmin = std::numeric_limits<float>::infinity();
+  constexpr bool operator<(const sphere_hit& o) const noexcept { return tmin < o.tmin; }
+  constexpr bool operator<(float d) const noexcept { return tmin < d; }
+};
+
+// 64-bit LCG mapped to [0,1) floats: deterministic scene bytes.
+struct lcg
+{
+  std::uint64_t s;
+  float next()
+  {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    return float(s >> 40) * (1.0f / 16777216.0f);
+  }
+};
+
+} // namespace
+
+int
+main(int argc, char** argv)
+{
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s out.bin\n", argv[0]);
+    return 2;
+  }
+  lcg g{ 12345 };
+  const std::size_t n = 20000;
+  std::vector<lbvh::triangle<float>> tris;
+  tris.reserve(n);
+  for (std::size_t i = 0; i < n; i++) {
+    lbvh::vec3<float> c{ g.next(), g.next(), g.next() };
+    lbvh::vec3<float> v[3];
+    for (auto& p : v)
+      p = { c.x + (g.next() - 0.5f) * 0.02f, c.y + (g.next() - 0.5f) * 0.02f,
+            c.z + (g.next() - 0.5f) * 0.02f };
+    tris.emplace_back(v[0], v[1], v[2]);
+  }
+
+  lbvh::triangle_aabb_converter<float> cvt;
+  lbvh::builder<float> builder;
+  auto bvh = builder(tris.data(), tris.size(), cvt);
+
+  // the reference example's pattern: user primitive + lambda converter
+  sphere spheres[3]{ { 1, { 5, 1, -5 } }, { 1, { 0, 2, -5 } }, { 1, { -5, 1, -5 } } };
+  auto sphere_to_box = [](sphere s) -> lbvh::aabb<float> {
+    return lbvh::aabb<float>{ { s.pos[0] - s.radius, s.pos[1] - s.radius, s.pos[2] - s.radius },
+                              { s.pos[0] + s.radius, s.pos[1] + s.radius, s.pos[2] + s.radius } };
+  };
+  auto sbvh = builder(spheres, 3, sphere_to_box);
+  lbvh::traverser<float, sphere, sphere_hit> straverser(sbvh, spheres);
+  auto isect_sphere = [](const sphere& s, const lbvh::ray<float>& r) -> sphere_hit {
+    using namespace lbvh::math;
+    lbvh::vec3<float> c{ s.pos[0], s.pos[1], s.pos[2] };
+    auto d = r.pos - c;
+    float a = dot(r.dir, r.dir), b = 2 * dot(r.dir, d), cc = dot(d, d) - s.radius * s.radius;
+    float disc = b * b - 4 * a * cc;
+    if (disc < 0)
+      return sphere_hit{};
+    float t = (-b - std::sqrt(disc)) / (2 * a);
+    return sphere_hit{ t };
+  };
+  lbvh::ray<float> sr{ { 0, 0, 5 }, { 0, 0.4f, -1 } };
+  auto sh = straverser(sr, isect_sphere);
+  if (!sh || sh.primitive != 1) {
+    std::fprintf(stderr, "sphere scene: expected to hit sphere 1, got %u\n", unsigned(sh.primitive));
+    return 1;
+  }
+
+  // closest hit: host traverser over the downloaded tree vs the device tree
+  const std::size_t m = 4096;
+  std::vector<lbvh::ray<float>> rays(m);
+  for (std::size_t i = 0; i < m; i++) {
+    float x = float(i % 64) / 63.0f, y = float(i / 64) / 63.0f;
+    rays[i].pos = { 0.5f, 0.5f, -1.0f };
+    rays[i].dir = lbvh::math::normalize(lbvh::vec3<float>{ x - 0.5f, y - 0.5f, 1.0f });
+  }
+  lbvh::traverser<float, lbvh::triangle<float>, lbvh::triangle_intersection<float>> tr(
+    bvh, tris.data());
+  lbvh::triangle_intersector<float> isect;
+  std::vector<std::uint32_t> hp(m);
+  std::vector<float> ht(m);
+  for (std::size_t i = 0; i < m; i++) {
+    auto h = tr(rays[i], isect);
+    hp[i] = h.primitive;
+    ht[i] = h.info.distance;
+  }
+  lbvh::gpu::device_tree dt(tris.data(), tris.size());
+  std::vector<std::uint32_t> dp(m);
+  std::vector<float> dtt(m);
+  dt.trace(rays.data(), m, dp.data(), dtt.data());
+  for (std::size_t i = 0; i < m; i++) {
+    if (dp[i] != hp[i] || std::memcmp(&dtt[i], &ht[i], 4) != 0) {
+      std::fprintf(stderr, "ray %zu: device (%u, %g) != host (%u, %g)\n", i, dp[i], dtt[i], hp[i],
+                   ht[i]);
+      return 1;
+    }
+  }
+  auto back = dt.download();
+  if (std::memcmp(back.data(), bvh.data(), (n - 1) * 32) != 0) {
+    std::fprintf(stderr, "device_tree nodes differ from builder nodes\n");
+    return 1;
+  }
+
+  FILE* f = std::fopen(argv[1], "wb");
+  if (!f)
+    return 2;
+  std::uint64_t nn = n, mm = m;
+  std::fwrite(&nn, 8, 1, f);
+  std::fwrite(tris.data(), 48, n, f);
+  std::fwrite(bvh.data(), 32, n - 1, f);
+  std::fwrite(&mm, 8, 1, f);
+  std::fwrite(rays.data(), 32, m, f);
+  std::fwrite(hp.data(), 4, m, f);
+  std::fwrite(ht.data(), 4, m, f);
+  std::fwrite(sbvh.data(), 32, 2, f);
+  std::fclose(f);
+  std::printf("dropin ok: %zu triangles, %zu rays, %zu hits\n", n, m,
+              std::size_t(std::count_if(hp.begin(), hp.end(),
+                                        [](std::uint32_t p) { return p != 0xffffffffu; })));
+  return 0;
+}

@@ -203,21 +203,97 @@ def test_refit_equals_oracle_fit(ctx, leaf):
 
 
 def test_device_path_equals_host_path(ctx):
-    import torch
+    from lbvh_amd import hip
     v, tri = W.scene_uniform(1, 65536, 0.01)
     rays = W.primary_rays(128, 128)
     host = ctx.build_triangles(tri, code_bits=63, leaf_order=1)
     hp, ht, huv = ctx.trace(host, rays, 1)
-    dtri = torch.from_numpy(tri).cuda()
-    drays = torch.from_numpy(rays).cuda()
-    ctx.set_stream(torch.cuda.current_stream())
+    dtri = hip.DeviceArray.from_host(tri)
+    drays = hip.DeviceArray.from_host(rays)
     dev = ctx.build_triangles(dtri, code_bits=63, leaf_order=1)
     dp, dt, duv = ctx.trace(dev, drays, 1)
     ctx.synchronize()
-    ctx.set_stream(None)
     assert dev.nodes().tobytes() == host.nodes().tobytes()
-    assert np.array_equal(dp.cpu().numpy().view(np.uint32), hp)
-    assert np.array_equal(dt.cpu().numpy().view(np.uint32), ht.view(np.uint32))
+    assert np.array_equal(dp.numpy(), hp)
+    assert np.array_equal(dt.numpy().view(np.uint32), ht.view(np.uint32))
+    assert np.array_equal(duv.numpy().view(np.uint32), huv.view(np.uint32))
+    # device refit from a device buffer, then trace again
+    v2 = W.perturb(2, v, 1e-4)
+    tri2 = W.make_triangles(v2)
+    ctx.refit_triangles(dev, hip.DeviceArray.from_host(tri2))
+    ctx.synchronize()
+    assert dev.nodes().tobytes() == oracle.fit(host.nodes(), tri2).tobytes()
+
+
+TORCH_INTEROP = r"""
+import sys, numpy as np
+import torch  # first: the engine then shares torch's HIP runtime
+sys.path.insert(0, %r)
+import lbvh_amd, oracle
+from lbvh_amd import workload as W
+v, tri = W.scene_uniform(1, 20000, 0.01)
+rays = W.primary_rays(64, 64)
+ctx = lbvh_amd.Context(0)
+ctx.set_stream(torch.cuda.current_stream())
+b = ctx.build_triangles(torch.from_numpy(tri).cuda(), code_bits=30, leaf_order=1)
+prim = torch.empty(rays.shape[0], dtype=torch.int32, device="cuda")
+t = torch.empty(rays.shape[0], dtype=torch.float32, device="cuda")
+uv = torch.empty((rays.shape[0], 2), dtype=torch.float32, device="cuda")
+ctx.trace(b, torch.from_numpy(rays).cuda(), 0, out=(prim, t, uv))
+torch.cuda.synchronize()
+on, _, _, _ = oracle.build(tri, 30, 1)
+op, ot, _ = oracle.trace(on, tri, rays, 0)
+assert b.nodes().tobytes() == on.tobytes()
+assert np.array_equal(prim.cpu().numpy().view(np.uint32), op)
+assert np.array_equal(t.cpu().numpy().view(np.uint32), ot.view(np.uint32))
+print("torch interop ok")
+"""
+
+
+def test_torch_tensors_share_the_runtime(tmp_path):
+    """torch CUDA tensors work as device arrays when torch is imported first
+    (run in a fresh process: one HIP runtime per process)."""
+    import subprocess
+    import sys
+    from conftest import ROOT
+    r = subprocess.run([sys.executable, "-c", TORCH_INTEROP % ROOT], capture_output=True, text=True,
+                       timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "torch interop ok" in r.stdout
+
+
+def test_cpp_dropin_conformance(tmp_path):
+    """The C++ drop-in header (include/lbvh_mi355x.hpp) used with the
+    reference's call shapes: builder over triangles and over a user primitive
+    with a lambda converter, host traverser, device tree; nodes and hits
+    against the oracle."""
+    import os
+    import subprocess
+    from lbvh_amd import build as B
+    if not os.path.exists(B.DROPIN_BIN):
+        pytest.skip("drop-in conformance program not built")
+    out = tmp_path / "dropin.bin"
+    r = subprocess.run([B.DROPIN_BIN, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = out.read_bytes()
+    n = int(np.frombuffer(raw[:8], np.uint64)[0])
+    off = 8
+    tri = np.frombuffer(raw[off:off + 48 * n], np.float32).reshape(n, 12)
+    off += 48 * n
+    nodes = np.frombuffer(raw[off:off + 32 * (n - 1)], oracle.NODE_DTYPE)
+    off += 32 * (n - 1)
+    m = int(np.frombuffer(raw[off:off + 8], np.uint64)[0])
+    off += 8
+    rays = np.frombuffer(raw[off:off + 32 * m], np.float32).reshape(m, 8)
+    off += 32 * m
+    hp = np.frombuffer(raw[off:off + 4 * m], np.uint32)
+    off += 4 * m
+    ht = np.frombuffer(raw[off:off + 4 * m], np.float32)
+    on, _, _, _ = oracle.build(tri, 30, 0)
+    assert nodes.tobytes() == on.tobytes()
+    op, ot, _ = oracle.trace(on, tri, rays, 0)
+    assert np.array_equal(hp, op)
+    assert np.array_equal(ht.view(np.uint32), ot.view(np.uint32))
 
 
 def test_aabb_build_equals_triangle_build(ctx):

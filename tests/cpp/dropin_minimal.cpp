@@ -85,7 +85,7 @@ main(int argc, char** argv)
     float t = (-b - std::sqrt(disc)) / (2 * a);
     return sphere_hit{ t };
   };
-  lbvh::ray<float> sr{ { 0, 0, 5 }, { 0, 0.4f, -1 } };
+  lbvh::ray<float> sr{ { 0, 0, 5 }, { 0, 0.2f, -1 } }; // aims at the centre of sphere 1
   auto sh = straverser(sr, isect_sphere);
   if (!sh || sh.primitive != 1) {
     std::fprintf(stderr, "sphere scene: expected to hit sphere 1, got %u\n", unsigned(sh.primitive));

@@ -1,15 +1,17 @@
-// gfx950 build kernels: centroid bounds, Morton encode, Karras radix tree and
-// the bottom-up box fit.  Each kernel cites the reference code whose results
-// it reproduces bit for bit.
+// gfx950 build kernels: centroid bounds, Morton encode (+ the radix sort's
+// digit histograms), Karras radix tree and the bottom-up box fit.  Each
+// kernel cites the reference code whose results it reproduces bit for bit.
 #include "device_common.hpp"
 #include "engine.hpp"
 
 namespace lbvh_dev {
 
 constexpr int kBlock = 256;
-constexpr int kCentroidMaxBlocks = 2048;
+constexpr int kCentroidMaxBlocks = 1024;
+constexpr int kEncodeMaxBlocks = 256;
+constexpr int kFitTile = 1024; // leaves (sorted positions) per fit workgroup
 
-// ---- centroid bounds --------------------------------------------------------
+// ---- centroid bounds ---------------------------------------------------------
 // Reproduces centroid_bounds_kernel + the serial union of per-thread boxes
 // (lbvh.h:1282-1293, 1539-1543): the union of (min + max) * 0.5 over every
 // primitive box (center_of, lbvh.h:1160-1167).  min/max are exact, so the
@@ -28,6 +30,30 @@ wave_reduce_box(float lo[3], float hi[3])
       lo[k] = std_min(lo[k], ol);
       hi[k] = std_max(hi[k], oh);
     }
+  }
+}
+
+// Reduces one box per thread to thread 0 of the workgroup.
+__device__ __forceinline__ void
+block_reduce_box(float lo[3], float hi[3], float (*s_box)[6])
+{
+  wave_reduce_box(lo, hi);
+  const int w = threadIdx.x / kWave;
+  if (__lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      s_box[w][k] = lo[k];
+      s_box[w][3 + k] = hi[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int ww = 1; ww < kBlock / kWave; ww++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        lo[k] = std_min(lo[k], s_box[ww][k]);
+        hi[k] = std_max(hi[k], s_box[ww][3 + k]);
+      }
   }
 }
 
@@ -62,33 +88,12 @@ k_centroid_bounds(const float* __restrict__ prims, int kind, uint64_t n,
     cy[i] = c[1];
     cz[i] = c[2];
   }
-  wave_reduce_box(lo, hi);
-  const int w = threadIdx.x / kWave;
-  if (__lane_id() == 0) {
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      s_box[w][k] = lo[k];
-      s_box[w][3 + k] = hi[k];
-    }
-  }
-  __syncthreads();
+  block_reduce_box(lo, hi, s_box);
   if (threadIdx.x == 0) {
-    float blo[3], bhi[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      blo[k] = s_box[0][k];
-      bhi[k] = s_box[0][3 + k];
-    }
-    for (int ww = 1; ww < kBlock / kWave; ww++)
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        blo[k] = std_min(blo[k], s_box[ww][k]);
-        bhi[k] = std_max(bhi[k], s_box[ww][3 + k]);
-      }
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      coherent_store_f32(&partials[6 * blockIdx.x + k], blo[k]);
-      coherent_store_f32(&partials[6 * blockIdx.x + 3 + k], bhi[k]);
+      coherent_store_f32(&partials[6 * blockIdx.x + k], lo[k]);
+      coherent_store_f32(&partials[6 * blockIdx.x + 3 + k], hi[k]);
     }
     vmem_drain();
     uint32_t t = atomic_add_agent(ticket, 1u);
@@ -97,32 +102,35 @@ k_centroid_bounds(const float* __restrict__ prims, int kind, uint64_t n,
   __syncthreads();
   if (!s_last)
     return;
-  // Last workgroup: union of the per-workgroup boxes in workgroup order.
-  if (threadIdx.x == 0) {
-    float blo[3] = { INFINITY, INFINITY, INFINITY };
-    float bhi[3] = { -INFINITY, -INFINITY, -INFINITY };
-    for (uint32_t g = 0; g < gridDim.x; g++)
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        blo[k] = std_min(blo[k], coherent_load_f32(&partials[6 * g + k]));
-        bhi[k] = std_max(bhi[k], coherent_load_f32(&partials[6 * g + 3 + k]));
-      }
+  // Last workgroup: union of the per-workgroup boxes.
+  float glo[3] = { INFINITY, INFINITY, INFINITY };
+  float ghi[3] = { -INFINITY, -INFINITY, -INFINITY };
+  for (uint32_t g = threadIdx.x; g < gridDim.x; g += kBlock)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      bounds[k] = blo[k];
-      bounds[3 + k] = bhi[k];
+      glo[k] = std_min(glo[k], coherent_load_f32(&partials[6 * g + k]));
+      ghi[k] = std_max(ghi[k], coherent_load_f32(&partials[6 * g + 3 + k]));
+    }
+  __syncthreads(); // s_box reuse
+  block_reduce_box(glo, ghi, s_box);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      bounds[k] = glo[k];
+      bounds[3 + k] = ghi[k];
     }
     *ticket = 0u;
   }
 }
 
-// ---- Morton encode ----------------------------------------------------------
+// ---- Morton encode --------------------------------------------------------------
 // morton_curve_kernel (lbvh.h:1432-1473): scale = D / (extent + 1) with
 // D = 1024 (morton_domain<4>) or 2^20 (morton_domain<8>); q = trunc((c - min)
 // * scale); code = expand(qx) << 2 | expand(qy) << 1 | expand(qz) with the
 // magic-mask expansions of morton_encoder<4>/<8> (lbvh.h:1352-1358,
 // 1377-1385).  The 63-bit variant applies morton_encoder<8> to FP32 centres.
-// Also accumulates the radix sort's digit histograms for every pass.
+// Each workgroup also counts the digits of every radix pass in LDS and writes
+// its counts (plain stores) for k_hist_reduce.
 
 __device__ __forceinline__ uint32_t
 expand10(uint32_t n)
@@ -166,13 +174,14 @@ encode<uint64_t>(float qx, float qy, float qz)
 }
 
 constexpr int kHistMaxWords = 8192; // passes * radix, 32 KiB of LDS
+constexpr int kEncodeUnroll = 4;
 
 template<typename KeyT>
 __global__ __launch_bounds__(kBlock) void
 k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
          const float* __restrict__ cz, uint64_t n,
          const float* __restrict__ bounds, KeyT* __restrict__ codes,
-         uint32_t* __restrict__ hist, int digit_bits, int passes)
+         uint32_t* __restrict__ part_hist, int digit_bits, int passes)
 {
   extern __shared__ uint32_t s_hist[];
   const int radix = 1 << digit_bits;
@@ -191,29 +200,58 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
   __syncthreads();
   const uint64_t stride = uint64_t(gridDim.x) * kBlock;
   const KeyT mask = KeyT(radix - 1);
-  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
-    float dx = cx[i] - bmin[0];
-    float dy = cy[i] - bmin[1];
-    float dz = cz[i] - bmin[2];
-    float qx = dx * scale[0];
-    float qy = dy * scale[1];
-    float qz = dz * scale[2];
-    KeyT code = encode<KeyT>(qx, qy, qz);
-    codes[i] = code;
-    for (int p = 0; p < passes; p++) {
-      uint32_t d = uint32_t((code >> (p * digit_bits)) & mask);
-      atomicAdd(&s_hist[p * radix + d], 1u);
+  for (uint64_t i0 = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i0 < n;
+       i0 += stride * kEncodeUnroll) {
+    float x[kEncodeUnroll], y[kEncodeUnroll], z[kEncodeUnroll];
+#pragma unroll
+    for (int u = 0; u < kEncodeUnroll; u++) {
+      uint64_t i = i0 + u * stride;
+      if (i < n) {
+        x[u] = cx[i];
+        y[u] = cy[i];
+        z[u] = cz[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kEncodeUnroll; u++) {
+      uint64_t i = i0 + u * stride;
+      if (i >= n)
+        continue;
+      float dx = x[u] - bmin[0];
+      float dy = y[u] - bmin[1];
+      float dz = z[u] - bmin[2];
+      float qx = dx * scale[0];
+      float qy = dy * scale[1];
+      float qz = dz * scale[2];
+      KeyT code = encode<KeyT>(qx, qy, qz);
+      codes[i] = code;
+      for (int p = 0; p < passes; p++) {
+        uint32_t d = uint32_t((code >> (p * digit_bits)) & mask);
+        atomicAdd(&s_hist[p * radix + d], 1u);
+      }
     }
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < words; j += kBlock) {
-    uint32_t v = s_hist[j];
-    if (v)
-      atomicAdd(&hist[j], v);
-  }
+  uint32_t* out = part_hist + size_t(blockIdx.x) * words;
+  for (int j = threadIdx.x; j < words; j += kBlock)
+    out[j] = s_hist[j];
 }
 
-// ---- Karras radix tree -------------------------------------------------------
+// Sums the per-workgroup digit counts: hist[w] = sum over g of part[g][w].
+__global__ __launch_bounds__(kBlock) void
+k_hist_reduce(const uint32_t* __restrict__ part, uint32_t groups, uint32_t words,
+              uint32_t* __restrict__ hist)
+{
+  const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+  if (w >= words)
+    return;
+  uint32_t s = 0;
+  for (uint32_t g = 0; g < groups; g++)
+    s += part[size_t(g) * words + w];
+  hist[w] = s;
+}
+
+// ---- Karras radix tree -----------------------------------------------------------
 // divide_node + builder_kernel (lbvh.h:1588-1688, 1713-1731), one internal
 // node per thread:
 //   delta(j, k) = -1 if k is outside [0, n); clz(code_j ^ code_k) if the codes
@@ -225,6 +263,13 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
 // left = gamma | leaf if min(i, j) == gamma; right = gamma + 1 | leaf if
 // max(i, j) == gamma + 1.  With PRIMITIVE_ID leaves, a leaf stores the id of
 // the primitive at that sorted position.
+//
+// The codes a workgroup's nodes search within +-128 positions are staged in
+// LDS; farther probes (large ranges near the root) read global memory.
+// Besides left/right it writes, for the fit: parent links (internal node c
+// at c, the leaf at sorted position k at n-1+k), with bit 31 set when the
+// parent's range straddles a fit tile; the primitive of each leaf position
+// (primitive-id leaves); and clears the fit flags.
 
 template<typename KeyT>
 struct KeyTraits;
@@ -241,54 +286,77 @@ struct KeyTraits<uint64_t>
   __device__ static int clz(uint64_t x) { return __clzll((long long)x); }
 };
 
+constexpr int kKarrasHalo = 128;
+constexpr int kKarrasWindow = kBlock + 2 * kKarrasHalo;
+
 template<typename KeyT>
-__device__ __forceinline__ int
-calc_delta(const KeyT* __restrict__ codes, int64_t n, int64_t j, KeyT cj, int64_t k)
+struct CodeWindow
 {
-  if (k < 0 || k >= n)
-    return -1;
-  KeyT ck = codes[k];
-  if (cj == ck)
-    return KeyTraits<KeyT>::clz(KeyT(uint64_t(j ^ k))) + KeyTraits<KeyT>::bits;
-  return KeyTraits<KeyT>::clz(cj ^ ck);
-}
+  const KeyT* __restrict__ g;
+  const KeyT* s; // LDS copy of g[w0, w0 + kKarrasWindow)
+  int64_t w0;
+  int64_t n;
+  __device__ KeyT at(int64_t k) const
+  {
+    int64_t o = k - w0;
+    return (o >= 0 && o < kKarrasWindow) ? s[o] : g[k];
+  }
+  __device__ int delta(int64_t j, KeyT cj, int64_t k) const
+  {
+    if (k < 0 || k >= n)
+      return -1;
+    KeyT ck = at(k);
+    if (cj == ck)
+      return KeyTraits<KeyT>::clz(KeyT(uint64_t(j ^ k))) + KeyTraits<KeyT>::bits;
+    return KeyTraits<KeyT>::clz(cj ^ ck);
+  }
+};
 
 template<typename KeyT>
 __global__ __launch_bounds__(kBlock) void
 k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64_t n,
          Node* __restrict__ nodes, uint32_t* __restrict__ parent,
-         uint32_t* __restrict__ flags)
+         uint32_t* __restrict__ leafprim, uint32_t* __restrict__ flags)
 {
-  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  __shared__ KeyT s_codes[kKarrasWindow];
+  const int64_t i0 = int64_t(blockIdx.x) * kBlock;
+  const int64_t w0 = i0 - kKarrasHalo;
+  for (int t = threadIdx.x; t < kKarrasWindow; t += kBlock) {
+    int64_t k = w0 + t;
+    s_codes[t] = (k >= 0 && k < n) ? codes[k] : KeyT(0);
+  }
+  __syncthreads();
+  const int64_t i = i0 + threadIdx.x;
   if (i >= n - 1)
     return;
-  const KeyT ci = codes[i];
-  const int dd = calc_delta(codes, n, i, ci, i + 1) - calc_delta(codes, n, i, ci, i - 1);
+  const CodeWindow<KeyT> cw{ codes, s_codes, w0, n };
+  const KeyT ci = cw.at(i);
+  const int dd = cw.delta(i, ci, i + 1) - cw.delta(i, ci, i - 1);
   const int64_t d = (dd < 0) ? -1 : 1;
-  const int delta_min = calc_delta(codes, n, i, ci, i - d);
+  const int delta_min = cw.delta(i, ci, i - d);
   int64_t l_max = 128;
   for (;;) {
     int64_t k = i + l_max * d;
     if (k >= n)
       break;
-    if (!(calc_delta(codes, n, i, ci, k) > delta_min))
+    if (!(cw.delta(i, ci, k) > delta_min))
       break;
     l_max *= 4;
   }
   int64_t l = 0;
   for (int64_t div = 2;; div *= 2) {
     int64_t t = l_max / div;
-    if (calc_delta(codes, n, i, ci, i + (l + t) * d) > delta_min)
+    if (cw.delta(i, ci, i + (l + t) * d) > delta_min)
       l += t;
     if (t == 1)
       break;
   }
   const int64_t j = i + l * d;
-  const int delta_node = calc_delta(codes, n, i, ci, j);
+  const int delta_node = cw.delta(i, ci, j);
   int64_t s = 0;
   for (int64_t div = 2;; div *= 2) {
     int64_t t = (l + (div - 1)) / div;
-    if (calc_delta(codes, n, i, ci, i + (s + t) * d) > delta_node)
+    if (cw.delta(i, ci, i + (s + t) * d) > delta_node)
       s += t;
     if (div >= l)
       break;
@@ -300,24 +368,28 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
   const int64_t mx = (j > i) ? j : i;
   const bool lleaf = (mn == gamma);
   const bool rleaf = (mx == gamma + 1);
+  // my range [mn, mx] straddles a fit tile: my children climb to me globally
+  const uint32_t up = uint32_t(i) | ((mn / kFitTile != mx / kFitTile) ? kLeafBit : 0u);
   uint32_t left, right;
   if (lleaf) {
     uint32_t v = ids ? ids[gamma] : uint32_t(gamma);
     left = v | kLeafBit;
-    if (v < uint64_t(n))
-      parent[(n - 1) + v] = uint32_t(i);
+    parent[(n - 1) + gamma] = up;
+    if (leafprim)
+      leafprim[gamma] = v;
   } else {
     left = uint32_t(gamma);
-    parent[gamma] = uint32_t(i);
+    parent[gamma] = up;
   }
   if (rleaf) {
     uint32_t v = ids ? ids[gamma + 1] : uint32_t(gamma + 1);
     right = v | kLeafBit;
-    if (v < uint64_t(n))
-      parent[(n - 1) + v] = uint32_t(i);
+    parent[(n - 1) + gamma + 1] = up;
+    if (leafprim)
+      leafprim[gamma + 1] = v;
   } else {
     right = uint32_t(gamma + 1);
-    parent[gamma + 1] = uint32_t(i);
+    parent[gamma + 1] = up;
   }
   *reinterpret_cast<uint2*>(&nodes[i].left) = make_uint2(left, right);
   flags[i] = 0u;
@@ -327,11 +399,18 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
 // builder::fit_boxes (lbvh.h:1827-1866) computes every internal box as
 // union(left box, right box), left operand first, where a leaf's box is the
 // converter applied to the primitive its index names.  Here one thread per
-// leaf climbs toward the root; the second thread to reach a node computes its
-// box (the sibling's box is re-derived from the primitive for a leaf child or
-// read coherently for an internal child), publishes it, and continues.  The
-// second arrival also clears the node's flag, so flags are zero again when
-// the kernel ends (ready for a refit).
+// leaf (by sorted position) climbs toward the root; the second thread to
+// reach a node computes its box (a leaf sibling's box is re-derived from its
+// primitive) and continues.
+//
+// A workgroup owns a tile of kFitTile sorted positions.  A node whose range
+// lies inside the tile has its index inside the tile too (a Karras node is
+// numbered by one end of its range), so its visit flag and box live in LDS
+// and the exchange is workgroup-local.  Nodes whose range straddles tiles
+// (bit 31 of the parent link) use device-scope flags and device-coherent box
+// stores/loads; a node whose parent straddles publishes its box coherently
+// before it arrives there.  Global flags are cleared by their second
+// arrival, so a refit needs no memset.
 
 __device__ __forceinline__ Box
 load_node_box_coherent(const Node* nd)
@@ -355,20 +434,46 @@ store_node_box_coherent(Node* nd, const Box& b)
   }
 }
 
-__global__ __launch_bounds__(kBlock) void
-k_fit(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
-      const uint32_t* __restrict__ parent, uint32_t* __restrict__ flags)
+__device__ __forceinline__ void
+store_node_box(Node* nd, const Box& b)
 {
-  const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (v >= n)
+  *reinterpret_cast<float4*>(&nd->bmin[0]) = make_float4(b.lo[0], b.lo[1], b.lo[2], b.hi[0]);
+  *reinterpret_cast<float2*>(&nd->bmax[1]) = make_float2(b.hi[1], b.hi[2]);
+}
+
+__global__ __launch_bounds__(kFitTile) void
+k_fit(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
+      const uint32_t* __restrict__ parent, const uint32_t* __restrict__ leafprim,
+      uint32_t* __restrict__ flags)
+{
+  __shared__ uint32_t s_flag[kFitTile];
+  __shared__ Box s_box[kFitTile];
+  const int64_t tile0 = int64_t(blockIdx.x) * kFitTile;
+  s_flag[threadIdx.x] = 0u;
+  __syncthreads();
+  const int64_t k = tile0 + threadIdx.x;
+  if (k >= n)
+    return;
+  const uint32_t v = leafprim ? leafprim[k] : uint32_t(k);
+  if (int64_t(v) >= n)
     return;
   Box mine = load_prim_box(prims, kind, uint64_t(v));
-  uint32_t my_ref = uint32_t(v) | kLeafBit;
-  uint32_t p = parent[(n - 1) + v];
+  uint32_t my_ref = v | kLeafBit;
+  uint32_t link = parent[(n - 1) + k];
   for (;;) {
+    const uint32_t p = link & ~kLeafBit;
+    const bool global_node = (link & kLeafBit) != 0;
     if (int64_t(p) >= n - 1)
       return; // parent links come from the tree kernel; never taken for a valid tree
-    uint32_t old = atomic_add_agent(&flags[p], 1u);
+    uint32_t old;
+    if (global_node) {
+      old = atomic_add_agent(&flags[p], 1u);
+    } else {
+      if (int64_t(p) < tile0 || int64_t(p) >= tile0 + kFitTile)
+        return; // a tile-local node is numbered inside its tile
+      old = __hip_atomic_fetch_add(&s_flag[p - tile0], 1u, __ATOMIC_ACQ_REL,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
     if (old == 0u)
       return; // first arrival: the sibling's thread completes p
     const uint2 lr = *reinterpret_cast<const uint2*>(&nodes[p].left);
@@ -379,20 +484,35 @@ k_fit(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ n
       if (int64_t(other & ~kLeafBit) >= n)
         return;
       ob = load_prim_box(prims, kind, uint64_t(other & ~kLeafBit));
-    } else {
-      if (int64_t(other) >= n - 1)
-        return;
-      ob = load_node_box_coherent(&nodes[other]);
-    }
-    Box b = i_am_left ? box_union(mine, ob) : box_union(ob, mine);
-    store_node_box_coherent(&nodes[p], b);
-    coherent_store_u32(&flags[p], 0u);
-    if (p == 0u)
+    } else if (int64_t(other) >= n - 1) {
       return;
-    vmem_drain();
+    } else if (global_node) {
+      ob = load_node_box_coherent(&nodes[other]);
+    } else {
+      ob = s_box[other - tile0];
+    }
+    const Box b = i_am_left ? box_union(mine, ob) : box_union(ob, mine);
+    const uint32_t next = (p == 0u) ? 0u : parent[p];
+    const bool next_global = (next & kLeafBit) != 0;
+    if (global_node) {
+      coherent_store_u32(&flags[p], 0u);
+    } else {
+      s_box[p - tile0] = b;
+    }
+    if (p == 0u) {
+      store_node_box(&nodes[p], b);
+      return;
+    }
+    if (next_global) {
+      // the parent is completed by whichever tile arrives second
+      store_node_box_coherent(&nodes[p], b);
+      vmem_drain();
+    } else {
+      store_node_box(&nodes[p], b);
+    }
     mine = b;
     my_ref = p;
-    p = parent[p];
+    link = next;
   }
 }
 
@@ -411,9 +531,9 @@ namespace lbvh_eng {
 using namespace lbvh_dev;
 
 static inline uint32_t
-grid_for(uint64_t n, uint32_t cap)
+grid_for(uint64_t work, uint64_t per_block, uint32_t cap)
 {
-  uint64_t g = (n + kBlock - 1) / kBlock;
+  uint64_t g = (work + per_block - 1) / per_block;
   if (g < 1)
     g = 1;
   if (g > cap)
@@ -424,7 +544,13 @@ grid_for(uint64_t n, uint32_t cap)
 uint32_t
 centroid_partials_needed(uint64_t n)
 {
-  return grid_for(n, kCentroidMaxBlocks);
+  return grid_for(n, uint64_t(kBlock) * 4, kCentroidMaxBlocks);
+}
+
+uint32_t
+encode_groups(uint64_t n)
+{
+  return grid_for(n, uint64_t(kBlock) * 16, kEncodeMaxBlocks);
 }
 
 hipError_t
@@ -432,7 +558,7 @@ launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx, floa
                        float* cz, float* partials, uint32_t* ticket, float* bounds,
                        uint32_t* zero_ptr, uint32_t zero_words, hipStream_t s)
 {
-  uint32_t g = grid_for(n, kCentroidMaxBlocks);
+  uint32_t g = centroid_partials_needed(n);
   hipLaunchKernelGGL(k_centroid_bounds, dim3(g), dim3(kBlock), 0, s, prims, kind, n, cx, cy,
                      cz, partials, ticket, bounds, zero_ptr, zero_words);
   return hipGetLastError();
@@ -440,46 +566,52 @@ launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx, floa
 
 hipError_t
 launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, uint64_t n,
-              const float* bounds, void* codes, uint32_t* hist, int digit_bits, int passes,
-              hipStream_t s)
+              const float* bounds, void* codes, uint32_t* part_hist, uint32_t* hist,
+              int digit_bits, int passes, hipStream_t s)
 {
-  if ((1 << digit_bits) * passes > kHistMaxWords)
+  const uint32_t words = uint32_t(1u << digit_bits) * uint32_t(passes);
+  if (words > uint32_t(kHistMaxWords))
     return hipErrorInvalidValue;
-  size_t lds = size_t(1 << digit_bits) * passes * sizeof(uint32_t);
-  uint32_t g = grid_for(n, 2048);
+  size_t lds = size_t(words) * sizeof(uint32_t);
+  uint32_t g = encode_groups(n);
   if (code_bits == 30)
     hipLaunchKernelGGL(k_encode<uint32_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
-                       bounds, static_cast<uint32_t*>(codes), hist, digit_bits, passes);
+                       bounds, static_cast<uint32_t*>(codes), part_hist, digit_bits, passes);
   else
     hipLaunchKernelGGL(k_encode<uint64_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
-                       bounds, static_cast<uint64_t*>(codes), hist, digit_bits, passes);
+                       bounds, static_cast<uint64_t*>(codes), part_hist, digit_bits, passes);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_hist_reduce, dim3((words + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     part_hist, g, words, hist);
   return hipGetLastError();
 }
 
 hipError_t
 launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n, void* nodes,
-              uint32_t* parent, uint32_t* flags, hipStream_t s)
+              uint32_t* parent, uint32_t* leafprim, uint32_t* flags, hipStream_t s)
 {
   uint64_t m = n - 1;
   uint32_t g = uint32_t((m + kBlock - 1) / kBlock);
   if (code_bits == 30)
     hipLaunchKernelGGL(k_karras<uint32_t>, dim3(g), dim3(kBlock), 0, s,
                        static_cast<const uint32_t*>(sorted), ids, int64_t(n),
-                       static_cast<Node*>(nodes), parent, flags);
+                       static_cast<Node*>(nodes), parent, leafprim, flags);
   else
     hipLaunchKernelGGL(k_karras<uint64_t>, dim3(g), dim3(kBlock), 0, s,
                        static_cast<const uint64_t*>(sorted), ids, int64_t(n),
-                       static_cast<Node*>(nodes), parent, flags);
+                       static_cast<Node*>(nodes), parent, leafprim, flags);
   return hipGetLastError();
 }
 
 hipError_t
 launch_fit(const float* prims, int kind, uint64_t n, void* nodes, const uint32_t* parent,
-           uint32_t* flags, hipStream_t s)
+           const uint32_t* leafprim, uint32_t* flags, hipStream_t s)
 {
-  uint32_t g = uint32_t((n + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_fit, dim3(g), dim3(kBlock), 0, s, prims, kind, int64_t(n),
-                     static_cast<Node*>(nodes), parent, flags);
+  uint32_t g = uint32_t((n + kFitTile - 1) / kFitTile);
+  hipLaunchKernelGGL(k_fit, dim3(g), dim3(kFitTile), 0, s, prims, kind, int64_t(n),
+                     static_cast<Node*>(nodes), parent, leafprim, flags);
   return hipGetLastError();
 }
 
@@ -488,7 +620,7 @@ launch_fill_u32(uint32_t* p, uint64_t words, uint32_t value, hipStream_t s)
 {
   if (words == 0)
     return hipSuccess;
-  uint32_t g = grid_for(words, 4096);
+  uint32_t g = grid_for(words, kBlock, 4096);
   hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, words, value);
   return hipGetLastError();
 }

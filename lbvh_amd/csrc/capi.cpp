@@ -47,7 +47,8 @@ struct lbvh_ctx
   std::string last_error;
   uint32_t epoch_next = 1;
   // workspace
-  DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, small, status, io_in, io_out;
+  DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, part_hist, small, status, io_in,
+    io_out;
   bool status_zeroed_words = 0;
 };
 
@@ -66,6 +67,7 @@ struct lbvh_bvh
   bool owns_prims = false;
   void* d_codes = nullptr;
   uint32_t* d_ids = nullptr;
+  uint32_t* d_leafprim = nullptr; // primitive of each leaf position (primitive-id leaves)
   uint64_t bytes = 0;
 };
 
@@ -216,6 +218,7 @@ alloc_bvh(lbvh_ctx* c, lbvh_bvh* b, bool keep_codes)
     { reinterpret_cast<void**>(&b->d_bounds), 32 },
     { &b->d_codes, keep_codes ? n * cw : 0 },
     { reinterpret_cast<void**>(&b->d_ids), keep_codes ? n * 4 : 0 },
+    { reinterpret_cast<void**>(&b->d_leafprim), b->leaf_order == LBVH_LEAF_PRIMITIVE_ID ? n * 4 : 0 },
   };
   for (auto& r : req) {
     if (!r.bytes)
@@ -244,6 +247,7 @@ free_bvh(lbvh_bvh* b)
   dev_free(b->d_bounds);
   dev_free(b->d_codes);
   dev_free(b->d_ids);
+  dev_free(b->d_leafprim);
   if (b->owns_prims)
     dev_free(b->d_prims);
   (void)hipSetDevice(cur);
@@ -289,6 +293,9 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
       return st;
   }
   if ((st = ensure(c, c->partials, 24ull * centroid_partials_needed(n))) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->part_hist, 4ull * encode_groups(n) * uint64_t(passes) * radix)) !=
+      LBVH_OK)
     return st;
   const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
   if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
@@ -342,7 +349,8 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
                                         c->stream);
   if (e == hipSuccess)
     e = launch_encode(opts.code_bits, cent, cent + n, cent + 2 * n, n, b->d_bounds,
-                      c->keys_a.p, ws.hist, cfg.digit_bits, passes, c->stream);
+                      c->keys_a.p, static_cast<uint32_t*>(c->part_hist.p), ws.hist,
+                      cfg.digit_bits, passes, c->stream);
   record(c, 1);
   bool in_b = false;
   if (e == hipSuccess)
@@ -357,10 +365,11 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   if (e == hipSuccess)
     e = launch_karras(opts.code_bits, sorted,
                       opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr, n,
-                      b->d_nodes, b->d_parent, b->d_flags, c->stream);
+                      b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags, c->stream);
   record(c, 3);
   if (e == hipSuccess)
-    e = launch_fit(d_prims, kind, n, b->d_nodes, b->d_parent, b->d_flags, c->stream);
+    e = launch_fit(d_prims, kind, n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
+                   c->stream);
   record(c, 4);
   if (e == hipSuccess && opts.keep_codes) {
     e = hipMemcpyAsync(b->d_codes, sorted, kw * n, hipMemcpyDeviceToDevice, c->stream);
@@ -472,7 +481,7 @@ lbvh_ctx_destroy(lbvh_ctx* c)
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = { &c->cent, &c->keys_a, &c->keys_b, &c->ids_a, &c->ids_b, &c->partials,
-                     &c->small, &c->status, &c->io_in, &c->io_out };
+                     &c->part_hist, &c->small, &c->status, &c->io_in, &c->io_out };
   for (auto* b : bufs)
     dev_free(b->p);
   for (auto& es : c->sets)
@@ -584,7 +593,8 @@ lbvh_ctx_workspace_bytes(const lbvh_ctx* c)
   if (!c)
     return 0;
   return c->cent.cap + c->keys_a.cap + c->keys_b.cap + c->ids_a.cap + c->ids_b.cap +
-         c->partials.cap + c->small.cap + c->status.cap + c->io_in.cap + c->io_out.cap;
+         c->partials.cap + c->part_hist.cap + c->small.cap + c->status.cap + c->io_in.cap +
+         c->io_out.cap;
 }
 
 lbvh_status
@@ -681,7 +691,8 @@ lbvh_refit_triangles_device(lbvh_ctx* c, lbvh_bvh* b, const float* d_tri)
   }
   begin_timed(c, 3);
   record(c, 0);
-  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_flags, c->stream));
+  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
+                          c->stream));
   record(c, 1);
   return LBVH_OK;
 }
@@ -709,7 +720,8 @@ lbvh_refit_triangles(lbvh_ctx* c, lbvh_bvh* b, const float* tri)
   HIP_TRY(c, hipMemcpyAsync(b->d_prims, tri, b->n * 48, hipMemcpyHostToDevice, c->stream));
   begin_timed(c, 3);
   record(c, 0);
-  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_flags, c->stream));
+  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
+                          c->stream));
   record(c, 1);
   return sync_and_check(c);
 }

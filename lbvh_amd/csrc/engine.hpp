@@ -39,10 +39,13 @@ hipError_t launch_centroid_bounds(const float* prims, int kind, uint64_t n, floa
                                   hipStream_t s);
 uint32_t centroid_partials_needed(uint64_t n);
 
-// Morton codes (u32 for 30-bit, u64 for 63-bit) and per-pass digit histograms.
+// Morton codes (u32 for 30-bit, u64 for 63-bit) and per-pass digit histograms
+// (per-workgroup counts in part_hist[encode_groups(n)][passes << digit_bits],
+// summed into hist).
 hipError_t launch_encode(int code_bits, const float* cx, const float* cy, const float* cz,
-                         uint64_t n, const float* bounds, void* codes, uint32_t* hist,
-                         int digit_bits, int passes, hipStream_t s);
+                         uint64_t n, const float* bounds, void* codes, uint32_t* part_hist,
+                         uint32_t* hist, int digit_bits, int passes, hipStream_t s);
+uint32_t encode_groups(uint64_t n);
 
 // LSD onesweep radix sort of (key, id) pairs in place across ping-pong
 // buffers; ids_in == nullptr means ids = 0..n-1; ids_a == nullptr sorts keys
@@ -54,14 +57,19 @@ hipError_t launch_sort(int key_bits, uint64_t n, void* keys_a, void* keys_b, uin
 uint64_t sort_status_words(uint64_t n, const SortConfig& cfg);
 
 // Karras radix tree: left/right of every internal node, parent links
-// (internal node c at c, leaf value v at n-1+v), fit flags cleared.
+// (internal node c at c, the leaf at sorted position k at n-1+k; bit 31 marks
+// a parent whose range straddles a fit tile), the primitive of each leaf
+// position when ids != nullptr (leafprim), fit flags cleared.
 hipError_t launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
-                         void* nodes, uint32_t* parent, uint32_t* flags, hipStream_t s);
+                         void* nodes, uint32_t* parent, uint32_t* leafprim, uint32_t* flags,
+                         hipStream_t s);
 
-// Bottom-up box fit with atomic visit flags (flags must be zero; they are
-// left zero again).
+// Bottom-up box fit with visit flags (global flags must be zero; they are
+// left zero again).  leafprim == nullptr: the leaf at position k is
+// primitive k.
 hipError_t launch_fit(const float* prims, int kind, uint64_t n, void* nodes,
-                      const uint32_t* parent, uint32_t* flags, hipStream_t s);
+                      const uint32_t* parent, const uint32_t* leafprim, uint32_t* flags,
+                      hipStream_t s);
 
 // Closest-hit traversal.
 hipError_t launch_trace(int mode, const void* nodes, const float* tris, uint64_t n,

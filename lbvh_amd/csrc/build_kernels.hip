@@ -11,6 +11,13 @@ constexpr int kCentroidMaxBlocks = 1024;
 constexpr int kEncodeMaxBlocks = 256;
 constexpr int kFitTile = 1024; // leaves (sorted positions) per fit workgroup
 
+// Parent link of a child (internal node c at parent[c], the leaf at sorted
+// position k at parent[n - 1 + k]): the parent's index in bits 0-28 and
+constexpr uint32_t kLinkGlobal = 1u << 31;  // the parent's range straddles fit tiles
+constexpr uint32_t kLinkRight = 1u << 30;   // the child is the parent's right child
+constexpr uint32_t kLinkSibLeaf = 1u << 29; // the child's sibling is a leaf
+constexpr uint32_t kLinkIndex = kLinkSibLeaf - 1;
+
 // ---- centroid bounds ---------------------------------------------------------
 // Reproduces centroid_bounds_kernel + the serial union of per-thread boxes
 // (lbvh.h:1282-1293, 1539-1543): the union of (min + max) * 0.5 over every
@@ -237,7 +244,10 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
     out[j] = s_hist[j];
 }
 
-// Sums the per-workgroup digit counts: hist[w] = sum over g of part[g][w].
+// Sums the per-workgroup digit counts into the (zeroed) histogram:
+// hist[w] += sum of part[g][w] over a slice of kHistSlice workgroups.
+constexpr uint32_t kHistSlice = 16;
+
 __global__ __launch_bounds__(kBlock) void
 k_hist_reduce(const uint32_t* __restrict__ part, uint32_t groups, uint32_t words,
               uint32_t* __restrict__ hist)
@@ -245,10 +255,14 @@ k_hist_reduce(const uint32_t* __restrict__ part, uint32_t groups, uint32_t words
   const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
   if (w >= words)
     return;
+  const uint32_t g0 = blockIdx.y * kHistSlice;
   uint32_t s = 0;
-  for (uint32_t g = 0; g < groups; g++)
-    s += part[size_t(g) * words + w];
-  hist[w] = s;
+#pragma unroll
+  for (uint32_t g = 0; g < kHistSlice; g++)
+    if (g0 + g < groups)
+      s += part[size_t(g0 + g) * words + w];
+  if (s)
+    atomicAdd(&hist[w], s);
 }
 
 // ---- Karras radix tree -----------------------------------------------------------
@@ -266,10 +280,9 @@ k_hist_reduce(const uint32_t* __restrict__ part, uint32_t groups, uint32_t words
 //
 // The codes a workgroup's nodes search within +-128 positions are staged in
 // LDS; farther probes (large ranges near the root) read global memory.
-// Besides left/right it writes, for the fit: parent links (internal node c
-// at c, the leaf at sorted position k at n-1+k), with bit 31 set when the
-// parent's range straddles a fit tile; the primitive of each leaf position
-// (primitive-id leaves); and clears the fit flags.
+// Besides left/right it writes, for the fit: parent links (see kLink*); the
+// primitive of each leaf position (primitive-id leaves); and clears the fit
+// flags.
 
 template<typename KeyT>
 struct KeyTraits;
@@ -336,32 +349,34 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
   const int delta_min = cw.delta(i, ci, i - d);
   int64_t l_max = 128;
   for (;;) {
-    int64_t k = i + l_max * d;
+    int64_t k = i + (d > 0 ? l_max : -l_max);
     if (k >= n)
       break;
     if (!(cw.delta(i, ci, k) > delta_min))
       break;
     l_max *= 4;
   }
+  // l_max and every div are powers of two and l >= 0, so l_max / div and
+  // ceil_div(l, div) are exact shifts.
   int64_t l = 0;
-  for (int64_t div = 2;; div *= 2) {
-    int64_t t = l_max / div;
-    if (cw.delta(i, ci, i + (l + t) * d) > delta_min)
+  for (int sh = 1;; sh++) {
+    int64_t t = l_max >> sh;
+    if (cw.delta(i, ci, i + (d > 0 ? (l + t) : -(l + t))) > delta_min)
       l += t;
     if (t == 1)
       break;
   }
-  const int64_t j = i + l * d;
+  const int64_t j = i + (d > 0 ? l : -l);
   const int delta_node = cw.delta(i, ci, j);
   int64_t s = 0;
-  for (int64_t div = 2;; div *= 2) {
-    int64_t t = (l + (div - 1)) / div;
-    if (cw.delta(i, ci, i + (s + t) * d) > delta_node)
+  for (int sh = 1;; sh++) {
+    int64_t t = (l + ((int64_t(1) << sh) - 1)) >> sh;
+    if (cw.delta(i, ci, i + (d > 0 ? (s + t) : -(s + t))) > delta_node)
       s += t;
-    if (div >= l)
+    if ((int64_t(1) << sh) >= l)
       break;
   }
-  const int64_t gamma = i + s * d + (d < 0 ? d : 0);
+  const int64_t gamma = (d > 0) ? (i + s) : (i - s - 1);
   if (gamma < 0 || gamma + 1 > n - 1)
     return; // unreachable for sorted codes; keeps every index below in range
   const int64_t mn = (i < j) ? i : j;
@@ -369,27 +384,29 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
   const bool lleaf = (mn == gamma);
   const bool rleaf = (mx == gamma + 1);
   // my range [mn, mx] straddles a fit tile: my children climb to me globally
-  const uint32_t up = uint32_t(i) | ((mn / kFitTile != mx / kFitTile) ? kLeafBit : 0u);
+  const uint32_t up = uint32_t(i) | ((mn / kFitTile != mx / kFitTile) ? kLinkGlobal : 0u);
+  const uint32_t left_link = up | (rleaf ? kLinkSibLeaf : 0u);
+  const uint32_t right_link = up | kLinkRight | (lleaf ? kLinkSibLeaf : 0u);
   uint32_t left, right;
   if (lleaf) {
     uint32_t v = ids ? ids[gamma] : uint32_t(gamma);
     left = v | kLeafBit;
-    parent[(n - 1) + gamma] = up;
+    parent[(n - 1) + gamma] = left_link;
     if (leafprim)
       leafprim[gamma] = v;
   } else {
     left = uint32_t(gamma);
-    parent[gamma] = up;
+    parent[gamma] = left_link;
   }
   if (rleaf) {
     uint32_t v = ids ? ids[gamma + 1] : uint32_t(gamma + 1);
     right = v | kLeafBit;
-    parent[(n - 1) + gamma + 1] = up;
+    parent[(n - 1) + gamma + 1] = right_link;
     if (leafprim)
       leafprim[gamma + 1] = v;
   } else {
     right = uint32_t(gamma + 1);
-    parent[gamma + 1] = up;
+    parent[gamma + 1] = right_link;
   }
   *reinterpret_cast<uint2*>(&nodes[i].left) = make_uint2(left, right);
   flags[i] = 0u;
@@ -400,14 +417,17 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
 // union(left box, right box), left operand first, where a leaf's box is the
 // converter applied to the primitive its index names.  Here one thread per
 // leaf (by sorted position) climbs toward the root; the second thread to
-// reach a node computes its box (a leaf sibling's box is re-derived from its
-// primitive) and continues.
+// reach a node computes its box and continues.  In the Karras layout the
+// children of a node with split g are g and g + 1 (leaf positions or node
+// indices), so the parent link's side and sibling-is-leaf bits locate the
+// sibling without reading the node.
 //
-// A workgroup owns a tile of kFitTile sorted positions.  A node whose range
-// lies inside the tile has its index inside the tile too (a Karras node is
-// numbered by one end of its range), so its visit flag and box live in LDS
-// and the exchange is workgroup-local.  Nodes whose range straddles tiles
-// (bit 31 of the parent link) use device-scope flags and device-coherent box
+// A workgroup owns a tile of kFitTile sorted positions; it first stages the
+// boxes of its leaves and the links of its internal nodes in LDS.  A node
+// whose range lies inside the tile is numbered inside it too (a Karras node
+// is numbered by one end of its range), so its visit flag and box live in
+// LDS and the climb through it touches no global memory.  Nodes whose range
+// straddles tiles use device-scope flags and device-coherent box
 // stores/loads; a node whose parent straddles publishes its box coherently
 // before it arrives there.  Global flags are cleared by their second
 // arrival, so a refit needs no memset.
@@ -447,71 +467,85 @@ k_fit(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ n
       uint32_t* __restrict__ flags)
 {
   __shared__ uint32_t s_flag[kFitTile];
-  __shared__ Box s_box[kFitTile];
+  __shared__ uint32_t s_link[kFitTile]; // link of internal node tile0 + t
+  __shared__ Box s_leaf[kFitTile];      // box of the leaf at position tile0 + t
+  __shared__ Box s_box[kFitTile];       // box of tile-local internal node tile0 + t
   const int64_t tile0 = int64_t(blockIdx.x) * kFitTile;
-  s_flag[threadIdx.x] = 0u;
+  const int t = threadIdx.x;
+  const int64_t k = tile0 + t;
+  s_flag[t] = 0u;
+  uint32_t link = 0u;
+  if (k < n) {
+    const uint32_t v = leafprim ? leafprim[k] : uint32_t(k);
+    if (int64_t(v) < n)
+      s_leaf[t] = load_prim_box(prims, kind, uint64_t(v));
+    link = parent[(n - 1) + k];
+  }
+  if (k > 0 && k < n - 1)
+    s_link[t] = parent[k];
   __syncthreads();
-  const int64_t k = tile0 + threadIdx.x;
   if (k >= n)
     return;
-  const uint32_t v = leafprim ? leafprim[k] : uint32_t(k);
-  if (int64_t(v) >= n)
-    return;
-  Box mine = load_prim_box(prims, kind, uint64_t(v));
-  uint32_t my_ref = v | kLeafBit;
-  uint32_t link = parent[(n - 1) + k];
+  Box mine = s_leaf[t];
+  int64_t cur = k;
   for (;;) {
-    const uint32_t p = link & ~kLeafBit;
-    const bool global_node = (link & kLeafBit) != 0;
-    if (int64_t(p) >= n - 1)
-      return; // parent links come from the tree kernel; never taken for a valid tree
+    const int64_t p = int64_t(link & kLinkIndex);
+    const bool global_node = (link & kLinkGlobal) != 0;
+    const bool right = (link & kLinkRight) != 0;
+    const int64_t sib = right ? cur - 1 : cur + 1;
+    // links come from the tree kernel; these exits are never taken for a valid tree
+    if (p >= n - 1 || sib < 0 || sib >= n)
+      return;
+    const bool p_in_tile = p >= tile0 && p < tile0 + kFitTile;
     uint32_t old;
     if (global_node) {
       old = atomic_add_agent(&flags[p], 1u);
     } else {
-      if (int64_t(p) < tile0 || int64_t(p) >= tile0 + kFitTile)
-        return; // a tile-local node is numbered inside its tile
+      if (!p_in_tile)
+        return;
       old = __hip_atomic_fetch_add(&s_flag[p - tile0], 1u, __ATOMIC_ACQ_REL,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     if (old == 0u)
       return; // first arrival: the sibling's thread completes p
-    const uint2 lr = *reinterpret_cast<const uint2*>(&nodes[p].left);
-    const bool i_am_left = (lr.x == my_ref);
-    const uint32_t other = i_am_left ? lr.y : lr.x;
+    const bool sib_in_tile = sib >= tile0 && sib < tile0 + kFitTile;
     Box ob;
-    if (other & kLeafBit) {
-      if (int64_t(other & ~kLeafBit) >= n)
-        return;
-      ob = load_prim_box(prims, kind, uint64_t(other & ~kLeafBit));
-    } else if (int64_t(other) >= n - 1) {
-      return;
+    if (link & kLinkSibLeaf) {
+      if (sib_in_tile) {
+        ob = s_leaf[sib - tile0];
+      } else {
+        const uint32_t v = leafprim ? leafprim[sib] : uint32_t(sib);
+        if (int64_t(v) >= n)
+          return;
+        ob = load_prim_box(prims, kind, uint64_t(v));
+      }
     } else if (global_node) {
-      ob = load_node_box_coherent(&nodes[other]);
+      ob = load_node_box_coherent(&nodes[sib]);
     } else {
-      ob = s_box[other - tile0];
+      if (!sib_in_tile)
+        return;
+      ob = s_box[sib - tile0];
     }
-    const Box b = i_am_left ? box_union(mine, ob) : box_union(ob, mine);
-    const uint32_t next = (p == 0u) ? 0u : parent[p];
-    const bool next_global = (next & kLeafBit) != 0;
-    if (global_node) {
+    const Box b = right ? box_union(ob, mine) : box_union(mine, ob);
+    if (global_node)
       coherent_store_u32(&flags[p], 0u);
-    } else {
-      s_box[p - tile0] = b;
-    }
-    if (p == 0u) {
-      store_node_box(&nodes[p], b);
+    if (p == 0) {
+      store_node_box(&nodes[0], b);
       return;
     }
-    if (next_global) {
-      // the parent is completed by whichever tile arrives second
+    const uint32_t next = p_in_tile ? s_link[p - tile0] : parent[p];
+    if (next & kLinkGlobal) {
+      // p's parent is completed by whichever tile arrives second
       store_node_box_coherent(&nodes[p], b);
       vmem_drain();
     } else {
+      if (!p_in_tile)
+        return; // a node under a tile-local parent is itself tile-local
       store_node_box(&nodes[p], b);
+      s_box[p - tile0] = b;
     }
     mine = b;
-    my_ref = p;
+    cur = p;
     link = next;
   }
 }
@@ -583,8 +617,9 @@ launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
-  hipLaunchKernelGGL(k_hist_reduce, dim3((words + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                     part_hist, g, words, hist);
+  hipLaunchKernelGGL(k_hist_reduce,
+                     dim3((words + kBlock - 1) / kBlock, (g + kHistSlice - 1) / kHistSlice),
+                     dim3(kBlock), 0, s, part_hist, g, words, hist);
   return hipGetLastError();
 }
 

@@ -268,8 +268,8 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
     return fail(c, LBVH_E_INVALID, "unknown leaf order");
   if (n < 2)
     return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
-  if (n >= (uint64_t(1) << 30))
-    return fail(c, LBVH_E_INVALID, "at most 2^30 - 1 primitives per tree");
+  if (n >= (uint64_t(1) << 29))
+    return fail(c, LBVH_E_INVALID, "at most 2^29 - 1 primitives per tree");
   SortConfig cfg = tune_sort(n, opts.code_bits, &opts);
   if (!valid_sort_config(cfg))
     return fail(c, LBVH_E_INVALID, "unsupported sort digit width / tile");

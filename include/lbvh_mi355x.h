@@ -73,7 +73,9 @@ typedef struct lbvh_build_opts
   uint32_t sort_digit_bits; /* 0 = tuned default; else 6..10 */
   uint32_t sort_tile;       /* 0 = tuned default; else keys per workgroup */
   uint32_t keep_codes;      /* keep sorted codes / ids for download */
-  uint32_t reserved[3];
+  uint32_t sort_strategy;   /* 0 = tuned; 1 = onesweep (decoupled look-back);
+                               2 = reduce-then-scan */
+  uint32_t reserved[2];
 } lbvh_build_opts;
 
 typedef struct lbvh_bvh_info

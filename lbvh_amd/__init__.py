@@ -43,7 +43,8 @@ class LbvhError(RuntimeError):
 class BuildOpts(ctypes.Structure):
     _fields_ = [("code_bits", ctypes.c_uint32), ("leaf_order", ctypes.c_uint32),
                 ("sort_digit_bits", ctypes.c_uint32), ("sort_tile", ctypes.c_uint32),
-                ("keep_codes", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("keep_codes", ctypes.c_uint32), ("sort_strategy", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
 
 
 class BvhInfo(ctypes.Structure):
@@ -155,7 +156,10 @@ def _hptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-def _make_opts(code_bits, leaf_order, keep_codes, sort_digit_bits=0, sort_tile=0):
+SORT_AUTO, SORT_ONESWEEP, SORT_REDUCE_SCAN = 0, 1, 2
+
+
+def _make_opts(code_bits, leaf_order, keep_codes, sort_digit_bits=0, sort_tile=0, sort_strategy=0):
     o = BuildOpts()
     lib().lbvh_build_opts_init(ctypes.byref(o))
     o.code_bits = code_bits
@@ -163,6 +167,7 @@ def _make_opts(code_bits, leaf_order, keep_codes, sort_digit_bits=0, sort_tile=0
     o.keep_codes = 1 if keep_codes else 0
     o.sort_digit_bits = sort_digit_bits
     o.sort_tile = sort_tile
+    o.sort_strategy = sort_strategy
     return o
 
 
@@ -277,17 +282,17 @@ class Context:
 
     # -- build --------------------------------------------------------------
     def build_triangles(self, tri, code_bits=30, leaf_order=LEAF_REFERENCE, keep_codes=False,
-                        sort_digit_bits=0, sort_tile=0):
+                        sort_digit_bits=0, sort_tile=0, sort_strategy=SORT_AUTO):
         return self._build(tri, 12, "lbvh_build_triangles", code_bits, leaf_order, keep_codes,
-                           sort_digit_bits, sort_tile)
+                           sort_digit_bits, sort_tile, sort_strategy)
 
     def build_aabbs(self, boxes, code_bits=30, leaf_order=LEAF_REFERENCE, keep_codes=False,
-                    sort_digit_bits=0, sort_tile=0):
+                    sort_digit_bits=0, sort_tile=0, sort_strategy=SORT_AUTO):
         return self._build(boxes, 6, "lbvh_build_aabbs", code_bits, leaf_order, keep_codes,
-                           sort_digit_bits, sort_tile)
+                           sort_digit_bits, sort_tile, sort_strategy)
 
-    def _build(self, prims, width, fn, code_bits, leaf_order, keep_codes, sdb, stile):
-        opts = _make_opts(code_bits, leaf_order, keep_codes, sdb, stile)
+    def _build(self, prims, width, fn, code_bits, leaf_order, keep_codes, sdb, stile, sstrat):
+        opts = _make_opts(code_bits, leaf_order, keep_codes, sdb, stile, sstrat)
         out = ctypes.c_void_p()
         if _is_device(prims):
             n = _numel(prims) // width

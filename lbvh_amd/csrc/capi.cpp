@@ -47,8 +47,8 @@ struct lbvh_ctx
   std::string last_error;
   uint32_t epoch_next = 1;
   // workspace
-  DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, part_hist, small, status, io_in,
-    io_out;
+  DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, part_hist, small, status, counts,
+    io_in, io_out;
   bool status_zeroed_words = 0;
 };
 
@@ -182,18 +182,26 @@ tune_sort(uint64_t n, uint32_t code_bits, const lbvh_build_opts* o)
 {
   SortConfig c;
   c.digit_bits = 8;
-  c.items = (n <= (uint64_t(1) << 21)) ? 8 : 16;
+  c.items = 16;
+  c.strategy = 1;
+  (void)n;
   (void)code_bits;
   if (o && o->sort_digit_bits)
     c.digit_bits = int(o->sort_digit_bits);
   if (o && o->sort_tile)
     c.items = int(o->sort_tile / 256u);
+  if (o && o->sort_strategy)
+    c.strategy = int(o->sort_strategy) - 1;
   return c;
 }
 
 bool
 valid_sort_config(const SortConfig& c)
 {
+  if (c.strategy < 0 || c.strategy > 1)
+    return false;
+  if (c.strategy == 1)
+    return (c.digit_bits == 8 || c.digit_bits == 10) && (c.items == 8 || c.items == 16);
   if (c.digit_bits == 8)
     return c.items == 4 || c.items == 8 || c.items == 16;
   if (c.digit_bits == 10)
@@ -300,6 +308,9 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
   if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
     return st;
+  const uint64_t counts_words = uint64_t(rts_groups(n, cfg, nullptr)) * radix;
+  if ((st = ensure(c, c->counts, 4 * counts_words)) != LBVH_OK)
+    return st;
   const uint64_t sw = sort_status_words(n, cfg);
   if (c->status.cap < sw * 8) {
     if ((st = ensure(c, c->status, sw * 8, true)) != LBVH_OK)
@@ -339,6 +350,8 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   ws.tickets = small + kSmallTickets;
   ws.hist = small + kSmallHist;
   ws.err = small + kSmallErr;
+  ws.counts = static_cast<uint32_t*>(c->counts.p);
+  ws.counts_words = c->counts.cap / 4;
 
   begin_timed(c, 1);
   record(c, 0);
@@ -481,7 +494,7 @@ lbvh_ctx_destroy(lbvh_ctx* c)
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = { &c->cent, &c->keys_a, &c->keys_b, &c->ids_a, &c->ids_b, &c->partials,
-                     &c->part_hist, &c->small, &c->status, &c->io_in, &c->io_out };
+                     &c->part_hist, &c->small, &c->status, &c->counts, &c->io_in, &c->io_out };
   for (auto* b : bufs)
     dev_free(b->p);
   for (auto& es : c->sets)
@@ -593,8 +606,8 @@ lbvh_ctx_workspace_bytes(const lbvh_ctx* c)
   if (!c)
     return 0;
   return c->cent.cap + c->keys_a.cap + c->keys_b.cap + c->ids_a.cap + c->ids_b.cap +
-         c->partials.cap + c->part_hist.cap + c->small.cap + c->status.cap + c->io_in.cap +
-         c->io_out.cap;
+         c->partials.cap + c->part_hist.cap + c->small.cap + c->status.cap + c->counts.cap +
+         c->io_in.cap + c->io_out.cap;
 }
 
 lbvh_status

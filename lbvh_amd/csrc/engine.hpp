@@ -13,6 +13,8 @@ struct SortConfig
 {
   int digit_bits = 8; // 8 or 10 (radix 256 / 1024)
   int items = 8;      // keys per thread; tile = 256 * items
+  int strategy = 0;   // 0: onesweep (decoupled look-back), 1: reduce-then-scan
+  int groups = 0;     // reduce-then-scan: target number of chunks (0 = 512)
 };
 
 inline int
@@ -29,6 +31,8 @@ struct SortWorkspace
   uint32_t* tickets = nullptr; // [kMaxSortPasses] virtual block counters
   uint32_t* hist = nullptr;    // [passes * radix] global digit histograms
   uint32_t* err = nullptr;     // bounded-wait failure flag
+  uint32_t* counts = nullptr;  // reduce-then-scan chunk counts [radix * groups]
+  uint64_t counts_words = 0;
 };
 
 // centroid of every primitive box (SoA) and the centroid bounds; also zeroes
@@ -55,6 +59,8 @@ hipError_t launch_sort(int key_bits, uint64_t n, void* keys_a, void* keys_b, uin
                        uint32_t* ids_b, const SortConfig& cfg, const SortWorkspace& ws,
                        uint32_t epoch, bool* result_in_b, hipStream_t s);
 uint64_t sort_status_words(uint64_t n, const SortConfig& cfg);
+// Number of chunks of the reduce-then-scan strategy (and tiles per chunk).
+uint32_t rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles);
 
 // Karras radix tree: left/right of every internal node, parent links
 // (internal node c at c, the leaf at sorted position k at n-1+k; bit 31 marks

@@ -19,6 +19,12 @@
 //      consecutive threads write consecutive addresses.
 // Every wait is bounded: past a spin limit the kernel raises an error flag
 // and drains, and the host reports a timeout instead of hanging.
+//
+// A second strategy, reduce-then-scan, has no inter-workgroup waits: an
+// upsweep counts each chunk's digits, a scan turns the counts into per-chunk
+// digit offsets, and a downsweep ranks and scatters each chunk tile by tile
+// with running offsets in LDS.  It reads the keys once more per pass but
+// avoids the look-back latency; the tuner picks between the two.
 #include "device_common.hpp"
 #include "engine.hpp"
 
@@ -259,6 +265,220 @@ k_onesweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
   }
 }
 
+// ---- reduce-then-scan --------------------------------------------------------
+
+constexpr int kUpsweepUnroll = 8;
+
+// counts[d * groups + g] = number of keys of chunk g whose digit is d.
+template<typename KeyT, int RBITS>
+__global__ __launch_bounds__(kSortBlock) void
+k_rts_upsweep(const KeyT* __restrict__ keys, uint32_t n, int shift, uint32_t chunk_keys,
+              uint32_t groups, uint32_t* __restrict__ counts)
+{
+  constexpr int RADIX = 1 << RBITS;
+  __shared__ uint32_t s_h[kSortWaves][RADIX];
+  const int tid = threadIdx.x;
+  const int w = tid / kWave;
+  for (int j = tid; j < kSortWaves * RADIX; j += kSortBlock)
+    (&s_h[0][0])[j] = 0u;
+  __syncthreads();
+  const uint32_t g = blockIdx.x;
+  const uint64_t begin64 = uint64_t(g) * chunk_keys;
+  const uint32_t begin = begin64 < n ? uint32_t(begin64) : n;
+  const uint32_t end = (uint64_t(begin) + chunk_keys < n) ? begin + chunk_keys : n;
+  const KeyT dmask = KeyT(RADIX - 1);
+  const uint64_t lt = lanemask_lt();
+  for (uint32_t base = begin; base < end; base += kSortBlock * kUpsweepUnroll) {
+    KeyT k[kUpsweepUnroll];
+#pragma unroll
+    for (int u = 0; u < kUpsweepUnroll; u++) {
+      uint32_t i = base + u * kSortBlock + tid;
+      k[u] = (i < end) ? keys[i] : KeyT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kUpsweepUnroll; u++) {
+      const uint32_t i = base + u * kSortBlock + tid;
+      const bool valid = i < end;
+      const uint32_t d = uint32_t((k[u] >> shift) & dmask);
+      uint64_t peers = ballot(valid);
+#pragma unroll
+      for (int b = 0; b < RBITS; b++) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = ballot(valid && bit);
+        peers &= bit ? m : ~m;
+      }
+      if (valid && (peers & lt) == 0)
+        s_h[w][d] += uint32_t(popc64(peers));
+    }
+  }
+  __syncthreads();
+  for (int d = tid; d < RADIX; d += kSortBlock) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int ww = 0; ww < kSortWaves; ww++)
+      c += s_h[ww][d];
+    counts[size_t(d) * groups + g] = c;
+  }
+}
+
+// One workgroup per digit d: counts[d][g] <- (keys with a digit below d) +
+// (keys of digit d in chunks before g).
+template<int RBITS>
+__global__ __launch_bounds__(kSortBlock) void
+k_rts_scan(uint32_t* __restrict__ counts, uint32_t groups, const uint32_t* __restrict__ ghist)
+{
+  constexpr int RADIX = 1 << RBITS;
+  __shared__ uint32_t s_tmp[kSortWaves];
+  const int d = blockIdx.x;
+  const int tid = threadIdx.x;
+  uint32_t below = 0;
+  for (int dd = tid; dd < d; dd += kSortBlock)
+    below += ghist[dd];
+  uint32_t total;
+  block_exclusive_scan(below, s_tmp, &total); // total = keys with a smaller digit
+  const uint32_t base_d = total;
+  uint32_t* row = counts + size_t(d) * groups;
+  const uint32_t per = (groups + kSortBlock - 1) / kSortBlock;
+  const uint32_t g0 = tid * per;
+  uint32_t sum = 0;
+  for (uint32_t g = g0; g < g0 + per && g < groups; g++)
+    sum += row[g];
+  uint32_t all;
+  uint32_t run = block_exclusive_scan(sum, s_tmp, &all) + base_d;
+  for (uint32_t g = g0; g < g0 + per && g < groups; g++) {
+    uint32_t c = row[g];
+    row[g] = run;
+    run += c;
+  }
+  (void)RADIX;
+}
+
+// Ranks and scatters chunk g, tile by tile, from its per-digit offsets.
+template<typename KeyT, bool VALS, int RBITS, int ITEMS>
+__global__ __launch_bounds__(kSortBlock) void
+k_rts_downsweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
+                const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout, uint32_t n,
+                int shift, uint32_t chunk_tiles, uint32_t groups,
+                const uint32_t* __restrict__ offsets)
+{
+  constexpr int RADIX = 1 << RBITS;
+  constexpr int TILE = kSortBlock * ITEMS;
+  constexpr int DPT = RADIX / kSortBlock;
+  __shared__ KeyT s_keys[TILE];
+  __shared__ uint32_t s_vals[VALS ? TILE : 1];
+  __shared__ uint32_t s_wcnt[kSortWaves][RADIX];
+  __shared__ uint32_t s_bstart[RADIX];
+  __shared__ uint32_t s_goff[RADIX];
+  __shared__ uint32_t s_tmp[kSortWaves];
+
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int w = tid / kWave;
+  const KeyT dmask = KeyT(RADIX - 1);
+  const uint32_t g = blockIdx.x;
+  uint32_t running[DPT];
+#pragma unroll
+  for (int dd = 0; dd < DPT; dd++)
+    running[dd] = offsets[size_t(tid * DPT + dd) * groups + g];
+  const uint64_t lt = lanemask_lt();
+
+  for (uint32_t tt = 0; tt < chunk_tiles; tt++) {
+    const uint64_t tile_begin64 = (uint64_t(g) * chunk_tiles + tt) * TILE;
+    if (tile_begin64 >= n)
+      break; // uniform across the workgroup
+    const uint32_t tile_begin = uint32_t(tile_begin64);
+    for (int j = tid; j < kSortWaves * RADIX; j += kSortBlock)
+      (&s_wcnt[0][0])[j] = 0u;
+    __syncthreads();
+    const uint32_t base = tile_begin + uint32_t(w) * (ITEMS * kWave) + uint32_t(lane);
+    KeyT key[ITEMS];
+    uint32_t val[ITEMS];
+    uint32_t rank[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+      uint32_t idx = base + uint32_t(j) * kWave;
+      bool valid = idx < n;
+      key[j] = valid ? kin[idx] : KeyT(0);
+      if (VALS)
+        val[j] = valid ? (vin ? vin[idx] : idx) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+      const uint32_t idx = base + uint32_t(j) * kWave;
+      const bool valid = idx < n;
+      const uint32_t d = uint32_t((key[j] >> shift) & dmask);
+      uint64_t peers = ballot(valid);
+#pragma unroll
+      for (int b = 0; b < RBITS; b++) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = ballot(valid && bit);
+        peers &= bit ? m : ~m;
+      }
+      uint32_t r = 0;
+      if (valid) {
+        const uint32_t below = uint32_t(popc64(peers & lt));
+        const uint32_t cnt = uint32_t(popc64(peers));
+        const uint32_t wbase = s_wcnt[w][d];
+        r = wbase + below;
+        if (below == 0)
+          s_wcnt[w][d] = wbase + cnt;
+      }
+      rank[j] = r;
+    }
+    __syncthreads();
+    uint32_t tot[DPT];
+    uint32_t my_sum = 0;
+#pragma unroll
+    for (int dd = 0; dd < DPT; dd++) {
+      const int d = tid * DPT + dd;
+      uint32_t acc = 0;
+#pragma unroll
+      for (int ww = 0; ww < kSortWaves; ww++) {
+        uint32_t c = s_wcnt[ww][d];
+        s_wcnt[ww][d] = acc;
+        acc += c;
+      }
+      tot[dd] = acc;
+      my_sum += acc;
+    }
+    uint32_t tile_total;
+    uint32_t run = block_exclusive_scan(my_sum, s_tmp, &tile_total);
+#pragma unroll
+    for (int dd = 0; dd < DPT; dd++) {
+      const int d = tid * DPT + dd;
+      s_bstart[d] = run;
+      s_goff[d] = running[dd] - run;
+      run += tot[dd];
+      running[dd] += tot[dd];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+      const uint32_t idx = base + uint32_t(j) * kWave;
+      if (idx < n) {
+        const uint32_t d = uint32_t((key[j] >> shift) & dmask);
+        const uint32_t pos = s_bstart[d] + s_wcnt[w][d] + rank[j];
+        s_keys[pos] = key[j];
+        if (VALS)
+          s_vals[pos] = val[j];
+      }
+    }
+    __syncthreads();
+    const uint32_t tile_n = (n - tile_begin < uint32_t(TILE)) ? (n - tile_begin) : uint32_t(TILE);
+    for (uint32_t i = tid; i < tile_n; i += kSortBlock) {
+      const KeyT k = s_keys[i];
+      const uint32_t d = uint32_t((k >> shift) & dmask);
+      const uint32_t dst = s_goff[d] + i;
+      if (dst < n) {
+        kout[dst] = k;
+        if (VALS)
+          vout[dst] = s_vals[i];
+      }
+    }
+    __syncthreads(); // LDS is reused by the next tile
+  }
+}
+
 } // namespace lbvh_dev
 
 namespace lbvh_eng {
@@ -308,12 +528,79 @@ run_passes(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t*
   return hipSuccess;
 }
 
+uint32_t
+rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles)
+{
+  const uint64_t tile = uint64_t(kSortBlock) * cfg.items;
+  const uint64_t tiles = (n + tile - 1) / tile;
+  uint64_t target = cfg.groups > 0 ? uint64_t(cfg.groups) : 512;
+  uint64_t ct = (tiles + target - 1) / target;
+  if (ct < 1)
+    ct = 1;
+  if (chunk_tiles)
+    *chunk_tiles = uint32_t(ct);
+  return uint32_t((tiles + ct - 1) / ct);
+}
+
+template<typename KeyT, bool VALS, int RBITS, int ITEMS>
+static hipError_t
+run_rts(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb,
+        const SortConfig& cfg, const SortWorkspace& ws, bool* in_b, hipStream_t s)
+{
+  constexpr int RADIX = 1 << RBITS;
+  constexpr uint32_t TILE = kSortBlock * ITEMS;
+  const int passes = sort_passes(key_bits, RBITS);
+  uint32_t chunk_tiles = 1;
+  const uint32_t groups = rts_groups(n, cfg, &chunk_tiles);
+  if (uint64_t(groups) * RADIX > ws.counts_words)
+    return hipErrorInvalidValue;
+  KeyT* src = ka;
+  KeyT* dst = kb;
+  uint32_t* vsrc = nullptr;
+  uint32_t* vdst = vb;
+  bool cur_b = false;
+  for (int p = 0; p < passes; p++) {
+    const int shift = p * RBITS;
+    hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(groups), dim3(kSortBlock), 0, s, src,
+                       uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts);
+    hipLaunchKernelGGL((k_rts_scan<RBITS>), dim3(RADIX), dim3(kSortBlock), 0, s, ws.counts,
+                       groups, ws.hist + size_t(p) * RADIX);
+    hipLaunchKernelGGL((k_rts_downsweep<KeyT, VALS, RBITS, ITEMS>), dim3(groups),
+                       dim3(kSortBlock), 0, s, src, dst, vsrc, vdst, uint32_t(n), shift,
+                       chunk_tiles, groups, ws.counts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
+    cur_b = !cur_b;
+    KeyT* t = src;
+    src = dst;
+    dst = t;
+    if (VALS) {
+      vsrc = vdst;
+      vdst = (vdst == vb) ? va : vb;
+    }
+  }
+  *in_b = cur_b;
+  return hipSuccess;
+}
+
 template<typename KeyT, bool VALS>
 static hipError_t
 dispatch(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb,
          const SortConfig& cfg, const SortWorkspace& ws, uint32_t epoch, bool* in_b,
          hipStream_t s)
 {
+  if (cfg.strategy == 1) {
+    if (cfg.digit_bits == 8 && cfg.items == 8)
+      return run_rts<KeyT, VALS, 8, 8>(key_bits, n, ka, kb, va, vb, cfg, ws, in_b, s);
+    if (cfg.digit_bits == 8 && cfg.items == 16)
+      return run_rts<KeyT, VALS, 8, 16>(key_bits, n, ka, kb, va, vb, cfg, ws, in_b, s);
+    if (cfg.digit_bits == 10 && cfg.items == 8)
+      return run_rts<KeyT, VALS, 10, 8>(key_bits, n, ka, kb, va, vb, cfg, ws, in_b, s);
+    if (cfg.digit_bits == 10 && cfg.items == 16)
+      return run_rts<KeyT, VALS, 10, 16>(key_bits, n, ka, kb, va, vb, cfg, ws, in_b, s);
+    return hipErrorInvalidValue;
+  }
   if (cfg.digit_bits == 8) {
     if (cfg.items == 8)
       return run_passes<KeyT, VALS, 8, 8>(key_bits, n, ka, kb, va, vb, ws, epoch, in_b, s);

@@ -76,13 +76,21 @@ def test_build_matches_oracle_ragged(ctx, n, bits, leaf):
     assert np.array_equal(ids, oids)
 
 
-@pytest.mark.parametrize("digit_bits,tile", [(8, 1024), (8, 2048), (8, 4096), (10, 2048), (10, 4096)])
+SORT_CONFIGS = [(1, 8, 1024), (1, 8, 2048), (1, 8, 4096), (1, 10, 2048), (1, 10, 4096),
+                (2, 8, 2048), (2, 8, 4096), (2, 10, 2048), (2, 10, 4096)]
+
+
+@pytest.mark.parametrize("strategy,digit_bits,tile", SORT_CONFIGS,
+                         ids=["%s-d%d-t%d" % ("onesweep" if c[0] == 1 else "rts", c[1], c[2])
+                              for c in SORT_CONFIGS])
 @pytest.mark.parametrize("bits", [30, 63])
-def test_sort_configurations_agree(ctx, digit_bits, tile, bits):
+def test_sort_configurations_agree(ctx, strategy, digit_bits, tile, bits):
+    """Both sort strategies at every digit width and tile size give the
+    oracle's stable (code, id) order and tree."""
     v = W.cluster_vertices(77, 300000)
     tri = W.make_triangles(v)
     bvh = ctx.build_triangles(tri, code_bits=bits, leaf_order=1, keep_codes=True,
-                              sort_digit_bits=digit_bits, sort_tile=tile)
+                              sort_digit_bits=digit_bits, sort_tile=tile, sort_strategy=strategy)
     on, _, osc, oids = oracle.build(tri, bits, 1)
     sc, ids, _ = bvh.codes()
     assert np.array_equal(sc, osc)

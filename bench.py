@@ -156,10 +156,10 @@ def main():
     ctx.synchronize()
     log("bench: warmup done (%d steps)" % args.warmup)
 
-    # timed region: K steps back to back; per-phase HIP events are recorded
-    # by the engine on the stream its kernels run on (the context's stream)
+    # timed region: K steps back to back; the engine records one HIP event
+    # pair per call on the stream its kernels run on (the context's stream)
     trees = []
-    ctx.set_timing(True)
+    ctx.set_timing(2)
     hip.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -167,19 +167,31 @@ def main():
     hip.synchronize()
     wall = time.perf_counter() - t0
     ctx.synchronize()
-    build_ph, nb = ctx.phase_stats(1)
-    trace_ph, nt = ctx.phase_stats(2)
-    ctx.set_timing(False)
+    build_tot, nb = ctx.phase_stats(1)
+    trace_tot, nt = ctx.phase_stats(2)
+    ctx.set_timing(0)
     for b in trees:
         b.destroy()
     if prev is not None:
         prev.destroy()
     log("bench: timed %d steps in %.3f s" % (args.steps, wall))
+    # phase split of the build, measured separately (events between phases
+    # add short idle gaps, so they stay out of the timed steps)
+    split = []
+    ctx.set_timing(1)
+    for i in range(5):
+        split.append(ctx.build_triangles(d_tri, code_bits=63, leaf_order=lbvh_amd.LEAF_PRIMITIVE_ID))
+    ctx.synchronize()
+    split_ph, ns = ctx.phase_stats(1)
+    ctx.set_timing(0)
+    for b in split:
+        b.destroy()
 
     K = args.steps
-    build_ms = sum(build_ph) / K
-    phases = [x / K for x in build_ph]
-    trace_ms = sum(trace_ph) / K  # primary + bounce per step
+    build_ms = sum(build_tot) / K
+    split_sum = sum(split_ph)
+    phases = [build_ms * x / split_sum for x in split_ph]  # split scaled to the timed total
+    trace_ms = sum(trace_tot) / K  # primary + bounce per step
     ms_per_step = wall * 1e3 / K
     mprims = N_PRIMS / (build_ms * 1e-3) / 1e6
     mrays = (n_primary + n_bounce) / (trace_ms * 1e-3) / 1e6

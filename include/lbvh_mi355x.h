@@ -101,8 +101,10 @@ lbvh_status lbvh_ctx_create(int device, lbvh_ctx** out);
 void lbvh_ctx_destroy(lbvh_ctx* ctx);
 lbvh_status lbvh_ctx_set_stream(lbvh_ctx* ctx, void* hip_stream);
 lbvh_status lbvh_ctx_synchronize(lbvh_ctx* ctx);
-/* Per-phase device timing of the next builds/traces (HIP events). */
-lbvh_status lbvh_ctx_set_timing(lbvh_ctx* ctx, int enable);
+/* Device timing of the following calls with HIP events on the context
+ * stream: mode 0 off, 1 every phase, 2 one interval per call (no events
+ * between the kernels of a call).  Setting a mode starts a new record. */
+lbvh_status lbvh_ctx_set_timing(lbvh_ctx* ctx, int mode);
 /* Milliseconds of the phases of the last timed call: for a build
  * [bounds+encode, sort, tree, fit], for a trace [trace], for a refit [fit]. */
 lbvh_status lbvh_ctx_phase_ms(lbvh_ctx* ctx, float* ms, uint32_t cap,

@@ -258,8 +258,9 @@ class Context:
     def synchronize(self):
         self._check(lib().lbvh_ctx_synchronize(self.handle))
 
-    def set_timing(self, enable=True):
-        self._check(lib().lbvh_ctx_set_timing(self.handle, 1 if enable else 0))
+    def set_timing(self, mode=1):
+        """0 off, 1 (or True) per phase, 2 one interval per call."""
+        self._check(lib().lbvh_ctx_set_timing(self.handle, int(mode)))
 
     def phase_ms(self):
         buf = (ctypes.c_float * 8)()

@@ -9,11 +9,14 @@ namespace lbvh_dev {
 constexpr int kBlock = 256;
 constexpr int kCentroidMaxBlocks = 1024;
 constexpr int kEncodeMaxBlocks = 256;
-constexpr int kFitTile = 1024; // leaves (sorted positions) per fit workgroup
+// Largest subtree (in leaves) fitted inside one workgroup's LDS.  Nodes whose
+// range holds more leaves form the "top" tree; their children with at most
+// kTreelet leaves root the treelets.
+constexpr int kTreelet = 1024;
 
 // Parent link of a child (internal node c at parent[c], the leaf at sorted
 // position k at parent[n - 1 + k]): the parent's index in bits 0-28 and
-constexpr uint32_t kLinkGlobal = 1u << 31;  // the parent's range straddles fit tiles
+constexpr uint32_t kLinkGlobal = 1u << 31;  // the parent is a top node
 constexpr uint32_t kLinkRight = 1u << 30;   // the child is the parent's right child
 constexpr uint32_t kLinkSibLeaf = 1u << 29; // the child's sibling is a leaf
 constexpr uint32_t kLinkIndex = kLinkSibLeaf - 1;
@@ -325,11 +328,39 @@ struct CodeWindow
   }
 };
 
+// Largest x in [0, span) with pred(x), for a predicate that holds on a
+// prefix of [1, span) and fails after it (x = 0 when pred(1) fails); span is
+// a power of two.  8-way steps issue 7 independent probes each.
+template<class Pred>
+__device__ __forceinline__ int64_t
+search_prefix(int64_t span, const Pred& pred)
+{
+  int64_t x = 0;
+  while (span >= 8) {
+    const int64_t step = span >> 3;
+    int cnt = 0;
+#pragma unroll
+    for (int q = 1; q < 8; q++)
+      cnt += pred(x + q * step) ? 1 : 0;
+    x += cnt * step;
+    span = step;
+  }
+  while (span > 1) {
+    const int64_t t = span >> 1;
+    if (pred(x + t))
+      x += t;
+    span = t;
+  }
+  return x;
+}
+
 template<typename KeyT>
 __global__ __launch_bounds__(kBlock) void
 k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64_t n,
          Node* __restrict__ nodes, uint32_t* __restrict__ parent,
-         uint32_t* __restrict__ leafprim, uint32_t* __restrict__ flags)
+         uint32_t* __restrict__ leafprim, uint32_t* __restrict__ flags,
+         uint32_t* __restrict__ other_end, uint2* __restrict__ treelets,
+         uint32_t* __restrict__ treelet_count)
 {
   __shared__ KeyT s_codes[kKarrasWindow];
   const int64_t i0 = int64_t(blockIdx.x) * kBlock;
@@ -347,35 +378,33 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
   const int dd = cw.delta(i, ci, i + 1) - cw.delta(i, ci, i - 1);
   const int64_t d = (dd < 0) ? -1 : 1;
   const int delta_min = cw.delta(i, ci, i - d);
-  int64_t l_max = 128;
-  for (;;) {
-    int64_t k = i + (d > 0 ? l_max : -l_max);
-    if (k >= n)
-      break;
-    if (!(cw.delta(i, ci, k) > delta_min))
-      break;
-    l_max *= 4;
+  // The reference grows l_max from 128 by x4 while delta(i, i + l_max d)
+  // exceeds delta_min (stopping when i + l_max d >= n), then binary-searches
+  // the largest l < l_max with delta(i, i + l d) > delta_min, then the
+  // largest split s < l with delta(i, i + s d) > delta_node.  For sorted
+  // keys delta(i, k) does not grow as k moves away from i, so each of these
+  // predicates is true on a prefix and false after it: the searches below
+  // find the same boundaries with independent probes (all l_max candidates
+  // at once, then 7 probes per 8-way step) instead of chains of dependent
+  // loads.
+  int m = 0;
+#pragma unroll
+  for (int q = 0; q < 13; q++) {
+    const int64_t span = int64_t(128) << (2 * q);
+    const int64_t k = i + (d > 0 ? span : -span);
+    const bool go = (k < n) && (cw.delta(i, ci, k) > delta_min);
+    m += go ? 1 : 0; // prefix-true: m = index of the first failing candidate
   }
-  // l_max and every div are powers of two and l >= 0, so l_max / div and
-  // ceil_div(l, div) are exact shifts.
-  int64_t l = 0;
-  for (int sh = 1;; sh++) {
-    int64_t t = l_max >> sh;
-    if (cw.delta(i, ci, i + (d > 0 ? (l + t) : -(l + t))) > delta_min)
-      l += t;
-    if (t == 1)
-      break;
-  }
+  const int64_t l_max = int64_t(128) << (2 * m);
+  auto pred_l = [&](int64_t x) { return cw.delta(i, ci, i + (d > 0 ? x : -x)) > delta_min; };
+  const int64_t l = search_prefix(l_max, pred_l);
   const int64_t j = i + (d > 0 ? l : -l);
   const int delta_node = cw.delta(i, ci, j);
-  int64_t s = 0;
-  for (int sh = 1;; sh++) {
-    int64_t t = (l + ((int64_t(1) << sh) - 1)) >> sh;
-    if (cw.delta(i, ci, i + (d > 0 ? (s + t) : -(s + t))) > delta_node)
-      s += t;
-    if ((int64_t(1) << sh) >= l)
-      break;
-  }
+  int64_t span_s = 1;
+  while (span_s < l)
+    span_s <<= 1;
+  auto pred_s = [&](int64_t x) { return cw.delta(i, ci, i + (d > 0 ? x : -x)) > delta_node; };
+  const int64_t s = search_prefix(span_s, pred_s);
   const int64_t gamma = (d > 0) ? (i + s) : (i - s - 1);
   if (gamma < 0 || gamma + 1 > n - 1)
     return; // unreachable for sorted codes; keeps every index below in range
@@ -383,8 +412,22 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
   const int64_t mx = (j > i) ? j : i;
   const bool lleaf = (mn == gamma);
   const bool rleaf = (mx == gamma + 1);
-  // my range [mn, mx] straddles a fit tile: my children climb to me globally
-  const uint32_t up = uint32_t(i) | ((mn / kFitTile != mx / kFitTile) ? kLinkGlobal : 0u);
+  // a top node: its children hand their boxes up through the top-tree pass
+  const bool top = (mx - mn + 1) > kTreelet;
+  const uint32_t up = uint32_t(i) | (top ? kLinkGlobal : 0u);
+  other_end[i] = uint32_t(j);
+  if (top) {
+    // children with at most kTreelet leaves root treelets; an entry holds the
+    // leaf range and, in bit 31 of .y, whether its root is a right child (the
+    // root is numbered hi for a left child, lo for a right child)
+    if (gamma - mn + 1 <= kTreelet)
+      treelets[atomicAdd(treelet_count, 1u)] = make_uint2(uint32_t(mn), uint32_t(gamma));
+    if (mx - gamma <= kTreelet)
+      treelets[atomicAdd(treelet_count, 1u)] =
+        make_uint2(uint32_t(gamma + 1), uint32_t(mx) | 0x80000000u);
+  } else if (i == 0) {
+    treelets[atomicAdd(treelet_count, 1u)] = make_uint2(0u, uint32_t(n - 1));
+  }
   const uint32_t left_link = up | (rleaf ? kLinkSibLeaf : 0u);
   const uint32_t right_link = up | kLinkRight | (lleaf ? kLinkSibLeaf : 0u);
   uint32_t left, right;
@@ -415,22 +458,21 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
 // ---- bottom-up fit -------------------------------------------------------------
 // builder::fit_boxes (lbvh.h:1827-1866) computes every internal box as
 // union(left box, right box), left operand first, where a leaf's box is the
-// converter applied to the primitive its index names.  Here one thread per
-// leaf (by sorted position) climbs toward the root; the second thread to
-// reach a node computes its box and continues.  In the Karras layout the
-// children of a node with split g are g and g + 1 (leaf positions or node
-// indices), so the parent link's side and sibling-is-leaf bits locate the
-// sibling without reading the node.
+// converter applied to the primitive its index names.  Here the second of
+// the two child computations to finish computes the parent's box.  In the
+// Karras layout the children of a node with split g are g and g + 1 (leaf
+// positions or node indices), so the parent link's side and sibling-is-leaf
+// bits locate the sibling without reading the node.
 //
-// A workgroup owns a tile of kFitTile sorted positions; it first stages the
-// boxes of its leaves and the links of its internal nodes in LDS.  A node
-// whose range lies inside the tile is numbered inside it too (a Karras node
-// is numbered by one end of its range), so its visit flag and box live in
-// LDS and the climb through it touches no global memory.  Nodes whose range
-// straddles tiles use device-scope flags and device-coherent box
-// stores/loads; a node whose parent straddles publishes its box coherently
-// before it arrives there.  Global flags are cleared by their second
-// arrival, so a refit needs no memset.
+// Pass 1 (k_fit_treelets): each treelet -- a subtree of at most kTreelet
+// leaves under a top node -- is fitted by one workgroup entirely in LDS: the
+// leaves' boxes and the internal nodes' links are staged once, then one
+// thread per leaf climbs with LDS visit flags.  Workgroups pull treelets
+// from the list the tree kernel wrote.
+// Pass 2 (k_fit_top): one thread per treelet climbs from the treelet root
+// through the top nodes with device-scope visit flags and device-coherent
+// box stores/loads (the top tree is shallow).  Flags are cleared by their
+// second arrival, so a refit needs no memset.
 
 __device__ __forceinline__ Box
 load_node_box_coherent(const Node* nd)
@@ -461,92 +503,177 @@ store_node_box(Node* nd, const Box& b)
   *reinterpret_cast<float2*>(&nd->bmax[1]) = make_float2(b.hi[1], b.hi[2]);
 }
 
-__global__ __launch_bounds__(kFitTile) void
-k_fit(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
-      const uint32_t* __restrict__ parent, const uint32_t* __restrict__ leafprim,
-      uint32_t* __restrict__ flags)
+__device__ __forceinline__ Box
+leaf_box(const float* __restrict__ prims, int kind, const uint32_t* __restrict__ leafprim,
+         int64_t pos)
 {
-  __shared__ uint32_t s_flag[kFitTile];
-  __shared__ uint32_t s_link[kFitTile]; // link of internal node tile0 + t
-  __shared__ Box s_leaf[kFitTile];      // box of the leaf at position tile0 + t
-  __shared__ Box s_box[kFitTile];       // box of tile-local internal node tile0 + t
-  const int64_t tile0 = int64_t(blockIdx.x) * kFitTile;
+  const uint32_t v = leafprim ? leafprim[pos] : uint32_t(pos);
+  return load_prim_box(prims, kind, uint64_t(v));
+}
+
+__global__ __launch_bounds__(kTreelet) void
+k_fit_treelets(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
+               const uint32_t* __restrict__ parent, const uint32_t* __restrict__ leafprim,
+               const uint2* __restrict__ treelets, const uint32_t* __restrict__ count_ptr,
+               uint32_t* __restrict__ queue)
+{
+  __shared__ uint32_t s_flag[kTreelet];
+  __shared__ uint32_t s_link[kTreelet]; // link of internal node lo + t
+  __shared__ Box s_leaf[kTreelet];      // box of the leaf at position lo + t
+  __shared__ Box s_box[kTreelet];       // box of internal node lo + t
+  __shared__ uint32_t s_item;
   const int t = threadIdx.x;
-  const int64_t k = tile0 + t;
-  s_flag[t] = 0u;
-  uint32_t link = 0u;
-  if (k < n) {
-    const uint32_t v = leafprim ? leafprim[k] : uint32_t(k);
-    if (int64_t(v) < n)
-      s_leaf[t] = load_prim_box(prims, kind, uint64_t(v));
-    link = parent[(n - 1) + k];
-  }
-  if (k > 0 && k < n - 1)
-    s_link[t] = parent[k];
-  __syncthreads();
-  if (k >= n)
-    return;
-  Box mine = s_leaf[t];
-  int64_t cur = k;
+  const uint32_t count = *count_ptr;
   for (;;) {
-    const int64_t p = int64_t(link & kLinkIndex);
-    const bool global_node = (link & kLinkGlobal) != 0;
-    const bool right = (link & kLinkRight) != 0;
-    const int64_t sib = right ? cur - 1 : cur + 1;
-    // links come from the tree kernel; these exits are never taken for a valid tree
-    if (p >= n - 1 || sib < 0 || sib >= n)
-      return;
-    const bool p_in_tile = p >= tile0 && p < tile0 + kFitTile;
-    uint32_t old;
-    if (global_node) {
-      old = atomic_add_agent(&flags[p], 1u);
-    } else {
-      if (!p_in_tile)
-        return;
-      old = __hip_atomic_fetch_add(&s_flag[p - tile0], 1u, __ATOMIC_ACQ_REL,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    if (old == 0u)
-      return; // first arrival: the sibling's thread completes p
-    const bool sib_in_tile = sib >= tile0 && sib < tile0 + kFitTile;
-    Box ob;
-    if (link & kLinkSibLeaf) {
-      if (sib_in_tile) {
-        ob = s_leaf[sib - tile0];
-      } else {
-        const uint32_t v = leafprim ? leafprim[sib] : uint32_t(sib);
-        if (int64_t(v) >= n)
-          return;
-        ob = load_prim_box(prims, kind, uint64_t(v));
+    if (t == 0)
+      s_item = atomic_add_agent(queue, 1u);
+    __syncthreads();
+    const uint32_t item = s_item;
+    if (item >= count)
+      return; // uniform across the workgroup
+    const uint2 e = treelets[item];
+    const int64_t lo = e.x;
+    const int64_t hi = e.y & 0x7fffffffu;
+    const int64_t m = hi - lo + 1;
+    if (m >= 2 && m <= kTreelet && hi < n) {
+      const int64_t k = lo + t;
+      uint32_t link = 0u;
+      if (t < m) {
+        s_leaf[t] = leaf_box(prims, kind, leafprim, k);
+        link = parent[(n - 1) + k];
+        if (k < n - 1)
+          s_link[t] = parent[k];
       }
-    } else if (global_node) {
-      ob = load_node_box_coherent(&nodes[sib]);
+      s_flag[t] = 0u;
+      __syncthreads();
+      if (t < m) {
+        Box mine = s_leaf[t];
+        int64_t cur = k;
+        for (;;) {
+          if (link & kLinkGlobal)
+            break; // cur is the treelet root
+          const int64_t p = int64_t(link & kLinkIndex);
+          const bool right = (link & kLinkRight) != 0;
+          const int64_t sib = right ? cur - 1 : cur + 1;
+          // links come from the tree kernel; never taken for a valid tree
+          if (p < lo || p > hi || sib < lo || sib > hi)
+            break;
+          const uint32_t old = __hip_atomic_fetch_add(&s_flag[p - lo], 1u, __ATOMIC_ACQ_REL,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (old == 0u)
+            break; // first arrival: the sibling's thread completes p
+          const Box ob = (link & kLinkSibLeaf) ? s_leaf[sib - lo] : s_box[sib - lo];
+          const Box b = right ? box_union(ob, mine) : box_union(mine, ob);
+          store_node_box(&nodes[p], b);
+          s_box[p - lo] = b;
+          if (p == 0)
+            break; // the whole tree was one treelet
+          mine = b;
+          cur = p;
+          link = s_link[p - lo];
+        }
+      }
+    }
+    __syncthreads(); // LDS and s_item are reused by the next treelet
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void
+k_fit_top(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
+          const uint32_t* __restrict__ parent, const uint32_t* __restrict__ leafprim,
+          const uint2* __restrict__ treelets, const uint32_t* __restrict__ count_ptr,
+          uint32_t* __restrict__ flags)
+{
+  const uint32_t count = *count_ptr;
+  for (uint32_t item = blockIdx.x * kBlock + threadIdx.x; item < count;
+       item += gridDim.x * kBlock) {
+    const uint2 e = treelets[item];
+    const int64_t lo = e.x;
+    const int64_t hi = e.y & 0x7fffffffu;
+    if (hi >= n || lo > hi || (lo == 0 && hi == n - 1))
+      continue; // the whole tree was one treelet (or a corrupt entry)
+    Box mine;
+    int64_t cur;
+    uint32_t link;
+    if (lo == hi) {
+      cur = lo;
+      mine = leaf_box(prims, kind, leafprim, lo);
+      link = parent[(n - 1) + lo];
     } else {
-      if (!sib_in_tile)
-        return;
-      ob = s_box[sib - tile0];
+      cur = (e.y & 0x80000000u) ? lo : hi;
+      if (cur >= n - 1)
+        continue;
+      const float4 a = *reinterpret_cast<const float4*>(&nodes[cur].bmin[0]);
+      const float2 c = *reinterpret_cast<const float2*>(&nodes[cur].bmax[1]);
+      mine.lo[0] = a.x;
+      mine.lo[1] = a.y;
+      mine.lo[2] = a.z;
+      mine.hi[0] = a.w;
+      mine.hi[1] = c.x;
+      mine.hi[2] = c.y;
+      link = parent[cur];
     }
-    const Box b = right ? box_union(ob, mine) : box_union(mine, ob);
-    if (global_node)
+    for (;;) {
+      const int64_t p = int64_t(link & kLinkIndex);
+      const bool right = (link & kLinkRight) != 0;
+      const int64_t sib = right ? cur - 1 : cur + 1;
+      if (!(link & kLinkGlobal) || p >= n - 1 || sib < 0 || sib >= n)
+        break; // never taken for a valid tree
+      if (atomic_add_agent(&flags[p], 1u) == 0u)
+        break; // first arrival: the sibling's thread completes p
+      Box ob;
+      if (link & kLinkSibLeaf) {
+        ob = leaf_box(prims, kind, leafprim, sib);
+      } else {
+        if (sib >= n - 1)
+          break;
+        ob = load_node_box_coherent(&nodes[sib]);
+      }
+      const Box b = right ? box_union(ob, mine) : box_union(mine, ob);
       coherent_store_u32(&flags[p], 0u);
-    if (p == 0) {
-      store_node_box(&nodes[0], b);
-      return;
-    }
-    const uint32_t next = p_in_tile ? s_link[p - tile0] : parent[p];
-    if (next & kLinkGlobal) {
-      // p's parent is completed by whichever tile arrives second
+      if (p == 0) {
+        store_node_box(&nodes[0], b);
+        break;
+      }
       store_node_box_coherent(&nodes[p], b);
       vmem_drain();
-    } else {
-      if (!p_in_tile)
-        return; // a node under a tile-local parent is itself tile-local
-      store_node_box(&nodes[p], b);
-      s_box[p - tile0] = b;
+      mine = b;
+      cur = p;
+      link = parent[p];
     }
-    mine = b;
-    cur = p;
-    link = next;
+  }
+}
+
+// Rebuilds the treelet list of an existing tree (refit): every child that is
+// not a top node but whose parent is roots a treelet; its leaf range is
+// [c, other_end[c]] (either order) for an internal node, [k, k] for a leaf.
+__global__ __launch_bounds__(kBlock) void
+k_enum_treelets(int64_t n, const uint32_t* __restrict__ parent,
+                const uint32_t* __restrict__ other_end, uint2* __restrict__ treelets,
+                uint32_t* __restrict__ count)
+{
+  const int64_t x = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (x >= 2 * n - 1)
+    return;
+  if (x == 0) {
+    if (n <= kTreelet)
+      treelets[atomicAdd(count, 1u)] = make_uint2(0u, uint32_t(n - 1));
+    return;
+  }
+  const uint32_t link = parent[x];
+  if (!(link & kLinkGlobal))
+    return;
+  if (x < n - 1) {
+    const int64_t j = other_end[x];
+    const int64_t lo = x < j ? x : j;
+    const int64_t hi = x < j ? j : x;
+    if (hi - lo + 1 > kTreelet)
+      return; // a top node itself
+    treelets[atomicAdd(count, 1u)] =
+      make_uint2(uint32_t(lo), uint32_t(hi) | ((link & kLinkRight) ? 0x80000000u : 0u));
+  } else {
+    const uint32_t k = uint32_t(x - (n - 1));
+    treelets[atomicAdd(count, 1u)] = make_uint2(k, k);
   }
 }
 
@@ -625,28 +752,53 @@ launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, 
 
 hipError_t
 launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n, void* nodes,
-              uint32_t* parent, uint32_t* leafprim, uint32_t* flags, hipStream_t s)
+              uint32_t* parent, uint32_t* leafprim, uint32_t* flags, uint32_t* other_end,
+              void* treelets, uint32_t* treelet_count, hipStream_t s)
 {
   uint64_t m = n - 1;
   uint32_t g = uint32_t((m + kBlock - 1) / kBlock);
+  uint2* tl = static_cast<uint2*>(treelets);
   if (code_bits == 30)
     hipLaunchKernelGGL(k_karras<uint32_t>, dim3(g), dim3(kBlock), 0, s,
                        static_cast<const uint32_t*>(sorted), ids, int64_t(n),
-                       static_cast<Node*>(nodes), parent, leafprim, flags);
+                       static_cast<Node*>(nodes), parent, leafprim, flags, other_end, tl,
+                       treelet_count);
   else
     hipLaunchKernelGGL(k_karras<uint64_t>, dim3(g), dim3(kBlock), 0, s,
                        static_cast<const uint64_t*>(sorted), ids, int64_t(n),
-                       static_cast<Node*>(nodes), parent, leafprim, flags);
+                       static_cast<Node*>(nodes), parent, leafprim, flags, other_end, tl,
+                       treelet_count);
   return hipGetLastError();
 }
 
 hipError_t
 launch_fit(const float* prims, int kind, uint64_t n, void* nodes, const uint32_t* parent,
-           const uint32_t* leafprim, uint32_t* flags, hipStream_t s)
+           const uint32_t* leafprim, uint32_t* flags, const void* treelets,
+           uint32_t* treelet_count, uint32_t* queue, hipStream_t s)
 {
-  uint32_t g = uint32_t((n + kFitTile - 1) / kFitTile);
-  hipLaunchKernelGGL(k_fit, dim3(g), dim3(kFitTile), 0, s, prims, kind, int64_t(n),
-                     static_cast<Node*>(nodes), parent, leafprim, flags);
+  const uint2* tl = static_cast<const uint2*>(treelets);
+  // typical trees hold about 2 n / kTreelet treelets; the loops are
+  // persistent, so any count is covered
+  uint32_t ga = grid_for(2 * n, kTreelet, 512);
+  hipLaunchKernelGGL(k_fit_treelets, dim3(ga), dim3(kTreelet), 0, s, prims, kind, int64_t(n),
+                     static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, queue);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  uint32_t gb = grid_for(4 * n / kTreelet + 64, kBlock, 1024);
+  hipLaunchKernelGGL(k_fit_top, dim3(gb), dim3(kBlock), 0, s, prims, kind, int64_t(n),
+                     static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, flags);
+  return hipGetLastError();
+}
+
+hipError_t
+launch_enum_treelets(uint64_t n, const uint32_t* parent, const uint32_t* other_end,
+                     void* treelets, uint32_t* treelet_count, hipStream_t s)
+{
+  uint64_t x = 2 * n - 1;
+  uint32_t g = uint32_t((x + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_enum_treelets, dim3(g), dim3(kBlock), 0, s, int64_t(n), parent,
+                     other_end, static_cast<uint2*>(treelets), treelet_count);
   return hipGetLastError();
 }
 

@@ -41,14 +41,14 @@ struct lbvh_ctx
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
-  bool timing = false;
+  int timing = 0; // 0 off, 1 per phase, 2 per call
   std::vector<EventSet> sets; // pool; sets[0..used) belong to timed calls
   size_t used = 0;
   std::string last_error;
   uint32_t epoch_next = 1;
   // workspace
   DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, part_hist, small, status, counts,
-    io_in, io_out;
+    treelets, io_in, io_out;
   bool status_zeroed_words = 0;
 };
 
@@ -68,6 +68,7 @@ struct lbvh_bvh
   void* d_codes = nullptr;
   uint32_t* d_ids = nullptr;
   uint32_t* d_leafprim = nullptr; // primitive of each leaf position (primitive-id leaves)
+  uint32_t* d_other_end = nullptr; // other end of each internal node's leaf range
   uint64_t bytes = 0;
 };
 
@@ -77,7 +78,9 @@ namespace {
 constexpr uint32_t kSmallBoundsTicket = 0; // self-resetting
 constexpr uint32_t kSmallErr = 1;          // cleared by the host after reading
 constexpr uint32_t kSmallTickets = 2;      // kMaxSortPasses words
-constexpr uint32_t kSmallHist = kSmallTickets + kMaxSortPasses;
+constexpr uint32_t kSmallTreeletCount = kSmallTickets + kMaxSortPasses;
+constexpr uint32_t kSmallTreeletQueue = kSmallTreeletCount + 1;
+constexpr uint32_t kSmallHist = kSmallTreeletQueue + 1;
 
 lbvh_status
 fail(lbvh_ctx* c, lbvh_status s, const std::string& msg)
@@ -165,14 +168,21 @@ begin_timed(lbvh_ctx* c, int kind)
   es.recorded = 0;
 }
 
+// Records phase boundary `idx` of the current timed call; `final` marks the
+// call's last boundary.  Per-call timing keeps only the first and last.
 void
-record(lbvh_ctx* c, int idx)
+record(lbvh_ctx* c, int idx, bool final = false)
 {
-  if (c->timing && c->used > 0 && idx < 8) {
-    EventSet& es = c->sets[c->used - 1];
-    (void)hipEventRecord(es.ev[idx], c->stream);
-    es.recorded = idx + 1;
+  if (!c->timing || c->used == 0 || idx >= 8)
+    return;
+  EventSet& es = c->sets[c->used - 1];
+  if (c->timing == 2) {
+    if (idx != 0 && !final)
+      return;
+    idx = final ? 1 : 0;
   }
+  (void)hipEventRecord(es.ev[idx], c->stream);
+  es.recorded = idx + 1;
 }
 
 // Sort tuning: digit width and tile by size and key width (see
@@ -223,6 +233,7 @@ alloc_bvh(lbvh_ctx* c, lbvh_bvh* b, bool keep_codes)
     { &b->d_nodes, m * 32 },
     { reinterpret_cast<void**>(&b->d_parent), (2 * n - 1) * 4 },
     { reinterpret_cast<void**>(&b->d_flags), m * 4 },
+    { reinterpret_cast<void**>(&b->d_other_end), m * 4 },
     { reinterpret_cast<void**>(&b->d_bounds), 32 },
     { &b->d_codes, keep_codes ? n * cw : 0 },
     { reinterpret_cast<void**>(&b->d_ids), keep_codes ? n * 4 : 0 },
@@ -256,6 +267,7 @@ free_bvh(lbvh_bvh* b)
   dev_free(b->d_codes);
   dev_free(b->d_ids);
   dev_free(b->d_leafprim);
+  dev_free(b->d_other_end);
   if (b->owns_prims)
     dev_free(b->d_prims);
   (void)hipSetDevice(cur);
@@ -307,6 +319,8 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
     return st;
   const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
   if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->treelets, 8 * n)) != LBVH_OK)
     return st;
   const uint64_t counts_words = uint64_t(rts_groups(n, cfg, nullptr)) * radix;
   if ((st = ensure(c, c->counts, 4 * counts_words)) != LBVH_OK)
@@ -378,12 +392,14 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   if (e == hipSuccess)
     e = launch_karras(opts.code_bits, sorted,
                       opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr, n,
-                      b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags, c->stream);
+                      b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end,
+                      c->treelets.p, small + kSmallTreeletCount, c->stream);
   record(c, 3);
   if (e == hipSuccess)
     e = launch_fit(d_prims, kind, n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
+                   c->treelets.p, small + kSmallTreeletCount, small + kSmallTreeletQueue,
                    c->stream);
-  record(c, 4);
+  record(c, 4, true);
   if (e == hipSuccess && opts.keep_codes) {
     e = hipMemcpyAsync(b->d_codes, sorted, kw * n, hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess)
@@ -494,7 +510,8 @@ lbvh_ctx_destroy(lbvh_ctx* c)
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = { &c->cent, &c->keys_a, &c->keys_b, &c->ids_a, &c->ids_b, &c->partials,
-                     &c->part_hist, &c->small, &c->status, &c->counts, &c->io_in, &c->io_out };
+                     &c->part_hist, &c->small, &c->status, &c->counts, &c->treelets,
+                     &c->io_in, &c->io_out };
   for (auto* b : bufs)
     dev_free(b->p);
   for (auto& es : c->sets)
@@ -531,7 +548,9 @@ lbvh_ctx_set_timing(lbvh_ctx* c, int enable)
 {
   if (!c)
     return LBVH_E_INVALID;
-  c->timing = enable != 0;
+  if (enable < 0 || enable > 2)
+    return fail(c, LBVH_E_INVALID, "timing mode is 0, 1 or 2");
+  c->timing = enable;
   c->used = 0;
   return LBVH_OK;
 }
@@ -607,7 +626,7 @@ lbvh_ctx_workspace_bytes(const lbvh_ctx* c)
     return 0;
   return c->cent.cap + c->keys_a.cap + c->keys_b.cap + c->ids_a.cap + c->ids_b.cap +
          c->partials.cap + c->part_hist.cap + c->small.cap + c->status.cap + c->counts.cap +
-         c->io_in.cap + c->io_out.cap;
+         c->treelets.cap + c->io_in.cap + c->io_out.cap;
 }
 
 lbvh_status
@@ -687,6 +706,28 @@ lbvh_build_aabbs(lbvh_ctx* c, const float* boxes, uint64_t n, const lbvh_build_o
   return build_host(c, boxes, 1, n, opts, out);
 }
 
+// Queues a refit of `b` over its current primitives.
+static lbvh_status
+queue_refit(lbvh_ctx* c, lbvh_bvh* b)
+{
+  lbvh_status st;
+  if ((st = ensure(c, c->treelets, 8 * b->n)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->small, 4ull * (kSmallHist + 256), true)) != LBVH_OK)
+    return st;
+  uint32_t* small = static_cast<uint32_t*>(c->small.p);
+  begin_timed(c, 3);
+  record(c, 0);
+  HIP_TRY(c, launch_fill_u32(small + kSmallTreeletCount, 2, 0u, c->stream));
+  HIP_TRY(c, launch_enum_treelets(b->n, b->d_parent, b->d_other_end, c->treelets.p,
+                                  small + kSmallTreeletCount, c->stream));
+  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
+                        c->treelets.p, small + kSmallTreeletCount, small + kSmallTreeletQueue,
+                        c->stream));
+  record(c, 1, true);
+  return LBVH_OK;
+}
+
 lbvh_status
 lbvh_refit_triangles_device(lbvh_ctx* c, lbvh_bvh* b, const float* d_tri)
 {
@@ -702,12 +743,7 @@ lbvh_refit_triangles_device(lbvh_ctx* c, lbvh_bvh* b, const float* d_tri)
   } else {
     b->d_prims = const_cast<float*>(d_tri);
   }
-  begin_timed(c, 3);
-  record(c, 0);
-  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
-                          c->stream));
-  record(c, 1);
-  return LBVH_OK;
+  return queue_refit(c, b);
 }
 
 lbvh_status
@@ -731,11 +767,8 @@ lbvh_refit_triangles(lbvh_ctx* c, lbvh_bvh* b, const float* tri)
     b->bytes += b->n * 48;
   }
   HIP_TRY(c, hipMemcpyAsync(b->d_prims, tri, b->n * 48, hipMemcpyHostToDevice, c->stream));
-  begin_timed(c, 3);
-  record(c, 0);
-  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
-                          c->stream));
-  record(c, 1);
+  if ((st = queue_refit(c, b)) != LBVH_OK)
+    return st;
   return sync_and_check(c);
 }
 
@@ -813,7 +846,7 @@ lbvh_trace_closest_device(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, u
   record(c, 0);
   HIP_TRY(c, launch_trace(mode, b->d_nodes, b->d_prims, b->n, d_rays, nrays, d_prim, d_t, d_uv,
                           c->stream));
-  record(c, 1);
+  record(c, 1, true);
   return LBVH_OK;
 }
 

@@ -62,20 +62,30 @@ uint64_t sort_status_words(uint64_t n, const SortConfig& cfg);
 // Number of chunks of the reduce-then-scan strategy (and tiles per chunk).
 uint32_t rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles);
 
-// Karras radix tree: left/right of every internal node, parent links
-// (internal node c at c, the leaf at sorted position k at n-1+k; bit 31 marks
-// a parent whose range straddles a fit tile), the primitive of each leaf
-// position when ids != nullptr (leafprim), fit flags cleared.
+// Karras radix tree: left/right of every internal node; parent links
+// (internal node c at c, the leaf at sorted position k at n-1+k; bit 31:
+// the parent is a top node, bit 30: right child, bit 29: sibling is a leaf);
+// the primitive of each leaf position when ids != nullptr (leafprim); the
+// other end of each node's leaf range (other_end); the treelet list
+// (uint2 {lo, hi | right-root << 31}, at most n entries; treelet_count must
+// be zero); fit flags cleared.
 hipError_t launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
                          void* nodes, uint32_t* parent, uint32_t* leafprim, uint32_t* flags,
+                         uint32_t* other_end, void* treelets, uint32_t* treelet_count,
                          hipStream_t s);
 
-// Bottom-up box fit with visit flags (global flags must be zero; they are
-// left zero again).  leafprim == nullptr: the leaf at position k is
-// primitive k.
+// Bottom-up box fit: treelets in LDS, then the top tree with device-scope
+// flags (flags must be zero and are left zero; *queue must be zero).
+// leafprim == nullptr: the leaf at position k is primitive k.
 hipError_t launch_fit(const float* prims, int kind, uint64_t n, void* nodes,
                       const uint32_t* parent, const uint32_t* leafprim, uint32_t* flags,
+                      const void* treelets, uint32_t* treelet_count, uint32_t* queue,
                       hipStream_t s);
+
+// Rebuilds the treelet list of a built tree (for a refit); *treelet_count
+// must be zero.
+hipError_t launch_enum_treelets(uint64_t n, const uint32_t* parent, const uint32_t* other_end,
+                                void* treelets, uint32_t* treelet_count, hipStream_t s);
 
 // Closest-hit traversal.
 hipError_t launch_trace(int mode, const void* nodes, const float* tris, uint64_t n,

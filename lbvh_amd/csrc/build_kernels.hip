@@ -360,7 +360,7 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
          Node* __restrict__ nodes, uint32_t* __restrict__ parent,
          uint32_t* __restrict__ leafprim, uint32_t* __restrict__ flags,
          uint32_t* __restrict__ other_end, uint2* __restrict__ treelets,
-         uint32_t* __restrict__ treelet_count)
+         uint32_t* __restrict__ treelet_count, WideNode* __restrict__ wide)
 {
   __shared__ KeyT s_codes[kKarrasWindow];
   const int64_t i0 = int64_t(blockIdx.x) * kBlock;
@@ -387,13 +387,17 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
   // find the same boundaries with independent probes (all l_max candidates
   // at once, then 7 probes per 8-way step) instead of chains of dependent
   // loads.
-  int m = 0;
-#pragma unroll
-  for (int q = 0; q < 13; q++) {
+  auto candidate = [&](int q) {
     const int64_t span = int64_t(128) << (2 * q);
     const int64_t k = i + (d > 0 ? span : -span);
-    const bool go = (k < n) && (cw.delta(i, ci, k) > delta_min);
-    m += go ? 1 : 0; // prefix-true: m = index of the first failing candidate
+    return (k < n) && (cw.delta(i, ci, k) > delta_min);
+  };
+  int m = 0;
+  if (candidate(0)) { // most nodes stop here, inside the LDS window
+    m = 1;
+#pragma unroll
+    for (int q = 1; q < 13; q++)
+      m += candidate(q) ? 1 : 0; // prefix-true: m = index of the first failing candidate
   }
   const int64_t l_max = int64_t(128) << (2 * m);
   auto pred_l = [&](int64_t x) { return cw.delta(i, ci, i + (d > 0 ? x : -x)) > delta_min; };
@@ -452,6 +456,7 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
     parent[gamma + 1] = right_link;
   }
   *reinterpret_cast<uint2*>(&nodes[i].left) = make_uint2(left, right);
+  *reinterpret_cast<uint2*>(&wide[i].left) = make_uint2(left, right);
   flags[i] = 0u;
 }
 
@@ -503,6 +508,17 @@ store_node_box(Node* nd, const Box& b)
   *reinterpret_cast<float2*>(&nd->bmax[1]) = make_float2(b.hi[1], b.hi[2]);
 }
 
+// The box of node p into its parent's traversal record (link = p's link).
+__device__ __forceinline__ void
+store_wide_child(WideNode* __restrict__ wide, uint32_t link, const Box& b)
+{
+  WideNode* w = &wide[link & kLinkIndex];
+  float* dst = (link & kLinkRight) ? w->rbox : w->lbox;
+  *reinterpret_cast<float2*>(dst) = make_float2(b.lo[0], b.lo[1]);
+  *reinterpret_cast<float2*>(dst + 2) = make_float2(b.lo[2], b.hi[0]);
+  *reinterpret_cast<float2*>(dst + 4) = make_float2(b.hi[1], b.hi[2]);
+}
+
 __device__ __forceinline__ Box
 leaf_box(const float* __restrict__ prims, int kind, const uint32_t* __restrict__ leafprim,
          int64_t pos)
@@ -515,7 +531,7 @@ __global__ __launch_bounds__(kTreelet) void
 k_fit_treelets(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
                const uint32_t* __restrict__ parent, const uint32_t* __restrict__ leafprim,
                const uint2* __restrict__ treelets, const uint32_t* __restrict__ count_ptr,
-               uint32_t* __restrict__ queue)
+               uint32_t* __restrict__ queue, WideNode* __restrict__ wide)
 {
   __shared__ uint32_t s_flag[kTreelet];
   __shared__ uint32_t s_link[kTreelet]; // link of internal node lo + t
@@ -571,6 +587,7 @@ k_fit_treelets(const float* __restrict__ prims, int kind, int64_t n, Node* __res
           mine = b;
           cur = p;
           link = s_link[p - lo];
+          store_wide_child(wide, link, b);
         }
       }
     }
@@ -582,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void
 k_fit_top(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
           const uint32_t* __restrict__ parent, const uint32_t* __restrict__ leafprim,
           const uint2* __restrict__ treelets, const uint32_t* __restrict__ count_ptr,
-          uint32_t* __restrict__ flags)
+          uint32_t* __restrict__ flags, WideNode* __restrict__ wide)
 {
   const uint32_t count = *count_ptr;
   for (uint32_t item = blockIdx.x * kBlock + threadIdx.x; item < count;
@@ -640,6 +657,7 @@ k_fit_top(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict
       mine = b;
       cur = p;
       link = parent[p];
+      store_wide_child(wide, link, b);
     }
   }
 }
@@ -753,7 +771,7 @@ launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, 
 hipError_t
 launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n, void* nodes,
               uint32_t* parent, uint32_t* leafprim, uint32_t* flags, uint32_t* other_end,
-              void* treelets, uint32_t* treelet_count, hipStream_t s)
+              void* treelets, uint32_t* treelet_count, void* wide, hipStream_t s)
 {
   uint64_t m = n - 1;
   uint32_t g = uint32_t((m + kBlock - 1) / kBlock);
@@ -762,32 +780,34 @@ launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n
     hipLaunchKernelGGL(k_karras<uint32_t>, dim3(g), dim3(kBlock), 0, s,
                        static_cast<const uint32_t*>(sorted), ids, int64_t(n),
                        static_cast<Node*>(nodes), parent, leafprim, flags, other_end, tl,
-                       treelet_count);
+                       treelet_count, static_cast<WideNode*>(wide));
   else
     hipLaunchKernelGGL(k_karras<uint64_t>, dim3(g), dim3(kBlock), 0, s,
                        static_cast<const uint64_t*>(sorted), ids, int64_t(n),
                        static_cast<Node*>(nodes), parent, leafprim, flags, other_end, tl,
-                       treelet_count);
+                       treelet_count, static_cast<WideNode*>(wide));
   return hipGetLastError();
 }
 
 hipError_t
 launch_fit(const float* prims, int kind, uint64_t n, void* nodes, const uint32_t* parent,
            const uint32_t* leafprim, uint32_t* flags, const void* treelets,
-           uint32_t* treelet_count, uint32_t* queue, hipStream_t s)
+           uint32_t* treelet_count, uint32_t* queue, void* wide, hipStream_t s)
 {
   const uint2* tl = static_cast<const uint2*>(treelets);
   // typical trees hold about 2 n / kTreelet treelets; the loops are
   // persistent, so any count is covered
   uint32_t ga = grid_for(2 * n, kTreelet, 512);
   hipLaunchKernelGGL(k_fit_treelets, dim3(ga), dim3(kTreelet), 0, s, prims, kind, int64_t(n),
-                     static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, queue);
+                     static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, queue,
+                     static_cast<WideNode*>(wide));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
   uint32_t gb = grid_for(4 * n / kTreelet + 64, kBlock, 1024);
   hipLaunchKernelGGL(k_fit_top, dim3(gb), dim3(kBlock), 0, s, prims, kind, int64_t(n),
-                     static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, flags);
+                     static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, flags,
+                     static_cast<WideNode*>(wide));
   return hipGetLastError();
 }
 

@@ -69,6 +69,7 @@ struct lbvh_bvh
   uint32_t* d_ids = nullptr;
   uint32_t* d_leafprim = nullptr; // primitive of each leaf position (primitive-id leaves)
   uint32_t* d_other_end = nullptr; // other end of each internal node's leaf range
+  void* d_wide = nullptr;          // traversal records (64 B per internal node)
   uint64_t bytes = 0;
 };
 
@@ -234,6 +235,7 @@ alloc_bvh(lbvh_ctx* c, lbvh_bvh* b, bool keep_codes)
     { reinterpret_cast<void**>(&b->d_parent), (2 * n - 1) * 4 },
     { reinterpret_cast<void**>(&b->d_flags), m * 4 },
     { reinterpret_cast<void**>(&b->d_other_end), m * 4 },
+    { &b->d_wide, m * 64 },
     { reinterpret_cast<void**>(&b->d_bounds), 32 },
     { &b->d_codes, keep_codes ? n * cw : 0 },
     { reinterpret_cast<void**>(&b->d_ids), keep_codes ? n * 4 : 0 },
@@ -268,6 +270,7 @@ free_bvh(lbvh_bvh* b)
   dev_free(b->d_ids);
   dev_free(b->d_leafprim);
   dev_free(b->d_other_end);
+  dev_free(b->d_wide);
   if (b->owns_prims)
     dev_free(b->d_prims);
   (void)hipSetDevice(cur);
@@ -393,12 +396,12 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
     e = launch_karras(opts.code_bits, sorted,
                       opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr, n,
                       b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end,
-                      c->treelets.p, small + kSmallTreeletCount, c->stream);
+                      c->treelets.p, small + kSmallTreeletCount, b->d_wide, c->stream);
   record(c, 3);
   if (e == hipSuccess)
     e = launch_fit(d_prims, kind, n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
                    c->treelets.p, small + kSmallTreeletCount, small + kSmallTreeletQueue,
-                   c->stream);
+                   b->d_wide, c->stream);
   record(c, 4, true);
   if (e == hipSuccess && opts.keep_codes) {
     e = hipMemcpyAsync(b->d_codes, sorted, kw * n, hipMemcpyDeviceToDevice, c->stream);
@@ -723,7 +726,7 @@ queue_refit(lbvh_ctx* c, lbvh_bvh* b)
                                   small + kSmallTreeletCount, c->stream));
   HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
                         c->treelets.p, small + kSmallTreeletCount, small + kSmallTreeletQueue,
-                        c->stream));
+                        b->d_wide, c->stream));
   record(c, 1, true);
   return LBVH_OK;
 }
@@ -844,7 +847,7 @@ lbvh_trace_closest_device(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, u
     return st;
   begin_timed(c, 2);
   record(c, 0);
-  HIP_TRY(c, launch_trace(mode, b->d_nodes, b->d_prims, b->n, d_rays, nrays, d_prim, d_t, d_uv,
+  HIP_TRY(c, launch_trace(mode, b->d_wide, b->d_prims, b->n, d_rays, nrays, d_prim, d_t, d_uv,
                           c->stream));
   record(c, 1, true);
   return LBVH_OK;

@@ -72,7 +72,7 @@ uint32_t rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles);
 hipError_t launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
                          void* nodes, uint32_t* parent, uint32_t* leafprim, uint32_t* flags,
                          uint32_t* other_end, void* treelets, uint32_t* treelet_count,
-                         hipStream_t s);
+                         void* wide, hipStream_t s);
 
 // Bottom-up box fit: treelets in LDS, then the top tree with device-scope
 // flags (flags must be zero and are left zero; *queue must be zero).
@@ -80,15 +80,15 @@ hipError_t launch_karras(int code_bits, const void* sorted, const uint32_t* ids,
 hipError_t launch_fit(const float* prims, int kind, uint64_t n, void* nodes,
                       const uint32_t* parent, const uint32_t* leafprim, uint32_t* flags,
                       const void* treelets, uint32_t* treelet_count, uint32_t* queue,
-                      hipStream_t s);
+                      void* wide, hipStream_t s);
 
 // Rebuilds the treelet list of a built tree (for a refit); *treelet_count
 // must be zero.
 hipError_t launch_enum_treelets(uint64_t n, const uint32_t* parent, const uint32_t* other_end,
                                 void* treelets, uint32_t* treelet_count, hipStream_t s);
 
-// Closest-hit traversal.
-hipError_t launch_trace(int mode, const void* nodes, const float* tris, uint64_t n,
+// Closest-hit traversal over the 64-byte traversal records (wide).
+hipError_t launch_trace(int mode, const void* wide, const float* tris, uint64_t n,
                         const float* rays, uint64_t nrays, uint32_t* prim, float* t, float* uv,
                         hipStream_t s);
 

@@ -172,12 +172,46 @@ better(float t, uint32_t id, float best_t, uint32_t best_id)
   return (t < best_t) || (t == best_t && id < best_id && t != INFINITY);
 }
 
+// The traversal stack: capacity 128 with pushes dropped when full, as the
+// reference's.  Its top kStackLds entries live in an LDS ring per lane; older
+// entries spill to a private array (only trees deeper than the ring touch
+// it).  Entries: (node index, tmin bits).
+constexpr int kStackLds = 16;
+
+struct Stack
+{
+  uint2* ring;        // this lane's kStackLds LDS slots (stride kTraceBlock)
+  uint2 spill[kStack];
+  int sp = 0;         // entries on the stack
+  int base = 0;       // lowest entry index held in the ring
+  __device__ void push(uint32_t node, float t)
+  {
+    if (sp >= kStack)
+      return;
+    if (sp - base == kStackLds) {
+      spill[base] = ring[(base % kStackLds) * kTraceBlock];
+      base++;
+    }
+    ring[(sp % kStackLds) * kTraceBlock] = make_uint2(node, __float_as_uint(t));
+    sp++;
+  }
+  __device__ uint2 pop()
+  {
+    sp--;
+    if (sp >= base)
+      return ring[(sp % kStackLds) * kTraceBlock];
+    base = sp;
+    return spill[sp];
+  }
+};
+
 template<int MODE>
 __global__ __launch_bounds__(kTraceBlock) void
-k_trace(const Node* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
+k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
         const float* __restrict__ rays, uint64_t nrays, uint32_t* __restrict__ prim_out,
         float* __restrict__ t_out, float* __restrict__ uv_out)
 {
+  __shared__ uint2 s_ring[kStackLds * kTraceBlock];
   const uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
   if (i >= nrays)
     return;
@@ -204,28 +238,31 @@ k_trace(const Node* __restrict__ nodes, const float* __restrict__ tris, uint32_t
     pos_dir[k] = r.d[k] > 0.0f;
   }
 
-  uint32_t st_node[kStack];
-  float st_t[kStack];
-  int sp = 0;
-  st_node[0] = 0u;
-  st_t[0] = INFINITY;
-  sp = 1;
+  Stack st;
+  st.ring = s_ring + threadIdx.x;
+  st.push(0u, INFINITY);
 
   uint32_t best = 0xffffffffu;
   float best_t = INFINITY, best_u = 0.0f, best_v = 0.0f;
 
-  while (sp > 0) {
-    sp--;
-    const uint32_t ni = st_node[sp];
-    const float etmin = st_t[sp];
+  while (st.sp > 0) {
+    const uint2 e = st.pop();
+    const uint32_t ni = e.x;
+    const float etmin = __uint_as_float(e.y);
     if (best_t < etmin)
       continue;
-    const uint2 lr = *reinterpret_cast<const uint2*>(&nodes[ni].left);
-    bool lh = false, rh = false;
-    float lt = 0.0f, rt = 0.0f;
+    if (ni + 1 >= nprims)
+      continue; // only possible for a corrupt node array
+    const float4* wp = reinterpret_cast<const float4*>(&nodes[ni]);
+    const float4 w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
+    const uint32_t lr[2] = { __float_as_uint(w3.x), __float_as_uint(w3.y) };
+    const float bx[2][6] = { { w0.x, w0.y, w0.z, w0.w, w1.x, w1.y },
+                             { w1.z, w1.w, w2.x, w2.y, w2.z, w2.w } };
+    bool hit2[2] = { false, false };
+    float tn2[2] = { 0.0f, 0.0f };
 #pragma unroll
     for (int side = 0; side < 2; side++) {
-      const uint32_t ch = side ? lr.y : lr.x;
+      const uint32_t ch = lr[side];
       if ((ch & kLeafBit) ? ((ch & ~kLeafBit) >= nprims) : (ch + 1 >= nprims))
         continue; // out-of-range child: only possible for a corrupt node array
       if (ch & kLeafBit) {
@@ -240,10 +277,8 @@ k_trace(const Node* __restrict__ nodes, const float* __restrict__ tris, uint32_t
           best_v = v;
         }
       } else {
-        const float4 b0 = *reinterpret_cast<const float4*>(&nodes[ch].bmin[0]);
-        const float2 b1 = *reinterpret_cast<const float2*>(&nodes[ch].bmax[1]);
-        const float lo[3] = { b0.x, b0.y, b0.z };
-        const float hi[3] = { b0.w, b1.x, b1.y };
+        const float* lo = &bx[side][0];
+        const float* hi = &bx[side][3];
         float tn[3], tf[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -256,46 +291,24 @@ k_trace(const Node* __restrict__ nodes, const float* __restrict__ tris, uint32_t
         }
         float tmin = m_max(m_max(tn[0], tn[1]), tn[2]);
         float tmax = m_min(m_min(tf[0], tf[1]), tf[2]);
-        bool h = tmax >= m_max(0.0f, tmin);
-        if (side) {
-          rh = h;
-          rt = tmin;
-        } else {
-          lh = h;
-          lt = tmin;
-        }
+        hit2[side] = tmax >= m_max(0.0f, tmin);
+        tn2[side] = tmin;
       }
     }
+    const bool lh = hit2[0], rh = hit2[1];
+    const float lt = tn2[0], rt = tn2[1];
     if (lh && rh) {
-      uint32_t first_n, second_n;
-      float first_t, second_t;
       if (lt < rt) {
-        first_n = lr.y;
-        first_t = rt;
-        second_n = lr.x;
-        second_t = lt;
+        st.push(lr[1], rt);
+        st.push(lr[0], lt);
       } else {
-        first_n = lr.x;
-        first_t = lt;
-        second_n = lr.y;
-        second_t = rt;
+        st.push(lr[0], lt);
+        st.push(lr[1], rt);
       }
-      if (sp < kStack) {
-        st_node[sp] = first_n;
-        st_t[sp] = first_t;
-        sp++;
-      }
-      if (sp < kStack) {
-        st_node[sp] = second_n;
-        st_t[sp] = second_t;
-        sp++;
-      }
-    } else if (lh || rh) {
-      if (sp < kStack) {
-        st_node[sp] = lh ? lr.x : lr.y;
-        st_t[sp] = lh ? lt : rt;
-        sp++;
-      }
+    } else if (lh) {
+      st.push(lr[0], lt);
+    } else if (rh) {
+      st.push(lr[1], rt);
     }
   }
   prim_out[i] = best;
@@ -317,7 +330,7 @@ launch_trace(int mode, const void* nodes, const float* tris, uint64_t n, const f
   if (nrays == 0)
     return hipSuccess;
   uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
-  const Node* nd = static_cast<const Node*>(nodes);
+  const WideNode* nd = static_cast<const WideNode*>(nodes);
   switch (mode) {
     case 0:
       hipLaunchKernelGGL(k_trace<0>, dim3(g), dim3(kTraceBlock), 0, s, nd, tris, uint32_t(n), rays,

@@ -176,32 +176,44 @@ better(float t, uint32_t id, float best_t, uint32_t best_id)
 // reference's.  Its top kStackLds entries live in an LDS ring per lane; older
 // entries spill to a private array (only trees deeper than the ring touch
 // it).  Entries: (node index, tmin bits).
-constexpr int kStackLds = 16;
+#ifndef LBVH_TRACE_RING
+#define LBVH_TRACE_RING 0
+#endif
+constexpr int kStackLds = LBVH_TRACE_RING;
 
 struct Stack
 {
   uint2* ring;        // this lane's kStackLds LDS slots (stride kTraceBlock)
   uint2 spill[kStack];
-  int sp = 0;         // entries on the stack
-  int base = 0;       // lowest entry index held in the ring
+  uint32_t sp = 0;    // entries on the stack
+  uint32_t base = 0;  // lowest entry index held in the ring
   __device__ void push(uint32_t node, float t)
   {
     if (sp >= kStack)
       return;
-    if (sp - base == kStackLds) {
-      spill[base] = ring[(base % kStackLds) * kTraceBlock];
-      base++;
+    const uint2 e = make_uint2(node, __float_as_uint(t));
+    if constexpr (kStackLds == 0) {
+      spill[sp++] = e;
+    } else {
+      if (sp - base == kStackLds) {
+        spill[base] = ring[(base % kStackLds) * kTraceBlock];
+        base++;
+      }
+      ring[(sp % kStackLds) * kTraceBlock] = e;
+      sp++;
     }
-    ring[(sp % kStackLds) * kTraceBlock] = make_uint2(node, __float_as_uint(t));
-    sp++;
   }
   __device__ uint2 pop()
   {
     sp--;
-    if (sp >= base)
-      return ring[(sp % kStackLds) * kTraceBlock];
-    base = sp;
-    return spill[sp];
+    if constexpr (kStackLds == 0) {
+      return spill[sp];
+    } else {
+      if (sp >= base)
+        return ring[(sp % kStackLds) * kTraceBlock];
+      base = sp;
+      return spill[sp];
+    }
   }
 };
 
@@ -211,7 +223,7 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
         const float* __restrict__ rays, uint64_t nrays, uint32_t* __restrict__ prim_out,
         float* __restrict__ t_out, float* __restrict__ uv_out)
 {
-  __shared__ uint2 s_ring[kStackLds * kTraceBlock];
+  __shared__ uint2 s_ring[(kStackLds > 0 ? kStackLds : 1) * kTraceBlock];
   const uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
   if (i >= nrays)
     return;
@@ -240,17 +252,26 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
 
   Stack st;
   st.ring = s_ring + threadIdx.x;
-  st.push(0u, INFINITY);
 
   uint32_t best = 0xffffffffu;
   float best_t = INFINITY, best_u = 0.0f, best_v = 0.0f;
 
-  while (st.sp > 0) {
-    const uint2 e = st.pop();
-    const uint32_t ni = e.x;
-    const float etmin = __uint_as_float(e.y);
-    if (best_t < etmin)
-      continue;
+  // The reference pushes the root and pops it at once; a node it would push
+  // and pop again straight away (the near child, or a single hit child) is
+  // taken directly here, but only when that push would have found room, so
+  // the visit order and the drop-when-full rule are unchanged.
+  uint32_t ni = 0u;
+  bool have = true;
+  for (;;) {
+    if (!have) {
+      if (st.sp == 0)
+        break;
+      const uint2 e = st.pop();
+      if (best_t < __uint_as_float(e.y))
+        continue;
+      ni = e.x;
+    }
+    have = false;
     if (ni + 1 >= nprims)
       continue; // only possible for a corrupt node array
     const float4* wp = reinterpret_cast<const float4*>(&nodes[ni]);
@@ -296,19 +317,24 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
       }
     }
     const bool lh = hit2[0], rh = hit2[1];
-    const float lt = tn2[0], rt = tn2[1];
+    uint32_t next_n;
+    float next_t;
     if (lh && rh) {
-      if (lt < rt) {
-        st.push(lr[1], rt);
-        st.push(lr[0], lt);
-      } else {
-        st.push(lr[0], lt);
-        st.push(lr[1], rt);
-      }
-    } else if (lh) {
-      st.push(lr[0], lt);
-    } else if (rh) {
-      st.push(lr[1], rt);
+      const bool left_near = tn2[0] < tn2[1];
+      st.push(left_near ? lr[1] : lr[0], left_near ? tn2[1] : tn2[0]);
+      next_n = left_near ? lr[0] : lr[1];
+      next_t = left_near ? tn2[0] : tn2[1];
+    } else if (lh || rh) {
+      next_n = lh ? lr[0] : lr[1];
+      next_t = lh ? tn2[0] : tn2[1];
+    } else {
+      continue;
+    }
+    // push-then-pop of next_n: its push needs room; its pop is then pruned
+    // when a closer hit is already known
+    if (st.sp < kStack && !(best_t < next_t)) {
+      ni = next_n;
+      have = true;
     }
   }
   prim_out[i] = best;

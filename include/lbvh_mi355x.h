@@ -176,6 +176,86 @@ lbvh_status lbvh_trace_closest_device(lbvh_ctx* ctx, const lbvh_bvh* bvh,
                                       int mode, uint32_t* d_prim, float* d_t,
                                       float* d_uv);
 
+/* -- scene batches -----------------------------------------------------------
+ * Many independent triangle scenes built in one segmented pass (per-scene
+ * bounds and codes, one radix sort keyed by (scene, code), then tree and fit
+ * over every scene at once).  Each scene's tree is the tree
+ * lbvh::builder<float>::operator() (lbvh.h:1793-1822) builds for that scene
+ * alone with 30-bit codes, as lbvh_build_triangles gives it.  The reference
+ * has no batched builder: a batch replaces one builder call per scene.
+ *
+ * Scene s is triangles [prim_off[s], prim_off[s + 1]) of one array, with
+ * prim_off[0] = 0, at least 2 triangles per scene and fewer than 2^31 in all.
+ * Scene s's nodes start at node prim_off[s] - s of the batch's node array;
+ * node indices, leaf values and hit ids are scene-local.  opts.code_bits
+ * must be 30; sort_digit_bits / sort_tile pick the sort (0 = tuned), which
+ * always uses the reduce-then-scan strategy.
+ *
+ * A batch owns every device buffer its rebuilds, refits and traversals need,
+ * so lbvh_batch_rebuild and lbvh_batch_refit allocate nothing and do not
+ * synchronize.  With use_graph != 0 the work is captured into a HIP graph on
+ * first use (per stream and primitive pointer) and replayed afterwards. */
+typedef struct lbvh_batch lbvh_batch;
+
+typedef struct lbvh_batch_info
+{
+  uint32_t n_scenes;
+  uint32_t leaf_order;
+  uint64_t n_prims;
+  uint64_t n_nodes;         /* n_prims - n_scenes */
+  uint32_t sort_digit_bits; /* the sort configuration in use */
+  uint32_t sort_tile;
+  uint32_t sort_passes;
+  uint32_t reserved;
+  uint64_t device_bytes;  /* device memory held by the handle */
+  const void* d_nodes;    /* device lbvh::node<float>[n_nodes] */
+  const void* d_prims;    /* device triangles */
+} lbvh_batch_info;
+
+/* prim_off: host array of n_scenes + 1 offsets. */
+lbvh_status lbvh_build_batch(lbvh_ctx* ctx, const float* tri48,
+                             const uint64_t* prim_off, uint32_t n_scenes,
+                             const lbvh_build_opts* opts, lbvh_batch** out);
+lbvh_status lbvh_build_batch_device(lbvh_ctx* ctx, const float* d_tri48,
+                                    const uint64_t* prim_off,
+                                    uint32_t n_scenes,
+                                    const lbvh_build_opts* opts,
+                                    lbvh_batch** out);
+/* Rebuild every scene (same scene sizes) from the current triangles, or from
+ * d_tri48 when not NULL (device memory; copied into owned storage, or
+ * borrowed in place of the previous pointer). */
+lbvh_status lbvh_batch_rebuild(lbvh_ctx* ctx, lbvh_batch* batch,
+                               const float* d_tri48, int use_graph);
+/* Refit every scene on its unchanged topology (the box fit of
+ * lbvh.h:1827-1866; the reference has no refit entry point). */
+lbvh_status lbvh_batch_refit(lbvh_ctx* ctx, lbvh_batch* batch,
+                             const float* d_tri48, int use_graph);
+lbvh_status lbvh_batch_get_info(const lbvh_batch* batch, lbvh_batch_info* out);
+/* One scene's node array ((n_s - 1) x 32 B, as lbvh::bvh<float>), sorted
+ * 30-bit codes, scene-local primitive ids in sorted order (needs
+ * PRIMITIVE_ID leaves or keep_codes) and centroid bounds (6 floats).  Any
+ * output may be NULL. */
+lbvh_status lbvh_batch_download_scene(lbvh_ctx* ctx, const lbvh_batch* batch,
+                                      uint32_t scene, void* nodes,
+                                      uint64_t cap_bytes, uint32_t* codes,
+                                      uint32_t* ids, float* bounds6);
+void lbvh_batch_destroy(lbvh_batch* batch);
+/* Closest hits for rays of every scene in one launch: rays
+ * [ray_off[s], ray_off[s + 1]) go to scene s (n_scenes + 1 offsets, host
+ * memory for lbvh_batch_trace_closest, device memory for the _device
+ * variant); outputs as lbvh_trace_closest with scene-local ids. */
+lbvh_status lbvh_batch_trace_closest(lbvh_ctx* ctx, const lbvh_batch* batch,
+                                     const float* rays32,
+                                     const uint64_t* ray_off, int mode,
+                                     uint32_t* prim, float* t, float* uv);
+lbvh_status lbvh_batch_trace_closest_device(lbvh_ctx* ctx,
+                                            const lbvh_batch* batch,
+                                            const float* d_rays32,
+                                            const uint64_t* d_ray_off,
+                                            uint64_t nrays, int mode,
+                                            uint32_t* d_prim, float* d_t,
+                                            float* d_uv);
+
 #ifdef __cplusplus
 }
 #endif

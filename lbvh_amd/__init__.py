@@ -54,6 +54,15 @@ class BvhInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64)]
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("n_scenes", ctypes.c_uint32), ("leaf_order", ctypes.c_uint32),
+                ("n_prims", ctypes.c_uint64), ("n_nodes", ctypes.c_uint64),
+                ("sort_digit_bits", ctypes.c_uint32), ("sort_tile", ctypes.c_uint32),
+                ("sort_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64), ("d_nodes", ctypes.c_void_p),
+                ("d_prims", ctypes.c_void_p)]
+
+
 VP = ctypes.c_void_p
 U64 = ctypes.c_uint64
 F32P = ctypes.POINTER(ctypes.c_float)
@@ -88,6 +97,18 @@ SIGNATURES = {
     "lbvh_bvh_destroy": (None, [VP]),
     "lbvh_trace_closest": (ctypes.c_int, [VP, VP, VP, U64, ctypes.c_int, VP, VP, VP]),
     "lbvh_trace_closest_device": (ctypes.c_int, [VP, VP, VP, U64, ctypes.c_int, VP, VP, VP]),
+    "lbvh_build_batch": (ctypes.c_int, [VP, VP, VP, ctypes.c_uint32, ctypes.POINTER(BuildOpts),
+                                        ctypes.POINTER(VP)]),
+    "lbvh_build_batch_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_uint32,
+                                               ctypes.POINTER(BuildOpts), ctypes.POINTER(VP)]),
+    "lbvh_batch_rebuild": (ctypes.c_int, [VP, VP, VP, ctypes.c_int]),
+    "lbvh_batch_refit": (ctypes.c_int, [VP, VP, VP, ctypes.c_int]),
+    "lbvh_batch_get_info": (ctypes.c_int, [VP, ctypes.POINTER(BatchInfo)]),
+    "lbvh_batch_download_scene": (ctypes.c_int, [VP, VP, ctypes.c_uint32, VP, U64, VP, VP, VP]),
+    "lbvh_batch_destroy": (None, [VP]),
+    "lbvh_batch_trace_closest": (ctypes.c_int, [VP, VP, VP, VP, ctypes.c_int, VP, VP, VP]),
+    "lbvh_batch_trace_closest_device": (ctypes.c_int, [VP, VP, VP, VP, U64, ctypes.c_int, VP,
+                                                       VP, VP]),
 }
 
 _lib = None
@@ -221,6 +242,110 @@ class Bvh:
             pass
 
 
+class Batch:
+    """Many scenes built in one segmented pass (lbvh_batch).  Scene s is
+    primitives [offsets[s], offsets[s + 1]); ids, nodes and hits are
+    scene-local."""
+
+    def __init__(self, ctx, handle, offsets, leaf_order, keep=None):
+        self.ctx = ctx
+        self.handle = handle
+        self.offsets = np.asarray(offsets, dtype=np.uint64)
+        self.n_scenes = len(self.offsets) - 1
+        self.n = int(self.offsets[-1])
+        self.leaf_order = leaf_order
+        self._keep = keep
+
+    def info(self):
+        inf = BatchInfo()
+        self.ctx._check(lib().lbvh_batch_get_info(self.handle, ctypes.byref(inf)))
+        return inf
+
+    def scene_size(self, s):
+        return int(self.offsets[s + 1] - self.offsets[s])
+
+    def nodes(self, s):
+        out = np.empty(self.scene_size(s) - 1, dtype=NODE_DTYPE)
+        self.ctx._check(lib().lbvh_batch_download_scene(self.ctx.handle, self.handle, s,
+                                                        _hptr(out), out.nbytes, None, None, None))
+        return out
+
+    def codes(self, s, with_ids=True):
+        """(sorted 30-bit codes, scene-local ids or None, centroid bounds[6])."""
+        m = self.scene_size(s)
+        codes = np.empty(m, dtype=np.uint32)
+        ids = np.empty(m, dtype=np.uint32) if with_ids else None
+        bounds = np.empty(6, dtype=np.float32)
+        self.ctx._check(lib().lbvh_batch_download_scene(
+            self.ctx.handle, self.handle, s, None, 0, _hptr(codes),
+            _hptr(ids) if ids is not None else None, _hptr(bounds)))
+        return codes, ids, bounds
+
+    def rebuild(self, tri=None, graph=False):
+        """Queues a rebuild of every scene (from `tri` when given: device memory)."""
+        p = None
+        if tri is not None:
+            if not _is_device(tri):
+                raise ValueError("rebuild takes device triangles")
+            p = _dptr(tri)
+            self._keep = tri
+        self.ctx._check(lib().lbvh_batch_rebuild(self.ctx.handle, self.handle, p, int(graph)))
+
+    def refit(self, tri=None, graph=False):
+        """Queues a refit of every scene on its unchanged topology."""
+        p = None
+        if tri is not None:
+            if not _is_device(tri):
+                raise ValueError("refit takes device triangles")
+            p = _dptr(tri)
+            self._keep = tri
+        self.ctx._check(lib().lbvh_batch_refit(self.ctx.handle, self.handle, p, int(graph)))
+
+    def trace(self, rays, ray_offsets, mode=ISECT_REFERENCE, out=None, with_uv=True):
+        """Closest hits; rays [ray_offsets[s], ray_offsets[s + 1]) go to scene
+        s.  numpy rays and offsets -> numpy outputs after the work finished;
+        device rays and device offsets -> device outputs, queued."""
+        if _is_device(rays):
+            if not _is_device(ray_offsets):
+                raise ValueError("device rays need device ray offsets")
+            m = _numel(rays) // 8
+            if out is None:
+                from .hip import DeviceArray
+                prim = DeviceArray(m, np.uint32)
+                t = DeviceArray(m, np.float32)
+                uv = DeviceArray((m, 2), np.float32) if with_uv else None
+            else:
+                prim, t, uv = out
+            self.ctx._check(lib().lbvh_batch_trace_closest_device(
+                self.ctx.handle, self.handle, _dptr(rays), _dptr(ray_offsets), m, mode,
+                _dptr(prim), _dptr(t), _dptr(uv) if uv is not None else None))
+            return prim, t, uv
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        off = np.ascontiguousarray(ray_offsets, dtype=np.uint64)
+        if off.shape[0] != self.n_scenes + 1 or int(off[-1]) != r.shape[0]:
+            raise ValueError("ray offsets must have n_scenes + 1 entries ending at the ray count")
+        m = r.shape[0]
+        prim = np.empty(m, dtype=np.uint32)
+        t = np.empty(m, dtype=np.float32)
+        uv = np.empty((m, 2), dtype=np.float32) if with_uv else None
+        self.ctx._check(lib().lbvh_batch_trace_closest(self.ctx.handle, self.handle, _hptr(r),
+                                                       _hptr(off), mode, _hptr(prim), _hptr(t),
+                                                       _hptr(uv) if uv is not None else None))
+        return prim, t, uv
+
+    def destroy(self):
+        if self.handle:
+            lib().lbvh_batch_destroy(self.handle)
+            self.handle = None
+            self._keep = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class Context:
     """An engine context on one gfx950 device (lbvh_ctx)."""
 
@@ -305,6 +430,25 @@ class Context:
         self._check(getattr(lib(), fn)(self.handle, _hptr(a), n, ctypes.byref(opts),
                                        ctypes.byref(out)))
         return Bvh(self, out, n, code_bits, leaf_order)
+
+    def build_batch(self, tri, offsets, leaf_order=LEAF_REFERENCE, keep_codes=False,
+                    sort_digit_bits=0, sort_tile=0):
+        """Builds every scene of `tri` (scene s = triangles offsets[s] ..
+        offsets[s + 1] - 1) in one segmented pass with 30-bit codes."""
+        opts = _make_opts(30, leaf_order, keep_codes, sort_digit_bits, sort_tile, SORT_AUTO)
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        nseg = off.shape[0] - 1
+        out = ctypes.c_void_p()
+        if _is_device(tri):
+            self._check(lib().lbvh_build_batch_device(self.handle, _dptr(tri), _hptr(off), nseg,
+                                                      ctypes.byref(opts), ctypes.byref(out)))
+            return Batch(self, out, off, leaf_order, keep=tri)
+        a = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 12)
+        if a.shape[0] != int(off[-1]):
+            raise ValueError("offsets must end at the triangle count")
+        self._check(lib().lbvh_build_batch(self.handle, _hptr(a), _hptr(off), nseg,
+                                           ctypes.byref(opts), ctypes.byref(out)))
+        return Batch(self, out, off, leaf_order)
 
     def refit_triangles(self, bvh, tri):
         if _is_device(tri):

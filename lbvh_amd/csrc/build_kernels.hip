@@ -287,44 +287,62 @@ k_hist_reduce(const uint32_t* __restrict__ part, uint32_t groups, uint32_t words
 // primitive of each leaf position (primitive-id leaves); and clears the fit
 // flags.
 
-template<typename KeyT>
-struct KeyTraits;
+template<typename CodeT>
+struct CodeTraits;
 template<>
-struct KeyTraits<uint32_t>
+struct CodeTraits<uint32_t>
 {
   static constexpr int bits = 32;
   __device__ static int clz(uint32_t x) { return __clz(int(x)); }
 };
 template<>
-struct KeyTraits<uint64_t>
+struct CodeTraits<uint64_t>
 {
   static constexpr int bits = 64;
   __device__ static int clz(uint64_t x) { return __clzll((long long)x); }
 };
 
-constexpr int kKarrasHalo = 128;
-constexpr int kKarrasWindow = kBlock + 2 * kKarrasHalo;
+// Sorted keys hold the Morton code alone (one scene) or (scene << 30 | code)
+// for a batch of scenes with 30-bit codes; the tree compares codes only.
+template<typename KeyT, typename CodeT, bool SEG>
+__device__ __forceinline__ CodeT
+key_code(KeyT k)
+{
+  if constexpr (SEG)
+    return CodeT(k & 0x3fffffffull);
+  else
+    return CodeT(k);
+}
 
-template<typename KeyT>
+constexpr int kKarrasHalo = 128;
+constexpr int kKarrasWindow = kBlock + 2 * kKarrasHalo + 64; // + room for scene boundaries
+
+// Codes of one scene (positions base .. base + n - 1 of the sorted array),
+// served from an LDS copy of sorted positions [w0, w0 + kKarrasWindow) when
+// possible.  Positions passed in are scene-local.
+template<typename KeyT, typename CodeT, bool SEG>
 struct CodeWindow
 {
   const KeyT* __restrict__ g;
-  const KeyT* s; // LDS copy of g[w0, w0 + kKarrasWindow)
+  const KeyT* s;
   int64_t w0;
+  int64_t base;
   int64_t n;
-  __device__ KeyT at(int64_t k) const
+  __device__ CodeT at(int64_t k) const
   {
-    int64_t o = k - w0;
-    return (o >= 0 && o < kKarrasWindow) ? s[o] : g[k];
+    const int64_t x = base + k;
+    const int64_t o = x - w0;
+    return key_code<KeyT, CodeT, SEG>((o >= 0 && o < kKarrasWindow) ? s[o] : g[x]);
   }
-  __device__ int delta(int64_t j, KeyT cj, int64_t k) const
+  // calc_delta (lbvh.h:1602-1621) over scene-local positions
+  __device__ int delta(int64_t j, CodeT cj, int64_t k) const
   {
     if (k < 0 || k >= n)
       return -1;
-    KeyT ck = at(k);
+    CodeT ck = at(k);
     if (cj == ck)
-      return KeyTraits<KeyT>::clz(KeyT(uint64_t(j ^ k))) + KeyTraits<KeyT>::bits;
-    return KeyTraits<KeyT>::clz(cj ^ ck);
+      return CodeTraits<CodeT>::clz(CodeT(uint64_t(j ^ k))) + CodeTraits<CodeT>::bits;
+    return CodeTraits<CodeT>::clz(cj ^ ck);
   }
 };
 
@@ -369,27 +387,39 @@ search_binary(int64_t span, const Pred& pred)
   return x;
 }
 
-template<typename KeyT>
+template<typename KeyT, typename CodeT, bool SEG>
 __global__ __launch_bounds__(kBlock) void
-k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64_t n,
+k_karras(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes sc,
          Node* __restrict__ nodes, uint32_t* __restrict__ parent,
          uint32_t* __restrict__ leafprim, uint32_t* __restrict__ flags,
          uint32_t* __restrict__ other_end, uint2* __restrict__ treelets,
          uint32_t* __restrict__ treelet_count, WideNode* __restrict__ wide)
 {
-  __shared__ KeyT s_codes[kKarrasWindow];
-  const int64_t i0 = int64_t(blockIdx.x) * kBlock;
-  const int64_t w0 = i0 - kKarrasHalo;
+  __shared__ KeyT s_keys[kKarrasWindow];
+  const int64_t nodes_total = sc.n - (SEG ? int64_t(sc.nseg) : 1);
+  const int64_t g0 = int64_t(blockIdx.x) * kBlock;
+  const int64_t g = g0 + threadIdx.x;
+  // scene of the workgroup's first node sets the window (sorted position of
+  // node g of scene s is g + s)
+  const uint32_t s0 = SEG ? find_scene(sc.prim_off, sc.nseg, g0, 1, true) : 0u;
+  const int64_t w0 = g0 + s0 - kKarrasHalo;
   for (int t = threadIdx.x; t < kKarrasWindow; t += kBlock) {
-    int64_t k = w0 + t;
-    s_codes[t] = (k >= 0 && k < n) ? codes[k] : KeyT(0);
+    const int64_t x = w0 + t;
+    s_keys[t] = (x >= 0 && x < sc.n) ? keys[x] : KeyT(0);
   }
   __syncthreads();
-  const int64_t i = i0 + threadIdx.x;
-  if (i >= n - 1)
+  if (g >= nodes_total)
     return;
-  const CodeWindow<KeyT> cw{ codes, s_codes, w0, n };
-  const KeyT ci = cw.at(i);
+  const uint32_t seg = SEG ? find_scene(sc.prim_off, sc.nseg, g, 1, true) : 0u;
+  const int64_t P = SEG ? int64_t(sc.prim_off[seg]) : 0;
+  const int64_t n = SEG ? int64_t(sc.prim_off[seg + 1]) - P : sc.n;
+  const int64_t node_base = P - seg;
+  const int64_t link_base = 2 * P - seg;
+  const int64_t i = g - node_base;
+  if (n < 2 || i < 0 || i >= n - 1)
+    return; // scenes have at least 2 primitives (checked by the host)
+  const CodeWindow<KeyT, CodeT, SEG> cw{ keys, s_keys, w0, P, n };
+  const CodeT ci = cw.at(i);
   const int dd = cw.delta(i, ci, i + 1) - cw.delta(i, ci, i - 1);
   const int64_t d = (dd < 0) ? -1 : 1;
   const int delta_min = cw.delta(i, ci, i - d);
@@ -399,9 +429,9 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
   // largest split s < l with delta(i, i + s d) > delta_node.  For sorted
   // keys delta(i, k) does not grow as k moves away from i, so each of these
   // predicates is true on a prefix and false after it: the searches below
-  // find the same boundaries with independent probes (all l_max candidates
-  // at once, then 7 probes per 8-way step) instead of chains of dependent
-  // loads.
+  // find the same boundaries, with independent probes (all l_max candidates
+  // at once, then 7 probes per 8-way step) where the span leaves the LDS
+  // window.
   auto candidate = [&](int q) {
     const int64_t span = int64_t(128) << (2 * q);
     const int64_t k = i + (d > 0 ? span : -span);
@@ -416,8 +446,6 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
   }
   const int64_t l_max = int64_t(128) << (2 * m);
   auto pred_l = [&](int64_t x) { return cw.delta(i, ci, i + (d > 0 ? x : -x)) > delta_min; };
-  // spans inside the LDS window: plain binary search (fewest probes); wider
-  // spans: 8-way steps (fewest dependent global loads)
   const int64_t l = (m == 0) ? search_binary(l_max, pred_l) : search_prefix(l_max, pred_l);
   const int64_t j = i + (d > 0 ? l : -l);
   const int delta_node = cw.delta(i, ci, j);
@@ -437,45 +465,46 @@ k_karras(const KeyT* __restrict__ codes, const uint32_t* __restrict__ ids, int64
   // a top node: its children hand their boxes up through the top-tree pass
   const bool top = (mx - mn + 1) > kTreelet;
   const uint32_t up = uint32_t(i) | (top ? kLinkGlobal : 0u);
-  other_end[i] = uint32_t(j);
-  if (top) {
-    // children with at most kTreelet leaves root treelets; an entry holds the
-    // leaf range and, in bit 31 of .y, whether its root is a right child (the
-    // root is numbered hi for a left child, lo for a right child)
-    if (gamma - mn + 1 <= kTreelet)
-      treelets[atomicAdd(treelet_count, 1u)] = make_uint2(uint32_t(mn), uint32_t(gamma));
-    if (mx - gamma <= kTreelet)
-      treelets[atomicAdd(treelet_count, 1u)] =
-        make_uint2(uint32_t(gamma + 1), uint32_t(mx) | 0x80000000u);
-  } else if (i == 0) {
-    treelets[atomicAdd(treelet_count, 1u)] = make_uint2(0u, uint32_t(n - 1));
-  }
   const uint32_t left_link = up | (rleaf ? kLinkSibLeaf : 0u);
   const uint32_t right_link = up | kLinkRight | (lleaf ? kLinkSibLeaf : 0u);
+  other_end[node_base + i] = uint32_t(j);
+  if (top) {
+    // children with at most kTreelet leaves root treelets; an entry holds the
+    // leaf range as sorted positions of the whole batch, and in bit 31 of .y
+    // whether its root is a right child (the root is numbered hi for a left
+    // child, lo for a right child)
+    if (gamma - mn + 1 <= kTreelet)
+      treelets[atomicAdd(treelet_count, 1u)] = make_uint2(uint32_t(P + mn), uint32_t(P + gamma));
+    if (mx - gamma <= kTreelet)
+      treelets[atomicAdd(treelet_count, 1u)] =
+        make_uint2(uint32_t(P + gamma + 1), uint32_t(P + mx) | 0x80000000u);
+  } else if (i == 0) {
+    treelets[atomicAdd(treelet_count, 1u)] = make_uint2(uint32_t(P), uint32_t(P + n - 1));
+  }
   uint32_t left, right;
   if (lleaf) {
-    uint32_t v = ids ? ids[gamma] : uint32_t(gamma);
+    const uint32_t v = ids ? uint32_t(int64_t(ids[P + gamma]) - P) : uint32_t(gamma);
     left = v | kLeafBit;
-    parent[(n - 1) + gamma] = left_link;
+    parent[link_base + (n - 1) + gamma] = left_link;
     if (leafprim)
-      leafprim[gamma] = v;
+      leafprim[P + gamma] = v;
   } else {
     left = uint32_t(gamma);
-    parent[gamma] = left_link;
+    parent[link_base + gamma] = left_link;
   }
   if (rleaf) {
-    uint32_t v = ids ? ids[gamma + 1] : uint32_t(gamma + 1);
+    const uint32_t v = ids ? uint32_t(int64_t(ids[P + gamma + 1]) - P) : uint32_t(gamma + 1);
     right = v | kLeafBit;
-    parent[(n - 1) + gamma + 1] = right_link;
+    parent[link_base + (n - 1) + gamma + 1] = right_link;
     if (leafprim)
-      leafprim[gamma + 1] = v;
+      leafprim[P + gamma + 1] = v;
   } else {
     right = uint32_t(gamma + 1);
-    parent[gamma + 1] = right_link;
+    parent[link_base + gamma + 1] = right_link;
   }
-  *reinterpret_cast<uint2*>(&nodes[i].left) = make_uint2(left, right);
-  *reinterpret_cast<uint2*>(&wide[i].left) = make_uint2(left, right);
-  flags[i] = 0u;
+  *reinterpret_cast<uint2*>(&nodes[node_base + i].left) = make_uint2(left, right);
+  *reinterpret_cast<uint2*>(&wide[node_base + i].left) = make_uint2(left, right);
+  flags[node_base + i] = 0u;
 }
 
 // ---- bottom-up fit -------------------------------------------------------------
@@ -545,11 +574,28 @@ leaf_box(const float* __restrict__ prims, int kind, const uint32_t* __restrict__
   return load_prim_box(prims, kind, uint64_t(v));
 }
 
+// The scene of a treelet entry (batch positions) and its arrays.
+struct TreeletScene
+{
+  int64_t lo, hi, n; // scene-local leaf range, scene size
+  int64_t P, nb, lb;
+};
+
+__device__ __forceinline__ TreeletScene
+treelet_scene(const Scenes& sc, uint2 e)
+{
+  const int64_t glo = e.x;
+  const int64_t ghi = e.y & 0x7fffffffu;
+  const uint32_t seg = sc.prim_off ? find_scene(sc.prim_off, sc.nseg, glo, 1, false) : 0u;
+  const SceneView v = scene_view(sc, seg);
+  return TreeletScene{ glo - v.P, ghi - v.P, v.n, v.P, v.nb, v.lb };
+}
+
 __global__ __launch_bounds__(kTreelet) void
-k_fit_treelets(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
-               const uint32_t* __restrict__ parent, const uint32_t* __restrict__ leafprim,
+k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __restrict__ nodes_all,
+               const uint32_t* __restrict__ parent_all, const uint32_t* __restrict__ leafprim_all,
                const uint2* __restrict__ treelets, const uint32_t* __restrict__ count_ptr,
-               uint32_t* __restrict__ queue, WideNode* __restrict__ wide)
+               uint32_t* __restrict__ queue, WideNode* __restrict__ wide_all)
 {
   __shared__ uint32_t s_flag[kTreelet];
   __shared__ uint32_t s_link[kTreelet]; // link of internal node lo + t
@@ -565,11 +611,15 @@ k_fit_treelets(const float* __restrict__ prims, int kind, int64_t n, Node* __res
     const uint32_t item = s_item;
     if (item >= count)
       return; // uniform across the workgroup
-    const uint2 e = treelets[item];
-    const int64_t lo = e.x;
-    const int64_t hi = e.y & 0x7fffffffu;
+    const TreeletScene ts = treelet_scene(sc, treelets[item]);
+    const int64_t lo = ts.lo, hi = ts.hi, n = ts.n;
     const int64_t m = hi - lo + 1;
-    if (m >= 2 && m <= kTreelet && hi < n) {
+    if (lo >= 0 && m >= 2 && m <= kTreelet && hi < n) {
+      const float* prims = prims_all + prim_stride(kind) * ts.P;
+      Node* nodes = nodes_all + ts.nb;
+      const uint32_t* parent = parent_all + ts.lb;
+      const uint32_t* leafprim = leafprim_all ? leafprim_all + ts.P : nullptr;
+      WideNode* wide = wide_all + ts.nb;
       const int64_t k = lo + t;
       uint32_t link = 0u;
       if (t < m) {
@@ -614,19 +664,25 @@ k_fit_treelets(const float* __restrict__ prims, int kind, int64_t n, Node* __res
 }
 
 __global__ __launch_bounds__(kBlock) void
-k_fit_top(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict__ nodes,
-          const uint32_t* __restrict__ parent, const uint32_t* __restrict__ leafprim,
+k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __restrict__ nodes_all,
+          const uint32_t* __restrict__ parent_all, const uint32_t* __restrict__ leafprim_all,
           const uint2* __restrict__ treelets, const uint32_t* __restrict__ count_ptr,
-          uint32_t* __restrict__ flags, WideNode* __restrict__ wide)
+          uint32_t* __restrict__ flags_all, WideNode* __restrict__ wide_all)
 {
   const uint32_t count = *count_ptr;
   for (uint32_t item = blockIdx.x * kBlock + threadIdx.x; item < count;
        item += gridDim.x * kBlock) {
     const uint2 e = treelets[item];
-    const int64_t lo = e.x;
-    const int64_t hi = e.y & 0x7fffffffu;
-    if (hi >= n || lo > hi || (lo == 0 && hi == n - 1))
+    const TreeletScene ts = treelet_scene(sc, e);
+    const int64_t lo = ts.lo, hi = ts.hi, n = ts.n;
+    if (lo < 0 || hi >= n || lo > hi || (lo == 0 && hi == n - 1))
       continue; // the whole tree was one treelet (or a corrupt entry)
+    const float* prims = prims_all + prim_stride(kind) * ts.P;
+    Node* nodes = nodes_all + ts.nb;
+    const uint32_t* parent = parent_all + ts.lb;
+    const uint32_t* leafprim = leafprim_all ? leafprim_all + ts.P : nullptr;
+    uint32_t* flags = flags_all + ts.nb;
+    WideNode* wide = wide_all + ts.nb;
     Box mine;
     int64_t cur;
     uint32_t link;
@@ -683,32 +739,41 @@ k_fit_top(const float* __restrict__ prims, int kind, int64_t n, Node* __restrict
 // Rebuilds the treelet list of an existing tree (refit): every child that is
 // not a top node but whose parent is roots a treelet; its leaf range is
 // [c, other_end[c]] (either order) for an internal node, [k, k] for a leaf.
+// One thread per parent link of the batch.
 __global__ __launch_bounds__(kBlock) void
-k_enum_treelets(int64_t n, const uint32_t* __restrict__ parent,
-                const uint32_t* __restrict__ other_end, uint2* __restrict__ treelets,
+k_enum_treelets(Scenes sc, const uint32_t* __restrict__ parent_all,
+                const uint32_t* __restrict__ other_end_all, uint2* __restrict__ treelets,
                 uint32_t* __restrict__ count)
 {
-  const int64_t x = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (x >= 2 * n - 1)
+  const int64_t links = 2 * sc.n - (sc.prim_off ? int64_t(sc.nseg) : 1);
+  const int64_t gx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (gx >= links)
     return;
+  const uint32_t seg = sc.prim_off ? find_scene(sc.prim_off, sc.nseg, gx, 2, true) : 0u;
+  const SceneView v = scene_view(sc, seg);
+  const int64_t n = v.n;
+  const int64_t x = gx - v.lb;
+  if (x < 0 || x >= 2 * n - 1)
+    return;
+  const uint32_t P = uint32_t(v.P);
   if (x == 0) {
     if (n <= kTreelet)
-      treelets[atomicAdd(count, 1u)] = make_uint2(0u, uint32_t(n - 1));
+      treelets[atomicAdd(count, 1u)] = make_uint2(P, P + uint32_t(n - 1));
     return;
   }
-  const uint32_t link = parent[x];
+  const uint32_t link = parent_all[v.lb + x];
   if (!(link & kLinkGlobal))
     return;
   if (x < n - 1) {
-    const int64_t j = other_end[x];
+    const int64_t j = other_end_all[v.nb + x];
     const int64_t lo = x < j ? x : j;
     const int64_t hi = x < j ? j : x;
     if (hi - lo + 1 > kTreelet)
       return; // a top node itself
-    treelets[atomicAdd(count, 1u)] =
-      make_uint2(uint32_t(lo), uint32_t(hi) | ((link & kLinkRight) ? 0x80000000u : 0u));
+    treelets[atomicAdd(count, 1u)] = make_uint2(
+      P + uint32_t(lo), (P + uint32_t(hi)) | ((link & kLinkRight) ? 0x80000000u : 0u));
   } else {
-    const uint32_t k = uint32_t(x - (n - 1));
+    const uint32_t k = P + uint32_t(x - (n - 1));
     treelets[atomicAdd(count, 1u)] = make_uint2(k, k);
   }
 }
@@ -786,57 +851,80 @@ launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, 
   return hipGetLastError();
 }
 
-hipError_t
-launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n, void* nodes,
-              uint32_t* parent, uint32_t* leafprim, uint32_t* flags, uint32_t* other_end,
-              void* treelets, uint32_t* treelet_count, void* wide, hipStream_t s)
+static inline Scenes
+make_scenes(const uint64_t* prim_off, uint32_t nseg, uint64_t n)
 {
-  uint64_t m = n - 1;
-  uint32_t g = uint32_t((m + kBlock - 1) / kBlock);
+  Scenes sc;
+  sc.prim_off = prim_off;
+  sc.nseg = prim_off ? nseg : 1u;
+  sc.n = int64_t(n);
+  return sc;
+}
+
+hipError_t
+launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
+              const uint64_t* prim_off, uint32_t nseg, void* nodes, uint32_t* parent,
+              uint32_t* leafprim, uint32_t* flags, uint32_t* other_end, void* treelets,
+              uint32_t* treelet_count, void* wide, hipStream_t s)
+{
+  const Scenes sc = make_scenes(prim_off, nseg, n);
+  const uint64_t m = n - sc.nseg;
+  const uint32_t g = uint32_t((m + kBlock - 1) / kBlock);
   uint2* tl = static_cast<uint2*>(treelets);
-  if (code_bits == 30)
-    hipLaunchKernelGGL(k_karras<uint32_t>, dim3(g), dim3(kBlock), 0, s,
-                       static_cast<const uint32_t*>(sorted), ids, int64_t(n),
-                       static_cast<Node*>(nodes), parent, leafprim, flags, other_end, tl,
-                       treelet_count, static_cast<WideNode*>(wide));
-  else
-    hipLaunchKernelGGL(k_karras<uint64_t>, dim3(g), dim3(kBlock), 0, s,
-                       static_cast<const uint64_t*>(sorted), ids, int64_t(n),
-                       static_cast<Node*>(nodes), parent, leafprim, flags, other_end, tl,
-                       treelet_count, static_cast<WideNode*>(wide));
+  Node* nd = static_cast<Node*>(nodes);
+  WideNode* wd = static_cast<WideNode*>(wide);
+  if (prim_off) {
+    if (code_bits != 30)
+      return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_karras<uint64_t, uint32_t, true>), dim3(g), dim3(kBlock), 0, s,
+                       static_cast<const uint64_t*>(sorted), ids, sc, nd, parent, leafprim, flags,
+                       other_end, tl, treelet_count, wd);
+  } else if (code_bits == 30) {
+    hipLaunchKernelGGL((k_karras<uint32_t, uint32_t, false>), dim3(g), dim3(kBlock), 0, s,
+                       static_cast<const uint32_t*>(sorted), ids, sc, nd, parent, leafprim, flags,
+                       other_end, tl, treelet_count, wd);
+  } else {
+    hipLaunchKernelGGL((k_karras<uint64_t, uint64_t, false>), dim3(g), dim3(kBlock), 0, s,
+                       static_cast<const uint64_t*>(sorted), ids, sc, nd, parent, leafprim, flags,
+                       other_end, tl, treelet_count, wd);
+  }
   return hipGetLastError();
 }
 
 hipError_t
-launch_fit(const float* prims, int kind, uint64_t n, void* nodes, const uint32_t* parent,
-           const uint32_t* leafprim, uint32_t* flags, const void* treelets,
-           uint32_t* treelet_count, uint32_t* queue, void* wide, hipStream_t s)
+launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off, uint32_t nseg,
+           void* nodes, const uint32_t* parent, const uint32_t* leafprim, uint32_t* flags,
+           const void* treelets, uint32_t* treelet_count, uint32_t* queue, void* wide,
+           hipStream_t s)
 {
+  const Scenes sc = make_scenes(prim_off, nseg, n);
   const uint2* tl = static_cast<const uint2*>(treelets);
-  // typical trees hold about 2 n / kTreelet treelets; the loops are
-  // persistent, so any count is covered
-  uint32_t ga = grid_for(2 * n, kTreelet, 512);
-  hipLaunchKernelGGL(k_fit_treelets, dim3(ga), dim3(kTreelet), 0, s, prims, kind, int64_t(n),
+  // typical trees hold about 2 n / kTreelet treelets (plus one per small
+  // scene); the loops are persistent, so any count is covered
+  const uint32_t ga = grid_for(2 * n + uint64_t(sc.nseg) * kTreelet, kTreelet, 512);
+  hipLaunchKernelGGL(k_fit_treelets, dim3(ga), dim3(kTreelet), 0, s, prims, kind, sc,
                      static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, queue,
                      static_cast<WideNode*>(wide));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
-  uint32_t gb = grid_for(4 * n / kTreelet + 64, kBlock, 1024);
-  hipLaunchKernelGGL(k_fit_top, dim3(gb), dim3(kBlock), 0, s, prims, kind, int64_t(n),
+  const uint32_t gb = grid_for(4 * n / kTreelet + 64, kBlock, 1024);
+  hipLaunchKernelGGL(k_fit_top, dim3(gb), dim3(kBlock), 0, s, prims, kind, sc,
                      static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, flags,
                      static_cast<WideNode*>(wide));
   return hipGetLastError();
 }
 
 hipError_t
-launch_enum_treelets(uint64_t n, const uint32_t* parent, const uint32_t* other_end,
-                     void* treelets, uint32_t* treelet_count, hipStream_t s)
+launch_enum_treelets(uint64_t n, const uint64_t* prim_off, uint32_t nseg, const uint32_t* parent,
+                     const uint32_t* other_end, void* treelets, uint32_t* treelet_count,
+                     hipStream_t s)
 {
-  uint64_t x = 2 * n - 1;
-  uint32_t g = uint32_t((x + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_enum_treelets, dim3(g), dim3(kBlock), 0, s, int64_t(n), parent,
-                     other_end, static_cast<uint2*>(treelets), treelet_count);
+  const Scenes sc = make_scenes(prim_off, nseg, n);
+  const uint64_t x = 2 * n - sc.nseg;
+  const uint32_t g = uint32_t((x + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_enum_treelets, dim3(g), dim3(kBlock), 0, s, sc, parent, other_end,
+                     static_cast<uint2*>(treelets), treelet_count);
   return hipGetLastError();
 }
 

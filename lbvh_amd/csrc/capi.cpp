@@ -15,42 +15,10 @@
 
 #include "engine.hpp"
 #include "lbvh_mi355x.h"
+#include "runtime.hpp"
 
 using namespace lbvh_eng;
-
-namespace {
-
-struct DevBuf
-{
-  void* p = nullptr;
-  uint64_t cap = 0;
-};
-
-} // namespace
-
-// One timed call: its kind and the events bracketing its phases.
-struct EventSet
-{
-  int kind = 0; // 1 build, 2 trace, 3 refit
-  int recorded = 0;
-  hipEvent_t ev[8] = {};
-};
-
-struct lbvh_ctx
-{
-  int device = 0;
-  hipStream_t own_stream = nullptr;
-  hipStream_t stream = nullptr;
-  int timing = 0; // 0 off, 1 per phase, 2 per call
-  std::vector<EventSet> sets; // pool; sets[0..used) belong to timed calls
-  size_t used = 0;
-  std::string last_error;
-  uint32_t epoch_next = 1;
-  // workspace
-  DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, part_hist, small, status, counts,
-    treelets, io_in, io_out;
-  bool status_zeroed_words = 0;
-};
+using namespace lbvh_rt;
 
 struct lbvh_bvh
 {
@@ -73,15 +41,7 @@ struct lbvh_bvh
   uint64_t bytes = 0;
 };
 
-namespace {
-
-// small workspace region layout (32-bit words)
-constexpr uint32_t kSmallBoundsTicket = 0; // self-resetting
-constexpr uint32_t kSmallErr = 1;          // cleared by the host after reading
-constexpr uint32_t kSmallTickets = 2;      // kMaxSortPasses words
-constexpr uint32_t kSmallTreeletCount = kSmallTickets + kMaxSortPasses;
-constexpr uint32_t kSmallTreeletQueue = kSmallTreeletCount + 1;
-constexpr uint32_t kSmallHist = kSmallTreeletQueue + 1;
+namespace lbvh_rt {
 
 lbvh_status
 fail(lbvh_ctx* c, lbvh_status s, const std::string& msg)
@@ -97,15 +57,8 @@ hip_fail(lbvh_ctx* c, hipError_t e, const char* what)
   return fail(c, LBVH_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(ctx, expr)                                                     \
-  do {                                                                         \
-    hipError_t e_ = (expr);                                                    \
-    if (e_ != hipSuccess)                                                      \
-      return hip_fail((ctx), e_, #expr);                                       \
-  } while (0)
-
 lbvh_status
-ensure(lbvh_ctx* c, DevBuf& b, uint64_t bytes, bool zero_new = false)
+ensure(lbvh_ctx* c, DevBuf& b, uint64_t bytes, bool zero_new)
 {
   if (bytes == 0)
     bytes = 256;
@@ -172,7 +125,7 @@ begin_timed(lbvh_ctx* c, int kind)
 // Records phase boundary `idx` of the current timed call; `final` marks the
 // call's last boundary.  Per-call timing keeps only the first and last.
 void
-record(lbvh_ctx* c, int idx, bool final = false)
+record(lbvh_ctx* c, int idx, bool final)
 {
   if (!c->timing || c->used == 0 || idx >= 8)
     return;
@@ -395,11 +348,11 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   if (e == hipSuccess)
     e = launch_karras(opts.code_bits, sorted,
                       opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr, n,
-                      b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end,
+                      nullptr, 1, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end,
                       c->treelets.p, small + kSmallTreeletCount, b->d_wide, c->stream);
   record(c, 3);
   if (e == hipSuccess)
-    e = launch_fit(d_prims, kind, n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
+    e = launch_fit(d_prims, kind, n, nullptr, 1, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
                    c->treelets.p, small + kSmallTreeletCount, small + kSmallTreeletQueue,
                    b->d_wide, c->stream);
   record(c, 4, true);
@@ -436,7 +389,7 @@ sync_and_check(lbvh_ctx* c)
   return LBVH_OK;
 }
 
-} // namespace
+} // namespace lbvh_rt
 
 extern "C" {
 
@@ -722,9 +675,9 @@ queue_refit(lbvh_ctx* c, lbvh_bvh* b)
   begin_timed(c, 3);
   record(c, 0);
   HIP_TRY(c, launch_fill_u32(small + kSmallTreeletCount, 2, 0u, c->stream));
-  HIP_TRY(c, launch_enum_treelets(b->n, b->d_parent, b->d_other_end, c->treelets.p,
+  HIP_TRY(c, launch_enum_treelets(b->n, nullptr, 1, b->d_parent, b->d_other_end, c->treelets.p,
                                   small + kSmallTreeletCount, c->stream));
-  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
+  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, nullptr, 1, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
                         c->treelets.p, small + kSmallTreeletCount, small + kSmallTreeletQueue,
                         b->d_wide, c->stream));
   record(c, 1, true);
@@ -847,7 +800,7 @@ lbvh_trace_closest_device(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, u
     return st;
   begin_timed(c, 2);
   record(c, 0);
-  HIP_TRY(c, launch_trace(mode, b->d_wide, b->d_prims, b->n, d_rays, nrays, d_prim, d_t, d_uv,
+  HIP_TRY(c, launch_trace(mode, b->d_wide, b->d_prims, b->n, nullptr, nullptr, 1, d_rays, nrays, d_prim, d_t, d_uv,
                           c->stream));
   record(c, 1, true);
   return LBVH_OK;

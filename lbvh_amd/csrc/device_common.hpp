@@ -131,6 +131,69 @@ load_prim_box(const float* __restrict__ prims, int kind, uint64_t i)
   return o;
 }
 
+// ---- scene batches ---------------------------------------------------------
+// A batch concatenates scenes: scene s owns primitives (and sorted positions)
+// [P, P + n_s) with P = prim_off[s], internal nodes [P - s, P - s + n_s - 1)
+// and parent links [2P - s, 2P - s + 2 n_s - 1); every n_s >= 2.  A single
+// scene is prim_off == nullptr, n_0 = n.
+struct Scenes
+{
+  const uint64_t* prim_off; // [nseg + 1] or nullptr
+  uint32_t nseg;
+  int64_t n; // all primitives
+};
+
+// Largest s in [0, nseg) with key(s) <= x, where key(s) = off[s] * mul -
+// (minus_s ? s : 0) is strictly increasing and key(0) = 0: the scene of a
+// primitive position (mul 1), an internal node (mul 1, minus_s), a parent
+// link (mul 2, minus_s), or of a ray given per-scene ray offsets (mul 1).
+__device__ __forceinline__ uint32_t
+find_scene(const uint64_t* __restrict__ off, uint32_t nseg, int64_t x, int mul, bool minus_s)
+{
+  uint32_t lo = 0, hi = nseg; // key(lo) <= x < key(hi)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const int64_t key = int64_t(off[mid]) * mul - (minus_s ? int64_t(mid) : 0);
+    if (key <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Where one scene's arrays start inside the batch.
+struct SceneView
+{
+  int64_t P;  // first primitive / sorted position
+  int64_t n;  // primitives
+  int64_t nb; // first internal node
+  int64_t lb; // first parent link
+};
+
+__device__ __forceinline__ SceneView
+scene_view(const Scenes& sc, uint32_t seg)
+{
+  SceneView v;
+  if (sc.prim_off) {
+    v.P = int64_t(sc.prim_off[seg]);
+    v.n = int64_t(sc.prim_off[seg + 1]) - v.P;
+  } else {
+    seg = 0;
+    v.P = 0;
+    v.n = sc.n;
+  }
+  v.nb = v.P - int64_t(seg);
+  v.lb = 2 * v.P - int64_t(seg);
+  return v;
+}
+
+__device__ __forceinline__ int
+prim_stride(int kind)
+{
+  return kind == kPrimTriangle ? 12 : 6;
+}
+
 // ---- inter-workgroup communication ----------------------------------------
 // Stores and loads that must be seen by workgroups on other XCDs use relaxed
 // agent-scope atomics (device-coherent, bypassing the per-XCD L2 for the line

@@ -62,35 +62,63 @@ uint64_t sort_status_words(uint64_t n, const SortConfig& cfg);
 // Number of chunks of the reduce-then-scan strategy (and tiles per chunk).
 uint32_t rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles);
 
+// Scene batches: prim_off == nullptr is one scene of n primitives; otherwise
+// scene s holds primitives [prim_off[s], prim_off[s + 1]) (n = prim_off[nseg],
+// every scene at least 2), its internal nodes start at prim_off[s] - s and its
+// parent links at 2 prim_off[s] - s.  Batched sorted keys are
+// (scene << 30 | 30-bit code) in u64.
+
 // Karras radix tree: left/right of every internal node; parent links
-// (internal node c at c, the leaf at sorted position k at n-1+k; bit 31:
-// the parent is a top node, bit 30: right child, bit 29: sibling is a leaf);
-// the primitive of each leaf position when ids != nullptr (leafprim); the
-// other end of each node's leaf range (other_end); the treelet list
-// (uint2 {lo, hi | right-root << 31}, at most n entries; treelet_count must
-// be zero); fit flags cleared.
+// (internal node c at c, the leaf at sorted position k at n-1+k, per scene;
+// bit 31: the parent is a top node, bit 30: right child, bit 29: sibling is
+// a leaf; the index is scene-local); the scene-local primitive of each leaf
+// position when ids != nullptr (leafprim); the other end of each node's leaf
+// range (other_end); the treelet list (uint2 {lo, hi | right-root << 31} as
+// batch positions, at most n entries; treelet_count must be zero); fit flags
+// cleared.
 hipError_t launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
-                         void* nodes, uint32_t* parent, uint32_t* leafprim, uint32_t* flags,
-                         uint32_t* other_end, void* treelets, uint32_t* treelet_count,
-                         void* wide, hipStream_t s);
+                         const uint64_t* prim_off, uint32_t nseg, void* nodes, uint32_t* parent,
+                         uint32_t* leafprim, uint32_t* flags, uint32_t* other_end, void* treelets,
+                         uint32_t* treelet_count, void* wide, hipStream_t s);
 
 // Bottom-up box fit: treelets in LDS, then the top tree with device-scope
 // flags (flags must be zero and are left zero; *queue must be zero).
-// leafprim == nullptr: the leaf at position k is primitive k.
-hipError_t launch_fit(const float* prims, int kind, uint64_t n, void* nodes,
-                      const uint32_t* parent, const uint32_t* leafprim, uint32_t* flags,
-                      const void* treelets, uint32_t* treelet_count, uint32_t* queue,
-                      void* wide, hipStream_t s);
+// leafprim == nullptr: the leaf at position k is primitive k of its scene.
+hipError_t launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off,
+                      uint32_t nseg, void* nodes, const uint32_t* parent,
+                      const uint32_t* leafprim, uint32_t* flags, const void* treelets,
+                      uint32_t* treelet_count, uint32_t* queue, void* wide, hipStream_t s);
 
 // Rebuilds the treelet list of a built tree (for a refit); *treelet_count
 // must be zero.
-hipError_t launch_enum_treelets(uint64_t n, const uint32_t* parent, const uint32_t* other_end,
+hipError_t launch_enum_treelets(uint64_t n, const uint64_t* prim_off, uint32_t nseg,
+                                const uint32_t* parent, const uint32_t* other_end,
                                 void* treelets, uint32_t* treelet_count, hipStream_t s);
 
-// Closest-hit traversal over the 64-byte traversal records (wide).
+// Closest-hit traversal over the 64-byte traversal records (wide).  For a
+// batch (prim_off != nullptr) ray r belongs to the scene s with ray_off[s] <=
+// r < ray_off[s + 1] and hits report scene-local primitive ids.
 hipError_t launch_trace(int mode, const void* wide, const float* tris, uint64_t n,
+                        const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg,
                         const float* rays, uint64_t nrays, uint32_t* prim, float* t, float* uv,
                         hipStream_t s);
+
+// Batched centroid bounds: the centre (SoA) of every primitive and per-scene
+// centroid bounds as order-preserving integers (6 per scene: min xyz, max
+// xyz; launch_scene_bounds_init resets them first).
+hipError_t launch_scene_bounds_init(int32_t* bounds_key, uint32_t nseg, hipStream_t s);
+hipError_t launch_scene_bounds(const float* prims, int kind, uint64_t n, const uint64_t* prim_off,
+                               uint32_t nseg, float* cx, float* cy, float* cz, int32_t* bounds_key,
+                               hipStream_t s);
+// Batched 30-bit codes keyed (scene << 30 | code) and the digit histograms
+// of `passes` radix passes (per-workgroup rows in part_hist
+// [scene_encode_groups(n)][passes << digit_bits], summed into hist, which
+// must be zero).
+hipError_t launch_scene_encode(const float* cx, const float* cy, const float* cz, uint64_t n,
+                               const uint64_t* prim_off, uint32_t nseg, const int32_t* bounds_key,
+                               uint64_t* keys, uint32_t* part_hist, uint32_t* hist,
+                               int digit_bits, int passes, hipStream_t s);
+uint32_t scene_encode_groups(uint64_t n);
 
 // Fill `words` 32-bit words with `value`.
 hipError_t launch_fill_u32(uint32_t* p, uint64_t words, uint32_t value, hipStream_t s);

@@ -217,16 +217,27 @@ struct Stack
   }
 };
 
-template<int MODE>
+// SEG: a batch of scenes; ray i belongs to the scene s with ray_off[s] <= i
+// < ray_off[s + 1], whose nodes start at prim_off[s] - s and whose triangles
+// start at prim_off[s]; reported ids are scene-local.
+template<int MODE, bool SEG>
 __global__ __launch_bounds__(kTraceBlock) void
 k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
-        const float* __restrict__ rays, uint64_t nrays, uint32_t* __restrict__ prim_out,
-        float* __restrict__ t_out, float* __restrict__ uv_out)
+        const uint64_t* __restrict__ prim_off, const uint64_t* __restrict__ ray_off,
+        uint32_t nseg, const float* __restrict__ rays, uint64_t nrays,
+        uint32_t* __restrict__ prim_out, float* __restrict__ t_out, float* __restrict__ uv_out)
 {
   __shared__ uint2 s_ring[(kStackLds > 0 ? kStackLds : 1) * kTraceBlock];
   const uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
   if (i >= nrays)
     return;
+  if constexpr (SEG) {
+    const uint32_t sc = find_scene(ray_off, nseg, int64_t(i), 1, false);
+    const uint64_t P = prim_off[sc];
+    nprims = uint32_t(prim_off[sc + 1] - P);
+    nodes += P - sc;
+    tris += 12 * P;
+  }
   const float4* rp = reinterpret_cast<const float4*>(rays + 8 * i);
   const float4 r0 = rp[0], r1 = rp[1];
   Ray r;
@@ -349,26 +360,40 @@ namespace lbvh_eng {
 
 using namespace lbvh_dev;
 
+template<int MODE>
+static void
+trace_mode(uint32_t g, hipStream_t s, const WideNode* nd, const float* tris, uint64_t n,
+           const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg, const float* rays,
+           uint64_t nrays, uint32_t* prim, float* t, float* uv)
+{
+  if (prim_off)
+    hipLaunchKernelGGL((k_trace<MODE, true>), dim3(g), dim3(kTraceBlock), 0, s, nd, tris,
+                       uint32_t(n), prim_off, ray_off, nseg, rays, nrays, prim, t, uv);
+  else
+    hipLaunchKernelGGL((k_trace<MODE, false>), dim3(g), dim3(kTraceBlock), 0, s, nd, tris,
+                       uint32_t(n), nullptr, nullptr, 1u, rays, nrays, prim, t, uv);
+}
+
 hipError_t
-launch_trace(int mode, const void* nodes, const float* tris, uint64_t n, const float* rays,
-             uint64_t nrays, uint32_t* prim, float* t, float* uv, hipStream_t s)
+launch_trace(int mode, const void* nodes, const float* tris, uint64_t n, const uint64_t* prim_off,
+             const uint64_t* ray_off, uint32_t nseg, const float* rays, uint64_t nrays,
+             uint32_t* prim, float* t, float* uv, hipStream_t s)
 {
   if (nrays == 0)
     return hipSuccess;
-  uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
+  if (prim_off && (!ray_off || nseg == 0))
+    return hipErrorInvalidValue;
+  const uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
   const WideNode* nd = static_cast<const WideNode*>(nodes);
   switch (mode) {
     case 0:
-      hipLaunchKernelGGL(k_trace<0>, dim3(g), dim3(kTraceBlock), 0, s, nd, tris, uint32_t(n), rays,
-                         nrays, prim, t, uv);
+      trace_mode<0>(g, s, nd, tris, n, prim_off, ray_off, nseg, rays, nrays, prim, t, uv);
       break;
     case 1:
-      hipLaunchKernelGGL(k_trace<1>, dim3(g), dim3(kTraceBlock), 0, s, nd, tris, uint32_t(n), rays,
-                         nrays, prim, t, uv);
+      trace_mode<1>(g, s, nd, tris, n, prim_off, ray_off, nseg, rays, nrays, prim, t, uv);
       break;
     case 2:
-      hipLaunchKernelGGL(k_trace<2>, dim3(g), dim3(kTraceBlock), 0, s, nd, tris, uint32_t(n), rays,
-                         nrays, prim, t, uv);
+      trace_mode<2>(g, s, nd, tris, n, prim_off, ray_off, nseg, rays, nrays, prim, t, uv);
       break;
     default:
       return hipErrorInvalidValue;

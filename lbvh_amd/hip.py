@@ -68,6 +68,19 @@ class DeviceArray:
         check(hip().hipMemcpy(d.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
         return d
 
+    def copy_from(self, src):
+        """Overwrites the buffer from a host array or another DeviceArray of
+        the same byte size."""
+        if isinstance(src, DeviceArray):
+            if src.nbytes != self.nbytes:
+                raise ValueError("size mismatch")
+            check(hip().hipMemcpy(self.ptr, src.ptr, self.nbytes, D2D), "hipMemcpy D2D")
+            return
+        a = np.ascontiguousarray(src)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        check(hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
+
     def numpy(self):
         out = np.empty(self.shape, dtype=self.dtype)
         check(hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, D2H), "hipMemcpy D2H")

@@ -373,3 +373,28 @@ wl_uniform_u53(uint64_t seed, uint64_t count, double* out)
   for (uint64_t i = 0; i < count; i++)
     out[i] = wl_u53(&r);
 }
+
+/* Bounds of the vertices a, b = a - e0, c = a + e1 of n triangles (48-byte
+ * lbvh::triangle<float> records), in float arithmetic. */
+void
+wl_tri_bounds(const float* tri, uint64_t n, float* lo, float* hi)
+{
+  for (int k = 0; k < 3; k++) {
+    lo[k] = INFINITY;
+    hi[k] = -INFINITY;
+  }
+  for (uint64_t i = 0; i < n; i++) {
+    const float* t = tri + 12 * i;
+    for (int k = 0; k < 3; k++) {
+      float a = t[k];
+      float b = a - t[3 + k];
+      float c = a + t[6 + k];
+      float mn = a < b ? a : b;
+      float mx = a < b ? b : a;
+      mn = c < mn ? c : mn;
+      mx = c > mx ? c : mx;
+      lo[k] = mn < lo[k] ? mn : lo[k];
+      hi[k] = mx > hi[k] ? mx : hi[k];
+    }
+  }
+}

@@ -86,6 +86,10 @@ class DeviceArray:
         check(hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, D2H), "hipMemcpy D2H")
         return out
 
+    def rows(self, start, stop):
+        """A non-owning view of rows [start, stop) along the first axis."""
+        return DeviceView(self, start, stop)
+
     def numel(self):
         return int(np.prod(self.shape))
 
@@ -102,3 +106,18 @@ class DeviceArray:
             self.close()
         except Exception:
             pass
+
+
+class DeviceView(DeviceArray):
+    """Rows [start, stop) of a DeviceArray; the parent keeps the memory."""
+
+    def __init__(self, parent, start, stop):
+        row = int(np.prod(parent.shape[1:])) if len(parent.shape) > 1 else 1
+        self.parent = parent
+        self.shape = (int(stop) - int(start),) + tuple(parent.shape[1:])
+        self.dtype = parent.dtype
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = parent.ptr + int(start) * row * self.dtype.itemsize
+
+    def close(self):
+        self.ptr = None

@@ -155,6 +155,7 @@ def bench_batched(args):
     ctx.synchronize()
     first_build_s = time.perf_counter() - t0
     info = b.info()
+    tune_log = ctx.tune_log()
     log("bench: batch built (first call incl. allocation and tuning: %.1f ms); sort %d-bit "
         "digits, tile %d, %d passes" % (first_build_s * 1e3, info.sort_digit_bits, info.sort_tile,
                                         info.sort_passes))
@@ -244,7 +245,9 @@ def bench_batched(args):
                                         "fit": round(split[3], 4)} if len(split) == 4 else split,
                     "closest_hit_mrays_s": round(m / (trace_ms * 1e-3) / 1e6, 3),
                     "sort": {"digit_bits": info.sort_digit_bits, "tile": info.sort_tile,
-                             "passes": info.sort_passes, "picked_by": "autotuner"},
+                             "passes": info.sort_passes, "picked_by": "autotuner",
+                             "candidates_ms": {"d%d-t%d" % (r["digit_bits"], r["tile"]): r["ms"]
+                                               for r in tune_log if r["batch"]}},
                     "first_build_ms": round(first_build_s * 1e3, 2),
                     "latency_ms_p50": round(wall_b * 1e3, 4),
                     "latency_ms_p99": round(wall_b * 1e3, 4)},
@@ -284,9 +287,12 @@ def main():
     d_rays = hip.DeviceArray.from_host(rays)
 
     # bounce rays from the GPU's own primary hits (identical to the oracle's)
+    # (the first build of this size also runs the sort autotuner)
     bvh = ctx.build_triangles(d_tri, code_bits=63, leaf_order=lbvh_amd.LEAF_PRIMITIVE_ID)
     p, t, _ = ctx.trace(bvh, d_rays, lbvh_amd.ISECT_REFERENCE)
     ctx.synchronize()
+    sort_info = bvh.info()
+    tune_log = ctx.tune_log()
     bvh.destroy()
     hp = p.numpy()
     ht = t.numpy()
@@ -355,7 +361,8 @@ def main():
     mprims = N_PRIMS / (build_ms * 1e-3) / 1e6
     mrays = (n_primary + n_bounce) / (trace_ms * 1e-3) / 1e6
 
-    sort_passes = 8  # 63-bit codes (60 bits used), 8-bit digits
+    digit = sort_info.sort_digit_bits
+    sort_passes = (60 + digit - 1) // digit  # 63-bit codes use 60 key bits
     sort_pass_ms = phases[1] / sort_passes
     sort_pass_bytes = N_PRIMS * SORT_PASS_BYTES_PER_PRIM_63
     kernels = {
@@ -370,7 +377,7 @@ def main():
     build_frac = build_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
     if dominant.startswith("sort"):
         ach = sort_pass_bytes / (sort_pass_ms * 1e-3) / 1e9
-        roof = {"kernel": "k_onesweep (one digit pass, u64 key + u32 id)", "bound": "hbm",
+        roof = {"kernel": "radix sort (one digit pass, u64 key + u32 id)", "bound": "hbm",
                 "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": sort_pass_bytes, "avg_launch_ms": round(sort_pass_ms, 5)}
@@ -404,6 +411,11 @@ def main():
         "build_ms": round(build_ms, 4),
         "build_phases_ms": {"bounds+encode": round(phases[0], 4), "sort": round(phases[1], 4),
                             "tree": round(phases[2], 4), "fit": round(phases[3], 4)},
+        "sort": {"digit_bits": digit, "tile": sort_info.sort_tile,
+                 "strategy": "onesweep" if sort_info.sort_strategy == 1 else "reduce-then-scan",
+                 "passes": sort_passes, "picked_by": "autotuner",
+                 "candidates_ms": {"%s-d%d-t%d" % (r["strategy"], r["digit_bits"], r["tile"]): r["ms"]
+                                   for r in tune_log if not r["batch"]}},
         "build_roofline_frac": round(build_frac, 4),
         "build_algorithmic_bytes_per_prim": BUILD_BYTES_PER_PRIM_63,
         "closest_hit_mrays_s": round(mrays, 3),

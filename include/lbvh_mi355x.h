@@ -87,7 +87,24 @@ typedef struct lbvh_bvh_info
   const void* d_nodes;   /* device lbvh::node<float>[n_nodes] */
   const void* d_prims;   /* device triangles (NULL for AABB builds) */
   uint64_t device_bytes; /* device memory held by the handle */
+  uint32_t sort_digit_bits; /* the sort configuration the build used */
+  uint32_t sort_tile;
+  uint32_t sort_strategy;   /* 1 onesweep, 2 reduce-then-scan */
+  uint32_t reserved;
 } lbvh_bvh_info;
+
+/* One configuration timed by the context's sort autotuner. */
+typedef struct lbvh_tune_record
+{
+  uint32_t batch;    /* 1: a scene batch, 0: a single-scene build */
+  uint32_t key_bits; /* sorted key width */
+  uint32_t log2_n;   /* size bucket: ceil(log2(primitives)) */
+  uint32_t sort_digit_bits;
+  uint32_t sort_tile;
+  uint32_t sort_strategy;
+  uint32_t picked;   /* the configuration kept for this bucket */
+  float ms;          /* device time of the whole build with it */
+} lbvh_tune_record;
 
 /* -- status / options ---------------------------------------------------- */
 
@@ -117,6 +134,17 @@ lbvh_status lbvh_ctx_phase_stats(lbvh_ctx* ctx, int kind, float* sum_ms,
                                  uint32_t cap, uint32_t* count,
                                  uint32_t* calls);
 const char* lbvh_ctx_last_error(const lbvh_ctx* ctx);
+/* Sort autotuner (on by default).  A build whose options leave
+ * sort_digit_bits, sort_tile and sort_strategy at 0 uses the configuration
+ * tuned for its key width and size bucket; the first such build of a bucket
+ * times every candidate configuration on its own data (each twice, on the
+ * device) and keeps the fastest.  Results are identical for every
+ * configuration; only the speed differs.  Disabled: a fixed default. */
+lbvh_status lbvh_ctx_set_autotune(lbvh_ctx* ctx, int enable);
+/* Copies up to cap records of every configuration timed so far; returns the
+ * total number of records. */
+uint32_t lbvh_ctx_tune_log(const lbvh_ctx* ctx, lbvh_tune_record* out,
+                           uint32_t cap);
 /* Device bytes held by the context's scratch workspace. */
 uint64_t lbvh_ctx_workspace_bytes(const lbvh_ctx* ctx);
 

@@ -51,7 +51,16 @@ class BvhInfo(ctypes.Structure):
     _fields_ = [("n_prims", ctypes.c_uint64), ("n_nodes", ctypes.c_uint64),
                 ("code_bits", ctypes.c_uint32), ("leaf_order", ctypes.c_uint32),
                 ("d_nodes", ctypes.c_void_p), ("d_prims", ctypes.c_void_p),
-                ("device_bytes", ctypes.c_uint64)]
+                ("device_bytes", ctypes.c_uint64), ("sort_digit_bits", ctypes.c_uint32),
+                ("sort_tile", ctypes.c_uint32), ("sort_strategy", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class TuneRecord(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
+                ("log2_n", ctypes.c_uint32), ("sort_digit_bits", ctypes.c_uint32),
+                ("sort_tile", ctypes.c_uint32), ("sort_strategy", ctypes.c_uint32),
+                ("picked", ctypes.c_uint32), ("ms", ctypes.c_float)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -82,6 +91,8 @@ SIGNATURES = {
                                             ctypes.POINTER(ctypes.c_uint32),
                                             ctypes.POINTER(ctypes.c_uint32)]),
     "lbvh_ctx_last_error": (ctypes.c_char_p, [VP]),
+    "lbvh_ctx_set_autotune": (ctypes.c_int, [VP, ctypes.c_int]),
+    "lbvh_ctx_tune_log": (ctypes.c_uint32, [VP, VP, ctypes.c_uint32]),
     "lbvh_ctx_workspace_bytes": (ctypes.c_uint64, [VP]),
     "lbvh_build_triangles": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), ctypes.POINTER(VP)]),
     "lbvh_build_triangles_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts),
@@ -402,6 +413,20 @@ class Context:
         self._check(lib().lbvh_ctx_phase_stats(self.handle, kind, buf, 8, ctypes.byref(cnt),
                                                ctypes.byref(calls)))
         return [buf[i] for i in range(cnt.value)], calls.value
+
+    def set_autotune(self, enable=True):
+        """Sort autotuning for builds that leave the sort options at 0."""
+        self._check(lib().lbvh_ctx_set_autotune(self.handle, int(bool(enable))))
+
+    def tune_log(self):
+        """Every configuration the sort autotuner timed: a list of dicts."""
+        n = lib().lbvh_ctx_tune_log(self.handle, None, 0)
+        buf = (TuneRecord * max(n, 1))()
+        lib().lbvh_ctx_tune_log(self.handle, buf, n)
+        return [{"batch": bool(r.batch), "key_bits": r.key_bits, "log2_n": r.log2_n,
+                 "digit_bits": r.sort_digit_bits, "tile": r.sort_tile,
+                 "strategy": "onesweep" if r.sort_strategy == 1 else "reduce-then-scan",
+                 "picked": bool(r.picked), "ms": round(r.ms, 4)} for r in buf[:n]]
 
     def workspace_bytes(self):
         return int(lib().lbvh_ctx_workspace_bytes(self.handle))

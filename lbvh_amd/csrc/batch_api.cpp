@@ -112,14 +112,27 @@ ceil_log2(uint32_t x)
   return b;
 }
 
+// Scratch is sized for every sort configuration in `cfgs` (the tuner's
+// candidates, or the one configuration in use).
 lbvh_status
-alloc_batch(lbvh_ctx* c, lbvh_batch* b)
+alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
 {
   const uint64_t n = b->n;
   const uint64_t m = n - b->nseg; // internal nodes
-  const uint32_t radix = 1u << b->cfg.digit_bits;
-  b->small_words = kSmallHist + uint32_t(b->passes) * radix;
-  b->counts_words = uint64_t(rts_groups(n, b->cfg, nullptr)) * radix;
+  uint64_t hist_words = 0;
+  b->small_words = 0;
+  b->counts_words = 0;
+  for (const SortConfig& cfg : cfgs) {
+    const uint32_t radix = 1u << cfg.digit_bits;
+    const uint32_t words = uint32_t(sort_passes(b->key_bits, cfg.digit_bits)) * radix;
+    if (words > hist_words)
+      hist_words = words;
+    if (kSmallHist + words > b->small_words)
+      b->small_words = kSmallHist + words;
+    const uint64_t cw = uint64_t(rts_groups(n, cfg, nullptr)) * radix;
+    if (cw > b->counts_words)
+      b->counts_words = cw;
+  }
   struct
   {
     void** p;
@@ -139,8 +152,7 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b)
     { reinterpret_cast<void**>(&b->d_keys_b), 8 * n },
     { reinterpret_cast<void**>(&b->d_ids_a), b->with_ids ? 4 * n : 0 },
     { reinterpret_cast<void**>(&b->d_ids_b), b->with_ids ? 4 * n : 0 },
-    { reinterpret_cast<void**>(&b->d_part_hist),
-      4ull * scene_encode_groups(n) * uint64_t(b->passes) * radix },
+    { reinterpret_cast<void**>(&b->d_part_hist), 4ull * scene_encode_groups(n) * hist_words },
     { reinterpret_cast<void**>(&b->d_small), 4ull * b->small_words },
     { reinterpret_cast<void**>(&b->d_counts), 4 * b->counts_words },
     { &b->d_treelets, 8 * n },
@@ -283,12 +295,30 @@ build_batch_common(lbvh_ctx* c, const float* d_tri, bool owns, const uint64_t* o
   const uint64_t n = off[nseg];
   if (n >= (uint64_t(1) << 31))
     return fail(c, LBVH_E_INVALID, "a batch holds fewer than 2^31 triangles");
-  SortConfig cfg = tune_sort(n, 30, &opts);
-  cfg.strategy = 1; // reduce-then-scan: no look-back state, so replays stay valid
   if (opts.sort_strategy == 1)
     return fail(c, LBVH_E_INVALID, "batches sort with the reduce-then-scan strategy");
-  if (!valid_sort_config(cfg))
-    return fail(c, LBVH_E_INVALID, "unsupported sort digit width / tile");
+  const int key_bits = 30 + ceil_log2(nseg);
+  // explicit options, else the tuned choice for this key width and size,
+  // else every candidate (timed below on this batch)
+  std::vector<SortConfig> cands;
+  bool tuning = false;
+  if (sort_overridden(opts)) {
+    SortConfig cfg = sort_from_opts(opts);
+    cfg.strategy = 1; // reduce-then-scan: no look-back state, so replays stay valid
+    if (!valid_sort_config(cfg))
+      return fail(c, LBVH_E_INVALID, "unsupported sort digit width / tile");
+    cands.push_back(cfg);
+  } else if (!c->autotune || n < kTuneMinPrims) {
+    cands.push_back(default_sort());
+  } else {
+    auto it = c->tuned.find(tune_key(true, key_bits, n));
+    if (it != c->tuned.end()) {
+      cands.push_back(it->second);
+    } else {
+      cands = sort_candidates(true);
+      tuning = true;
+    }
+  }
   auto* b = new (std::nothrow) lbvh_batch;
   if (!b)
     return fail(c, LBVH_E_OOM, "host allocation failed");
@@ -297,15 +327,15 @@ build_batch_common(lbvh_ctx* c, const float* d_tri, bool owns, const uint64_t* o
   b->nseg = nseg;
   b->leaf_order = opts.leaf_order;
   b->with_ids = opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID || opts.keep_codes;
-  b->key_bits = 30 + ceil_log2(nseg);
-  b->cfg = cfg;
-  b->passes = sort_passes(b->key_bits, cfg.digit_bits);
+  b->key_bits = key_bits;
+  b->cfg = cands[0];
+  b->passes = sort_passes(b->key_bits, b->cfg.digit_bits);
   b->h_off.assign(off, off + nseg + 1);
   b->d_prims = const_cast<float*>(d_tri);
   b->owns_prims = owns;
   if (owns)
     b->bytes += n * 48;
-  lbvh_status st = alloc_batch(c, b);
+  lbvh_status st = alloc_batch(c, b, cands);
   if (st != LBVH_OK) {
     b->owns_prims = false;
     free_batch(b);
@@ -314,10 +344,51 @@ build_batch_common(lbvh_ctx* c, const float* d_tri, bool owns, const uint64_t* o
   hipError_t e = hipMemcpyAsync(b->d_off, off, 8ull * (nseg + 1), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
     e = hipStreamSynchronize(c->stream); // `off` may be pageable caller memory
-  if (e == hipSuccess) {
-    begin_timed(c, 1);
-    e = queue_build(c, b, true);
+  if (e != hipSuccess) {
+    b->owns_prims = false;
+    free_batch(b);
+    return hip_fail(c, e, "batch offsets");
   }
+  if (tuning) {
+    // every candidate twice on this batch; the second run is timed
+    float best = -1.0f;
+    const size_t first = c->tune_log.size();
+    SortConfig pick = cands[0];
+    for (const SortConfig& cand : cands) {
+      b->cfg = cand;
+      b->passes = sort_passes(b->key_bits, cand.digit_bits);
+      float ms = -1.0f;
+      for (int rep = 0; rep < 2 && st == LBVH_OK; rep++) {
+        ms = time_call(c, [&] {
+          begin_timed(c, 1);
+          hipError_t qe = queue_build(c, b, true);
+          return qe == hipSuccess ? LBVH_OK : hip_fail(c, qe, "batch build");
+        }, &st);
+        if (st == LBVH_OK)
+          st = sync_and_check(c);
+      }
+      if (st != LBVH_OK) {
+        b->owns_prims = false;
+        free_batch(b);
+        return st;
+      }
+      c->tune_log.push_back(TuneRecord{ 1, b->key_bits, ceil_log2_u64(n), cand, ms, false });
+      if (ms >= 0.0f && (best < 0.0f || ms < best)) {
+        best = ms;
+        pick = cand;
+      }
+    }
+    for (size_t i = first; i < c->tune_log.size(); i++) {
+      const SortConfig& r = c->tune_log[i].cfg;
+      c->tune_log[i].picked =
+        r.digit_bits == pick.digit_bits && r.items == pick.items && r.strategy == pick.strategy;
+    }
+    c->tuned[tune_key(true, b->key_bits, n)] = pick;
+    b->cfg = pick;
+    b->passes = sort_passes(b->key_bits, pick.digit_bits);
+  }
+  begin_timed(c, 1);
+  e = queue_build(c, b, true);
   if (e != hipSuccess) {
     b->owns_prims = false;
     free_batch(b);

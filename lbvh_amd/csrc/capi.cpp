@@ -39,6 +39,7 @@ struct lbvh_bvh
   uint32_t* d_other_end = nullptr; // other end of each internal node's leaf range
   void* d_wide = nullptr;          // traversal records (64 B per internal node)
   uint64_t bytes = 0;
+  SortConfig sort;                 // the sort configuration the build used
 };
 
 namespace lbvh_rt {
@@ -139,24 +140,67 @@ record(lbvh_ctx* c, int idx, bool final)
   es.recorded = idx + 1;
 }
 
-// Sort tuning: digit width and tile by size and key width (see
-// tune_sort_config callers; overridable through the build options).
 SortConfig
-tune_sort(uint64_t n, uint32_t code_bits, const lbvh_build_opts* o)
+default_sort()
 {
   SortConfig c;
   c.digit_bits = 8;
   c.items = 16;
   c.strategy = 1;
-  (void)n;
-  (void)code_bits;
-  if (o && o->sort_digit_bits)
-    c.digit_bits = int(o->sort_digit_bits);
-  if (o && o->sort_tile)
-    c.items = int(o->sort_tile / 256u);
-  if (o && o->sort_strategy)
-    c.strategy = int(o->sort_strategy) - 1;
   return c;
+}
+
+std::vector<SortConfig>
+sort_candidates(bool batch)
+{
+  std::vector<SortConfig> v;
+  // reduce-then-scan at both digit widths and tiles; onesweep only for
+  // single builds (its look-back state is tied to an epoch per build, so a
+  // replayed graph could not use it)
+  for (int strategy = 1; strategy >= (batch ? 1 : 0); strategy--)
+    for (int digit : { 8, 10 })
+      for (int items : { 8, 16 }) {
+        SortConfig c;
+        c.digit_bits = digit;
+        c.items = items;
+        c.strategy = strategy;
+        v.push_back(c);
+      }
+  return v;
+}
+
+bool
+sort_overridden(const lbvh_build_opts& o)
+{
+  return o.sort_digit_bits != 0 || o.sort_tile != 0 || o.sort_strategy != 0;
+}
+
+SortConfig
+sort_from_opts(const lbvh_build_opts& o)
+{
+  SortConfig c = default_sort();
+  if (o.sort_digit_bits)
+    c.digit_bits = int(o.sort_digit_bits);
+  if (o.sort_tile)
+    c.items = int(o.sort_tile / 256u);
+  if (o.sort_strategy)
+    c.strategy = int(o.sort_strategy) - 1;
+  return c;
+}
+
+int
+ceil_log2_u64(uint64_t x)
+{
+  int b = 0;
+  while ((uint64_t(1) << b) < x)
+    b++;
+  return b;
+}
+
+uint64_t
+tune_key(bool batch, int key_bits, uint64_t n)
+{
+  return (uint64_t(batch) << 40) | (uint64_t(key_bits) << 32) | uint64_t(ceil_log2_u64(n));
 }
 
 bool
@@ -230,9 +274,65 @@ free_bvh(lbvh_bvh* b)
   delete b;
 }
 
+lbvh_status build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
+                         const lbvh_build_opts* opts_in, bool owns, lbvh_bvh** out,
+                         const SortConfig* forced = nullptr);
+
+// The sort configuration of a single-scene build: explicit options, else
+// the context's tuned choice for this key width and size, tuning now (every
+// candidate timed on this very build, twice) when there is none yet.
+lbvh_status
+choose_sort(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n, const lbvh_build_opts& opts,
+            SortConfig* cfg)
+{
+  if (sort_overridden(opts)) {
+    *cfg = sort_from_opts(opts);
+    return LBVH_OK;
+  }
+  *cfg = default_sort();
+  if (!c->autotune || n < kTuneMinPrims)
+    return LBVH_OK;
+  const int key_bits = opts.code_bits == 30 ? 30 : 60;
+  const uint64_t key = tune_key(false, key_bits, n);
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) {
+    *cfg = it->second;
+    return LBVH_OK;
+  }
+  float best = -1.0f;
+  size_t first = c->tune_log.size();
+  for (const SortConfig& cand : sort_candidates(false)) {
+    float ms = -1.0f;
+    for (int rep = 0; rep < 2; rep++) {
+      lbvh_bvh* tmp = nullptr;
+      lbvh_status st;
+      ms = time_call(c, [&] { return build_common(c, d_prims, kind, n, &opts, false, &tmp, &cand); },
+                     &st);
+      if (tmp)
+        free_bvh(tmp);
+      if (st != LBVH_OK)
+        return st;
+      if ((st = sync_and_check(c)) != LBVH_OK)
+        return st;
+    }
+    c->tune_log.push_back(TuneRecord{ 0, key_bits, ceil_log2_u64(n), cand, ms, false });
+    if (ms >= 0.0f && (best < 0.0f || ms < best)) {
+      best = ms;
+      *cfg = cand;
+    }
+  }
+  for (size_t i = first; i < c->tune_log.size(); i++) {
+    const SortConfig& r = c->tune_log[i].cfg;
+    c->tune_log[i].picked = r.digit_bits == cfg->digit_bits && r.items == cfg->items &&
+                            r.strategy == cfg->strategy;
+  }
+  c->tuned[key] = *cfg;
+  return LBVH_OK;
+}
+
 lbvh_status
 build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
-             const lbvh_build_opts* opts_in, bool owns, lbvh_bvh** out)
+             const lbvh_build_opts* opts_in, bool owns, lbvh_bvh** out, const SortConfig* forced)
 {
   lbvh_build_opts opts;
   lbvh_build_opts_init(&opts);
@@ -246,7 +346,14 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
     return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
   if (n >= (uint64_t(1) << 29))
     return fail(c, LBVH_E_INVALID, "at most 2^29 - 1 primitives per tree");
-  SortConfig cfg = tune_sort(n, opts.code_bits, &opts);
+  SortConfig cfg;
+  if (forced) {
+    cfg = *forced;
+  } else {
+    lbvh_status st0 = choose_sort(c, d_prims, kind, n, opts, &cfg);
+    if (st0 != LBVH_OK)
+      return st0;
+  }
   if (!valid_sort_config(cfg))
     return fail(c, LBVH_E_INVALID, "unsupported sort digit width / tile");
   const int key_bits = opts.code_bits == 30 ? 30 : 60;
@@ -302,6 +409,7 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   b->code_bits = opts.code_bits;
   b->leaf_order = opts.leaf_order;
   b->prim_kind = kind;
+  b->sort = cfg;
   b->d_prims = const_cast<float*>(d_prims);
   b->owns_prims = owns;
   if ((st = alloc_bvh(c, b, opts.keep_codes != 0)) != LBVH_OK) {
@@ -740,7 +848,40 @@ lbvh_bvh_get_info(const lbvh_bvh* b, lbvh_bvh_info* out)
   out->d_nodes = b->d_nodes;
   out->d_prims = b->prim_kind == 0 ? b->d_prims : nullptr;
   out->device_bytes = b->bytes;
+  out->sort_digit_bits = uint32_t(b->sort.digit_bits);
+  out->sort_tile = uint32_t(b->sort.items * 256);
+  out->sort_strategy = uint32_t(b->sort.strategy + 1);
+  out->reserved = 0;
   return LBVH_OK;
+}
+
+lbvh_status
+lbvh_ctx_set_autotune(lbvh_ctx* c, int enable)
+{
+  if (!c)
+    return LBVH_E_INVALID;
+  c->autotune = enable != 0;
+  return LBVH_OK;
+}
+
+uint32_t
+lbvh_ctx_tune_log(const lbvh_ctx* c, lbvh_tune_record* out, uint32_t cap)
+{
+  if (!c)
+    return 0;
+  const uint32_t n = uint32_t(c->tune_log.size());
+  for (uint32_t i = 0; out && i < n && i < cap; i++) {
+    const TuneRecord& r = c->tune_log[i];
+    out[i].batch = uint32_t(r.batch);
+    out[i].key_bits = uint32_t(r.key_bits);
+    out[i].log2_n = uint32_t(r.log2n);
+    out[i].sort_digit_bits = uint32_t(r.cfg.digit_bits);
+    out[i].sort_tile = uint32_t(r.cfg.items * 256);
+    out[i].sort_strategy = uint32_t(r.cfg.strategy + 1);
+    out[i].picked = r.picked ? 1u : 0u;
+    out[i].ms = r.ms;
+  }
+  return n;
 }
 
 lbvh_status

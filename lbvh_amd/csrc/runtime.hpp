@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,17 @@ constexpr uint32_t kSmallTreeletCount = kSmallTickets + kMaxSortPasses;
 constexpr uint32_t kSmallTreeletQueue = kSmallTreeletCount + 1;
 constexpr uint32_t kSmallHist = kSmallTreeletQueue + 1;
 
+// One timed candidate of the sort autotuner.
+struct TuneRecord
+{
+  int batch;
+  int key_bits;
+  int log2n;
+  SortConfig cfg;
+  float ms; // device time of the whole build with this configuration
+  bool picked;
+};
+
 } // namespace lbvh_rt
 
 struct lbvh_ctx
@@ -50,6 +62,9 @@ struct lbvh_ctx
   size_t used = 0;
   std::string last_error;
   uint32_t epoch_next = 1;
+  bool autotune = true;
+  std::map<uint64_t, lbvh_eng::SortConfig> tuned; // by tune_key()
+  std::vector<lbvh_rt::TuneRecord> tune_log;
   // workspace
   lbvh_rt::DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, part_hist, small, status, counts,
     treelets, io_in, io_out;
@@ -64,9 +79,24 @@ void dev_free(void* p);
 lbvh_status set_device(lbvh_ctx* c);
 void begin_timed(lbvh_ctx* c, int kind);
 void record(lbvh_ctx* c, int idx, bool final = false);
-SortConfig tune_sort(uint64_t n, uint32_t code_bits, const lbvh_build_opts* o);
 bool valid_sort_config(const SortConfig& c);
 lbvh_status sync_and_check(lbvh_ctx* c);
+
+// ---- sort autotuner ----------------------------------------------------------
+// The default configuration, the candidates timed by the tuner, and whether
+// build options override the choice (then the tuner is not consulted).
+SortConfig default_sort();
+std::vector<SortConfig> sort_candidates(bool batch);
+bool sort_overridden(const lbvh_build_opts& o);
+SortConfig sort_from_opts(const lbvh_build_opts& o);
+uint64_t tune_key(bool batch, int key_bits, uint64_t n);
+int ceil_log2_u64(uint64_t x);
+// Device milliseconds of the work `queue` puts on the context stream between
+// its begin_timed()/record() calls (per-call timing forced on meanwhile).
+template<class Q>
+float time_call(lbvh_ctx* c, const Q& queue, lbvh_status* st);
+// Sizes below this are launch-bound: the default is used without tuning.
+constexpr uint64_t kTuneMinPrims = 1u << 16;
 
 } // namespace lbvh_rt
 
@@ -76,3 +106,23 @@ lbvh_status sync_and_check(lbvh_ctx* c);
     if (e_ != hipSuccess)                                                      \
       return lbvh_rt::hip_fail((ctx), e_, #expr);                              \
   } while (0)
+
+template<class Q>
+float
+lbvh_rt::time_call(lbvh_ctx* c, const Q& queue, lbvh_status* st)
+{
+  const int saved_timing = c->timing;
+  const size_t saved_used = c->used;
+  c->timing = 2;
+  *st = queue();
+  float ms = -1.0f;
+  if (*st == LBVH_OK && c->used > saved_used) {
+    EventSet& es = c->sets[c->used - 1];
+    if (es.recorded >= 2 && hipEventSynchronize(es.ev[1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, es.ev[0], es.ev[1]) != hipSuccess)
+      ms = -1.0f;
+  }
+  c->timing = saved_timing;
+  c->used = saved_used;
+  return ms;
+}

@@ -319,3 +319,45 @@ def test_phase_timing_reports_four_phases(ctx):
     ms = ctx.phase_ms()
     ctx.set_timing(False)
     assert len(ms) == 4 and all(x >= 0 for x in ms)
+
+
+def test_autotuner_times_candidates_and_keeps_results_exact():
+    """A fresh context tunes the sort on the first build of a size bucket:
+    every candidate is timed, one is kept, and the tree equals the oracle's
+    whatever configuration was picked."""
+    c = lbvh_amd.Context(0)
+    v = W.cluster_vertices(91, 200000)
+    tri = W.make_triangles(v)
+    bvh = c.build_triangles(tri, code_bits=63, leaf_order=1, keep_codes=True)
+    log = c.tune_log()
+    assert len(log) == 8 and all(r["ms"] > 0 for r in log)
+    picked = [r for r in log if r["picked"]]
+    assert len(picked) == 1
+    inf = bvh.info()
+    assert inf.sort_digit_bits == picked[0]["digit_bits"]
+    assert inf.sort_tile == picked[0]["tile"]
+    on, _, osc, oids = oracle.build(tri, 63, 1)
+    assert bvh.nodes().tobytes() == on.tobytes()
+    sc, ids, _ = bvh.codes()
+    assert np.array_equal(sc, osc) and np.array_equal(ids, oids)
+    # the same bucket again: no new tuning
+    bvh2 = c.build_triangles(tri, code_bits=63, leaf_order=1)
+    assert len(c.tune_log()) == 8
+    assert bvh2.info().sort_tile == inf.sort_tile
+    # a batch of the same size tunes its own candidates (reduce-then-scan only)
+    sizes = W.loguniform_sizes(5, 64, 1000, 8000)
+    btri, off = W.batched_scenes(seed=5, sizes=sizes)
+    b = c.build_batch(btri, off, leaf_order=1)
+    blog = [r for r in c.tune_log() if r["batch"]]
+    assert len(blog) == 4 and sum(r["picked"] for r in blog) == 1
+    for s in (0, 31, 63):
+        on, _, _, _ = oracle.build(btri[int(off[s]):int(off[s + 1])], 30, 1)
+        assert b.nodes(s).tobytes() == on.tobytes()
+    # explicit options bypass the tuner
+    c.set_autotune(False)
+    bvh3 = c.build_triangles(W.make_triangles(W.cluster_vertices(92, 1 << 19)), code_bits=30)
+    assert len(c.tune_log()) == 12
+    assert bvh3.info().sort_strategy in (1, 2)
+    for x in (bvh, bvh2, bvh3, b):
+        x.destroy()
+    c.close()

@@ -284,6 +284,96 @@ lbvh_status lbvh_batch_trace_closest_device(lbvh_ctx* ctx,
                                             uint32_t* d_prim, float* d_t,
                                             float* d_uv);
 
+/* -- serving ------------------------------------------------------------------
+ * A request server for one GPU with continuous batching.  A request names a
+ * scene -- a new one (its triangles), or a resident one, optionally with new
+ * triangle data (a refit on the unchanged topology) -- and carries a ray
+ * batch; its result is the closest hit of every ray, as lbvh_trace_closest
+ * gives it on that scene.  Built trees stay resident.  Requests are admitted
+ * in arrival order as in-flight slots and device memory (under
+ * memory_cap_bytes) free up; idle resident scenes are evicted least recently
+ * used first when a new scene needs room.  Admitted requests build, refit and
+ * upload on several streams at once, and every request whose scene is ready
+ * joins the next traversal launch, which covers all of their trees and rays.
+ * Requests on one scene keep arrival order (a refit affects exactly the
+ * requests submitted after it), so each result equals the result of serving
+ * the requests one at a time.  The reference has no server; its traversal
+ * (lbvh.h:1869-1942) and builder (lbvh.h:1793-1822) are what each request
+ * runs.  Host buffers named by a request must stay valid until the request
+ * is collected by lbvh_server_wait or lbvh_server_poll. */
+typedef struct lbvh_server lbvh_server;
+
+typedef struct lbvh_server_opts
+{
+  uint64_t memory_cap_bytes; /* device bytes for scenes and requests (288 GiB,
+                                lowered to the device's free memory) */
+  uint32_t lanes;            /* build / refit / upload streams (4) */
+  uint32_t max_inflight;     /* admitted requests at once (64) */
+  uint32_t code_bits;        /* 30 or 63 (63) */
+  uint32_t leaf_order;       /* (LBVH_LEAF_PRIMITIVE_ID) */
+  uint32_t isect_mode;       /* LBVH_ISECT_* (REFERENCE) */
+  uint32_t reserved[5];
+} lbvh_server_opts;
+
+typedef struct lbvh_request
+{
+  uint64_t scene;      /* 0: a new scene from tri48; else a resident scene id */
+  const float* tri48;  /* new scene: its triangles; resident scene: NULL, or
+                          new triangle data of the same count (refit) */
+  uint64_t n_tri;
+  const float* rays32; /* host rays */
+  uint64_t n_rays;
+  uint32_t* prim;      /* host outputs, as lbvh_trace_closest */
+  float* t;
+  float* uv;           /* may be NULL */
+} lbvh_request;
+
+typedef struct lbvh_request_info
+{
+  uint64_t scene;
+  lbvh_status status;
+  uint32_t coalesced_with; /* requests in the traversal launch it joined */
+  double submit_us;        /* host clock, microseconds since server creation */
+  double admit_us;
+  double ready_us;         /* its build / refit / upload finished */
+  double done_us;          /* results in the caller's buffers */
+} lbvh_request_info;
+
+typedef struct lbvh_server_stats
+{
+  uint64_t completed;
+  uint64_t failed;
+  uint64_t launches;           /* traversal launches */
+  uint64_t coalesced_requests; /* requests summed over launches */
+  uint64_t max_coalesced;
+  uint64_t rays;
+  uint64_t evictions;
+  uint64_t resident_scenes;
+  uint64_t memory_cap_bytes;
+  uint64_t memory_used_bytes;
+  uint64_t memory_peak_bytes;
+} lbvh_server_stats;
+
+void lbvh_server_opts_init(lbvh_server_opts* o);
+lbvh_status lbvh_server_create(int device, const lbvh_server_opts* opts,
+                               lbvh_server** out);
+/* Finishes every submitted request, then frees everything. */
+void lbvh_server_destroy(lbvh_server* server);
+/* Queues a request; returns at once with its ticket and its scene id (the
+ * new scene's id for a new scene). */
+lbvh_status lbvh_server_submit(lbvh_server* server, const lbvh_request* req,
+                               uint64_t* ticket, uint64_t* scene);
+/* Waits for a request and collects it: its status is returned. */
+lbvh_status lbvh_server_wait(lbvh_server* server, uint64_t ticket,
+                             lbvh_request_info* info);
+/* Collects a finished request (its status), or LBVH_E_BUSY while it runs. */
+lbvh_status lbvh_server_poll(lbvh_server* server, uint64_t ticket,
+                             lbvh_request_info* info);
+/* Marks an idle resident scene as first to evict (LBVH_E_BUSY while
+ * requests name it). */
+lbvh_status lbvh_server_release_scene(lbvh_server* server, uint64_t scene);
+lbvh_status lbvh_server_get_stats(lbvh_server* server, lbvh_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

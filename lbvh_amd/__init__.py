@@ -72,6 +72,34 @@ class BatchInfo(ctypes.Structure):
                 ("d_prims", ctypes.c_void_p)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("memory_cap_bytes", ctypes.c_uint64), ("lanes", ctypes.c_uint32),
+                ("max_inflight", ctypes.c_uint32), ("code_bits", ctypes.c_uint32),
+                ("leaf_order", ctypes.c_uint32), ("isect_mode", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 5)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_uint64), ("tri48", ctypes.c_void_p),
+                ("n_tri", ctypes.c_uint64), ("rays32", ctypes.c_void_p),
+                ("n_rays", ctypes.c_uint64), ("prim", ctypes.c_void_p), ("t", ctypes.c_void_p),
+                ("uv", ctypes.c_void_p)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_uint64), ("status", ctypes.c_int),
+                ("coalesced_with", ctypes.c_uint32), ("submit_us", ctypes.c_double),
+                ("admit_us", ctypes.c_double), ("ready_us", ctypes.c_double),
+                ("done_us", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in (
+        "completed", "failed", "launches", "coalesced_requests", "max_coalesced", "rays",
+        "evictions", "resident_scenes", "memory_cap_bytes", "memory_used_bytes",
+        "memory_peak_bytes")]
+
+
 VP = ctypes.c_void_p
 U64 = ctypes.c_uint64
 F32P = ctypes.POINTER(ctypes.c_float)
@@ -120,6 +148,16 @@ SIGNATURES = {
     "lbvh_batch_trace_closest": (ctypes.c_int, [VP, VP, VP, VP, ctypes.c_int, VP, VP, VP]),
     "lbvh_batch_trace_closest_device": (ctypes.c_int, [VP, VP, VP, VP, U64, ctypes.c_int, VP,
                                                        VP, VP]),
+    "lbvh_server_opts_init": (None, [ctypes.POINTER(ServerOpts)]),
+    "lbvh_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ServerOpts),
+                                          ctypes.POINTER(VP)]),
+    "lbvh_server_destroy": (None, [VP]),
+    "lbvh_server_submit": (ctypes.c_int, [VP, ctypes.POINTER(Request), ctypes.POINTER(U64),
+                                          ctypes.POINTER(U64)]),
+    "lbvh_server_wait": (ctypes.c_int, [VP, U64, ctypes.POINTER(RequestInfo)]),
+    "lbvh_server_poll": (ctypes.c_int, [VP, U64, ctypes.POINTER(RequestInfo)]),
+    "lbvh_server_release_scene": (ctypes.c_int, [VP, U64]),
+    "lbvh_server_get_stats": (ctypes.c_int, [VP, ctypes.POINTER(ServerStats)]),
 }
 
 _lib = None
@@ -353,6 +391,98 @@ class Batch:
     def __del__(self):
         try:
             self.destroy()
+        except Exception:
+            pass
+
+
+class Server:
+    """A request server with continuous batching (lbvh_server).  submit()
+    returns at once; wait() / poll() collect a request's (prim, t, uv, info)."""
+
+    def __init__(self, device=0, memory_cap_bytes=None, lanes=None, max_inflight=None,
+                 code_bits=None, leaf_order=None, isect_mode=None):
+        o = ServerOpts()
+        lib().lbvh_server_opts_init(ctypes.byref(o))
+        for k, v in (("memory_cap_bytes", memory_cap_bytes), ("lanes", lanes),
+                     ("max_inflight", max_inflight), ("code_bits", code_bits),
+                     ("leaf_order", leaf_order), ("isect_mode", isect_mode)):
+            if v is not None:
+                setattr(o, k, v)
+        h = ctypes.c_void_p()
+        st = lib().lbvh_server_create(device, ctypes.byref(o), ctypes.byref(h))
+        if st != 0:
+            raise LbvhError(st, "lbvh_server_create")
+        self.handle = h
+        self._live = {}  # ticket -> host arrays the request reads and writes
+
+    def submit(self, rays, scene=0, tri=None, with_uv=True):
+        """scene 0 with tri: a new scene; a scene id with tri: refit, then
+        trace; a scene id alone: trace.  Returns (ticket, scene id)."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        m = r.shape[0]
+        prim = np.empty(m, dtype=np.uint32)
+        t = np.empty(m, dtype=np.float32)
+        uv = np.empty((m, 2), dtype=np.float32) if with_uv else None
+        req = Request()
+        req.scene = scene
+        tri_a = None
+        if tri is not None:
+            tri_a = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 12)
+            req.tri48 = tri_a.ctypes.data
+            req.n_tri = tri_a.shape[0]
+        req.rays32 = r.ctypes.data if m else None
+        req.n_rays = m
+        req.prim = prim.ctypes.data
+        req.t = t.ctypes.data
+        req.uv = uv.ctypes.data if uv is not None else None
+        ticket, sid = U64(), U64()
+        st = lib().lbvh_server_submit(self.handle, ctypes.byref(req), ctypes.byref(ticket),
+                                      ctypes.byref(sid))
+        if st != 0:
+            raise LbvhError(st, "lbvh_server_submit")
+        self._live[ticket.value] = (r, tri_a, prim, t, uv)
+        return ticket.value, sid.value
+
+    def _collect(self, fn, ticket):
+        info = RequestInfo()
+        st = fn(self.handle, ticket, ctypes.byref(info))
+        if st == 4 and fn is lib().lbvh_server_poll:
+            return None
+        _, _, prim, t, uv = self._live.pop(ticket)
+        if st != 0:
+            raise LbvhError(st, "request %d" % ticket)
+        return prim, t, uv, {"scene": info.scene, "coalesced_with": info.coalesced_with,
+                             "submit_us": info.submit_us, "admit_us": info.admit_us,
+                             "ready_us": info.ready_us, "done_us": info.done_us}
+
+    def wait(self, ticket):
+        return self._collect(lib().lbvh_server_wait, ticket)
+
+    def poll(self, ticket):
+        """None while the request runs."""
+        return self._collect(lib().lbvh_server_poll, ticket)
+
+    def release_scene(self, scene):
+        st = lib().lbvh_server_release_scene(self.handle, scene)
+        if st != 0:
+            raise LbvhError(st, "lbvh_server_release_scene")
+
+    def stats(self):
+        out = ServerStats()
+        st = lib().lbvh_server_get_stats(self.handle, ctypes.byref(out))
+        if st != 0:
+            raise LbvhError(st, "lbvh_server_get_stats")
+        return {k: int(getattr(out, k)) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().lbvh_server_destroy(self.handle)
+            self.handle = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

@@ -103,6 +103,27 @@ hipError_t launch_trace(int mode, const void* wide, const float* tris, uint64_t 
                         const float* rays, uint64_t nrays, uint32_t* prim, float* t, float* uv,
                         hipStream_t s);
 
+// One tree's share of a coalesced traversal launch: its rays are global rays
+// [ray_begin, next job's ray_begin) (the last job ends at the launch's ray
+// count) and read from / written to the job's own buffers.
+struct TraceJob
+{
+  const void* nodes; // 64-byte traversal records
+  const float* tris;
+  const float* rays;
+  uint32_t* prim;
+  float* t;
+  float* uv; // may be null
+  uint64_t ray_begin;
+  uint32_t nprims;
+  uint32_t pad;
+};
+
+// Closest hit over many independent trees in one launch; `jobs` is device
+// memory, sorted by ray_begin with jobs[0].ray_begin = 0.
+hipError_t launch_trace_jobs(int mode, const TraceJob* jobs, uint32_t njobs, uint64_t nrays,
+                             hipStream_t s);
+
 // Batched centroid bounds: the centre (SoA) of every primitive and per-scene
 // centroid bounds as order-preserving integers (6 per scene: min xyz, max
 // xyz; launch_scene_bounds_init resets them first).

@@ -70,6 +70,30 @@ struct lbvh_ctx
     treelets, io_in, io_out;
 };
 
+// A built tree: node array, parent links, fit flags, centroid bounds,
+// traversal records, and the primitives (owned copy or borrowed pointer).
+struct lbvh_bvh
+{
+  int device = 0;
+  uint64_t n = 0;
+  uint32_t code_bits = 30;
+  uint32_t leaf_order = 0;
+  int prim_kind = 0; // 0 triangles, 1 aabbs
+  void* d_nodes = nullptr;
+  uint32_t* d_parent = nullptr;
+  uint32_t* d_flags = nullptr;
+  float* d_bounds = nullptr;
+  float* d_prims = nullptr;
+  bool owns_prims = false;
+  void* d_codes = nullptr;
+  uint32_t* d_ids = nullptr;
+  uint32_t* d_leafprim = nullptr; // primitive of each leaf position (primitive-id leaves)
+  uint32_t* d_other_end = nullptr; // other end of each internal node's leaf range
+  void* d_wide = nullptr;          // traversal records (64 B per internal node)
+  uint64_t bytes = 0;
+  lbvh_eng::SortConfig sort;       // the sort configuration the build used
+};
+
 namespace lbvh_rt {
 
 lbvh_status fail(lbvh_ctx* c, lbvh_status s, const std::string& msg);

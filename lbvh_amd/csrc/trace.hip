@@ -13,6 +13,8 @@
 
 namespace lbvh_dev {
 
+using lbvh_eng::TraceJob;
+
 constexpr int kTraceBlock = 256;
 constexpr int kStack = 128;
 
@@ -217,26 +219,51 @@ struct Stack
   }
 };
 
-// SEG: a batch of scenes; ray i belongs to the scene s with ray_off[s] <= i
-// < ray_off[s + 1], whose nodes start at prim_off[s] - s and whose triangles
-// start at prim_off[s]; reported ids are scene-local.
-template<int MODE, bool SEG>
+// Ray layouts: kLayoutSingle, one tree; kLayoutBatch, a scene batch (ray i
+// belongs to the scene s with ray_off[s] <= i < ray_off[s + 1], whose nodes
+// start at prim_off[s] - s and whose triangles start at prim_off[s]; ids are
+// scene-local); kLayoutJobs, independent trees with their own ray and output
+// buffers (job k covers global rays [jobs[k].ray_begin, jobs[k + 1].ray_begin)).
+constexpr int kLayoutSingle = 0;
+constexpr int kLayoutBatch = 1;
+constexpr int kLayoutJobs = 2;
+
+template<int MODE, int LAYOUT>
 __global__ __launch_bounds__(kTraceBlock) void
 k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
         const uint64_t* __restrict__ prim_off, const uint64_t* __restrict__ ray_off,
-        uint32_t nseg, const float* __restrict__ rays, uint64_t nrays,
-        uint32_t* __restrict__ prim_out, float* __restrict__ t_out, float* __restrict__ uv_out)
+        uint32_t nseg, const TraceJob* __restrict__ jobs, const float* __restrict__ rays,
+        uint64_t nrays, uint32_t* __restrict__ prim_out, float* __restrict__ t_out,
+        float* __restrict__ uv_out)
 {
   __shared__ uint2 s_ring[(kStackLds > 0 ? kStackLds : 1) * kTraceBlock];
-  const uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+  uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
   if (i >= nrays)
     return;
-  if constexpr (SEG) {
+  if constexpr (LAYOUT == kLayoutBatch) {
     const uint32_t sc = find_scene(ray_off, nseg, int64_t(i), 1, false);
     const uint64_t P = prim_off[sc];
     nprims = uint32_t(prim_off[sc + 1] - P);
     nodes += P - sc;
     tris += 12 * P;
+  } else if constexpr (LAYOUT == kLayoutJobs) {
+    uint32_t lo = 0, hi = nseg; // jobs[lo].ray_begin <= i < jobs[hi].ray_begin
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (jobs[mid].ray_begin <= i)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const TraceJob& jb = jobs[lo];
+    nodes = static_cast<const WideNode*>(jb.nodes);
+    tris = jb.tris;
+    nprims = jb.nprims;
+    rays = jb.rays;
+    prim_out = jb.prim;
+    t_out = jb.t;
+    uv_out = jb.uv;
+    i -= jb.ray_begin;
   }
   const float4* rp = reinterpret_cast<const float4*>(rays + 8 * i);
   const float4 r0 = rp[0], r1 = rp[1];
@@ -363,15 +390,44 @@ using namespace lbvh_dev;
 template<int MODE>
 static void
 trace_mode(uint32_t g, hipStream_t s, const WideNode* nd, const float* tris, uint64_t n,
-           const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg, const float* rays,
-           uint64_t nrays, uint32_t* prim, float* t, float* uv)
+           const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg, const TraceJob* jobs,
+           const float* rays, uint64_t nrays, uint32_t* prim, float* t, float* uv)
 {
-  if (prim_off)
-    hipLaunchKernelGGL((k_trace<MODE, true>), dim3(g), dim3(kTraceBlock), 0, s, nd, tris,
-                       uint32_t(n), prim_off, ray_off, nseg, rays, nrays, prim, t, uv);
+  if (jobs)
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutJobs>), dim3(g), dim3(kTraceBlock), 0, s, nullptr,
+                       nullptr, 0u, nullptr, nullptr, nseg, jobs, nullptr, nrays, nullptr, nullptr,
+                       nullptr);
+  else if (prim_off)
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutBatch>), dim3(g), dim3(kTraceBlock), 0, s, nd, tris,
+                       uint32_t(n), prim_off, ray_off, nseg, nullptr, rays, nrays, prim, t, uv);
   else
-    hipLaunchKernelGGL((k_trace<MODE, false>), dim3(g), dim3(kTraceBlock), 0, s, nd, tris,
-                       uint32_t(n), nullptr, nullptr, 1u, rays, nrays, prim, t, uv);
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutSingle>), dim3(g), dim3(kTraceBlock), 0, s, nd, tris,
+                       uint32_t(n), nullptr, nullptr, 1u, nullptr, rays, nrays, prim, t, uv);
+}
+
+static hipError_t
+trace_dispatch(int mode, const void* nodes, const float* tris, uint64_t n, const uint64_t* prim_off,
+               const uint64_t* ray_off, uint32_t nseg, const TraceJob* jobs, const float* rays,
+               uint64_t nrays, uint32_t* prim, float* t, float* uv, hipStream_t s)
+{
+  if (nrays == 0)
+    return hipSuccess;
+  const uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
+  const WideNode* nd = static_cast<const WideNode*>(nodes);
+  switch (mode) {
+    case 0:
+      trace_mode<0>(g, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, nrays, prim, t, uv);
+      break;
+    case 1:
+      trace_mode<1>(g, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, nrays, prim, t, uv);
+      break;
+    case 2:
+      trace_mode<2>(g, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, nrays, prim, t, uv);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t
@@ -379,26 +435,19 @@ launch_trace(int mode, const void* nodes, const float* tris, uint64_t n, const u
              const uint64_t* ray_off, uint32_t nseg, const float* rays, uint64_t nrays,
              uint32_t* prim, float* t, float* uv, hipStream_t s)
 {
-  if (nrays == 0)
-    return hipSuccess;
   if (prim_off && (!ray_off || nseg == 0))
     return hipErrorInvalidValue;
-  const uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
-  const WideNode* nd = static_cast<const WideNode*>(nodes);
-  switch (mode) {
-    case 0:
-      trace_mode<0>(g, s, nd, tris, n, prim_off, ray_off, nseg, rays, nrays, prim, t, uv);
-      break;
-    case 1:
-      trace_mode<1>(g, s, nd, tris, n, prim_off, ray_off, nseg, rays, nrays, prim, t, uv);
-      break;
-    case 2:
-      trace_mode<2>(g, s, nd, tris, n, prim_off, ray_off, nseg, rays, nrays, prim, t, uv);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return trace_dispatch(mode, nodes, tris, n, prim_off, ray_off, nseg, nullptr, rays, nrays, prim,
+                        t, uv, s);
+}
+
+hipError_t
+launch_trace_jobs(int mode, const TraceJob* jobs, uint32_t njobs, uint64_t nrays, hipStream_t s)
+{
+  if (!jobs || njobs == 0)
+    return hipErrorInvalidValue;
+  return trace_dispatch(mode, nullptr, nullptr, 0, nullptr, nullptr, njobs, jobs, nullptr, nrays,
+                        nullptr, nullptr, nullptr, s);
 }
 
 } // namespace lbvh_eng

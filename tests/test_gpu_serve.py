@@ -1,0 +1,163 @@
+"""GPU tests of the request server: every request's hits equal the plain
+path's result for the same request served alone and in order (bit-exact ids,
+t and uv), while requests overlap and share traversal launches; admission
+under a memory cap evicts idle scenes; bad requests are rejected."""
+import numpy as np
+import pytest
+
+import lbvh_amd
+from lbvh_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(seed, n):
+    return W.make_triangles(W.cluster_vertices(seed, n))
+
+
+def _rays(seed, m):
+    """Primary rays of a small pinhole camera, then a random subset."""
+    r = W.primary_rays(96, 96)
+    rng = np.random.default_rng(seed)
+    return r[rng.integers(0, r.shape[0], m)]
+
+
+def _plan():
+    """(kind, scene index, triangles or None, rays) in arrival order."""
+    sizes = [3000, 50000, 200000, 1000, 120000, 7000]
+    tris = [_scene(500 + k, n) for k, n in enumerate(sizes)]
+    rng = np.random.default_rng(7)
+    plan = []
+    made = []
+    for step in range(40):
+        ray_n = int(rng.integers(1, 9000))
+        rays = _rays(step, ray_n)
+        if len(made) < len(sizes) and (step % 4 == 0 or not made):
+            k = len(made)
+            plan.append(("new", k, tris[k], rays))
+            made.append(k)
+        elif step % 5 == 3:
+            k = made[int(rng.integers(0, len(made)))]
+            v = W.perturb(900 + step, W.cluster_vertices(500 + k, sizes[k]), 2e-4)
+            plan.append(("refit", k, W.make_triangles(v), rays))
+        else:
+            k = made[int(rng.integers(0, len(made)))]
+            plan.append(("trace", k, None, rays))
+    return plan
+
+
+def _plain_results(ctx, plan):
+    trees = {}
+    out = []
+    for kind, k, tri, rays in plan:
+        if kind == "new":
+            trees[k] = ctx.build_triangles(tri, code_bits=63, leaf_order=1)
+        elif kind == "refit":
+            ctx.refit_triangles(trees[k], tri)
+        out.append(ctx.trace(trees[k], rays, lbvh_amd.ISECT_REFERENCE))
+    for b in trees.values():
+        b.destroy()
+    return out
+
+
+def test_server_results_equal_plain_path_in_order(ctx):
+    plan = _plan()
+    expect = _plain_results(ctx, plan)
+    srv = lbvh_amd.Server(0)
+    ids = {}
+    tickets = []
+    for kind, k, tri, rays in plan:
+        if kind == "new":
+            t, sid = srv.submit(rays, 0, tri)
+            ids[k] = sid
+        elif kind == "refit":
+            t, _ = srv.submit(rays, ids[k], tri)
+        else:
+            t, _ = srv.submit(rays, ids[k])
+        tickets.append(t)
+    coalesced = 0
+    for i, t in enumerate(tickets):
+        p, tt, uv, info = srv.wait(t)
+        ep, et, euv = expect[i]
+        assert np.array_equal(p, ep), "request %d (%s) ids" % (i, plan[i][0])
+        assert np.array_equal(tt.view(np.uint32), et.view(np.uint32)), "request %d t" % i
+        assert np.array_equal(uv.view(np.uint32), euv.view(np.uint32)), "request %d uv" % i
+        assert info["done_us"] >= info["ready_us"] >= info["admit_us"] >= info["submit_us"]
+        coalesced = max(coalesced, info["coalesced_with"])
+    st = srv.stats()
+    assert st["completed"] == len(plan) and st["failed"] == 0
+    assert st["resident_scenes"] == 6
+    assert st["launches"] <= len(plan)
+    srv.close()
+
+
+def test_server_coalesces_concurrent_ray_batches(ctx):
+    srv = lbvh_amd.Server(0)
+    a_tri, b_tri = _scene(11, 40000), _scene(12, 60000)
+    t, a = srv.submit(_rays(1, 100), 0, a_tri)
+    srv.wait(t)
+    t, b = srv.submit(_rays(2, 100), 0, b_tri)
+    srv.wait(t)
+    ta = ctx.build_triangles(a_tri, code_bits=63, leaf_order=1)
+    tb = ctx.build_triangles(b_tri, code_bits=63, leaf_order=1)
+    burst = [(a if i % 2 == 0 else b, _rays(100 + i, 2000 + 37 * i)) for i in range(24)]
+    tickets = [srv.submit(r, sid)[0] for sid, r in burst]
+    for (sid, r), t in zip(burst, tickets):
+        p, tt, uv, info = srv.wait(t)
+        ep, et, euv = ctx.trace(ta if sid == a else tb, r, lbvh_amd.ISECT_REFERENCE)
+        assert np.array_equal(p, ep)
+        assert np.array_equal(tt.view(np.uint32), et.view(np.uint32))
+    st = srv.stats()
+    assert st["max_coalesced"] >= 2, st
+    assert st["launches"] < 2 + len(burst)
+    ta.destroy()
+    tb.destroy()
+    srv.close()
+
+
+def test_server_memory_cap_evicts_idle_scenes(ctx):
+    cap = 40 << 20
+    srv = lbvh_amd.Server(0, memory_cap_bytes=cap, lanes=2)
+    ids = []
+    for k in range(6):
+        t, sid = srv.submit(_rays(k, 500), 0, _scene(300 + k, 50000))
+        srv.wait(t)
+        ids.append(sid)
+    st = srv.stats()
+    assert st["evictions"] > 0
+    assert st["memory_peak_bytes"] <= cap
+    # the first scene is gone: naming it is an error
+    with pytest.raises(lbvh_amd.LbvhError) as e:
+        srv.submit(_rays(9, 10), ids[0])
+    assert e.value.status == 1
+    # the last scene is resident and still answers correctly
+    ref = ctx.build_triangles(_scene(305, 50000), code_bits=63, leaf_order=1)
+    r = _rays(10, 3000)
+    p, tt, _, _ = srv.wait(srv.submit(r, ids[-1])[0])
+    ep, et, _ = ctx.trace(ref, r, 0)
+    assert np.array_equal(p, ep) and np.array_equal(tt.view(np.uint32), et.view(np.uint32))
+    ref.destroy()
+    # a request larger than the cap fails cleanly
+    t, _ = srv.submit(_rays(11, 10), 0, _scene(99, 400000))
+    with pytest.raises(lbvh_amd.LbvhError) as e:
+        srv.wait(t)
+    assert e.value.status == 2
+    srv.close()
+
+
+def test_server_rejects_bad_requests():
+    srv = lbvh_amd.Server(0)
+    with pytest.raises(lbvh_amd.LbvhError):
+        srv.submit(_rays(1, 10), 0, None)  # a new scene without triangles
+    with pytest.raises(lbvh_amd.LbvhError):
+        srv.submit(_rays(1, 10), 12345)  # unknown scene
+    t, sid = srv.submit(_rays(1, 10), 0, _scene(1, 2000))
+    srv.wait(t)
+    with pytest.raises(lbvh_amd.LbvhError):
+        srv.submit(_rays(1, 10), sid, _scene(1, 1999))  # refit with another count
+    # a request with no rays still builds its scene
+    t, sid2 = srv.submit(np.zeros((0, 8), np.float32), 0, _scene(2, 3000))
+    p, tt, uv, info = srv.wait(t)
+    assert p.shape == (0,)
+    assert srv.stats()["resident_scenes"] == 2
+    srv.close()

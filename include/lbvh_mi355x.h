@@ -315,17 +315,30 @@ typedef struct lbvh_server_opts
   uint32_t reserved[5];
 } lbvh_server_opts;
 
+/* lbvh_request.flags */
+enum
+{
+  /* tri48, rays32 and the outputs are device memory: nothing is staged or
+   * copied; a new scene borrows tri48 (as lbvh_build_triangles_device), a
+   * refit borrows its new triangles in place of the old ones, and results
+   * are in the output buffers when the request completes.  Borrowed memory
+   * must stay valid while the scene is resident. */
+  LBVH_REQUEST_DEVICE = 1
+};
+
 typedef struct lbvh_request
 {
   uint64_t scene;      /* 0: a new scene from tri48; else a resident scene id */
   const float* tri48;  /* new scene: its triangles; resident scene: NULL, or
                           new triangle data of the same count (refit) */
   uint64_t n_tri;
-  const float* rays32; /* host rays */
+  const float* rays32; /* rays */
   uint64_t n_rays;
-  uint32_t* prim;      /* host outputs, as lbvh_trace_closest */
+  uint32_t* prim;      /* outputs, as lbvh_trace_closest */
   float* t;
   float* uv;           /* may be NULL */
+  uint32_t flags;      /* LBVH_REQUEST_* */
+  uint32_t reserved;
 } lbvh_request;
 
 typedef struct lbvh_request_info

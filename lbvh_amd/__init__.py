@@ -83,7 +83,11 @@ class Request(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_uint64), ("tri48", ctypes.c_void_p),
                 ("n_tri", ctypes.c_uint64), ("rays32", ctypes.c_void_p),
                 ("n_rays", ctypes.c_uint64), ("prim", ctypes.c_void_p), ("t", ctypes.c_void_p),
-                ("uv", ctypes.c_void_p)]
+                ("uv", ctypes.c_void_p), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+REQUEST_DEVICE = 1
 
 
 class RequestInfo(ctypes.Structure):
@@ -415,32 +419,56 @@ class Server:
         self.handle = h
         self._live = {}  # ticket -> host arrays the request reads and writes
 
-    def submit(self, rays, scene=0, tri=None, with_uv=True):
+    def submit(self, rays, scene=0, tri=None, with_uv=True, out=None):
         """scene 0 with tri: a new scene; a scene id with tri: refit, then
-        trace; a scene id alone: trace.  Returns (ticket, scene id)."""
-        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
-        m = r.shape[0]
-        prim = np.empty(m, dtype=np.uint32)
-        t = np.empty(m, dtype=np.float32)
-        uv = np.empty((m, 2), dtype=np.float32) if with_uv else None
+        trace; a scene id alone: trace.  numpy rays: host request (results
+        copied back); device rays (DeviceArray or torch CUDA tensor, with
+        device triangles): device request, results in device outputs (`out`
+        = (prim, t, uv) or new DeviceArrays).  Returns (ticket, scene id)."""
         req = Request()
         req.scene = scene
-        tri_a = None
-        if tri is not None:
-            tri_a = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 12)
-            req.tri48 = tri_a.ctypes.data
-            req.n_tri = tri_a.shape[0]
-        req.rays32 = r.ctypes.data if m else None
+        keep = []
+        if _is_device(rays):
+            m = _numel(rays) // 8
+            if tri is not None and not _is_device(tri):
+                raise ValueError("a device request needs device triangles")
+            if out is None:
+                from .hip import DeviceArray
+                out = (DeviceArray(m, np.uint32), DeviceArray(m, np.float32),
+                       DeviceArray((m, 2), np.float32) if with_uv else None)
+            prim, t, uv = out
+            req.flags = REQUEST_DEVICE
+            req.rays32 = _dptr(rays).value if m else None
+            req.prim = _dptr(prim).value
+            req.t = _dptr(t).value
+            req.uv = _dptr(uv).value if uv is not None else None
+            if tri is not None:
+                req.tri48 = _dptr(tri).value
+                req.n_tri = _numel(tri) // 12
+            keep = [rays, tri]
+        else:
+            r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+            m = r.shape[0]
+            prim = np.empty(m, dtype=np.uint32)
+            t = np.empty(m, dtype=np.float32)
+            uv = np.empty((m, 2), dtype=np.float32) if with_uv else None
+            if tri is not None:
+                tri_a = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 12)
+                req.tri48 = tri_a.ctypes.data
+                req.n_tri = tri_a.shape[0]
+                keep.append(tri_a)
+            req.rays32 = r.ctypes.data if m else None
+            req.prim = prim.ctypes.data
+            req.t = t.ctypes.data
+            req.uv = uv.ctypes.data if uv is not None else None
+            keep.append(r)
         req.n_rays = m
-        req.prim = prim.ctypes.data
-        req.t = t.ctypes.data
-        req.uv = uv.ctypes.data if uv is not None else None
         ticket, sid = U64(), U64()
         st = lib().lbvh_server_submit(self.handle, ctypes.byref(req), ctypes.byref(ticket),
                                       ctypes.byref(sid))
         if st != 0:
             raise LbvhError(st, "lbvh_server_submit")
-        self._live[ticket.value] = (r, tri_a, prim, t, uv)
+        self._live[ticket.value] = (keep, prim, t, uv)
         return ticket.value, sid.value
 
     def _collect(self, fn, ticket):
@@ -448,7 +476,7 @@ class Server:
         st = fn(self.handle, ticket, ctypes.byref(info))
         if st == 4 and fn is lib().lbvh_server_poll:
             return None
-        _, _, prim, t, uv = self._live.pop(ticket)
+        _, prim, t, uv = self._live.pop(ticket)
         if st != 0:
             raise LbvhError(st, "request %d" % ticket)
         return prim, t, uv, {"scene": info.scene, "coalesced_with": info.coalesced_with,

@@ -110,6 +110,7 @@ struct Scene
   lbvh_bvh* bvh = nullptr;
   float* d_tri = nullptr;
   uint64_t bytes = 0;     // device bytes held (triangles + tree)
+  bool owns_tri = true;   // false: tri memory borrowed from device requests
   hipEvent_t ev_tree = nullptr;  // after the last build / refit
   hipEvent_t ev_trace = nullptr; // after the last traversal that read it
   bool has_tree_ev = false, has_trace_ev = false;
@@ -217,12 +218,19 @@ fail_req(lbvh_server* s, Req* q, lbvh_status st, const std::string& msg)
 
 // Device bytes a request needs while active (new scenes: the scene itself,
 // which stays resident afterwards and is then accounted to the scene).
+bool
+on_device(const Req* q)
+{
+  return (q->r.flags & LBVH_REQUEST_DEVICE) != 0;
+}
+
 uint64_t
 footprint(const lbvh_server* s, const Req* q)
 {
-  uint64_t b = Pool::size_class(q->r.n_rays * 32) + Pool::size_class(q->r.n_rays * 16);
+  const bool dev = on_device(q);
+  uint64_t b = dev ? 0 : Pool::size_class(q->r.n_rays * 32) + Pool::size_class(q->r.n_rays * 16);
   if (q->creates)
-    b += q->r.n_tri * 48 + tree_bytes_estimate(q->r.n_tri, s->opts.leaf_order) +
+    b += (dev ? 0 : q->r.n_tri * 48) + tree_bytes_estimate(q->r.n_tri, s->opts.leaf_order) +
          workspace_estimate(q->r.n_tri, s->opts.code_bits);
   return b;
 }
@@ -242,7 +250,7 @@ destroy_scene(Scene* sc)
 {
   if (sc->bvh)
     lbvh_bvh_destroy(sc->bvh);
-  if (sc->d_tri)
+  if (sc->d_tri && sc->owns_tri)
     (void)hipFree(sc->d_tri);
   if (sc->ev_tree)
     (void)hipEventDestroy(sc->ev_tree);
@@ -380,42 +388,52 @@ start(lbvh_server* s, Req* q, Scene* sc)
   q->h_in = Buf{};
   if (!q->creates && !sc->bvh)
     return bad(LBVH_E_INVALID, "the scene's build failed");
-  const uint64_t in_bytes = nr * 32 + ntri * 48;
-  if (in_bytes) {
-    q->h_in.p = s->host_pool.get(in_bytes, &q->h_in.cls);
-    if (!q->h_in.p)
-      return bad(LBVH_E_OOM, "pinned staging allocation failed");
+  const bool dev = on_device(q);
+  const float* tri_src = q->r.tri48;
+  if (!dev) {
+    const uint64_t in_bytes = nr * 32 + ntri * 48;
+    if (in_bytes) {
+      q->h_in.p = s->host_pool.get(in_bytes, &q->h_in.cls);
+      if (!q->h_in.p)
+        return bad(LBVH_E_OOM, "pinned staging allocation failed");
+      if (nr)
+        std::memcpy(q->h_in.p, q->r.rays32, nr * 32);
+      if (ntri)
+        std::memcpy(static_cast<char*>(q->h_in.p) + nr * 32, q->r.tri48, ntri * 48);
+    }
+    q->h_out.p = s->host_pool.get(nr * 16 + 16, &q->h_out.cls);
+    q->d_rays.p = s->dev_pool.get(nr * 32 + 32, &q->d_rays.cls);
+    q->d_out.p = s->dev_pool.get(nr * 16 + 16, &q->d_out.cls);
+    if (!q->h_out.p || !q->d_rays.p || !q->d_out.p)
+      return bad(LBVH_E_OOM, "request buffer allocation failed");
     if (nr)
-      std::memcpy(q->h_in.p, q->r.rays32, nr * 32);
-    if (ntri)
-      std::memcpy(static_cast<char*>(q->h_in.p) + nr * 32, q->r.tri48, ntri * 48);
+      if (hipMemcpyAsync(q->d_rays.p, q->h_in.p, nr * 32, hipMemcpyHostToDevice, st) !=
+          hipSuccess)
+        return bad(LBVH_E_HIP, "ray upload failed");
+    tri_src = reinterpret_cast<const float*>(static_cast<char*>(q->h_in.p) + nr * 32);
   }
-  q->h_out.p = s->host_pool.get(nr * 16 + 16, &q->h_out.cls);
-  q->d_rays.p = s->dev_pool.get(nr * 32 + 32, &q->d_rays.cls);
-  q->d_out.p = s->dev_pool.get(nr * 16 + 16, &q->d_out.cls);
-  if (!q->h_out.p || !q->d_rays.p || !q->d_out.p)
-    return bad(LBVH_E_OOM, "request buffer allocation failed");
-  if (nr)
-    if (hipMemcpyAsync(q->d_rays.p, q->h_in.p, nr * 32, hipMemcpyHostToDevice, st) != hipSuccess)
-      return bad(LBVH_E_HIP, "ray upload failed");
   lbvh_build_opts o;
   lbvh_build_opts_init(&o);
   o.code_bits = s->opts.code_bits;
   o.leaf_order = s->opts.leaf_order;
-  const float* tri_src = reinterpret_cast<const float*>(static_cast<char*>(q->h_in.p) + nr * 32);
   if (q->creates) {
-    if (hipMalloc(&sc->d_tri, ntri * 48) != hipSuccess) {
-      (void)hipGetLastError();
-      return bad(LBVH_E_OOM, "scene allocation failed");
+    if (dev) {
+      sc->d_tri = const_cast<float*>(tri_src);
+      sc->owns_tri = false;
+    } else {
+      if (hipMalloc(&sc->d_tri, ntri * 48) != hipSuccess) {
+        (void)hipGetLastError();
+        return bad(LBVH_E_OOM, "scene allocation failed");
+      }
+      if (hipMemcpyAsync(sc->d_tri, tri_src, ntri * 48, hipMemcpyHostToDevice, st) != hipSuccess)
+        return bad(LBVH_E_HIP, "triangle upload failed");
     }
-    if (hipMemcpyAsync(sc->d_tri, tri_src, ntri * 48, hipMemcpyHostToDevice, st) != hipSuccess)
-      return bad(LBVH_E_HIP, "triangle upload failed");
     const uint64_t ws0 = lbvh_ctx_workspace_bytes(lane);
     lbvh_status bs = lbvh_build_triangles_device(lane, sc->d_tri, ntri, &o, &sc->bvh);
     if (bs != LBVH_OK)
       return bad(bs, lbvh_ctx_last_error(lane));
     const uint64_t ws1 = lbvh_ctx_workspace_bytes(lane);
-    sc->bytes = ntri * 48 + sc->bvh->bytes;
+    sc->bytes = (sc->owns_tri ? ntri * 48 : 0) + sc->bvh->bytes;
     std::lock_guard<std::mutex> g(s->mu);
     s->lane_ws += ws1 - ws0;
   } else {
@@ -424,8 +442,13 @@ start(lbvh_server* s, Req* q, Scene* sc)
     if (q->refit) {
       if (sc->has_trace_ev)
         (void)hipStreamWaitEvent(st, sc->ev_trace, 0);
-      if (hipMemcpyAsync(sc->d_tri, tri_src, ntri * 48, hipMemcpyHostToDevice, st) != hipSuccess)
+      if (dev && !sc->owns_tri) {
+        sc->d_tri = const_cast<float*>(tri_src); // borrow the new triangles
+      } else if (hipMemcpyAsync(sc->d_tri, tri_src, ntri * 48,
+                                dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                st) != hipSuccess) {
         return bad(LBVH_E_HIP, "triangle upload failed");
+      }
       lbvh_status rs = lbvh_refit_triangles_device(lane, sc->bvh, sc->d_tri);
       if (rs != LBVH_OK)
         return bad(rs, lbvh_ctx_last_error(lane));
@@ -457,13 +480,20 @@ launch_traversal(lbvh_server* s, std::vector<Req*>& ready)
     Req* q = ready[k];
     Scene* sc = q->sc;
     const uint64_t nr = q->r.n_rays;
-    auto* out = static_cast<uint32_t*>(q->d_out.p);
     jobs[k].nodes = sc->bvh->d_wide;
     jobs[k].tris = sc->d_tri;
-    jobs[k].rays = static_cast<const float*>(q->d_rays.p);
-    jobs[k].prim = out;
-    jobs[k].t = reinterpret_cast<float*>(out + nr);
-    jobs[k].uv = reinterpret_cast<float*>(out + 2 * nr);
+    if (on_device(q)) {
+      jobs[k].rays = q->r.rays32;
+      jobs[k].prim = q->r.prim;
+      jobs[k].t = q->r.t;
+      jobs[k].uv = q->r.uv;
+    } else {
+      auto* out = static_cast<uint32_t*>(q->d_out.p);
+      jobs[k].rays = static_cast<const float*>(q->d_rays.p);
+      jobs[k].prim = out;
+      jobs[k].t = reinterpret_cast<float*>(out + nr);
+      jobs[k].uv = reinterpret_cast<float*>(out + 2 * nr);
+    }
     jobs[k].ray_begin = total;
     jobs[k].nprims = uint32_t(sc->n);
     jobs[k].pad = 0;
@@ -478,8 +508,9 @@ launch_traversal(lbvh_server* s, std::vector<Req*>& ready)
                            total, st) == hipSuccess;
   for (uint32_t k = 0; ok && k < nj; k++) {
     Req* q = ready[k];
-    ok = hipMemcpyAsync(q->h_out.p, q->d_out.p, q->r.n_rays * 16, hipMemcpyDeviceToHost, st) ==
-         hipSuccess;
+    if (!on_device(q))
+      ok = hipMemcpyAsync(q->h_out.p, q->d_out.p, q->r.n_rays * 16, hipMemcpyDeviceToHost, st) ==
+           hipSuccess;
   }
   if (ok)
     ok = hipEventRecord(L->done, st) == hipSuccess;
@@ -508,7 +539,7 @@ void
 finish(lbvh_server* s, Req* q)
 {
   const uint64_t nr = q->r.n_rays;
-  if (q->status == LBVH_OK && nr) {
+  if (q->status == LBVH_OK && nr && !on_device(q)) {
     const auto* o = static_cast<const uint32_t*>(q->h_out.p);
     std::memcpy(q->r.prim, o, nr * 4);
     std::memcpy(q->r.t, o + nr, nr * 4);

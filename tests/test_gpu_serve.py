@@ -161,3 +161,32 @@ def test_server_rejects_bad_requests():
     assert p.shape == (0,)
     assert srv.stats()["resident_scenes"] == 2
     srv.close()
+
+
+def test_server_device_requests_equal_device_plain_path(ctx):
+    """Device-resident requests (no staging): same plan, same results as the
+    plain device entry points one at a time."""
+    from lbvh_amd import hip
+    plan = _plan()[:24]
+    expect = _plain_results(ctx, plan)
+    srv = lbvh_amd.Server(0)
+    ids, tickets, keep = {}, [], []
+    for kind, k, tri, rays in plan:
+        dr = hip.DeviceArray.from_host(rays)
+        dt = hip.DeviceArray.from_host(tri) if tri is not None else None
+        keep.append((dr, dt))
+        if kind == "new":
+            t, sid = srv.submit(dr, 0, dt)
+            ids[k] = sid
+        elif kind == "refit":
+            t, _ = srv.submit(dr, ids[k], dt)
+        else:
+            t, _ = srv.submit(dr, ids[k])
+        tickets.append(t)
+    for i, t in enumerate(tickets):
+        p, tt, uv, info = srv.wait(t)
+        ep, et, euv = expect[i]
+        assert np.array_equal(p.numpy(), ep), "request %d" % i
+        assert np.array_equal(tt.numpy().view(np.uint32), et.view(np.uint32)), "request %d" % i
+        assert np.array_equal(uv.numpy().view(np.uint32), euv.view(np.uint32)), "request %d" % i
+    srv.close()

@@ -396,12 +396,26 @@ k_karras(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes
          uint32_t* __restrict__ treelet_count, WideNode* __restrict__ wide)
 {
   __shared__ KeyT s_keys[kKarrasWindow];
+  // batches: offsets of the scenes this workgroup's nodes can belong to (a
+  // scene has at least one node, so at most kBlock scenes, + 1 end offset)
+  __shared__ uint64_t s_off[SEG ? kBlock + 1 : 1];
+  __shared__ uint32_t s_first;
   const int64_t nodes_total = sc.n - (SEG ? int64_t(sc.nseg) : 1);
   const int64_t g0 = int64_t(blockIdx.x) * kBlock;
   const int64_t g = g0 + threadIdx.x;
+  uint32_t s0 = 0;
+  uint32_t span = 0; // scenes staged in s_off (entries span + 1)
+  if constexpr (SEG) {
+    if (threadIdx.x == 0)
+      s_first = find_scene(sc.prim_off, sc.nseg, g0, 1, true);
+    __syncthreads();
+    s0 = s_first;
+    span = sc.nseg - s0 < uint32_t(kBlock) ? sc.nseg - s0 : uint32_t(kBlock);
+    for (uint32_t t = threadIdx.x; t <= span; t += kBlock)
+      s_off[t] = sc.prim_off[s0 + t];
+  }
   // scene of the workgroup's first node sets the window (sorted position of
   // node g of scene s is g + s)
-  const uint32_t s0 = SEG ? find_scene(sc.prim_off, sc.nseg, g0, 1, true) : 0u;
   const int64_t w0 = g0 + s0 - kKarrasHalo;
   for (int t = threadIdx.x; t < kKarrasWindow; t += kBlock) {
     const int64_t x = w0 + t;
@@ -410,9 +424,22 @@ k_karras(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes
   __syncthreads();
   if (g >= nodes_total)
     return;
-  const uint32_t seg = SEG ? find_scene(sc.prim_off, sc.nseg, g, 1, true) : 0u;
-  const int64_t P = SEG ? int64_t(sc.prim_off[seg]) : 0;
-  const int64_t n = SEG ? int64_t(sc.prim_off[seg + 1]) - P : sc.n;
+  uint32_t seg = 0;
+  int64_t P = 0, n = sc.n;
+  if constexpr (SEG) {
+    // largest j < span with node base s_off[j] - (s0 + j) <= g
+    uint32_t lo = 0, hi = span;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (int64_t(s_off[mid]) - int64_t(s0 + mid) <= g)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    seg = s0 + lo;
+    P = int64_t(s_off[lo]);
+    n = int64_t(s_off[lo + 1]) - P;
+  }
   const int64_t node_base = P - seg;
   const int64_t link_base = 2 * P - seg;
   const int64_t i = g - node_base;
@@ -481,30 +508,24 @@ k_karras(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes
   } else if (i == 0) {
     treelets[atomicAdd(treelet_count, 1u)] = make_uint2(uint32_t(P), uint32_t(P + n - 1));
   }
-  uint32_t left, right;
+  // left / right are written by the fit together with the box (one full
+  // node and one full traversal record per internal node)
   if (lleaf) {
-    const uint32_t v = ids ? uint32_t(int64_t(ids[P + gamma]) - P) : uint32_t(gamma);
-    left = v | kLeafBit;
     parent[link_base + (n - 1) + gamma] = left_link;
     if (leafprim)
-      leafprim[P + gamma] = v;
+      leafprim[P + gamma] = uint32_t(int64_t(ids[P + gamma]) - P);
   } else {
-    left = uint32_t(gamma);
     parent[link_base + gamma] = left_link;
   }
   if (rleaf) {
-    const uint32_t v = ids ? uint32_t(int64_t(ids[P + gamma + 1]) - P) : uint32_t(gamma + 1);
-    right = v | kLeafBit;
     parent[link_base + (n - 1) + gamma + 1] = right_link;
     if (leafprim)
-      leafprim[P + gamma + 1] = v;
+      leafprim[P + gamma + 1] = uint32_t(int64_t(ids[P + gamma + 1]) - P);
   } else {
-    right = uint32_t(gamma + 1);
     parent[link_base + gamma + 1] = right_link;
   }
-  *reinterpret_cast<uint2*>(&nodes[node_base + i].left) = make_uint2(left, right);
-  *reinterpret_cast<uint2*>(&wide[node_base + i].left) = make_uint2(left, right);
-  flags[node_base + i] = 0u;
+  if (top)
+    flags[node_base + i] = 0u; // only top nodes use device-scope visit flags
 }
 
 // ---- bottom-up fit -------------------------------------------------------------
@@ -548,22 +569,39 @@ store_node_box_coherent(Node* nd, const Box& b)
   }
 }
 
+// A completed internal node: box and children, one full 32-byte record.
 __device__ __forceinline__ void
-store_node_box(Node* nd, const Box& b)
+store_node(Node* nd, const Box& b, uint32_t left, uint32_t right)
 {
-  *reinterpret_cast<float4*>(&nd->bmin[0]) = make_float4(b.lo[0], b.lo[1], b.lo[2], b.hi[0]);
-  *reinterpret_cast<float2*>(&nd->bmax[1]) = make_float2(b.hi[1], b.hi[2]);
+  uint4* q = reinterpret_cast<uint4*>(nd);
+  q[0] = make_uint4(__float_as_uint(b.lo[0]), __float_as_uint(b.lo[1]), __float_as_uint(b.lo[2]),
+                    __float_as_uint(b.hi[0]));
+  q[1] = make_uint4(__float_as_uint(b.hi[1]), __float_as_uint(b.hi[2]), left, right);
 }
 
-// The box of node p into its parent's traversal record (link = p's link).
+// Its traversal record: both child boxes and both children, one full
+// 64-byte record.
 __device__ __forceinline__ void
-store_wide_child(WideNode* __restrict__ wide, uint32_t link, const Box& b)
+store_wide(WideNode* w, const Box& lb, const Box& rb, uint32_t left, uint32_t right)
 {
-  WideNode* w = &wide[link & kLinkIndex];
-  float* dst = (link & kLinkRight) ? w->rbox : w->lbox;
-  *reinterpret_cast<float2*>(dst) = make_float2(b.lo[0], b.lo[1]);
-  *reinterpret_cast<float2*>(dst + 2) = make_float2(b.lo[2], b.hi[0]);
-  *reinterpret_cast<float2*>(dst + 4) = make_float2(b.hi[1], b.hi[2]);
+  uint4* q = reinterpret_cast<uint4*>(w);
+  q[0] = make_uint4(__float_as_uint(lb.lo[0]), __float_as_uint(lb.lo[1]),
+                    __float_as_uint(lb.lo[2]), __float_as_uint(lb.hi[0]));
+  q[1] = make_uint4(__float_as_uint(lb.hi[1]), __float_as_uint(lb.hi[2]),
+                    __float_as_uint(rb.lo[0]), __float_as_uint(rb.lo[1]));
+  q[2] = make_uint4(__float_as_uint(rb.lo[2]), __float_as_uint(rb.hi[0]),
+                    __float_as_uint(rb.hi[1]), __float_as_uint(rb.hi[2]));
+  q[3] = make_uint4(left, right, 0u, 0u);
+}
+
+// Child value of a node: an internal index, or a leaf (bit 31) holding the
+// sorted position (reference leaves) or the primitive id there.
+__device__ __forceinline__ uint32_t
+child_value(int64_t idx, bool is_leaf, const uint32_t* __restrict__ leafprim)
+{
+  if (!is_leaf)
+    return uint32_t(idx);
+  return (leafprim ? leafprim[idx] : uint32_t(idx)) | kLeafBit;
 }
 
 __device__ __forceinline__ Box
@@ -601,6 +639,7 @@ k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* _
   __shared__ uint32_t s_link[kTreelet]; // link of internal node lo + t
   __shared__ Box s_leaf[kTreelet];      // box of the leaf at position lo + t
   __shared__ Box s_box[kTreelet];       // box of internal node lo + t
+  __shared__ uint32_t s_leafv[kTreelet]; // child value of the leaf at position lo + t
   __shared__ uint32_t s_item;
   const int t = threadIdx.x;
   const uint32_t count = *count_ptr;
@@ -624,6 +663,7 @@ k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* _
       uint32_t link = 0u;
       if (t < m) {
         s_leaf[t] = leaf_box(prims, kind, leafprim, k);
+        s_leafv[t] = child_value(k, true, leafprim);
         link = parent[(n - 1) + k];
         if (k < n - 1)
           s_link[t] = parent[k];
@@ -633,6 +673,7 @@ k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* _
       if (t < m) {
         Box mine = s_leaf[t];
         int64_t cur = k;
+        uint32_t cur_v = s_leafv[t]; // child value of cur
         for (;;) {
           if (link & kLinkGlobal)
             break; // cur is the treelet root
@@ -646,16 +687,23 @@ k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* _
                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
           if (old == 0u)
             break; // first arrival: the sibling's thread completes p
-          const Box ob = (link & kLinkSibLeaf) ? s_leaf[sib - lo] : s_box[sib - lo];
-          const Box b = right ? box_union(ob, mine) : box_union(mine, ob);
-          store_node_box(&nodes[p], b);
+          const bool sib_leaf = (link & kLinkSibLeaf) != 0;
+          const Box ob = sib_leaf ? s_leaf[sib - lo] : s_box[sib - lo];
+          const uint32_t sib_v = sib_leaf ? s_leafv[sib - lo] : uint32_t(sib);
+          const Box& lb = right ? ob : mine;
+          const Box& rb = right ? mine : ob;
+          const uint32_t lv = right ? sib_v : cur_v;
+          const uint32_t rv = right ? cur_v : sib_v;
+          const Box b = box_union(lb, rb);
+          store_node(&nodes[p], b, lv, rv);
+          store_wide(&wide[p], lb, rb, lv, rv);
           s_box[p - lo] = b;
           if (p == 0)
             break; // the whole tree was one treelet
           mine = b;
           cur = p;
+          cur_v = uint32_t(p);
           link = s_link[p - lo];
-          store_wide_child(wide, link, b);
         }
       }
     }
@@ -685,15 +733,18 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
     WideNode* wide = wide_all + ts.nb;
     Box mine;
     int64_t cur;
+    uint32_t cur_v;
     uint32_t link;
     if (lo == hi) {
       cur = lo;
+      cur_v = child_value(lo, true, leafprim);
       mine = leaf_box(prims, kind, leafprim, lo);
       link = parent[(n - 1) + lo];
     } else {
       cur = (e.y & 0x80000000u) ? lo : hi;
       if (cur >= n - 1)
         continue;
+      cur_v = uint32_t(cur);
       const float4 a = *reinterpret_cast<const float4*>(&nodes[cur].bmin[0]);
       const float2 c = *reinterpret_cast<const float2*>(&nodes[cur].bmax[1]);
       mine.lo[0] = a.x;
@@ -712,26 +763,37 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
         break; // never taken for a valid tree
       if (atomic_add_agent(&flags[p], 1u) == 0u)
         break; // first arrival: the sibling's thread completes p
+      const bool sib_leaf = (link & kLinkSibLeaf) != 0;
       Box ob;
-      if (link & kLinkSibLeaf) {
+      if (sib_leaf) {
         ob = leaf_box(prims, kind, leafprim, sib);
       } else {
         if (sib >= n - 1)
           break;
         ob = load_node_box_coherent(&nodes[sib]);
       }
-      const Box b = right ? box_union(ob, mine) : box_union(mine, ob);
+      const uint32_t sib_v = child_value(sib, sib_leaf, leafprim);
+      const Box& lb = right ? ob : mine;
+      const Box& rb = right ? mine : ob;
+      const uint32_t lv = right ? sib_v : cur_v;
+      const uint32_t rv = right ? cur_v : sib_v;
+      const Box b = box_union(lb, rb);
       coherent_store_u32(&flags[p], 0u);
+      store_wide(&wide[p], lb, rb, lv, rv);
       if (p == 0) {
-        store_node_box(&nodes[0], b);
+        store_node(&nodes[0], b, lv, rv);
         break;
       }
+      // the box is read by the thread completing p's parent, possibly on
+      // another XCD
       store_node_box_coherent(&nodes[p], b);
+      nodes[p].left = lv;
+      nodes[p].right = rv;
       vmem_drain();
       mine = b;
       cur = p;
+      cur_v = uint32_t(p);
       link = parent[p];
-      store_wide_child(wide, link, b);
     }
   }
 }

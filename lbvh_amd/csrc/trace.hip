@@ -228,6 +228,21 @@ constexpr int kLayoutSingle = 0;
 constexpr int kLayoutBatch = 1;
 constexpr int kLayoutJobs = 2;
 
+// The last job with ray_begin <= i.
+__device__ __forceinline__ uint32_t
+job_of(const TraceJob* __restrict__ jobs, uint32_t njobs, uint64_t i)
+{
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (jobs[mid].ray_begin <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 template<int MODE, int LAYOUT>
 __global__ __launch_bounds__(kTraceBlock) void
 k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
@@ -237,25 +252,62 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
         float* __restrict__ uv_out)
 {
   __shared__ uint2 s_ring[(kStackLds > 0 ? kStackLds : 1) * kTraceBlock];
-  uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+  // batches and jobs: the ray offsets of the scenes / jobs this workgroup's
+  // rays fall in, staged once (s_rb) when they are at most kTraceBlock
+  constexpr bool kStage = LAYOUT != kLayoutSingle;
+  __shared__ uint64_t s_rb[kStage ? kTraceBlock + 1 : 1];
+  __shared__ uint32_t s_first, s_span;
+  const uint64_t i0 = uint64_t(blockIdx.x) * kTraceBlock;
+  uint64_t i = i0 + threadIdx.x;
+  uint32_t first = 0, span = 0;
+  if constexpr (kStage) {
+    if (threadIdx.x == 0) {
+      const uint64_t last = (i0 + kTraceBlock <= nrays ? i0 + kTraceBlock : nrays) - 1;
+      uint32_t a, b;
+      if constexpr (LAYOUT == kLayoutBatch) {
+        a = find_scene(ray_off, nseg, int64_t(i0), 1, false);
+        b = find_scene(ray_off, nseg, int64_t(last), 1, false);
+      } else {
+        a = job_of(jobs, nseg, i0);
+        b = job_of(jobs, nseg, last);
+      }
+      s_first = a;
+      s_span = (b - a + 1 <= uint32_t(kTraceBlock)) ? b - a + 1 : 0u; // 0: not staged
+    }
+    __syncthreads();
+    first = s_first;
+    span = s_span;
+    for (uint32_t t = threadIdx.x; t < span; t += kTraceBlock)
+      s_rb[t] = (LAYOUT == kLayoutBatch) ? ray_off[first + t] : jobs[first + t].ray_begin;
+    __syncthreads();
+  }
   if (i >= nrays)
     return;
+  uint32_t sel = 0; // the ray's scene or job
+  if constexpr (kStage) {
+    if (span) {
+      uint32_t lo = 0, hi = span; // s_rb[lo] <= i, the last such entry
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_rb[mid] <= i)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      sel = first + lo;
+    } else {
+      sel = (LAYOUT == kLayoutBatch) ? find_scene(ray_off, nseg, int64_t(i), 1, false)
+                                     : job_of(jobs, nseg, i);
+    }
+  }
   if constexpr (LAYOUT == kLayoutBatch) {
-    const uint32_t sc = find_scene(ray_off, nseg, int64_t(i), 1, false);
+    const uint32_t sc = sel;
     const uint64_t P = prim_off[sc];
     nprims = uint32_t(prim_off[sc + 1] - P);
     nodes += P - sc;
     tris += 12 * P;
   } else if constexpr (LAYOUT == kLayoutJobs) {
-    uint32_t lo = 0, hi = nseg; // jobs[lo].ray_begin <= i < jobs[hi].ray_begin
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (jobs[mid].ray_begin <= i)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const TraceJob& jb = jobs[lo];
+    const TraceJob& jb = jobs[sel];
     nodes = static_cast<const WideNode*>(jb.nodes);
     tris = jb.tris;
     nprims = jb.nprims;

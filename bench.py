@@ -46,11 +46,13 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU-baseline sample")
-    ap.add_argument("--workload", choices=["single", "batched", "serving"], default="single",
+    ap.add_argument("--workload", choices=["single", "batched", "serving", "large"],
+                    default="single",
                     help="single: the 1M-triangle scene (default); batched: 4,096 uneven "
                          "scenes built in one segmented pass against the plain per-scene path; "
                          "serving: an open-loop request stream through the request server "
-                         "against the plain path one request at a time")
+                         "against the plain path one request at a time; large: 16,777,216 "
+                         "triangles, per-phase 30/63-bit builds and a refit")
     ap.add_argument("--scenes", type=int, default=4096, help="batched: number of scenes")
     ap.add_argument("--requests", type=int, default=160, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=1000.0,
@@ -466,6 +468,95 @@ def bench_serving(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_large(args):
+    """16,777,216 triangles (seed 3): centres U[0,1]^3, offsets
+    U[-5e-4, 5e-4]^3.  Per-phase device times of builds with 30-bit and
+    63-bit codes (reference leaf order, as builder<float>), then a refit
+    after perturbing every vertex by U[-1e-4, 1e-4] with the topology kept.
+    Inputs resident in HBM; steps = builds per code width."""
+    import numpy as np
+
+    import lbvh_amd
+    from lbvh_amd import hip
+    from lbvh_amd import workload as W
+
+    n = 1 << 24
+    log("bench: generating the 16,777,216-triangle uniform scene (seed 3)")
+    v, tri = W.scene_uniform(3, n, 5e-4)
+    tri2 = W.make_triangles(W.perturb(33, v, 1e-4))
+    del v
+    ctx = lbvh_amd.Context(0)
+    d_tri = hip.DeviceArray.from_host(tri)
+    d_tri2 = hip.DeviceArray.from_host(tri2)
+    out = {}
+    for bits in (30, 63):
+        b = ctx.build_triangles(d_tri, code_bits=bits)  # tunes the sort for this size
+        ctx.synchronize()
+        b.destroy()
+        for _ in range(args.warmup):
+            ctx.build_triangles(d_tri, code_bits=bits).destroy()
+        ctx.synchronize()
+        ctx.set_timing(2)
+        trees = [ctx.build_triangles(d_tri, code_bits=bits) for _ in range(args.steps)]
+        ctx.synchronize()
+        tot, calls = ctx.phase_stats(1)
+        ctx.set_timing(1)
+        split = [ctx.build_triangles(d_tri, code_bits=bits) for _ in range(3)]
+        ctx.synchronize()
+        ph, _ = ctx.phase_stats(1)
+        ctx.set_timing(0)
+        build_ms = sum(tot) / calls
+        scale = build_ms / sum(ph)
+        inf = trees[0].info()
+        out[bits] = {"build_ms": round(build_ms, 4),
+                     "mprims_s": round(n / (build_ms * 1e-3) / 1e6, 2),
+                     "phases_ms": {k: round(x * scale, 4) for k, x in
+                                   zip(("encode (bounds+codes)", "sort", "tree", "fit"), ph)},
+                     "sort": {"digit_bits": inf.sort_digit_bits, "tile": inf.sort_tile,
+                              "strategy": "onesweep" if inf.sort_strategy == 1
+                              else "reduce-then-scan"}}
+        if bits == 30:
+            r = trees[0]
+            for _ in range(args.warmup):
+                ctx.refit_triangles(r, d_tri2)
+                ctx.refit_triangles(r, d_tri)
+            ctx.synchronize()
+            ctx.set_timing(2)
+            for k in range(args.steps):
+                ctx.refit_triangles(r, d_tri2 if k % 2 == 0 else d_tri)
+            ctx.synchronize()
+            rt, rc = ctx.phase_stats(3)
+            ctx.set_timing(0)
+            out["refit_ms"] = round(sum(rt) / rc, 4)
+        for b in trees + split:
+            b.destroy()
+        log("bench: %d-bit build %.3f ms" % (bits, build_ms))
+    ctx.close()
+    res = {
+        "metric": "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain",
+        "value": out[30]["mprims_s"],
+        "unit": "Mprims/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": out[30]["build_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "large build: 16,777,216 uniform FP32 triangles (seed 3, offsets "
+                               "U[-5e-4, 5e-4]^3), reference leaf order; 30- and 63-bit codes; "
+                               "refit after a U[-1e-4, 1e-4] vertex perturbation",
+                   "n_prims": n},
+        "build_30bit": out[30],
+        "build_63bit": out[63],
+        "refit_ms": out["refit_ms"],
+        "refit_mprims_s": round(n / (out["refit_ms"] * 1e-3) / 1e6, 2),
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -477,6 +568,8 @@ def main():
         return bench_batched(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "large":
+        return bench_large(args)
     import numpy as np
 
     import lbvh_amd

@@ -145,7 +145,7 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
     { reinterpret_cast<void**>(&b->d_other_end), m * 4 },
     { reinterpret_cast<void**>(&b->d_leafprim),
       b->leaf_order == LBVH_LEAF_PRIMITIVE_ID ? n * 4 : 0 },
-    { &b->d_wide, m * 64 },
+    { &b->d_wide, m * 128 },
     { reinterpret_cast<void**>(&b->d_bounds_key), 24ull * b->nseg },
     { reinterpret_cast<void**>(&b->d_cent), 12 * n },
     { reinterpret_cast<void**>(&b->d_keys_a), 8 * n },
@@ -201,7 +201,7 @@ queue_build(lbvh_ctx* c, lbvh_batch* b, bool timed)
     record(c, 1);
   bool in_b = false;
   if (e == hipSuccess)
-    e = launch_sort(b->key_bits, n, b->d_keys_a, b->d_keys_b, b->with_ids ? b->d_ids_a : nullptr,
+    e = launch_sort(b->key_bits, 8, n, b->d_keys_a, b->d_keys_b, b->with_ids ? b->d_ids_a : nullptr,
                     b->with_ids ? b->d_ids_b : nullptr, b->cfg, ws, 0u, &in_b, s);
   b->sorted_in_b = in_b;
   if (timed)

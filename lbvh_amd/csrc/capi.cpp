@@ -211,7 +211,7 @@ alloc_bvh(lbvh_ctx* c, lbvh_bvh* b, bool keep_codes)
     { reinterpret_cast<void**>(&b->d_parent), (2 * n - 1) * 4 },
     { reinterpret_cast<void**>(&b->d_flags), m * 4 },
     { reinterpret_cast<void**>(&b->d_other_end), m * 4 },
-    { &b->d_wide, m * 64 },
+    { &b->d_wide, m * 128 },
     { reinterpret_cast<void**>(&b->d_bounds), 32 },
     { &b->d_codes, keep_codes ? n * cw : 0 },
     { reinterpret_cast<void**>(&b->d_ids), keep_codes ? n * 4 : 0 },
@@ -424,7 +424,7 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   record(c, 1);
   bool in_b = false;
   if (e == hipSuccess)
-    e = launch_sort(key_bits, n, c->keys_a.p, c->keys_b.p,
+    e = launch_sort(key_bits, int(kw), n, c->keys_a.p, c->keys_b.p,
                     with_ids ? static_cast<uint32_t*>(c->ids_a.p) : nullptr,
                     with_ids ? static_cast<uint32_t*>(c->ids_b.p) : nullptr, cfg, ws, epoch,
                     &in_b, c->stream);

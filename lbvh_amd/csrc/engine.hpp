@@ -55,7 +55,9 @@ uint32_t encode_groups(uint64_t n);
 // buffers; ids_in == nullptr means ids = 0..n-1; ids_a == nullptr sorts keys
 // only.  Returns in *result_in_b whether the sorted data ended in the B
 // buffers.  `epoch` must change between sorts sharing `ws.status`.
-hipError_t launch_sort(int key_bits, uint64_t n, void* keys_a, void* keys_b, uint32_t* ids_a,
+// key_bytes: 4 (u32 keys) or 8 (u64 keys); key_bits: the low bits that vary.
+hipError_t launch_sort(int key_bits, int key_bytes, uint64_t n, void* keys_a, void* keys_b,
+                       uint32_t* ids_a,
                        uint32_t* ids_b, const SortConfig& cfg, const SortWorkspace& ws,
                        uint32_t epoch, bool* result_in_b, hipStream_t s);
 uint64_t sort_status_words(uint64_t n, const SortConfig& cfg);

@@ -618,14 +618,15 @@ dispatch(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* v
 }
 
 hipError_t
-launch_sort(int key_bits, uint64_t n, void* keys_a, void* keys_b, uint32_t* ids_a,
+launch_sort(int key_bits, int key_bytes, uint64_t n, void* keys_a, void* keys_b, uint32_t* ids_a,
             uint32_t* ids_b, const SortConfig& cfg, const SortWorkspace& ws, uint32_t epoch,
             bool* result_in_b, hipStream_t s)
 {
-  if (n == 0 || n >= (uint64_t(1) << 31))
+  if (n == 0 || n >= (uint64_t(1) << 31) || (key_bytes != 4 && key_bytes != 8) ||
+      key_bits > 8 * key_bytes)
     return hipErrorInvalidValue;
   const bool vals = ids_a != nullptr;
-  if (key_bits <= 32) {
+  if (key_bytes == 4) {
     auto* a = static_cast<uint32_t*>(keys_a);
     auto* b = static_cast<uint32_t*>(keys_b);
     return vals ? dispatch<uint32_t, true>(key_bits, n, a, b, ids_a, ids_b, cfg, ws, epoch,

@@ -247,3 +247,21 @@ def test_batch_of_many_scenes_matches_single_builds(ctx):
         assert b.nodes(s).tobytes() == single.nodes().tobytes(), "scene %d" % s
         single.destroy()
     b.destroy()
+
+
+@pytest.mark.parametrize("sizes", [[4, 5, 6], [2, 3], [3000, 7000, 2, 50000], [1025, 2]],
+                         ids=["3-tiny", "2-tiny", "4-mixed", "2-edge"])
+def test_batch_of_few_scenes_equals_oracle(ctx, sizes):
+    """Batches of 2-4 scenes have at most 32 key bits (30 + scene bits) in
+    64-bit keys: they must still sort as 64-bit keys."""
+    tri, off = W.batched_scenes(seed=21, sizes=sizes)
+    for leaf in (0, 1):
+        b = ctx.build_batch(tri, off, leaf_order=leaf, keep_codes=True)
+        for s in range(len(sizes)):
+            ts = _scene(tri, off, s)
+            on, ob, osc, oids = oracle.build(ts, 30, leaf)
+            codes, ids, bounds = b.codes(s)
+            assert np.array_equal(codes, osc), "scene %d codes" % s
+            assert np.array_equal(ids, oids), "scene %d ids" % s
+            assert b.nodes(s).tobytes() == on.tobytes(), "scene %d nodes" % s
+        b.destroy()

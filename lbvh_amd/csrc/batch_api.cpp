@@ -145,7 +145,7 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
     { reinterpret_cast<void**>(&b->d_other_end), m * 4 },
     { reinterpret_cast<void**>(&b->d_leafprim),
       b->leaf_order == LBVH_LEAF_PRIMITIVE_ID ? n * 4 : 0 },
-    { &b->d_wide, m * 128 },
+    { &b->d_wide, m * 64 },
     { reinterpret_cast<void**>(&b->d_bounds_key), 24ull * b->nseg },
     { reinterpret_cast<void**>(&b->d_cent), 12 * n },
     { reinterpret_cast<void**>(&b->d_keys_a), 8 * n },

@@ -579,41 +579,19 @@ store_node(Node* nd, const Box& b, uint32_t left, uint32_t right)
   q[1] = make_uint4(__float_as_uint(b.hi[1]), __float_as_uint(b.hi[2]), left, right);
 }
 
-// Its traversal record: the boxes of internal children and both child
-// values (leaf children's triangles are written by store_leaf_tri).
+// Its traversal record: both child boxes and both children, one full
+// 64-byte record.
 __device__ __forceinline__ void
 store_wide(WideNode* w, const Box& lb, const Box& rb, uint32_t left, uint32_t right)
 {
-  if (!(left & kLeafBit)) {
-    float4* q = reinterpret_cast<float4*>(&w->child[0][0]);
-    q[0] = make_float4(lb.lo[0], lb.lo[1], lb.lo[2], lb.hi[0]);
-    *reinterpret_cast<float2*>(&w->child[0][4]) = make_float2(lb.hi[1], lb.hi[2]);
-  }
-  if (!(right & kLeafBit)) {
-    float4* q = reinterpret_cast<float4*>(&w->child[1][0]);
-    q[0] = make_float4(rb.lo[0], rb.lo[1], rb.lo[2], rb.hi[0]);
-    *reinterpret_cast<float2*>(&w->child[1][4]) = make_float2(rb.hi[1], rb.hi[2]);
-  }
-  *reinterpret_cast<uint2*>(&w->left) = make_uint2(left, right);
-}
-
-// A leaf's triangle record into its parent's traversal record (triangle
-// trees only; `link` is the leaf's parent link, `prim` its primitive; the
-// parent lies in [p_min, p_max] for a valid tree, and nothing is written
-// otherwise).
-__device__ __forceinline__ void
-store_leaf_tri(WideNode* __restrict__ wide, uint32_t link, const float* __restrict__ prims,
-               int kind, uint32_t prim, int64_t p_min, int64_t p_max)
-{
-  const int64_t p = int64_t(link & kLinkIndex);
-  if (kind != kPrimTriangle || p < p_min || p > p_max)
-    return;
-  const float4* src = reinterpret_cast<const float4*>(prims + 12 * uint64_t(prim));
-  float4* dst =
-    reinterpret_cast<float4*>(&wide[link & kLinkIndex].child[(link & kLinkRight) ? 1 : 0][0]);
-  dst[0] = src[0];
-  dst[1] = src[1];
-  dst[2] = src[2];
+  uint4* q = reinterpret_cast<uint4*>(w);
+  q[0] = make_uint4(__float_as_uint(lb.lo[0]), __float_as_uint(lb.lo[1]),
+                    __float_as_uint(lb.lo[2]), __float_as_uint(lb.hi[0]));
+  q[1] = make_uint4(__float_as_uint(lb.hi[1]), __float_as_uint(lb.hi[2]),
+                    __float_as_uint(rb.lo[0]), __float_as_uint(rb.lo[1]));
+  q[2] = make_uint4(__float_as_uint(rb.lo[2]), __float_as_uint(rb.hi[0]),
+                    __float_as_uint(rb.hi[1]), __float_as_uint(rb.hi[2]));
+  q[3] = make_uint4(left, right, 0u, 0u);
 }
 
 // Child value of a node: an internal index, or a leaf (bit 31) holding the
@@ -687,10 +665,6 @@ k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* _
         s_leaf[t] = leaf_box(prims, kind, leafprim, k);
         s_leafv[t] = child_value(k, true, leafprim);
         link = parent[(n - 1) + k];
-        // a leaf of the treelet [lo, hi] has its parent among nodes lo .. hi
-        // (a treelet root that is a left child is numbered hi)
-        store_leaf_tri(wide, link, prims, kind, leafprim ? leafprim[k] : uint32_t(k), lo,
-                       hi < n - 2 ? hi : n - 2);
         if (k < n - 1)
           s_link[t] = parent[k];
       }
@@ -766,7 +740,6 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
       cur_v = child_value(lo, true, leafprim);
       mine = leaf_box(prims, kind, leafprim, lo);
       link = parent[(n - 1) + lo];
-      store_leaf_tri(wide, link, prims, kind, leafprim ? leafprim[lo] : uint32_t(lo), 0, n - 2);
     } else {
       cur = (e.y & 0x80000000u) ? lo : hi;
       if (cur >= n - 1)

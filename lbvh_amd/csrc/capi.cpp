@@ -211,7 +211,7 @@ alloc_bvh(lbvh_ctx* c, lbvh_bvh* b, bool keep_codes)
     { reinterpret_cast<void**>(&b->d_parent), (2 * n - 1) * 4 },
     { reinterpret_cast<void**>(&b->d_flags), m * 4 },
     { reinterpret_cast<void**>(&b->d_other_end), m * 4 },
-    { &b->d_wide, m * 128 },
+    { &b->d_wide, m * 64 },
     { reinterpret_cast<void**>(&b->d_bounds), 32 },
     { &b->d_codes, keep_codes ? n * cw : 0 },
     { reinterpret_cast<void**>(&b->d_ids), keep_codes ? n * 4 : 0 },

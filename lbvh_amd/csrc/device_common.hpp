@@ -24,19 +24,18 @@ struct alignas(32) Node
 };
 static_assert(sizeof(Node) == 32, "node<float> is 32 bytes");
 
-// Traversal layout of an internal node, one 128-byte line: per child either
-// its box (an internal child: lo xyz, hi xyz in the first 6 floats) or its
-// whole triangle record (a leaf child of a triangle tree: p0, e0, e1, normal,
-// as lbvh::triangle<float> stores them), then both child values.  One load
-// serves a whole traversal step, leaf tests included.
-struct alignas(128) WideNode
+// Traversal layout of an internal node: the boxes of both children beside
+// their indices, so one 64-byte load serves a whole traversal step.  The box
+// of a leaf child is not stored (leaves are intersected directly).
+struct alignas(64) WideNode
 {
-  float child[2][12];
+  float lbox[6];
+  float rbox[6];
   uint32_t left;
   uint32_t right;
-  uint32_t pad[6];
+  uint32_t pad[2];
 };
-static_assert(sizeof(WideNode) == 128, "WideNode is 128 bytes");
+static_assert(sizeof(WideNode) == 64, "WideNode is 64 bytes");
 
 struct Box
 {

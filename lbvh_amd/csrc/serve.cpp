@@ -196,9 +196,9 @@ us_since(const lbvh_server* s)
 uint64_t
 tree_bytes_estimate(uint64_t n, uint32_t leaf_order)
 {
-  // nodes 32, traversal records 128, parent links 8, flags 4, other end 4,
+  // nodes 32, traversal records 64, parent links 8, flags 4, other end 4,
   // centroid bounds, leaf primitives 4 (primitive-id leaves)
-  return n * (32 + 128 + 8 + 4 + 4) + (leaf_order == LBVH_LEAF_PRIMITIVE_ID ? 4 * n : 0) + 4096;
+  return n * (32 + 64 + 8 + 4 + 4) + (leaf_order == LBVH_LEAF_PRIMITIVE_ID ? 4 * n : 0) + 4096;
 }
 
 uint64_t

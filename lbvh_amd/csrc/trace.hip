@@ -56,11 +56,11 @@ struct Tri
   float p0[3], e0[3], e1[3], nrm[3];
 };
 
-// A triangle record held in registers (three float4: p0 e0.x | e0.yz e1.xy |
-// e1.z n).
 __device__ __forceinline__ Tri
-tri_from(float4 a, float4 b, float4 c)
+load_tri(const float* __restrict__ tris, uint32_t i)
 {
+  const float4* p = reinterpret_cast<const float4*>(tris + 12 * uint64_t(i));
+  float4 a = p[0], b = p[1], c = p[2];
   Tri t;
   t.p0[0] = a.x;
   t.p0[1] = a.y;
@@ -76,7 +76,6 @@ tri_from(float4 a, float4 b, float4 c)
   t.nrm[2] = c.w;
   return t;
 }
-
 
 // triangle_intersector (lbvh.h:2055-2075): two-sided, not watertight.
 __device__ __forceinline__ bool
@@ -365,14 +364,11 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     have = false;
     if (ni + 1 >= nprims)
       continue; // only possible for a corrupt node array
-    // the whole step's data in one 128-byte line: per child a box or the
-    // leaf's triangle, then the child values
     const float4* wp = reinterpret_cast<const float4*>(&nodes[ni]);
-    const float4 c00 = wp[0], c01 = wp[1], c02 = wp[2];
-    const float4 c10 = wp[3], c11 = wp[4], c12 = wp[5];
-    const uint2 lrv = *reinterpret_cast<const uint2*>(&nodes[ni].left);
-    const uint32_t lr[2] = { lrv.x, lrv.y };
-    const float4 cd[2][3] = { { c00, c01, c02 }, { c10, c11, c12 } };
+    const float4 w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
+    const uint32_t lr[2] = { __float_as_uint(w3.x), __float_as_uint(w3.y) };
+    const float bx[2][6] = { { w0.x, w0.y, w0.z, w0.w, w1.x, w1.y },
+                             { w1.z, w1.w, w2.x, w2.y, w2.z, w2.w } };
     bool hit2[2] = { false, false };
     float tn2[2] = { 0.0f, 0.0f };
 #pragma unroll
@@ -382,7 +378,7 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
         continue; // out-of-range child: only possible for a corrupt node array
       if (ch & kLeafBit) {
         const uint32_t p = ch & ~kLeafBit;
-        const Tri tr = tri_from(cd[side][0], cd[side][1], cd[side][2]);
+        const Tri tr = load_tri(tris, p);
         float t, u, v;
         bool hit = (MODE == 2) ? isect_watertight(tr, r, t, u, v) : isect_reference(tr, r, t, u, v);
         if (hit && better<MODE>(t, p, best_t, best)) {
@@ -392,8 +388,8 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
           best_v = v;
         }
       } else {
-        const float lo[3] = { cd[side][0].x, cd[side][0].y, cd[side][0].z };
-        const float hi[3] = { cd[side][0].w, cd[side][1].x, cd[side][1].y };
+        const float* lo = &bx[side][0];
+        const float* hi = &bx[side][3];
         float tn[3], tf[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {

@@ -66,6 +66,37 @@ def test_host_only_entry_points():
     # null handles are rejected without touching a device
     assert lib.lbvh_ctx_synchronize(None) == 1
     assert lib.lbvh_ctx_create(0, None) == 1
+    assert lib.lbvh_ctx_set_autotune(None, 1) == 1
+    assert lib.lbvh_ctx_tune_log(None, None, 0) == 0
+    assert lib.lbvh_batch_get_info(None, None) == 1
+    assert lib.lbvh_server_submit(None, None, None, None) == 1
+    assert lib.lbvh_server_get_stats(None, None) == 1
+    # server defaults: 288 GiB cap, 4 lanes, 64 in flight, 63-bit codes,
+    # primitive-id leaves, reference intersector
+    so = lbvh_amd.ServerOpts()
+    lib.lbvh_server_opts_init(ctypes.byref(so))
+    assert so.memory_cap_bytes == 288 << 30 and so.lanes == 4 and so.max_inflight == 64
+    assert so.code_bits == 63 and so.leaf_order == 1 and so.isect_mode == 0
+    # bad server options are rejected before any device is touched
+    so.lanes = 0
+    out = ctypes.c_void_p()
+    assert lib.lbvh_server_create(0, ctypes.byref(so), ctypes.byref(out)) == 1
+    so.lanes = 4
+    so.code_bits = 31
+    assert lib.lbvh_server_create(0, ctypes.byref(so), ctypes.byref(out)) == 1
+
+
+@needs_lib
+def test_struct_layouts_match_the_header():
+    """The ctypes mirrors have the C structs' sizes (x86-64 ABI)."""
+    assert ctypes.sizeof(lbvh_amd.BuildOpts) == 32
+    assert ctypes.sizeof(lbvh_amd.BvhInfo) == 64
+    assert ctypes.sizeof(lbvh_amd.BatchInfo) == 64
+    assert ctypes.sizeof(lbvh_amd.TuneRecord) == 32
+    assert ctypes.sizeof(lbvh_amd.ServerOpts) == 48
+    assert ctypes.sizeof(lbvh_amd.Request) == 72
+    assert ctypes.sizeof(lbvh_amd.RequestInfo) == 48
+    assert ctypes.sizeof(lbvh_amd.ServerStats) == 88
 
 
 def test_engine_fails_loudly_without_library(monkeypatch):

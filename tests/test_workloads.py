@@ -1,0 +1,89 @@
+"""CPU tests of the synthetic workload generators that feed the benchmark
+and the GPU tests: the batched scenes, their box-surface rays and the serving
+request stream are deterministic and have the sizes and fractions they
+state."""
+import math
+
+import numpy as np
+
+from lbvh_amd import workload as W
+
+
+def _vertices(tri):
+    a = tri[:, 0:3]
+    return a, a - tri[:, 3:6], a + tri[:, 6:9]
+
+
+def test_batched_scene_sizes_are_loguniform_in_range():
+    s = W.batched_scene_sizes(4, 4096, 64, 262144)
+    assert s.shape == (4096,) and s.min() >= 64 and s.max() <= 262144
+    # log-uniform: the median of log(size) sits near the midpoint of the log range
+    mid = 0.5 * (math.log(64) + math.log(262144))
+    assert abs(np.median(np.log(s.astype(np.float64))) - mid) < 0.2
+    # E[n] = (hi - lo) / ln(hi / lo) for a log-uniform size
+    expect = (262144 - 64) / math.log(262144 / 64)
+    assert abs(s.mean() / expect - 1.0) < 0.05
+    assert np.array_equal(s, W.batched_scene_sizes(4, 4096, 64, 262144))
+
+
+def test_batched_scenes_layout_and_determinism():
+    sizes = [2, 3, 100, 1000]
+    tri, off = W.batched_scenes(seed=7, sizes=sizes)
+    assert off.tolist() == [0, 2, 5, 105, 1105]
+    assert tri.shape == (1105, 12) and tri.dtype == np.float32
+    # every scene is a uniform scene in its own unit cube: centres in [0,1)^3,
+    # vertices within the 0.01 offset
+    for s in range(len(sizes)):
+        a, b, c = _vertices(tri[off[s]:off[s + 1]])
+        cen = (a + b + c) / 3.0
+        assert cen.min() > -0.011 and cen.max() < 1.011
+    # a scene equals the single-scene generator with its own seed
+    v = W.uniform_vertices((7 << 32) | 2, 100, 0.01)
+    assert np.array_equal(tri[5:105], W.make_triangles(v))
+    tri2, off2 = W.batched_scenes(seed=7, sizes=sizes)
+    assert np.array_equal(tri, tri2) and np.array_equal(off, off2)
+
+
+def test_batched_box_rays_start_on_each_scene_box():
+    tri, off = W.batched_scenes(seed=9, sizes=[50, 400, 3000])
+    rays, roff = W.batched_box_rays(9, tri, off, per_scene=256)
+    assert rays.shape == (768, 8) and roff.tolist() == [0, 256, 512, 768]
+    for s in range(3):
+        a, b, c = _vertices(tri[off[s]:off[s + 1]])
+        lo = np.minimum(np.minimum(a, b), c).min(axis=0)
+        hi = np.maximum(np.maximum(a, b), c).max(axis=0)
+        r = rays[roff[s]:roff[s + 1]]
+        o, d = r[:, 0:3], r[:, 3:6]
+        eps = 1e-5
+        assert np.all(o >= lo - eps) and np.all(o <= hi + eps)
+        # on the surface: at least one coordinate on a face
+        on_face = np.isclose(o, lo, atol=eps) | np.isclose(o, hi, atol=eps)
+        assert np.all(on_face.any(axis=1))
+        assert np.allclose(np.linalg.norm(d.astype(np.float64), axis=1), 1.0, atol=1e-5)
+        assert np.all(r[:, 6] == np.float32(1e-4)) and np.all(r[:, 7] == np.float32(1e30))
+
+
+def test_serving_stream_mix_and_sizes():
+    st = W.serving_stream(seed=5, count=4000, rate_hz=1000.0)
+    kinds = [r["kind"] for r in st]
+    assert kinds[0] == "new"
+    n = len(st)
+    refit = kinds.count("refit") / n
+    trace = kinds.count("trace") / n
+    assert abs(refit - 0.10) < 0.02 and abs(trace - 0.30) < 0.03
+    t = np.array([r["arrival_s"] for r in st])
+    assert np.all(np.diff(t) > 0)
+    # Poisson arrivals: mean gap 1 / rate
+    assert abs(np.diff(t).mean() * 1000.0 - 1.0) < 0.05
+    new = [r for r in st if r["kind"] == "new"]
+    assert min(r["n_tri"] for r in new) >= 1024 and max(r["n_tri"] for r in new) <= 4194304
+    assert all(4096 <= r["n_rays"] <= 2097152 for r in st)
+    # requests on a scene come after its creation and carry its size
+    made = {}
+    for r in st:
+        if r["kind"] == "new":
+            assert r["scene"] == len(made)
+            made[r["scene"]] = r["n_tri"]
+        else:
+            assert r["scene"] in made and r["n_tri"] == made[r["scene"]]
+    assert st == W.serving_stream(seed=5, count=4000, rate_hz=1000.0)

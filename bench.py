@@ -27,10 +27,25 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 N_PRIMS = 1 << 20
 WIDTH, HEIGHT = 1920, 1080
-# Algorithmic bytes per primitive of the build with 63-bit codes (u64 key +
-# u32 id): bounds 36, encode 48, sort 8 + 8 passes x 24, tree 16, fit 92.
-BUILD_BYTES_PER_PRIM_63 = 36 + 48 + (8 + 8 * 24) + 16 + 92
 SORT_PASS_BYTES_PER_PRIM_63 = 24  # read + write of an 8-byte key and a 4-byte id
+SORT_NAMES = {1: "onesweep", 2: "reduce-then-scan", 3: "hybrid"}
+
+
+def sort_data_passes(info, key_bits):
+    """Passes of the sort over every key and id in HBM: one per digit for
+    onesweep and reduce-then-scan; for the hybrid, one per global digit plus
+    the LDS bucket sort (which reads and writes each key and id once)."""
+    if info.sort_strategy == 3:
+        return info.sort_global_digits + 1
+    d = info.sort_digit_bits
+    return (key_bits + d - 1) // d
+
+
+def build_bytes_per_prim_63(passes):
+    """Algorithmic bytes per primitive of the build with 63-bit codes (u64 key
+    + u32 id): bounds 36, encode 48, sort 8 + 24 per data pass, tree 16,
+    fit 92."""
+    return 36 + 48 + (8 + passes * 24) + 16 + 92
 RAY_COMPULSORY_BYTES = 48  # 32-byte ray in, 16-byte hit out
 
 
@@ -252,8 +267,10 @@ def bench_batched(args):
                                         "fit": round(split[3], 4)} if len(split) == 4 else split,
                     "closest_hit_mrays_s": round(m / (trace_ms * 1e-3) / 1e6, 3),
                     "sort": {"digit_bits": info.sort_digit_bits, "tile": info.sort_tile,
-                             "passes": info.sort_passes, "picked_by": "autotuner",
-                             "candidates_ms": {"d%d-t%d" % (r["digit_bits"], r["tile"]): r["ms"]
+                             "strategy": SORT_NAMES.get(info.sort_strategy, "?"),
+                             "digit_passes": info.sort_passes, "picked_by": "autotuner",
+                             "candidates_ms": {"%s-d%d-t%d" % (r["strategy"], r["digit_bits"],
+                                                               r["tile"]): r["ms"]
                                                for r in tune_log if r["batch"]}},
                     "first_build_ms": round(first_build_s * 1e3, 2),
                     "latency_ms_p50": round(wall_b * 1e3, 4),
@@ -513,8 +530,8 @@ def bench_large(args):
                      "phases_ms": {k: round(x * scale, 4) for k, x in
                                    zip(("encode (bounds+codes)", "sort", "tree", "fit"), ph)},
                      "sort": {"digit_bits": inf.sort_digit_bits, "tile": inf.sort_tile,
-                              "strategy": "onesweep" if inf.sort_strategy == 1
-                              else "reduce-then-scan"}}
+                              "strategy": SORT_NAMES.get(inf.sort_strategy, "?"),
+                              "global_digits": inf.sort_global_digits}}
         if bits == 30:
             r = trees[0]
             for _ in range(args.warmup):
@@ -661,7 +678,7 @@ def main():
     mrays = (n_primary + n_bounce) / (trace_ms * 1e-3) / 1e6
 
     digit = sort_info.sort_digit_bits
-    sort_passes = (60 + digit - 1) // digit  # 63-bit codes use 60 key bits
+    sort_passes = sort_data_passes(sort_info, 60)  # 63-bit codes use 60 key bits
     sort_pass_ms = phases[1] / sort_passes
     sort_pass_bytes = N_PRIMS * SORT_PASS_BYTES_PER_PRIM_63
     kernels = {
@@ -672,11 +689,11 @@ def main():
         "fit": phases[3],
     }
     dominant = max(kernels, key=kernels.get)
-    build_bytes = N_PRIMS * BUILD_BYTES_PER_PRIM_63
+    build_bytes = N_PRIMS * build_bytes_per_prim_63(sort_passes)
     build_frac = build_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
     if dominant.startswith("sort"):
         ach = sort_pass_bytes / (sort_pass_ms * 1e-3) / 1e9
-        roof = {"kernel": "radix sort (one digit pass, u64 key + u32 id)", "bound": "hbm",
+        roof = {"kernel": "radix sort (one data pass, u64 key + u32 id)", "bound": "hbm",
                 "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": sort_pass_bytes, "avg_launch_ms": round(sort_pass_ms, 5)}
@@ -711,12 +728,13 @@ def main():
         "build_phases_ms": {"bounds+encode": round(phases[0], 4), "sort": round(phases[1], 4),
                             "tree": round(phases[2], 4), "fit": round(phases[3], 4)},
         "sort": {"digit_bits": digit, "tile": sort_info.sort_tile,
-                 "strategy": "onesweep" if sort_info.sort_strategy == 1 else "reduce-then-scan",
-                 "passes": sort_passes, "picked_by": "autotuner",
+                 "strategy": SORT_NAMES.get(sort_info.sort_strategy, "?"),
+                 "global_digits": sort_info.sort_global_digits,
+                 "data_passes": sort_passes, "picked_by": "autotuner",
                  "candidates_ms": {"%s-d%d-t%d" % (r["strategy"], r["digit_bits"], r["tile"]): r["ms"]
                                    for r in tune_log if not r["batch"]}},
         "build_roofline_frac": round(build_frac, 4),
-        "build_algorithmic_bytes_per_prim": BUILD_BYTES_PER_PRIM_63,
+        "build_algorithmic_bytes_per_prim": build_bytes_per_prim_63(sort_passes),
         "closest_hit_mrays_s": round(mrays, 3),
         "trace_ms": round(trace_ms, 4),
         "dominant_kernel": dominant,

@@ -74,7 +74,9 @@ typedef struct lbvh_build_opts
   uint32_t sort_tile;       /* 0 = tuned default; else keys per workgroup */
   uint32_t keep_codes;      /* keep sorted codes / ids for download */
   uint32_t sort_strategy;   /* 0 = tuned; 1 = onesweep (decoupled look-back);
-                               2 = reduce-then-scan */
+                               2 = reduce-then-scan; 3 = hybrid (top digits
+                               by reduce-then-scan, then each run of equal
+                               top bits sorted by the full key in LDS) */
   uint32_t reserved[2];
 } lbvh_build_opts;
 
@@ -89,8 +91,8 @@ typedef struct lbvh_bvh_info
   uint64_t device_bytes; /* device memory held by the handle */
   uint32_t sort_digit_bits; /* the sort configuration the build used */
   uint32_t sort_tile;
-  uint32_t sort_strategy;   /* 1 onesweep, 2 reduce-then-scan */
-  uint32_t reserved;
+  uint32_t sort_strategy;   /* 1 onesweep, 2 reduce-then-scan, 3 hybrid */
+  uint32_t sort_global_digits; /* hybrid: top digits sorted by global passes */
 } lbvh_bvh_info;
 
 /* One configuration timed by the context's sort autotuner. */
@@ -216,8 +218,9 @@ lbvh_status lbvh_trace_closest_device(lbvh_ctx* ctx, const lbvh_bvh* bvh,
  * prim_off[0] = 0, at least 2 triangles per scene and fewer than 2^31 in all.
  * Scene s's nodes start at node prim_off[s] - s of the batch's node array;
  * node indices, leaf values and hit ids are scene-local.  opts.code_bits
- * must be 30; sort_digit_bits / sort_tile pick the sort (0 = tuned), which
- * always uses the reduce-then-scan strategy.
+ * must be 30; sort_digit_bits / sort_tile / sort_strategy pick the sort
+ * (0 = tuned); the onesweep strategy is rejected (its look-back state
+ * belongs to one build, so a replayed graph could not reuse it).
  *
  * A batch owns every device buffer its rebuilds, refits and traversals need,
  * so lbvh_batch_rebuild and lbvh_batch_refit allocate nothing and do not
@@ -234,7 +237,7 @@ typedef struct lbvh_batch_info
   uint32_t sort_digit_bits; /* the sort configuration in use */
   uint32_t sort_tile;
   uint32_t sort_passes;
-  uint32_t reserved;
+  uint32_t sort_strategy;   /* 2 reduce-then-scan, 3 hybrid */
   uint64_t device_bytes;  /* device memory held by the handle */
   const void* d_nodes;    /* device lbvh::node<float>[n_nodes] */
   const void* d_prims;    /* device triangles */

@@ -53,7 +53,7 @@ class BvhInfo(ctypes.Structure):
                 ("d_nodes", ctypes.c_void_p), ("d_prims", ctypes.c_void_p),
                 ("device_bytes", ctypes.c_uint64), ("sort_digit_bits", ctypes.c_uint32),
                 ("sort_tile", ctypes.c_uint32), ("sort_strategy", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("sort_global_digits", ctypes.c_uint32)]
 
 
 class TuneRecord(ctypes.Structure):
@@ -67,7 +67,7 @@ class BatchInfo(ctypes.Structure):
     _fields_ = [("n_scenes", ctypes.c_uint32), ("leaf_order", ctypes.c_uint32),
                 ("n_prims", ctypes.c_uint64), ("n_nodes", ctypes.c_uint64),
                 ("sort_digit_bits", ctypes.c_uint32), ("sort_tile", ctypes.c_uint32),
-                ("sort_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("sort_passes", ctypes.c_uint32), ("sort_strategy", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64), ("d_nodes", ctypes.c_void_p),
                 ("d_prims", ctypes.c_void_p)]
 
@@ -230,7 +230,8 @@ def _hptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-SORT_AUTO, SORT_ONESWEEP, SORT_REDUCE_SCAN = 0, 1, 2
+SORT_AUTO, SORT_ONESWEEP, SORT_REDUCE_SCAN, SORT_HYBRID = 0, 1, 2, 3
+SORT_NAMES = {1: "onesweep", 2: "reduce-then-scan", 3: "hybrid"}
 
 
 def _make_opts(code_bits, leaf_order, keep_codes, sort_digit_bits=0, sort_tile=0, sort_strategy=0):
@@ -583,7 +584,7 @@ class Context:
         lib().lbvh_ctx_tune_log(self.handle, buf, n)
         return [{"batch": bool(r.batch), "key_bits": r.key_bits, "log2_n": r.log2_n,
                  "digit_bits": r.sort_digit_bits, "tile": r.sort_tile,
-                 "strategy": "onesweep" if r.sort_strategy == 1 else "reduce-then-scan",
+                 "strategy": SORT_NAMES.get(r.sort_strategy, "?"),
                  "picked": bool(r.picked), "ms": round(r.ms, 4)} for r in buf[:n]]
 
     def workspace_bytes(self):
@@ -615,10 +616,10 @@ class Context:
         return Bvh(self, out, n, code_bits, leaf_order)
 
     def build_batch(self, tri, offsets, leaf_order=LEAF_REFERENCE, keep_codes=False,
-                    sort_digit_bits=0, sort_tile=0):
+                    sort_digit_bits=0, sort_tile=0, sort_strategy=SORT_AUTO):
         """Builds every scene of `tri` (scene s = triangles offsets[s] ..
         offsets[s + 1] - 1) in one segmented pass with 30-bit codes."""
-        opts = _make_opts(30, leaf_order, keep_codes, sort_digit_bits, sort_tile, SORT_AUTO)
+        opts = _make_opts(30, leaf_order, keep_codes, sort_digit_bits, sort_tile, sort_strategy)
         off = np.ascontiguousarray(offsets, dtype=np.uint64)
         nseg = off.shape[0] - 1
         out = ctypes.c_void_p()

@@ -129,7 +129,7 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
       hist_words = words;
     if (kSmallHist + words > b->small_words)
       b->small_words = kSmallHist + words;
-    const uint64_t cw = uint64_t(rts_groups(n, cfg, nullptr)) * radix;
+    const uint64_t cw = sort_counts_words(n, cfg);
     if (cw > b->counts_words)
       b->counts_words = cw;
   }
@@ -295,17 +295,16 @@ build_batch_common(lbvh_ctx* c, const float* d_tri, bool owns, const uint64_t* o
   const uint64_t n = off[nseg];
   if (n >= (uint64_t(1) << 31))
     return fail(c, LBVH_E_INVALID, "a batch holds fewer than 2^31 triangles");
-  if (opts.sort_strategy == 1)
-    return fail(c, LBVH_E_INVALID, "batches sort with the reduce-then-scan strategy");
+  if (opts.sort_strategy == 1) // its look-back state is tied to one build: no graph replays
+    return fail(c, LBVH_E_INVALID, "batches do not use the onesweep sort");
   const int key_bits = 30 + ceil_log2(nseg);
   // explicit options, else the tuned choice for this key width and size,
   // else every candidate (timed below on this batch)
   std::vector<SortConfig> cands;
   bool tuning = false;
   if (sort_overridden(opts)) {
-    SortConfig cfg = sort_from_opts(opts);
-    cfg.strategy = 1; // reduce-then-scan: no look-back state, so replays stay valid
-    if (!valid_sort_config(cfg))
+    const SortConfig cfg = sort_from_opts(opts, key_bits, n);
+    if (!valid_sort_config(cfg, key_bits))
       return fail(c, LBVH_E_INVALID, "unsupported sort digit width / tile");
     cands.push_back(cfg);
   } else if (!c->autotune || n < kTuneMinPrims) {
@@ -315,7 +314,7 @@ build_batch_common(lbvh_ctx* c, const float* d_tri, bool owns, const uint64_t* o
     if (it != c->tuned.end()) {
       cands.push_back(it->second);
     } else {
-      cands = sort_candidates(true);
+      cands = sort_candidates(true, key_bits, n);
       tuning = true;
     }
   }
@@ -379,9 +378,7 @@ build_batch_common(lbvh_ctx* c, const float* d_tri, bool owns, const uint64_t* o
       }
     }
     for (size_t i = first; i < c->tune_log.size(); i++) {
-      const SortConfig& r = c->tune_log[i].cfg;
-      c->tune_log[i].picked =
-        r.digit_bits == pick.digit_bits && r.items == pick.items && r.strategy == pick.strategy;
+      c->tune_log[i].picked = same_sort(c->tune_log[i].cfg, pick);
     }
     c->tuned[tune_key(true, b->key_bits, n)] = pick;
     b->cfg = pick;
@@ -530,6 +527,7 @@ lbvh_batch_get_info(const lbvh_batch* b, lbvh_batch_info* out)
   out->sort_digit_bits = uint32_t(b->cfg.digit_bits);
   out->sort_tile = uint32_t(b->cfg.items * 256);
   out->sort_passes = uint32_t(b->passes);
+  out->sort_strategy = uint32_t(b->cfg.strategy + 1);
   out->device_bytes = b->bytes;
   out->d_nodes = b->d_nodes;
   out->d_prims = b->d_prims;

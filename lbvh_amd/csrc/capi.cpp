@@ -129,13 +129,26 @@ default_sort()
   return c;
 }
 
+int
+hybrid_global_digits(int key_bits, int digit_bits, uint64_t n)
+{
+  const int passes = sort_passes(key_bits, digit_bits);
+  for (int gd = 1; gd < passes - 1; gd++) {
+    const int hi_bits = key_bits - (passes - gd) * digit_bits;
+    if ((n >> hi_bits) <= kHybridMeanBucket)
+      return gd;
+  }
+  return passes - 1;
+}
+
 std::vector<SortConfig>
-sort_candidates(bool batch)
+sort_candidates(bool batch, int key_bits, uint64_t n)
 {
   std::vector<SortConfig> v;
   // reduce-then-scan at both digit widths and tiles; onesweep only for
   // single builds (its look-back state is tied to an epoch per build, so a
-  // replayed graph could not use it)
+  // replayed graph could not use it); the hybrid where its global passes
+  // leave at least two digits to the LDS sort
   for (int strategy = 1; strategy >= (batch ? 1 : 0); strategy--)
     for (int digit : { 8, 10 })
       for (int items : { 8, 16 }) {
@@ -145,7 +158,23 @@ sort_candidates(bool batch)
         c.strategy = strategy;
         v.push_back(c);
       }
+  for (int digit : { 8, 10 }) {
+    SortConfig c;
+    c.digit_bits = digit;
+    c.items = 16;
+    c.strategy = 2;
+    c.global_digits = hybrid_global_digits(key_bits, digit, n);
+    if (c.global_digits <= sort_passes(key_bits, digit) - 2)
+      v.push_back(c);
+  }
   return v;
+}
+
+bool
+same_sort(const SortConfig& a, const SortConfig& b)
+{
+  return a.digit_bits == b.digit_bits && a.items == b.items && a.strategy == b.strategy &&
+         a.global_digits == b.global_digits;
 }
 
 bool
@@ -155,7 +184,7 @@ sort_overridden(const lbvh_build_opts& o)
 }
 
 SortConfig
-sort_from_opts(const lbvh_build_opts& o)
+sort_from_opts(const lbvh_build_opts& o, int key_bits, uint64_t n)
 {
   SortConfig c = default_sort();
   if (o.sort_digit_bits)
@@ -164,6 +193,8 @@ sort_from_opts(const lbvh_build_opts& o)
     c.items = int(o.sort_tile / 256u);
   if (o.sort_strategy)
     c.strategy = int(o.sort_strategy) - 1;
+  if (c.strategy == 2 && (c.digit_bits == 8 || c.digit_bits == 10))
+    c.global_digits = hybrid_global_digits(key_bits, c.digit_bits, n);
   return c;
 }
 
@@ -183,11 +214,14 @@ tune_key(bool batch, int key_bits, uint64_t n)
 }
 
 bool
-valid_sort_config(const SortConfig& c)
+valid_sort_config(const SortConfig& c, int key_bits)
 {
-  if (c.strategy < 0 || c.strategy > 1)
+  if (c.strategy < 0 || c.strategy > 2)
     return false;
-  if (c.strategy == 1)
+  if (c.strategy == 2 && (c.global_digits < 1 ||
+                          c.global_digits >= sort_passes(key_bits, c.digit_bits)))
+    return false;
+  if (c.strategy >= 1)
     return (c.digit_bits == 8 || c.digit_bits == 10) && (c.items == 8 || c.items == 16);
   if (c.digit_bits == 8)
     return c.items == 4 || c.items == 8 || c.items == 16;
@@ -264,14 +298,14 @@ lbvh_status
 choose_sort(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n, const lbvh_build_opts& opts,
             SortConfig* cfg)
 {
+  const int key_bits = opts.code_bits == 30 ? 30 : 60;
   if (sort_overridden(opts)) {
-    *cfg = sort_from_opts(opts);
+    *cfg = sort_from_opts(opts, key_bits, n);
     return LBVH_OK;
   }
   *cfg = default_sort();
   if (!c->autotune || n < kTuneMinPrims)
     return LBVH_OK;
-  const int key_bits = opts.code_bits == 30 ? 30 : 60;
   const uint64_t key = tune_key(false, key_bits, n);
   auto it = c->tuned.find(key);
   if (it != c->tuned.end()) {
@@ -280,7 +314,7 @@ choose_sort(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n, const lbvh_
   }
   float best = -1.0f;
   size_t first = c->tune_log.size();
-  for (const SortConfig& cand : sort_candidates(false)) {
+  for (const SortConfig& cand : sort_candidates(false, key_bits, n)) {
     float ms = -1.0f;
     for (int rep = 0; rep < 2; rep++) {
       lbvh_bvh* tmp = nullptr;
@@ -301,9 +335,7 @@ choose_sort(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n, const lbvh_
     }
   }
   for (size_t i = first; i < c->tune_log.size(); i++) {
-    const SortConfig& r = c->tune_log[i].cfg;
-    c->tune_log[i].picked = r.digit_bits == cfg->digit_bits && r.items == cfg->items &&
-                            r.strategy == cfg->strategy;
+    c->tune_log[i].picked = same_sort(c->tune_log[i].cfg, *cfg);
   }
   c->tuned[key] = *cfg;
   return LBVH_OK;
@@ -333,9 +365,9 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
     if (st0 != LBVH_OK)
       return st0;
   }
-  if (!valid_sort_config(cfg))
-    return fail(c, LBVH_E_INVALID, "unsupported sort digit width / tile");
   const int key_bits = opts.code_bits == 30 ? 30 : 60;
+  if (!valid_sort_config(cfg, key_bits))
+    return fail(c, LBVH_E_INVALID, "unsupported sort digit width / tile");
   const int passes = sort_passes(key_bits, cfg.digit_bits);
   const uint32_t radix = 1u << cfg.digit_bits;
   const size_t kw = opts.code_bits == 30 ? 4 : 8;
@@ -364,7 +396,7 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
     return st;
   if ((st = ensure(c, c->treelets, 8 * n)) != LBVH_OK)
     return st;
-  const uint64_t counts_words = uint64_t(rts_groups(n, cfg, nullptr)) * radix;
+  const uint64_t counts_words = sort_counts_words(n, cfg);
   if ((st = ensure(c, c->counts, 4 * counts_words)) != LBVH_OK)
     return st;
   const uint64_t sw = sort_status_words(n, cfg);
@@ -830,7 +862,7 @@ lbvh_bvh_get_info(const lbvh_bvh* b, lbvh_bvh_info* out)
   out->sort_digit_bits = uint32_t(b->sort.digit_bits);
   out->sort_tile = uint32_t(b->sort.items * 256);
   out->sort_strategy = uint32_t(b->sort.strategy + 1);
-  out->reserved = 0;
+  out->sort_global_digits = uint32_t(b->sort.global_digits);
   return LBVH_OK;
 }
 

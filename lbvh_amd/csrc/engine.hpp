@@ -13,9 +13,16 @@ struct SortConfig
 {
   int digit_bits = 8; // 8 or 10 (radix 256 / 1024)
   int items = 8;      // keys per thread; tile = 256 * items
-  int strategy = 0;   // 0: onesweep (decoupled look-back), 1: reduce-then-scan
+  int strategy = 0;   // 0: onesweep (decoupled look-back), 1: reduce-then-scan,
+                      // 2: hybrid (top digits by reduce-then-scan, then every
+                      //    run of equal top bits sorted by the full key in LDS)
   int groups = 0;     // reduce-then-scan: target number of chunks (0 = 512)
+  int global_digits = 0; // hybrid: digits sorted by global passes (the top ones)
 };
+
+// Hybrid strategy: keys per workgroup of the LDS sort (nominal and maximum).
+constexpr uint32_t kHybridTileKeys = 2048;
+constexpr uint32_t kHybridTileCap = 4096;
 
 inline int
 sort_passes(int key_bits, int digit_bits)
@@ -61,6 +68,8 @@ hipError_t launch_sort(int key_bits, int key_bytes, uint64_t n, void* keys_a, vo
                        uint32_t* ids_b, const SortConfig& cfg, const SortWorkspace& ws,
                        uint32_t epoch, bool* result_in_b, hipStream_t s);
 uint64_t sort_status_words(uint64_t n, const SortConfig& cfg);
+// 32-bit words of ws.counts the sort needs for n keys.
+uint64_t sort_counts_words(uint64_t n, const SortConfig& cfg);
 // Number of chunks of the reduce-then-scan strategy (and tiles per chunk).
 uint32_t rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles);
 

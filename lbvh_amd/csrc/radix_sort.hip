@@ -325,12 +325,15 @@ k_rts_upsweep(const KeyT* __restrict__ keys, uint32_t n, int shift, uint32_t chu
 // (keys of digit d in chunks before g).
 template<int RBITS>
 __global__ __launch_bounds__(kSortBlock) void
-k_rts_scan(uint32_t* __restrict__ counts, uint32_t groups, const uint32_t* __restrict__ ghist)
+k_rts_scan(uint32_t* __restrict__ counts, uint32_t groups, const uint32_t* __restrict__ ghist,
+           uint32_t* __restrict__ zero_word)
 {
   constexpr int RADIX = 1 << RBITS;
   __shared__ uint32_t s_tmp[kSortWaves];
   const int d = blockIdx.x;
   const int tid = threadIdx.x;
+  if (zero_word && d == 0 && tid == 0)
+    *zero_word = 0u; // a counter of a later kernel of the same sort
   uint32_t below = 0;
   for (int dd = tid; dd < d; dd += kSortBlock)
     below += ghist[dd];
@@ -479,6 +482,425 @@ k_rts_downsweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
   }
 }
 
+// ---- hybrid: global passes on the top digits, then LDS sorts of buckets ------
+//
+// After reduce-then-scan passes on the top digits the keys are grouped into
+// buckets of equal top bits, in bucket order, and inside a bucket in input
+// order.  Each workgroup then takes the buckets that start in its nominal
+// tile of kHybridTileKeys keys (through the end of the last one) and sorts
+// that range by the full key in LDS with stable 8-bit passes, skipping
+// digits that do not vary in the range.  A range longer than
+// kHybridTileCap is listed and sorted by k_big_sort, one workgroup per
+// range, with passes through global memory.  Stable everywhere, so equal
+// keys keep the input (id) order, as in the plain LSD sort.
+
+constexpr int kTileThreads = 512;
+constexpr int kTileWaves = kTileThreads / kWave;
+constexpr int kTileItems = int(lbvh_eng::kHybridTileCap) / kTileThreads;
+constexpr int kTileRadix = 256;
+constexpr uint32_t kBigBlocks = 64;
+static_assert(kTileItems * kTileThreads == int(lbvh_eng::kHybridTileCap), "tile shape");
+static_assert(lbvh_eng::kHybridTileKeys <= lbvh_eng::kHybridTileCap, "tile shape");
+
+template<int WAVES>
+__device__ __forceinline__ uint32_t
+block_exclusive_scan_n(uint32_t x, uint32_t* s_tmp, uint32_t* total)
+{
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+  uint32_t incl = x;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t y = __shfl_up(incl, off, kWave);
+    if (lane >= off)
+      incl += y;
+  }
+  if (lane == kWave - 1)
+    s_tmp[w] = incl;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < WAVES; i++) {
+    uint32_t v = s_tmp[i];
+    woff += (i < w) ? v : 0u;
+    tot += v;
+  }
+  __syncthreads();
+  *total = tot;
+  return incl - x + woff;
+}
+
+template<typename KeyT>
+__device__ __forceinline__ KeyT
+wave_xor_shfl(KeyT v, int off)
+{
+  if constexpr (sizeof(KeyT) == 8) {
+    uint32_t lo = __shfl_xor(uint32_t(v), off, kWave);
+    uint32_t hi = __shfl_xor(uint32_t(uint64_t(v) >> 32), off, kWave);
+    return KeyT((uint64_t(hi) << 32) | lo);
+  } else {
+    return KeyT(__shfl_xor(uint32_t(v), off, kWave));
+  }
+}
+
+// The key bits that differ between any two of the block's keys (AND ^ OR
+// over the block; threads without keys pass ~0 and 0).
+template<typename KeyT>
+__device__ KeyT
+block_varying_bits(KeyT k_and, KeyT k_or, KeyT* s_and, KeyT* s_or)
+{
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    k_and &= wave_xor_shfl(k_and, off);
+    k_or |= wave_xor_shfl(k_or, off);
+  }
+  if (lane == 0) {
+    s_and[w] = k_and;
+    s_or[w] = k_or;
+  }
+  __syncthreads();
+  KeyT a = ~KeyT(0), o = KeyT(0);
+#pragma unroll
+  for (int i = 0; i < kTileWaves; i++) {
+    a &= s_and[i];
+    o |= s_or[i];
+  }
+  __syncthreads();
+  return a ^ o;
+}
+
+// First position p of [lo, hi) with (keys[p] >> shift) > h, or hi if there
+// is none; keys >> shift must not decrease along the array.  k-ary search:
+// each step probes kTileThreads evenly spaced positions of the interval.
+// Block-uniform arguments and result.
+template<typename KeyT>
+__device__ uint32_t
+block_upper_bound(const KeyT* __restrict__ keys, uint32_t lo, uint32_t hi, int shift, KeyT h,
+                  uint32_t* s_first)
+{
+  const uint32_t tid = threadIdx.x;
+  while (lo < hi) {
+    const uint32_t step = (hi - lo + kTileThreads - 1) / kTileThreads;
+    if (tid == 0)
+      *s_first = kTileThreads;
+    __syncthreads();
+    const uint64_t p = uint64_t(lo) + uint64_t(tid) * step;
+    if (p < hi && (keys[p] >> shift) > h)
+      atomicMin(s_first, tid);
+    __syncthreads();
+    const uint32_t t = *s_first;
+    __syncthreads();
+    if (t == uint32_t(kTileThreads)) {
+      lo += ((hi - lo - 1) / step) * step + 1; // past the last probe
+    } else {
+      hi = lo + t * step;
+      if (t > 0)
+        lo += (t - 1) * step + 1;
+    }
+  }
+  return lo;
+}
+
+// The first bucket start at or after `from` (limit if none): a window of
+// one probe per thread first, the k-ary search past it only when needed.
+template<typename KeyT>
+__device__ uint32_t
+next_bucket(const KeyT* __restrict__ keys, uint32_t from, uint32_t limit, int shift,
+            uint32_t* s_first)
+{
+  const KeyT h = keys[from - 1] >> shift;
+  const uint32_t wend = (limit - from > uint32_t(kTileThreads)) ? from + kTileThreads : limit;
+  uint32_t r = block_upper_bound(keys, from, wend, shift, h, s_first);
+  if (r == wend && wend < limit)
+    r = block_upper_bound(keys, wend, limit, shift, h, s_first);
+  return r;
+}
+
+// Stable ranks of the 8-bit digit at `shift` inside each wave (wave-striped
+// items); digit bits outside `dvar` are equal for all keys and skipped.
+// Leaves per-wave digit counts in s_wcnt.
+template<typename KeyT>
+__device__ __forceinline__ void
+rank_items(const KeyT (&key)[kTileItems], uint32_t lbase, uint32_t m, int shift, uint32_t dvar,
+           uint32_t (&rank)[kTileItems], uint32_t (*s_wcnt)[kTileRadix])
+{
+  const int w = threadIdx.x / kWave;
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < kTileItems; j++) {
+    const bool valid = lbase + uint32_t(j) * kWave < m;
+    const uint32_t d = uint32_t(key[j] >> shift) & uint32_t(kTileRadix - 1);
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      if ((dvar >> b) & 1u) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t mm = ballot(valid && bit);
+        peers &= bit ? mm : ~mm;
+      }
+    }
+    uint32_t r = 0;
+    if (valid) {
+      const uint32_t below = uint32_t(popc64(peers & lt));
+      const uint32_t cnt = uint32_t(popc64(peers));
+      const uint32_t wbase = s_wcnt[w][d];
+      r = wbase + below;
+      if (below == 0)
+        s_wcnt[w][d] = wbase + cnt;
+    }
+    rank[j] = r;
+  }
+}
+
+// Turns s_wcnt into exclusive per-wave offsets inside each digit and
+// returns (threads below kTileRadix) the digit's total.
+__device__ __forceinline__ uint32_t
+digit_wave_offsets(uint32_t (*s_wcnt)[kTileRadix])
+{
+  const int tid = threadIdx.x;
+  uint32_t acc = 0;
+  if (tid < kTileRadix) {
+#pragma unroll
+    for (int ww = 0; ww < kTileWaves; ww++) {
+      const uint32_t c = s_wcnt[ww][tid];
+      s_wcnt[ww][tid] = acc;
+      acc += c;
+    }
+  }
+  return acc;
+}
+
+template<typename KeyT, bool VALS>
+__global__ __launch_bounds__(kTileThreads) void
+k_tile_sort(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
+            const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout, uint32_t n,
+            int hi_shift, int key_bits, uint32_t* __restrict__ big,
+            uint32_t* __restrict__ big_count, uint32_t big_cap)
+{
+  constexpr uint32_t T = lbvh_eng::kHybridTileKeys;
+  constexpr uint32_t CAP = lbvh_eng::kHybridTileCap;
+  __shared__ KeyT s_keys[CAP];
+  __shared__ uint32_t s_vals[VALS ? CAP : 1];
+  __shared__ uint32_t s_wcnt[kTileWaves][kTileRadix];
+  __shared__ uint32_t s_bstart[kTileRadix];
+  __shared__ uint32_t s_tmp[kTileWaves];
+  __shared__ KeyT s_and[kTileWaves];
+  __shared__ KeyT s_or[kTileWaves];
+  __shared__ uint32_t s_first;
+
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int w = tid / kWave;
+  const uint64_t t0 = uint64_t(blockIdx.x) * T;
+  if (t0 >= n)
+    return;
+  const uint32_t tile0 = uint32_t(t0);
+  const uint32_t tile1 = (n - tile0 > T) ? tile0 + T : n;
+  uint32_t start = 0;
+  if (tile0 > 0) {
+    start = next_bucket(kin, tile0, tile1, hi_shift, &s_first);
+    if (start >= tile1)
+      return; // the whole tile continues a bucket that started earlier
+  }
+  const uint32_t end = tile1 < n ? next_bucket(kin, tile1, n, hi_shift, &s_first) : n;
+  const uint32_t m = end - start;
+  if (m > CAP) {
+    if (tid == 0) {
+      const uint32_t slot = atomicAdd(big_count, 1u);
+      if (slot < big_cap) {
+        big[2 * slot] = start;
+        big[2 * slot + 1] = end;
+      }
+    }
+    return;
+  }
+
+  const uint32_t lbase = uint32_t(w) * (kTileItems * kWave) + uint32_t(lane);
+  KeyT key[kTileItems];
+  uint32_t val[kTileItems];
+  KeyT k_and = ~KeyT(0), k_or = KeyT(0);
+#pragma unroll
+  for (int j = 0; j < kTileItems; j++) {
+    const uint32_t li = lbase + uint32_t(j) * kWave;
+    const bool valid = li < m;
+    key[j] = valid ? kin[start + li] : KeyT(0);
+    if (VALS)
+      val[j] = valid ? vin[start + li] : 0u;
+    if (valid) {
+      k_and &= key[j];
+      k_or |= key[j];
+    }
+  }
+  const KeyT var = block_varying_bits(k_and, k_or, s_and, s_or);
+  bool staged = false;
+  for (int shift = 0; shift < key_bits; shift += 8) {
+    const uint32_t dvar = uint32_t(var >> shift) & uint32_t(kTileRadix - 1);
+    if (dvar == 0)
+      continue; // block-uniform
+    if (staged) {
+#pragma unroll
+      for (int j = 0; j < kTileItems; j++) {
+        const uint32_t li = lbase + uint32_t(j) * kWave;
+        if (li < m) {
+          key[j] = s_keys[li];
+          if (VALS)
+            val[j] = s_vals[li];
+        }
+      }
+    }
+    for (int j = tid; j < kTileWaves * kTileRadix; j += kTileThreads)
+      (&s_wcnt[0][0])[j] = 0u;
+    __syncthreads();
+    uint32_t rank[kTileItems];
+    rank_items(key, lbase, m, shift, dvar, rank, s_wcnt);
+    __syncthreads();
+    const uint32_t tot = digit_wave_offsets(s_wcnt);
+    uint32_t all;
+    const uint32_t run = block_exclusive_scan_n<kTileWaves>(tot, s_tmp, &all);
+    if (tid < kTileRadix)
+      s_bstart[tid] = run;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kTileItems; j++) {
+      const uint32_t li = lbase + uint32_t(j) * kWave;
+      if (li < m) {
+        const uint32_t d = uint32_t(key[j] >> shift) & uint32_t(kTileRadix - 1);
+        const uint32_t pos = s_bstart[d] + s_wcnt[w][d] + rank[j];
+        s_keys[pos] = key[j];
+        if (VALS)
+          s_vals[pos] = val[j];
+      }
+    }
+    __syncthreads();
+    staged = true;
+  }
+  if (staged) {
+    for (uint32_t i = tid; i < m; i += kTileThreads) {
+      kout[start + i] = s_keys[i];
+      if (VALS)
+        vout[start + i] = s_vals[i];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kTileItems; j++) {
+      const uint32_t li = lbase + uint32_t(j) * kWave;
+      if (li < m) {
+        kout[start + li] = key[j];
+        if (VALS)
+          vout[start + li] = val[j];
+      }
+    }
+  }
+}
+
+// Ranges longer than kHybridTileCap: one workgroup per range, stable 8-bit
+// passes (histogram, then tiles in order with running digit offsets) that
+// ping-pong between the range's slices of the two buffers; the result ends
+// in kout.
+template<typename KeyT, bool VALS>
+__global__ __launch_bounds__(kTileThreads) void
+k_big_sort(KeyT* kin, KeyT* kout, uint32_t* vin, uint32_t* vout, int key_bits,
+           const uint32_t* __restrict__ big, const uint32_t* __restrict__ big_count,
+           uint32_t big_cap)
+{
+  constexpr uint32_t CAP = lbvh_eng::kHybridTileCap;
+  __shared__ uint32_t s_wcnt[kTileWaves][kTileRadix];
+  __shared__ uint32_t s_run[kTileRadix];
+  __shared__ uint32_t s_tmp[kTileWaves];
+  __shared__ KeyT s_and[kTileWaves];
+  __shared__ KeyT s_or[kTileWaves];
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int w = tid / kWave;
+  const uint32_t count = *big_count < big_cap ? *big_count : big_cap;
+  const uint32_t lbase = uint32_t(w) * (kTileItems * kWave) + uint32_t(lane);
+  for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+    const uint32_t start = big[2 * e];
+    const uint32_t m = big[2 * e + 1] - start;
+    KeyT* src = kin + start;
+    KeyT* dst = kout + start;
+    uint32_t* vs = VALS ? vin + start : nullptr;
+    uint32_t* vd = VALS ? vout + start : nullptr;
+    KeyT k_and = ~KeyT(0), k_or = KeyT(0);
+    for (uint32_t i = tid; i < m; i += kTileThreads) {
+      const KeyT k = src[i];
+      k_and &= k;
+      k_or |= k;
+    }
+    const KeyT var = block_varying_bits(k_and, k_or, s_and, s_or);
+    for (int shift = 0; shift < key_bits; shift += 8) {
+      const uint32_t dvar = uint32_t(var >> shift) & uint32_t(kTileRadix - 1);
+      if (dvar == 0)
+        continue;
+      if (tid < kTileRadix)
+        s_run[tid] = 0u;
+      __syncthreads();
+      for (uint32_t i = tid; i < m; i += kTileThreads)
+        atomicAdd(&s_run[uint32_t(src[i] >> shift) & uint32_t(kTileRadix - 1)], 1u);
+      __syncthreads();
+      const uint32_t h = tid < kTileRadix ? s_run[tid] : 0u;
+      uint32_t all;
+      const uint32_t run = block_exclusive_scan_n<kTileWaves>(h, s_tmp, &all);
+      if (tid < kTileRadix)
+        s_run[tid] = run;
+      for (uint32_t tb = 0; tb < m; tb += CAP) {
+        for (int j = tid; j < kTileWaves * kTileRadix; j += kTileThreads)
+          (&s_wcnt[0][0])[j] = 0u;
+        __syncthreads();
+        const uint32_t tm = (m - tb < CAP) ? m - tb : CAP;
+        KeyT key[kTileItems];
+        uint32_t val[kTileItems];
+#pragma unroll
+        for (int j = 0; j < kTileItems; j++) {
+          const uint32_t li = lbase + uint32_t(j) * kWave;
+          const bool valid = li < tm;
+          key[j] = valid ? src[tb + li] : KeyT(0);
+          if (VALS)
+            val[j] = valid ? vs[tb + li] : 0u;
+        }
+        uint32_t rank[kTileItems];
+        rank_items(key, lbase, tm, shift, dvar, rank, s_wcnt);
+        __syncthreads();
+        const uint32_t tot = digit_wave_offsets(s_wcnt);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kTileItems; j++) {
+          const uint32_t li = lbase + uint32_t(j) * kWave;
+          if (li < tm) {
+            const uint32_t d = uint32_t(key[j] >> shift) & uint32_t(kTileRadix - 1);
+            const uint32_t pos = s_run[d] + s_wcnt[w][d] + rank[j];
+            dst[pos] = key[j];
+            if (VALS)
+              vd[pos] = val[j];
+          }
+        }
+        __syncthreads();
+        if (tid < kTileRadix)
+          s_run[tid] += tot;
+      }
+      __threadfence();
+      __syncthreads();
+      KeyT* tk = src;
+      src = dst;
+      dst = tk;
+      uint32_t* tv = vs;
+      vs = vd;
+      vd = tv;
+    }
+    if (src != kout + start) {
+      for (uint32_t i = tid; i < m; i += kTileThreads) {
+        kout[start + i] = src[i];
+        if (VALS)
+          vout[start + i] = vs[i];
+      }
+    }
+    __threadfence();
+    __syncthreads();
+  }
+}
+
 } // namespace lbvh_dev
 
 namespace lbvh_eng {
@@ -564,7 +986,7 @@ run_rts(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb
     hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(groups), dim3(kSortBlock), 0, s, src,
                        uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts);
     hipLaunchKernelGGL((k_rts_scan<RBITS>), dim3(RADIX), dim3(kSortBlock), 0, s, ws.counts,
-                       groups, ws.hist + size_t(p) * RADIX);
+                       groups, ws.hist + size_t(p) * RADIX, nullptr);
     hipLaunchKernelGGL((k_rts_downsweep<KeyT, VALS, RBITS, ITEMS>), dim3(groups),
                        dim3(kSortBlock), 0, s, src, dst, vsrc, vdst, uint32_t(n), shift,
                        chunk_tiles, groups, ws.counts);
@@ -584,12 +1006,85 @@ run_rts(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb
   return hipSuccess;
 }
 
+template<typename KeyT, bool VALS, int RBITS, int ITEMS>
+static hipError_t
+run_hybrid(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb,
+           const SortConfig& cfg, const SortWorkspace& ws, bool* in_b, hipStream_t s)
+{
+  constexpr int RADIX = 1 << RBITS;
+  constexpr uint32_t TILE = kSortBlock * ITEMS;
+  const int passes = sort_passes(key_bits, RBITS);
+  const int gd = cfg.global_digits;
+  if (gd < 1 || gd >= passes)
+    return hipErrorInvalidValue;
+  uint32_t chunk_tiles = 1;
+  const uint32_t groups = rts_groups(n, cfg, &chunk_tiles);
+  const uint64_t big_cap = n / kHybridTileCap + 1; // ranges longer than the cap are disjoint
+  if (uint64_t(groups) * RADIX + 2 + 2 * big_cap > ws.counts_words)
+    return hipErrorInvalidValue;
+  uint32_t* big_count = ws.counts + size_t(groups) * RADIX;
+  uint32_t* big = big_count + 2;
+  KeyT* src = ka;
+  KeyT* dst = kb;
+  uint32_t* vsrc = nullptr;
+  uint32_t* vdst = vb;
+  for (int p = passes - gd; p < passes; p++) {
+    const int shift = p * RBITS;
+    hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(groups), dim3(kSortBlock), 0, s, src,
+                       uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts);
+    hipLaunchKernelGGL((k_rts_scan<RBITS>), dim3(RADIX), dim3(kSortBlock), 0, s, ws.counts,
+                       groups, ws.hist + size_t(p) * RADIX,
+                       p == passes - gd ? big_count : nullptr);
+    hipLaunchKernelGGL((k_rts_downsweep<KeyT, VALS, RBITS, ITEMS>), dim3(groups),
+                       dim3(kSortBlock), 0, s, src, dst, vsrc, vdst, uint32_t(n), shift,
+                       chunk_tiles, groups, ws.counts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
+    KeyT* t = src;
+    src = dst;
+    dst = t;
+    if (VALS) {
+      vsrc = vdst;
+      vdst = (vdst == vb) ? va : vb;
+    }
+  }
+  const uint32_t tiles = uint32_t((n + kHybridTileKeys - 1) / kHybridTileKeys);
+  hipLaunchKernelGGL((k_tile_sort<KeyT, VALS>), dim3(tiles), dim3(kTileThreads), 0, s, src, dst,
+                     vsrc, vdst, uint32_t(n), (passes - gd) * RBITS, key_bits, big, big_count,
+                     uint32_t(big_cap));
+  hipLaunchKernelGGL((k_big_sort<KeyT, VALS>), dim3(kBigBlocks), dim3(kTileThreads), 0, s, src,
+                     dst, vsrc, vdst, key_bits, big, big_count, uint32_t(big_cap));
+  *in_b = dst == kb;
+  return hipGetLastError();
+}
+
+uint64_t
+sort_counts_words(uint64_t n, const SortConfig& cfg)
+{
+  uint64_t w = uint64_t(rts_groups(n, cfg, nullptr)) << cfg.digit_bits;
+  if (cfg.strategy == 2)
+    w += 2 + 2 * (n / kHybridTileCap + 1);
+  return w;
+}
+
 template<typename KeyT, bool VALS>
 static hipError_t
 dispatch(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb,
          const SortConfig& cfg, const SortWorkspace& ws, uint32_t epoch, bool* in_b,
          hipStream_t s)
 {
+  if (cfg.strategy == 2) {
+    if (cfg.digit_bits == 8 && cfg.items == 8)
+      return run_hybrid<KeyT, VALS, 8, 8>(key_bits, n, ka, kb, va, vb, cfg, ws, in_b, s);
+    if (cfg.digit_bits == 8 && cfg.items == 16)
+      return run_hybrid<KeyT, VALS, 8, 16>(key_bits, n, ka, kb, va, vb, cfg, ws, in_b, s);
+    if (cfg.digit_bits == 10 && cfg.items == 8)
+      return run_hybrid<KeyT, VALS, 10, 8>(key_bits, n, ka, kb, va, vb, cfg, ws, in_b, s);
+    if (cfg.digit_bits == 10 && cfg.items == 16)
+      return run_hybrid<KeyT, VALS, 10, 16>(key_bits, n, ka, kb, va, vb, cfg, ws, in_b, s);
+    return hipErrorInvalidValue;
+  }
   if (cfg.strategy == 1) {
     if (cfg.digit_bits == 8 && cfg.items == 8)
       return run_rts<KeyT, VALS, 8, 8>(key_bits, n, ka, kb, va, vb, cfg, ws, in_b, s);

@@ -103,16 +103,22 @@ void dev_free(void* p);
 lbvh_status set_device(lbvh_ctx* c);
 void begin_timed(lbvh_ctx* c, int kind);
 void record(lbvh_ctx* c, int idx, bool final = false);
-bool valid_sort_config(const SortConfig& c);
+bool valid_sort_config(const SortConfig& c, int key_bits);
 lbvh_status sync_and_check(lbvh_ctx* c);
 
 // ---- sort autotuner ----------------------------------------------------------
 // The default configuration, the candidates timed by the tuner, and whether
 // build options override the choice (then the tuner is not consulted).
 SortConfig default_sort();
-std::vector<SortConfig> sort_candidates(bool batch);
+std::vector<SortConfig> sort_candidates(bool batch, int key_bits, uint64_t n);
 bool sort_overridden(const lbvh_build_opts& o);
-SortConfig sort_from_opts(const lbvh_build_opts& o);
+SortConfig sort_from_opts(const lbvh_build_opts& o, int key_bits, uint64_t n);
+bool same_sort(const SortConfig& a, const SortConfig& b);
+// Hybrid sort: the fewest top digits whose buckets hold at most
+// kHybridMeanBucket keys on average (uniform keys), leaving the rest to the
+// LDS sort.
+int hybrid_global_digits(int key_bits, int digit_bits, uint64_t n);
+constexpr uint64_t kHybridMeanBucket = 64;
 uint64_t tune_key(bool batch, int key_bits, uint64_t n);
 int ceil_log2_u64(uint64_t x);
 // Device milliseconds of the work `queue` puts on the context stream between

@@ -206,17 +206,18 @@ def test_batch_graph_rebuild_and_refit_replays(ctx, mixed):
 def test_batch_sort_configurations_agree(ctx, mixed):
     tri, off = mixed
     ref = None
-    for digit_bits in (8, 10):
-        for tile in (2048, 4096):
-            b = ctx.build_batch(tri, off, leaf_order=1, sort_digit_bits=digit_bits,
-                                sort_tile=tile)
-            inf = b.info()
-            assert inf.sort_digit_bits == digit_bits and inf.sort_tile == tile
-            got = b"".join(b.nodes(s).tobytes() for s in range(b.n_scenes))
-            if ref is None:
-                ref = got
-            assert got == ref
-            b.destroy()
+    for strategy, digit_bits, tile in [(2, 8, 2048), (2, 8, 4096), (2, 10, 2048), (2, 10, 4096),
+                                       (3, 8, 4096), (3, 10, 4096)]:
+        b = ctx.build_batch(tri, off, leaf_order=1, sort_digit_bits=digit_bits,
+                            sort_tile=tile, sort_strategy=strategy)
+        inf = b.info()
+        assert inf.sort_digit_bits == digit_bits and inf.sort_tile == tile
+        assert inf.sort_strategy == strategy
+        got = b"".join(b.nodes(s).tobytes() for s in range(b.n_scenes))
+        if ref is None:
+            ref = got
+        assert got == ref
+        b.destroy()
 
 
 def test_batch_rejects_bad_arguments(ctx):
@@ -251,12 +252,14 @@ def test_batch_of_many_scenes_matches_single_builds(ctx):
 
 @pytest.mark.parametrize("sizes", [[4, 5, 6], [2, 3], [3000, 7000, 2, 50000], [1025, 2]],
                          ids=["3-tiny", "2-tiny", "4-mixed", "2-edge"])
-def test_batch_of_few_scenes_equals_oracle(ctx, sizes):
+@pytest.mark.parametrize("strategy", [2, 3], ids=["rts", "hybrid"])
+def test_batch_of_few_scenes_equals_oracle(ctx, sizes, strategy):
     """Batches of 2-4 scenes have at most 32 key bits (30 + scene bits) in
     64-bit keys: they must still sort as 64-bit keys."""
     tri, off = W.batched_scenes(seed=21, sizes=sizes)
     for leaf in (0, 1):
-        b = ctx.build_batch(tri, off, leaf_order=leaf, keep_codes=True)
+        b = ctx.build_batch(tri, off, leaf_order=leaf, keep_codes=True, sort_strategy=strategy)
+        assert b.info().sort_strategy == strategy
         for s in range(len(sizes)):
             ts = _scene(tri, off, s)
             on, ob, osc, oids = oracle.build(ts, 30, leaf)

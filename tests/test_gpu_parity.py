@@ -77,15 +77,17 @@ def test_build_matches_oracle_ragged(ctx, n, bits, leaf):
 
 
 SORT_CONFIGS = [(1, 8, 1024), (1, 8, 2048), (1, 8, 4096), (1, 10, 2048), (1, 10, 4096),
-                (2, 8, 2048), (2, 8, 4096), (2, 10, 2048), (2, 10, 4096)]
+                (2, 8, 2048), (2, 8, 4096), (2, 10, 2048), (2, 10, 4096),
+                (3, 8, 2048), (3, 8, 4096), (3, 10, 4096)]
+SORT_NAMES = {1: "onesweep", 2: "rts", 3: "hybrid"}
 
 
 @pytest.mark.parametrize("strategy,digit_bits,tile", SORT_CONFIGS,
-                         ids=["%s-d%d-t%d" % ("onesweep" if c[0] == 1 else "rts", c[1], c[2])
+                         ids=["%s-d%d-t%d" % (SORT_NAMES[c[0]], c[1], c[2])
                               for c in SORT_CONFIGS])
 @pytest.mark.parametrize("bits", [30, 63])
 def test_sort_configurations_agree(ctx, strategy, digit_bits, tile, bits):
-    """Both sort strategies at every digit width and tile size give the
+    """Every sort strategy at every digit width and tile size gives the
     oracle's stable (code, id) order and tree."""
     v = W.cluster_vertices(77, 300000)
     tri = W.make_triangles(v)
@@ -96,6 +98,45 @@ def test_sort_configurations_agree(ctx, strategy, digit_bits, tile, bits):
     assert np.array_equal(sc, osc)
     assert np.array_equal(ids, oids)
     assert bvh.nodes().tobytes() == on.tobytes()
+
+
+def _hybrid_edge_scene(kind, n, seed):
+    """Key distributions that stress the hybrid sort's buckets."""
+    rng = np.random.default_rng(seed)
+    if kind == "one-cluster-and-outlier":
+        # every code but one shares its top bits: one bucket of n - 1 keys,
+        # sorted by the oversize-range path
+        v = rng.random((n, 3, 3), dtype=np.float32) * np.float32(1e-3)
+        v[-1] += np.float32(1e3)
+    elif kind == "duplicates":
+        # a handful of distinct triangles repeated: long runs of equal codes
+        base = rng.random((7, 3, 3), dtype=np.float32)
+        v = base[rng.integers(0, 7, n)]
+    elif kind == "two-clusters":
+        v = rng.random((n, 3, 3), dtype=np.float32) * np.float32(1e-2)
+        v[n // 3:] += np.float32(5.0)
+    else:  # "shells": clustered, buckets of every size
+        v = W.cluster_vertices(seed, n)
+    return W.make_triangles(v.reshape(n, 9))
+
+
+@pytest.mark.parametrize("kind", ["one-cluster-and-outlier", "duplicates", "two-clusters",
+                                  "shells"])
+@pytest.mark.parametrize("n", [2, 3, 2049, 4097, 9000, 250001])
+@pytest.mark.parametrize("bits", [30, 63])
+def test_hybrid_sort_edge_distributions(ctx, kind, n, bits):
+    """The hybrid sort on skewed keys (one bucket holding nearly every key,
+    runs of equal codes, ragged sizes) gives the oracle's order and tree."""
+    tri = _hybrid_edge_scene(kind, n, 600 + n)
+    bvh = ctx.build_triangles(tri, code_bits=bits, leaf_order=1, keep_codes=True,
+                              sort_strategy=lbvh_amd.SORT_HYBRID)
+    assert bvh.info().sort_strategy == 3
+    on, _, osc, oids = oracle.build(tri, bits, 1)
+    sc, ids, _ = bvh.codes()
+    assert np.array_equal(sc, osc)
+    assert np.array_equal(ids, oids)
+    assert bvh.nodes().tobytes() == on.tobytes()
+    bvh.destroy()
 
 
 def test_rejects_bad_arguments(ctx):
@@ -330,7 +371,8 @@ def test_autotuner_times_candidates_and_keeps_results_exact():
     tri = W.make_triangles(v)
     bvh = c.build_triangles(tri, code_bits=63, leaf_order=1, keep_codes=True)
     log = c.tune_log()
-    assert len(log) == 8 and all(r["ms"] > 0 for r in log)
+    # 4 reduce-then-scan, 4 onesweep and 2 hybrid configurations
+    assert len(log) == 10 and all(r["ms"] > 0 for r in log)
     picked = [r for r in log if r["picked"]]
     assert len(picked) == 1
     inf = bvh.info()
@@ -342,22 +384,22 @@ def test_autotuner_times_candidates_and_keeps_results_exact():
     assert np.array_equal(sc, osc) and np.array_equal(ids, oids)
     # the same bucket again: no new tuning
     bvh2 = c.build_triangles(tri, code_bits=63, leaf_order=1)
-    assert len(c.tune_log()) == 8
+    assert len(c.tune_log()) == 10
     assert bvh2.info().sort_tile == inf.sort_tile
-    # a batch of the same size tunes its own candidates (reduce-then-scan only)
+    # a batch of the same size tunes its own candidates (no onesweep)
     sizes = W.loguniform_sizes(5, 64, 1000, 8000)
     btri, off = W.batched_scenes(seed=5, sizes=sizes)
     b = c.build_batch(btri, off, leaf_order=1)
     blog = [r for r in c.tune_log() if r["batch"]]
-    assert len(blog) == 4 and sum(r["picked"] for r in blog) == 1
+    assert len(blog) == 6 and sum(r["picked"] for r in blog) == 1
     for s in (0, 31, 63):
         on, _, _, _ = oracle.build(btri[int(off[s]):int(off[s + 1])], 30, 1)
         assert b.nodes(s).tobytes() == on.tobytes()
     # explicit options bypass the tuner
     c.set_autotune(False)
     bvh3 = c.build_triangles(W.make_triangles(W.cluster_vertices(92, 1 << 19)), code_bits=30)
-    assert len(c.tune_log()) == 12
-    assert bvh3.info().sort_strategy in (1, 2)
+    assert len(c.tune_log()) == 16
+    assert bvh3.info().sort_strategy in (1, 2, 3)
     for x in (bvh, bvh2, bvh3, b):
         x.destroy()
     c.close()

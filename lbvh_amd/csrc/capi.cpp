@@ -158,15 +158,16 @@ sort_candidates(bool batch, int key_bits, uint64_t n)
         c.strategy = strategy;
         v.push_back(c);
       }
-  for (int digit : { 8, 10 }) {
-    SortConfig c;
-    c.digit_bits = digit;
-    c.items = 16;
-    c.strategy = 2;
-    c.global_digits = hybrid_global_digits(key_bits, digit, n);
-    if (c.global_digits <= sort_passes(key_bits, digit) - 2)
-      v.push_back(c);
-  }
+  for (int digit : { 8, 10 })
+    for (int items : { 8, 16 }) {
+      SortConfig c;
+      c.digit_bits = digit;
+      c.items = items;
+      c.strategy = 2;
+      c.global_digits = hybrid_global_digits(key_bits, digit, n);
+      if (c.global_digits <= sort_passes(key_bits, digit) - 2)
+        v.push_back(c);
+    }
   return v;
 }
 

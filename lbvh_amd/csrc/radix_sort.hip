@@ -603,35 +603,76 @@ block_upper_bound(const KeyT* __restrict__ keys, uint32_t lo, uint32_t hi, int s
   return lo;
 }
 
-// The first bucket start at or after `from` (limit if none): a window of
-// one probe per thread first, the k-ary search past it only when needed.
+// The range of whole buckets a workgroup sorts: the buckets that start in
+// its nominal tile [tile0, tile1), through the end of the last one.  One
+// round of probes covers the tile (for the start) and the same number of
+// keys after it (for the end); each probe compares a key's top bits with
+// its predecessor's.  Only a bucket that runs on past that window needs the
+// k-ary search.  Returns false (block-uniform) when no bucket starts in the
+// tile.
 template<typename KeyT>
-__device__ uint32_t
-next_bucket(const KeyT* __restrict__ keys, uint32_t from, uint32_t limit, int shift,
-            uint32_t* s_first)
+__device__ bool
+find_range(const KeyT* __restrict__ keys, uint32_t n, uint32_t tile0, uint32_t tile1, int shift,
+           uint32_t* s_first, uint32_t* start, uint32_t* end)
 {
-  const KeyT h = keys[from - 1] >> shift;
-  const uint32_t wend = (limit - from > uint32_t(kTileThreads)) ? from + kTileThreads : limit;
-  uint32_t r = block_upper_bound(keys, from, wend, shift, h, s_first);
-  if (r == wend && wend < limit)
-    r = block_upper_bound(keys, wend, limit, shift, h, s_first);
-  return r;
+  constexpr uint32_t W = lbvh_eng::kHybridTileKeys;
+  constexpr int PROBES = int(W) / kTileThreads;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    s_first[0] = 0xFFFFFFFFu;
+    s_first[1] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  uint32_t f0 = 0xFFFFFFFFu, f1 = 0xFFFFFFFFu;
+#pragma unroll
+  for (int k = 0; k < PROBES; k++) {
+    const uint32_t p = tile0 + tid + uint32_t(k) * kTileThreads;
+    if (tile0 > 0 && p < tile1 && f0 == 0xFFFFFFFFu && (keys[p] >> shift) != (keys[p - 1] >> shift))
+      f0 = p;
+    const uint64_t q = uint64_t(tile1) + tid + uint64_t(k) * kTileThreads;
+    if (q < n && f1 == 0xFFFFFFFFu && (keys[q] >> shift) != (keys[q - 1] >> shift))
+      f1 = uint32_t(q);
+  }
+  if (f0 != 0xFFFFFFFFu)
+    atomicMin(&s_first[0], f0);
+  if (f1 != 0xFFFFFFFFu)
+    atomicMin(&s_first[1], f1);
+  __syncthreads();
+  const uint32_t b0 = tile0 == 0 ? 0u : s_first[0];
+  const uint32_t b1 = s_first[1];
+  __syncthreads(); // s_first is reused by the search below
+  if (b0 == 0xFFFFFFFFu)
+    return false;
+  *start = b0;
+  if (tile1 >= n) {
+    *end = n;
+  } else if (b1 != 0xFFFFFFFFu) {
+    *end = b1;
+  } else {
+    const uint64_t from = uint64_t(tile1) + W;
+    *end = from >= n ? n
+                     : block_upper_bound(keys, uint32_t(from), n, shift,
+                                         KeyT(keys[tile1 - 1] >> shift), s_first);
+  }
+  return true;
 }
 
-// Stable ranks of the 8-bit digit at `shift` inside each wave (wave-striped
-// items); digit bits outside `dvar` are equal for all keys and skipped.
-// Leaves per-wave digit counts in s_wcnt.
-template<typename KeyT>
+// Stable ranks of 8-bit digits inside each wave (wave-striped items: item j
+// of lane l of wave w is element w * ipw * 64 + j * 64 + l, j < ipw); digit
+// bits outside `dvar` are equal for all items and skipped.  Leaves per-wave
+// digit counts in s_wcnt.
 __device__ __forceinline__ void
-rank_items(const KeyT (&key)[kTileItems], uint32_t lbase, uint32_t m, int shift, uint32_t dvar,
-           uint32_t (&rank)[kTileItems], uint32_t (*s_wcnt)[kTileRadix])
+rank_digits(const uint32_t (&dg)[kTileItems], int ipw, uint32_t lbase, uint32_t m, uint32_t dvar,
+            uint32_t (&rank)[kTileItems], uint32_t (*s_wcnt)[kTileRadix])
 {
   const int w = threadIdx.x / kWave;
   const uint64_t lt = lanemask_lt();
 #pragma unroll
   for (int j = 0; j < kTileItems; j++) {
+    if (j >= ipw)
+      break; // block-uniform
     const bool valid = lbase + uint32_t(j) * kWave < m;
-    const uint32_t d = uint32_t(key[j] >> shift) & uint32_t(kTileRadix - 1);
+    const uint32_t d = dg[j];
     uint64_t peers = ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
@@ -672,6 +713,58 @@ digit_wave_offsets(uint32_t (*s_wcnt)[kTileRadix])
   return acc;
 }
 
+// Exclusive count, in element order, of the items whose flag is set (the
+// wave-striped layout of rank_digits); *total = the block's count.  Uses
+// s_tmp and ends with a barrier.
+__device__ __forceinline__ void
+flag_prefix(const bool (&flag)[kTileItems], int ipw, uint32_t (&pre)[kTileItems],
+            uint32_t* s_tmp, uint32_t* total)
+{
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+  const uint64_t lt = lanemask_lt();
+  uint32_t running = 0;
+#pragma unroll
+  for (int j = 0; j < kTileItems; j++) {
+    if (j >= ipw)
+      break;
+    const uint64_t mask = ballot(flag[j]);
+    pre[j] = running + uint32_t(popc64(mask & lt));
+    running += uint32_t(popc64(mask));
+  }
+  if (lane == 0)
+    s_tmp[w] = running;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kTileWaves; i++) {
+    const uint32_t v = s_tmp[i];
+    woff += (i < w) ? v : 0u;
+    tot += v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kTileItems; j++) {
+    if (j >= ipw)
+      break;
+    pre[j] += woff;
+  }
+  *total = tot;
+}
+
+// Comparisons per key the counting rank may spend on average before the
+// workgroup falls back to LSD passes over every digit.
+constexpr uint32_t kCountRankBudget = 64;
+
+// Sorts one range of whole buckets by the full key, stably:
+//   1. bucket index b of every key (its top bits are the bucket's, and
+//      buckets are already in order);
+//   2. stable LDS passes on the top digit of the low bits, then on b: the
+//      range is now ordered by (bucket, split digit) in input order within;
+//   3. inside each run of equal (bucket, split digit) every key counts the
+//      keys below it (ties: earlier positions) and goes to run start + count.
+// When the runs are too long for step 3, LDS passes over every low digit
+// and then b finish the sort instead.
 template<typename KeyT, bool VALS>
 __global__ __launch_bounds__(kTileThreads) void
 k_tile_sort(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
@@ -683,12 +776,14 @@ k_tile_sort(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
   constexpr uint32_t CAP = lbvh_eng::kHybridTileCap;
   __shared__ KeyT s_keys[CAP];
   __shared__ uint32_t s_vals[VALS ? CAP : 1];
-  __shared__ uint32_t s_wcnt[kTileWaves][kTileRadix];
-  __shared__ uint32_t s_bstart[kTileRadix];
+  __shared__ uint16_t s_aux[CAP];                        // bucket index
+  __shared__ uint32_t s_wcnt[2][kTileWaves][kTileRadix]; // per pass, alternating
   __shared__ uint32_t s_tmp[kTileWaves];
   __shared__ KeyT s_and[kTileWaves];
   __shared__ KeyT s_or[kTileWaves];
-  __shared__ uint32_t s_first;
+  __shared__ uint32_t s_first[2];
+  __shared__ uint32_t s_work;
+  (void)key_bits;
 
   const int tid = threadIdx.x;
   const int lane = __lane_id();
@@ -698,13 +793,9 @@ k_tile_sort(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
     return;
   const uint32_t tile0 = uint32_t(t0);
   const uint32_t tile1 = (n - tile0 > T) ? tile0 + T : n;
-  uint32_t start = 0;
-  if (tile0 > 0) {
-    start = next_bucket(kin, tile0, tile1, hi_shift, &s_first);
-    if (start >= tile1)
-      return; // the whole tile continues a bucket that started earlier
-  }
-  const uint32_t end = tile1 < n ? next_bucket(kin, tile1, n, hi_shift, &s_first) : n;
+  uint32_t start = 0, end = 0;
+  if (!find_range(kin, n, tile0, tile1, hi_shift, s_first, &start, &end))
+    return; // the whole tile continues a bucket that started earlier
   const uint32_t m = end - start;
   if (m > CAP) {
     if (tid == 0) {
@@ -717,81 +808,236 @@ k_tile_sort(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
     return;
   }
 
-  const uint32_t lbase = uint32_t(w) * (kTileItems * kWave) + uint32_t(lane);
+  // items per wave: just enough slots for the range
+  const int ipw = int((m + kTileThreads - 1) / kTileThreads);
+  const uint32_t lbase = uint32_t(w) * uint32_t(ipw * kWave) + uint32_t(lane);
   KeyT key[kTileItems];
   uint32_t val[kTileItems];
+  uint32_t aux[kTileItems];
+  bool flag[kTileItems];
   KeyT k_and = ~KeyT(0), k_or = KeyT(0);
 #pragma unroll
   for (int j = 0; j < kTileItems; j++) {
+    if (j >= ipw)
+      break;
     const uint32_t li = lbase + uint32_t(j) * kWave;
     const bool valid = li < m;
     key[j] = valid ? kin[start + li] : KeyT(0);
     if (VALS)
       val[j] = valid ? vin[start + li] : 0u;
+    flag[j] = valid && (li == 0 || (key[j] >> hi_shift) != (kin[start + li - 1] >> hi_shift));
     if (valid) {
       k_and &= key[j];
       k_or |= key[j];
     }
   }
-  const KeyT var = block_varying_bits(k_and, k_or, s_and, s_or);
-  bool staged = false;
-  for (int shift = 0; shift < key_bits; shift += 8) {
-    const uint32_t dvar = uint32_t(var >> shift) & uint32_t(kTileRadix - 1);
-    if (dvar == 0)
-      continue; // block-uniform
-    if (staged) {
+  for (int j = tid; j < kTileWaves * kTileRadix; j += kTileThreads)
+    (&s_wcnt[0][0][0])[j] = 0u;
+  uint32_t nb;
+  flag_prefix(flag, ipw, aux, s_tmp, &nb); // exclusive count of bucket starts
 #pragma unroll
-      for (int j = 0; j < kTileItems; j++) {
-        const uint32_t li = lbase + uint32_t(j) * kWave;
-        if (li < m) {
-          key[j] = s_keys[li];
-          if (VALS)
-            val[j] = s_vals[li];
+  for (int j = 0; j < kTileItems; j++) {
+    if (j >= ipw)
+      break;
+    aux[j] += flag[j] ? 1u : 0u; // inclusive, minus the first start: the bucket index
+    aux[j] -= 1u;
+  }
+  const KeyT var = block_varying_bits(k_and, k_or, s_and, s_or); // also orders the clear
+  const uint32_t bmask = nb > 1 ? (0xFFFFFFFFu >> __clz(nb - 1)) : 0u;
+  const int kmax = (hi_shift + 7) / 8; // 8-bit digits below the top bits
+  bool staged = false;
+  int pb = 0;
+  int k = 1;
+  for (;;) {
+    // stable LDS passes: split digits from the lowest to the highest, then
+    // the bucket index (low byte, high byte)
+    for (int step = 0; step < k + 2; step++) {
+      uint32_t dvar;
+      int sh;
+      const bool on_aux = step >= k;
+      if (!on_aux) {
+        sh = hi_shift - 8 * (k - step);
+        if (sh < 0)
+          sh = 0;
+        dvar = uint32_t(var >> sh) & uint32_t(kTileRadix - 1);
+      } else {
+        sh = 8 * (step - k);
+        dvar = (bmask >> sh) & uint32_t(kTileRadix - 1);
+      }
+      if (dvar == 0)
+        continue; // block-uniform
+      uint32_t (*cnt)[kTileRadix] = s_wcnt[pb];
+      if (staged) {
+#pragma unroll
+        for (int j = 0; j < kTileItems; j++) {
+          if (j >= ipw)
+            break;
+          const uint32_t li = lbase + uint32_t(j) * kWave;
+          if (li < m) {
+            key[j] = s_keys[li];
+            aux[j] = s_aux[li];
+            if (VALS)
+              val[j] = s_vals[li];
+          }
         }
       }
+      uint32_t dg[kTileItems];
+#pragma unroll
+      for (int j = 0; j < kTileItems; j++) {
+        if (j >= ipw)
+          break;
+        dg[j] = on_aux ? (aux[j] >> sh) & uint32_t(kTileRadix - 1)
+                       : uint32_t(key[j] >> sh) & uint32_t(kTileRadix - 1);
+      }
+      uint32_t rank[kTileItems];
+      rank_digits(dg, ipw, lbase, m, dvar, rank, cnt);
+      __syncthreads();
+      // digit starts: exclusive scan of the digit totals (threads below the
+      // radix), folded into the per-wave offsets
+      const uint32_t tot = digit_wave_offsets(cnt);
+      uint32_t incl = tot;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, kWave);
+        if (lane >= off)
+          incl += y;
+      }
+      if (lane == kWave - 1)
+        s_tmp[w] = incl;
+      __syncthreads();
+      uint32_t woff = 0;
+#pragma unroll
+      for (int i = 0; i < kTileWaves; i++)
+        woff += (i < w) ? s_tmp[i] : 0u;
+      if (tid < kTileRadix) {
+        const uint32_t dstart = incl - tot + woff;
+#pragma unroll
+        for (int ww = 0; ww < kTileWaves; ww++)
+          cnt[ww][tid] += dstart;
+      }
+      for (int j = tid; j < kTileWaves * kTileRadix; j += kTileThreads)
+        (&s_wcnt[pb ^ 1][0][0])[j] = 0u; // the next pass's counters
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kTileItems; j++) {
+        if (j >= ipw)
+          break;
+        const uint32_t li = lbase + uint32_t(j) * kWave;
+        if (li < m) {
+          const uint32_t pos = cnt[w][dg[j]] + rank[j];
+          s_keys[pos] = key[j];
+          s_aux[pos] = uint16_t(aux[j]);
+          if (VALS)
+            s_vals[pos] = val[j];
+        }
+      }
+      __syncthreads();
+      staged = true;
+      pb ^= 1;
     }
-    for (int j = tid; j < kTileWaves * kTileRadix; j += kTileThreads)
-      (&s_wcnt[0][0])[j] = 0u;
-    __syncthreads();
-    uint32_t rank[kTileItems];
-    rank_items(key, lbase, m, shift, dvar, rank, s_wcnt);
-    __syncthreads();
-    const uint32_t tot = digit_wave_offsets(s_wcnt);
-    uint32_t all;
-    const uint32_t run = block_exclusive_scan_n<kTileWaves>(tot, s_tmp, &all);
-    if (tid < kTileRadix)
-      s_bstart[tid] = run;
+    if (!staged) {
+      // no digit varied: the input order is the order by (bucket, split
+      // digit); stage it for the steps below
+#pragma unroll
+      for (int j = 0; j < kTileItems; j++) {
+        if (j >= ipw)
+          break;
+        const uint32_t li = lbase + uint32_t(j) * kWave;
+        if (li < m) {
+          s_keys[li] = key[j];
+          s_aux[li] = uint16_t(aux[j]);
+          if (VALS)
+            s_vals[li] = val[j];
+        }
+      }
+      __syncthreads();
+      staged = true;
+    }
+    if (k >= kmax)
+      break; // every digit sorted
+    // runs of equal (bucket, split digits) in the current order
+    int cut = hi_shift - 8 * k;
+    if (cut < 0)
+      cut = 0;
+    uint32_t ss[kTileItems], se[kTileItems];
+    uint32_t my_work = 0;
+#pragma unroll
+    for (int j = 0; j < kTileItems; j++) {
+      if (j >= ipw)
+        break;
+      const uint32_t li = lbase + uint32_t(j) * kWave;
+      const bool valid = li < m;
+      if (valid) {
+        key[j] = s_keys[li];
+        if (VALS)
+          val[j] = s_vals[li];
+      }
+      flag[j] = valid && (li == 0 || (key[j] >> cut) != (s_keys[li - 1] >> cut));
+    }
+    uint32_t ns;
+    flag_prefix(flag, ipw, ss, s_tmp, &ns); // ss: exclusive count of run starts
+    uint16_t* s_run = reinterpret_cast<uint16_t*>(&s_wcnt[0][0][0]); // CAP entries
+    if (tid == 0)
+      s_work = 0u;
+#pragma unroll
+    for (int j = 0; j < kTileItems; j++) {
+      if (j >= ipw)
+        break;
+      const uint32_t li = lbase + uint32_t(j) * kWave;
+      if (flag[j])
+        s_run[ss[j]] = uint16_t(li);
+      if (li < m)
+        ss[j] = ss[j] + (flag[j] ? 1u : 0u) - 1u; // run index
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kTileItems; j++) {
+      if (j >= ipw)
+        break;
       const uint32_t li = lbase + uint32_t(j) * kWave;
       if (li < m) {
-        const uint32_t d = uint32_t(key[j] >> shift) & uint32_t(kTileRadix - 1);
-        const uint32_t pos = s_bstart[d] + s_wcnt[w][d] + rank[j];
-        s_keys[pos] = key[j];
-        if (VALS)
-          s_vals[pos] = val[j];
+        const uint32_t r = ss[j];
+        ss[j] = s_run[r];
+        se[j] = r + 1 < ns ? uint32_t(s_run[r + 1]) : m;
+        my_work += se[j] - ss[j];
       }
     }
+    atomicAdd(&s_work, my_work);
     __syncthreads();
-    staged = true;
+    const uint32_t work = s_work;
+    if (work <= kCountRankBudget * m) {
+      // counting rank inside each run; the output goes straight to HBM
+#pragma unroll
+      for (int j = 0; j < kTileItems; j++) {
+        if (j >= ipw)
+          break;
+        const uint32_t li = lbase + uint32_t(j) * kWave;
+        if (li < m) {
+          const KeyT kj = key[j];
+          uint32_t r = 0;
+          for (uint32_t t = ss[j]; t < se[j]; t++) {
+            const KeyT kt = s_keys[t];
+            r += (kt < kj || (kt == kj && t < li)) ? 1u : 0u;
+          }
+          const uint32_t dst = start + ss[j] + r;
+          kout[dst] = kj;
+          if (VALS)
+            vout[dst] = val[j];
+        }
+      }
+      return;
+    }
+    // long runs: sort by every low digit, then the bucket index
+    __syncthreads(); // s_run (in the counters) is read before the passes clear them
+    for (int j = tid; j < 2 * kTileWaves * kTileRadix; j += kTileThreads)
+      (&s_wcnt[0][0][0])[j] = 0u;
+    __syncthreads();
+    k = kmax;
   }
-  if (staged) {
-    for (uint32_t i = tid; i < m; i += kTileThreads) {
-      kout[start + i] = s_keys[i];
-      if (VALS)
-        vout[start + i] = s_vals[i];
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < kTileItems; j++) {
-      const uint32_t li = lbase + uint32_t(j) * kWave;
-      if (li < m) {
-        kout[start + li] = key[j];
-        if (VALS)
-          vout[start + li] = val[j];
-      }
-    }
+  for (uint32_t i = tid; i < m; i += kTileThreads) {
+    kout[start + i] = s_keys[i];
+    if (VALS)
+      vout[start + i] = s_vals[i];
   }
 }
 
@@ -860,8 +1106,12 @@ k_big_sort(KeyT* kin, KeyT* kout, uint32_t* vin, uint32_t* vout, int key_bits,
           if (VALS)
             val[j] = valid ? vs[tb + li] : 0u;
         }
+        uint32_t dg[kTileItems];
+#pragma unroll
+        for (int j = 0; j < kTileItems; j++)
+          dg[j] = uint32_t(key[j] >> shift) & uint32_t(kTileRadix - 1);
         uint32_t rank[kTileItems];
-        rank_items(key, lbase, tm, shift, dvar, rank, s_wcnt);
+        rank_digits(dg, kTileItems, lbase, tm, dvar, rank, s_wcnt);
         __syncthreads();
         const uint32_t tot = digit_wave_offsets(s_wcnt);
         __syncthreads();

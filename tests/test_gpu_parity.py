@@ -371,8 +371,8 @@ def test_autotuner_times_candidates_and_keeps_results_exact():
     tri = W.make_triangles(v)
     bvh = c.build_triangles(tri, code_bits=63, leaf_order=1, keep_codes=True)
     log = c.tune_log()
-    # 4 reduce-then-scan, 4 onesweep and 2 hybrid configurations
-    assert len(log) == 10 and all(r["ms"] > 0 for r in log)
+    # 4 configurations each of reduce-then-scan, onesweep and the hybrid
+    assert len(log) == 12 and all(r["ms"] > 0 for r in log)
     picked = [r for r in log if r["picked"]]
     assert len(picked) == 1
     inf = bvh.info()
@@ -384,21 +384,21 @@ def test_autotuner_times_candidates_and_keeps_results_exact():
     assert np.array_equal(sc, osc) and np.array_equal(ids, oids)
     # the same bucket again: no new tuning
     bvh2 = c.build_triangles(tri, code_bits=63, leaf_order=1)
-    assert len(c.tune_log()) == 10
+    assert len(c.tune_log()) == 12
     assert bvh2.info().sort_tile == inf.sort_tile
     # a batch of the same size tunes its own candidates (no onesweep)
     sizes = W.loguniform_sizes(5, 64, 1000, 8000)
     btri, off = W.batched_scenes(seed=5, sizes=sizes)
     b = c.build_batch(btri, off, leaf_order=1)
     blog = [r for r in c.tune_log() if r["batch"]]
-    assert len(blog) == 6 and sum(r["picked"] for r in blog) == 1
+    assert len(blog) == 8 and sum(r["picked"] for r in blog) == 1
     for s in (0, 31, 63):
         on, _, _, _ = oracle.build(btri[int(off[s]):int(off[s + 1])], 30, 1)
         assert b.nodes(s).tobytes() == on.tobytes()
     # explicit options bypass the tuner
     c.set_autotune(False)
     bvh3 = c.build_triangles(W.make_triangles(W.cluster_vertices(92, 1 << 19)), code_bits=30)
-    assert len(c.tune_log()) == 16
+    assert len(c.tune_log()) == 20
     assert bvh3.info().sort_strategy in (1, 2, 3)
     for x in (bvh, bvh2, bvh3, b):
         x.destroy()

@@ -74,6 +74,8 @@ struct lbvh_batch
   uint32_t* d_counts = nullptr;
   uint64_t counts_words = 0;
   void* d_treelets = nullptr;
+  void* d_tree = nullptr;           // tree kernel segments
+  uint32_t* d_tree_tickets = nullptr; // tree kernel tickets (zero at rest)
   bool sorted_in_b = false;
   uint64_t bytes = 0;
   Graph g_build, g_refit;
@@ -94,7 +96,7 @@ free_batch(lbvh_batch* b)
   void* bufs[] = { b->d_off,       b->d_nodes,  b->d_parent,      b->d_flags,   b->d_other_end,
                    b->d_leafprim,  b->d_wide,   b->d_bounds_key,  b->d_cent,    b->d_keys_a,
                    b->d_keys_b,    b->d_ids_a,  b->d_ids_b,       b->d_part_hist, b->d_small,
-                   b->d_counts,    b->d_treelets };
+                   b->d_counts,    b->d_treelets, b->d_tree, b->d_tree_tickets };
   for (void* p : bufs)
     dev_free(p);
   if (b->owns_prims)
@@ -156,6 +158,8 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
     { reinterpret_cast<void**>(&b->d_small), 4ull * b->small_words },
     { reinterpret_cast<void**>(&b->d_counts), 4 * b->counts_words },
     { &b->d_treelets, 8 * n },
+    { &b->d_tree, tree_scratch_bytes(n) },
+    { reinterpret_cast<void**>(&b->d_tree_tickets), 4 * tree_ticket_words(n) },
   };
   for (auto& r : req) {
     if (!r.bytes)
@@ -167,6 +171,9 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
     }
     b->bytes += r.bytes;
   }
+  hipError_t e = hipMemset(b->d_tree_tickets, 0, 4 * tree_ticket_words(n));
+  if (e != hipSuccess)
+    return hip_fail(c, e, "clearing the tree tickets");
   return LBVH_OK;
 }
 
@@ -208,18 +215,16 @@ queue_build(lbvh_ctx* c, lbvh_batch* b, bool timed)
     record(c, 2);
   const uint64_t* sorted = in_b ? b->d_keys_b : b->d_keys_a;
   const uint32_t* sorted_ids = b->with_ids ? (in_b ? b->d_ids_b : b->d_ids_a) : nullptr;
+  // tree and fit in one kernel (the fit phase is part of the tree phase)
+  const uint32_t* leaf_ids = b->leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr;
   if (e == hipSuccess)
-    e = launch_karras(30, sorted, b->leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr,
-                      n, b->d_off, b->nseg, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
-                      b->d_other_end, b->d_treelets, small + kSmallTreeletCount, b->d_wide, s);
-  if (timed)
+    e = launch_tree(30, sorted, leaf_ids, n, b->d_off, b->nseg, b->d_prims, 0, b->d_nodes,
+                    b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end, b->d_wide,
+                    b->d_tree, b->d_tree_tickets, s);
+  if (timed) {
     record(c, 3);
-  if (e == hipSuccess)
-    e = launch_fit(b->d_prims, 0, n, b->d_off, b->nseg, b->d_nodes, b->d_parent, b->d_leafprim,
-                   b->d_flags, b->d_treelets, small + kSmallTreeletCount,
-                   small + kSmallTreeletQueue, b->d_wide, s);
-  if (timed)
     record(c, 4, true);
+  }
   return e;
 }
 

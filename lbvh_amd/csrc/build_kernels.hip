@@ -798,6 +798,601 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
   }
 }
 
+// ---- bottom-up construction and fit in one pass ---------------------------------
+// The Karras tree of sorted codes is the binary radix tree of the keys
+// (code, position): node ranges and splits depend only on the adjacent
+// prefix lengths delta(x) = delta(x, x + 1) of calc_delta (lbvh.h:1602-1621),
+// with delta = -1 past either end of a scene.  A node [l, r] is the left
+// child of the node split at r when delta(r) > delta(l - 1), else the right
+// child of the node split at l - 1 (the two are never equal); a left child
+// is numbered r and a right child l, the root 0 -- the numbering of
+// builder_kernel (lbvh.h:1713-1731).  So the tree can be grown from the
+// leaves (Apetrei, "Fast and Simple Agglomerative LBVH Construction", 2014):
+// the second child to reach a split completes that node -- its range, its
+// box union(left, right) as fit_boxes computes it (lbvh.h:1827-1866), its
+// number, its records and its children's parent links.
+//
+// k_tree runs this one level of workgroups after another, in LDS:
+//   level 0: a workgroup per kTreeLeaves sorted positions; its elements are
+//     the leaves;
+//   level k + 1: the last of each kTreeFan level-k workgroups to finish
+//     (an agent-scope ticket; nobody waits) takes their listed subtrees, in
+//     position order, as its elements.
+// Within a workgroup, with d[e] the delta left of element e, the node split
+// between elements j and j + 1 spans elements [a, b] where d[a] and d[b + 1]
+// are the nearest deltas below d[j + 1] on either side (searched in a min
+// tree over d); it lies in the workgroup when both exist.  Its children are
+// the elements j or j + 1 or the nodes numbered j and j + 1 in element
+// space, so boxes and records follow in passes, each node once both
+// children are done (the reference's union order).  Nodes and elements
+// whose parent does not lie in the workgroup -- at most two per tree level,
+// so at most kTreeSegCap -- are listed in position order
+// ({l, r, delta(l - 1), delta(r), box}).  The single workgroup of the last
+// level completes every remaining node.  Segments and tickets cross
+// workgroups (and XCDs) through coherent stores, a vector-memory drain and
+// the ticket atomic; tickets reset themselves.
+//
+// Outputs match the Karras kernel plus the fit: node and traversal records,
+// parent links (kLink*), other_end, leafprim, and the fit flags left at 0.
+
+constexpr int kTreeLeaves = kTreelet;                   // level-0 positions per workgroup
+constexpr int kTreeFan = 8;                             // workgroups merged per level
+constexpr int kTreeSegCap = 192;                        // 2 x (tree height bound + 1)
+constexpr int kTreeSlots = kTreeFan * kTreeSegCap;      // elements of one workgroup
+static_assert(kTreeSlots >= kTreeLeaves, "level-0 slots");
+
+// A listed subtree: leaf range [l, r], delta(l - 1) and delta(r) (packed as
+// int16 pairs), box.
+struct TreeEntry
+{
+  uint32_t l, r;
+  uint32_t d; // (delta(l - 1) & 0xffff) | delta(r) << 16
+  float box[6];
+};
+static_assert(sizeof(TreeEntry) == 36, "TreeEntry layout");
+
+// delta(x, x + 1) over sorted batch positions; -1 at scene edges.
+template<typename KeyT, typename CodeT, bool SEG>
+__device__ __forceinline__ int
+adj_delta(const KeyT* __restrict__ keys, const uint64_t* __restrict__ prim_off, int64_t N,
+          int64_t x)
+{
+  if (x < 0 || x >= N - 1)
+    return -1;
+  const KeyT a = keys[x];
+  const KeyT b = keys[x + 1];
+  if constexpr (SEG) {
+    if ((a >> 30) != (b >> 30))
+      return -1;
+  }
+  const CodeT ca = key_code<KeyT, CodeT, SEG>(a);
+  const CodeT cb = key_code<KeyT, CodeT, SEG>(b);
+  if (ca != cb)
+    return CodeTraits<CodeT>::clz(ca ^ cb);
+  int64_t i = x;
+  if constexpr (SEG)
+    i -= int64_t(prim_off[uint32_t(a >> 30)]);
+  return CodeTraits<CodeT>::clz(CodeT(uint64_t(i ^ (i + 1)))) + CodeTraits<CodeT>::bits;
+}
+
+// One scene's arrays, located from any of its sorted positions.
+struct TreeScene
+{
+  int64_t P, n;
+  const float* prims;
+  Node* nodes;
+  WideNode* wide;
+  uint32_t* parent;
+  uint32_t* leafprim;
+  uint32_t* flags;
+  uint32_t* other_end;
+};
+
+template<typename KeyT, bool SEG>
+__device__ __forceinline__ TreeScene
+tree_scene(const KeyT* __restrict__ keys, const Scenes& sc, int64_t x, const float* prims_all,
+           int kind, Node* nodes_all, WideNode* wide_all, uint32_t* parent_all,
+           uint32_t* leafprim_all, uint32_t* flags_all, uint32_t* other_end_all)
+{
+  uint32_t seg = 0;
+  int64_t P = 0, n = sc.n;
+  if constexpr (SEG) {
+    seg = uint32_t(keys[x] >> 30);
+    P = int64_t(sc.prim_off[seg]);
+    n = int64_t(sc.prim_off[seg + 1]) - P;
+  }
+  const int64_t nb = P - int64_t(seg);
+  const int64_t lb = 2 * P - int64_t(seg);
+  TreeScene t;
+  t.P = P;
+  t.n = n;
+  t.prims = prims_all + int64_t(prim_stride(kind)) * P;
+  t.nodes = nodes_all + nb;
+  t.wide = wide_all + nb;
+  t.parent = parent_all + lb;
+  t.leafprim = leafprim_all ? leafprim_all + P : nullptr;
+  t.flags = flags_all + nb;
+  t.other_end = other_end_all + nb;
+  return t;
+}
+
+// Child value of the leaf at sorted position x.
+__device__ __forceinline__ uint32_t
+tree_leaf_value(const TreeScene& ts, const uint32_t* __restrict__ ids, int64_t x)
+{
+  return (ts.leafprim ? uint32_t(int64_t(ids[x]) - ts.P) : uint32_t(x - ts.P)) | kLeafBit;
+}
+
+struct TreeLevels
+{
+  int levels;
+  uint32_t nblocks[16];     // workgroups of each level (the last has 1)
+  uint32_t seg_base[16];    // first segment of each level
+  uint32_t ticket_base[16]; // first ticket of each level's groups
+};
+
+constexpr int kTreeLeaf0 = 2048; // leaves of the delta min-tree (>= kTreeSlots + 1)
+static_assert(kTreeLeaf0 >= kTreeSlots + 1, "delta tree size");
+constexpr int16_t kTreeInf = 0x7fff;
+
+// Largest i <= j with d[i] < x, or -1 (tree: heap-ordered minima of d,
+// leaves at kTreeLeaf0 + i).
+__device__ __forceinline__ int
+tree_left_less(const int16_t* __restrict__ tree, int j, int x)
+{
+  int p = j + kTreeLeaf0;
+  if (tree[p] < x)
+    return j;
+  while (p > 1) {
+    if ((p & 1) && tree[p - 1] < x) {
+      p = p - 1;
+      while (p < kTreeLeaf0)
+        p = (tree[2 * p + 1] < x) ? 2 * p + 1 : 2 * p;
+      return p - kTreeLeaf0;
+    }
+    p >>= 1;
+  }
+  return -1;
+}
+
+// Smallest i >= j with d[i] < x, or -1.
+__device__ __forceinline__ int
+tree_right_less(const int16_t* __restrict__ tree, int j, int x)
+{
+  int p = j + kTreeLeaf0;
+  if (tree[p] < x)
+    return j;
+  while (p > 1) {
+    if (!(p & 1) && tree[p + 1] < x) {
+      p = p + 1;
+      while (p < kTreeLeaf0)
+        p = (tree[2 * p] < x) ? 2 * p : 2 * p + 1;
+      return p - kTreeLeaf0;
+    }
+    p >>= 1;
+  }
+  return -1;
+}
+
+// Union of element boxes [a, b] left to right, from a bottom-up segment
+// tree over m elements (leaves at bt[m + e]; bt[p] = union(bt[2p],
+// bt[2p + 1])).  std::min / std::max with the left operand kept on ties
+// are associative on non-NaN values, so any order-preserving grouping gives
+// the bits of the reference's union of its two children, recursively.
+__device__ __forceinline__ Box
+bt_query(const Box* __restrict__ bt, int m, int a, int b)
+{
+  int l = a + m, r = b + m + 1;
+  Box rl, rr;
+  bool hl = false, hr = false;
+  while (l < r) {
+    if (l & 1) {
+      rl = hl ? box_union(rl, bt[l]) : bt[l];
+      hl = true;
+      l++;
+    }
+    if (r & 1) {
+      r--;
+      rr = hr ? box_union(bt[r], rr) : bt[r];
+      hr = true;
+    }
+    l >>= 1;
+    r >>= 1;
+  }
+  if (hl && hr)
+    return box_union(rl, rr);
+  return hl ? rl : rr;
+}
+
+__device__ __forceinline__ bool
+box_has_nan(const Box& b)
+{
+  bool x = false;
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+    x = x || (b.lo[q] != b.lo[q]) || (b.hi[q] != b.hi[q]);
+  return x;
+}
+
+// FIRST: the level-0 kernel (leaves, 1,024 elements: two workgroups fit a CU);
+// else the kernel of the levels above (its workgroup b starts as level-1
+// workgroup b).
+template<typename KeyT, typename CodeT, bool SEG, bool FIRST>
+__global__ __launch_bounds__(kTreeLeaves) void
+k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes sc,
+       const float* __restrict__ prims_all, int kind, Node* __restrict__ nodes_all,
+       WideNode* __restrict__ wide_all, uint32_t* __restrict__ parent_all,
+       uint32_t* __restrict__ leafprim_all, uint32_t* __restrict__ flags_all,
+       uint32_t* __restrict__ other_end_all, TreeLevels lv, uint32_t* __restrict__ seg_count,
+       TreeEntry* __restrict__ seg_entries, uint32_t* __restrict__ tickets)
+{
+  constexpr int SL = FIRST ? kTreeLeaves : kTreeSlots; // elements per workgroup
+  // elements (level 0: leaves; then listed subtrees), in position order
+  __shared__ uint32_t s_el_l[SL];
+  __shared__ uint32_t s_el_r[SL];
+  // element boxes at s_bt[m + e]; below m: the segment tree over them, or
+  // (NaN boxes present) node j's box at s_bt[1 + j]
+  __shared__ Box s_bt[2 * SL];
+  __shared__ uint32_t s_el_scene[SEG ? SL : 1];
+  __shared__ uint32_t s_el_val[SL]; // child value when the element is one leaf
+  // min tree over d (leaves at kTreeLeaf0): d[e] is the delta left of
+  // element e, d[m] the delta right of the last
+  __shared__ int16_t s_tree[2 * kTreeLeaf0];
+  int16_t* const s_d = s_tree + kTreeLeaf0;
+  // split j (between elements j and j + 1): its node's element range [a, b]
+  __shared__ uint16_t s_na[SL];
+  __shared__ uint16_t s_nb[SL];
+  __shared__ uint8_t s_inb[SL];     // 1: the node's range lies in the workgroup's
+  __shared__ uint8_t s_round[SL];   // pass in which its box was completed (0: not yet)
+  __shared__ uint16_t s_kid[SL];    // element-space node number -> split
+  __shared__ uint16_t s_esc[SL];    // listed subtree starting at element e (+1; 0: none)
+  __shared__ uint32_t s_wsum[kTreeLeaves / kWave];
+  __shared__ uint32_t s_pre[kTreeFan + 1];
+  __shared__ uint32_t s_last;
+
+  const int t = threadIdx.x;
+  const int lane = __lane_id();
+  const int w = t / kWave;
+  const int64_t N = sc.n;
+  int level = FIRST ? 0 : 1;
+  uint32_t blk = blockIdx.x;
+  int64_t lo = 0, hi = 0;
+  int m = 0;
+
+  // the listed subtrees of level-(level - 1) workgroups [blk * fan, ...), in order
+  auto load_group = [&]() {
+    int64_t span = kTreeLeaves;
+    for (int q = 0; q < level; q++)
+      span *= kTreeFan;
+    lo = int64_t(blk) * span;
+    hi = (N - lo < span) ? N : lo + span;
+    const uint32_t members = (lv.nblocks[level - 1] - blk * kTreeFan) < uint32_t(kTreeFan)
+                               ? lv.nblocks[level - 1] - blk * kTreeFan
+                               : uint32_t(kTreeFan);
+    if (t == 0) {
+      uint32_t acc = 0;
+      s_pre[0] = 0;
+      for (uint32_t c = 0; c < uint32_t(kTreeFan); c++) {
+        const uint32_t cnt =
+          c < members ? coherent_load_u32(&seg_count[lv.seg_base[level - 1] + blk * kTreeFan + c])
+                      : 0u;
+        acc += cnt;
+        s_pre[c + 1] = acc;
+      }
+    }
+    __syncthreads();
+    m = int(s_pre[kTreeFan]);
+    for (int e = t; e < m; e += kTreeLeaves) {
+      uint32_t c = 0;
+      while (c + 1 < uint32_t(kTreeFan) && uint32_t(e) >= s_pre[c + 1])
+        c++;
+      const uint32_t sgi = lv.seg_base[level - 1] + blk * kTreeFan + c;
+      const TreeEntry* en = &seg_entries[size_t(sgi) * kTreeSegCap + (uint32_t(e) - s_pre[c])];
+      const uint32_t el = coherent_load_u32(&en->l);
+      const uint32_t er = coherent_load_u32(&en->r);
+      const uint32_t d = coherent_load_u32(&en->d);
+      Box bx;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        bx.lo[q] = coherent_load_f32(&en->box[q]);
+        bx.hi[q] = coherent_load_f32(&en->box[3 + q]);
+      }
+      s_el_l[e] = el;
+      s_el_r[e] = er;
+      s_bt[m + e] = bx;
+      s_d[e + 1] = int16_t(d >> 16);
+      if (e == 0)
+        s_d[0] = int16_t(d & 0xffffu);
+      int64_t P = 0;
+      if constexpr (SEG) {
+        const uint32_t sg = uint32_t(keys[el] >> 30);
+        P = int64_t(sc.prim_off[sg]);
+        s_el_scene[e] = sg;
+      }
+      if (el == er)
+        s_el_val[e] = (leafprim_all ? uint32_t(int64_t(ids[el]) - P) : uint32_t(int64_t(el) - P)) |
+                      kLeafBit;
+    }
+    __syncthreads();
+  };
+
+  if constexpr (FIRST) {
+    // ---- level 0 elements: the leaves -------------------------------------------
+    lo = int64_t(blk) * kTreeLeaves;
+    hi = (N - lo < kTreeLeaves) ? N : lo + kTreeLeaves;
+    m = int(hi - lo);
+    s_d[t] = int16_t(adj_delta<KeyT, CodeT, SEG>(keys, sc.prim_off, N, lo - 1 + t));
+    if (t == 0)
+      s_d[kTreeLeaves] =
+        int16_t(adj_delta<KeyT, CodeT, SEG>(keys, sc.prim_off, N, lo - 1 + kTreeLeaves));
+    if (t < m) {
+      const int64_t x = lo + t;
+      const TreeScene ts = tree_scene<KeyT, SEG>(keys, sc, x, prims_all, kind, nodes_all,
+                                                 wide_all, parent_all, leafprim_all, flags_all,
+                                                 other_end_all);
+      const int64_t lx = x - ts.P;
+      if (lx < ts.n - 1)
+        ts.flags[lx] = 0u; // fit flags (refits) start at 0
+      uint32_t prim = uint32_t(lx);
+      if (ts.leafprim) {
+        prim = uint32_t(int64_t(ids[x]) - ts.P);
+        ts.leafprim[lx] = prim;
+      }
+      s_el_l[t] = uint32_t(x);
+      s_el_r[t] = uint32_t(x);
+      s_el_val[t] = prim | kLeafBit;
+      if constexpr (SEG)
+        s_el_scene[t] = uint32_t(keys[x] >> 30);
+      s_bt[m + t] = load_prim_box(ts.prims, kind, uint64_t(prim));
+    }
+    __syncthreads();
+  } else {
+    load_group();
+  }
+
+  for (;;) {
+    // ---- nodes of this workgroup: ranges from the nearest smaller deltas -------
+    for (int q = m + 1 + t; q < kTreeLeaf0; q += kTreeLeaves)
+      s_tree[kTreeLeaf0 + q] = kTreeInf;
+    __syncthreads();
+    // exact NaN semantics need the unions in tree order (passes below)
+    int any_nan = 0;
+    for (int e = t; e < m; e += kTreeLeaves)
+      any_nan |= box_has_nan(s_bt[m + e]) ? 1 : 0;
+    const bool nan = __syncthreads_or(any_nan) != 0;
+    for (int half = kTreeLeaf0 / 2; half >= 1; half >>= 1) {
+      for (int q = t; q < half; q += kTreeLeaves) {
+        const int p = half + q;
+        const int16_t a0 = s_tree[2 * p], a1 = s_tree[2 * p + 1];
+        s_tree[p] = a0 < a1 ? a0 : a1;
+        if (!nan && p < m)
+          s_bt[p] = box_union(s_bt[2 * p], s_bt[2 * p + 1]);
+      }
+      __syncthreads();
+    }
+    for (int j = t; j < m - 1; j += kTreeLeaves) {
+      const int x = s_d[j + 1];
+      uint8_t inb = 0;
+      if (x >= 0) {
+        const int i = tree_left_less(s_tree, j, x);
+        const int k = tree_right_less(s_tree, j + 2, x);
+        if (i >= 0 && k >= 0) {
+          inb = 1;
+          s_na[j] = uint16_t(i);
+          s_nb[j] = uint16_t(k - 1);
+        }
+      }
+      s_inb[j] = inb;
+      s_round[j] = 0;
+    }
+    for (int e = t; e < m; e += kTreeLeaves) {
+      s_esc[e] = 0;
+      s_kid[e] = 0xffff;
+    }
+    __syncthreads();
+    for (int j = t; j < m - 1; j += kTreeLeaves) {
+      if (s_inb[j]) {
+        const int a = s_na[j], b = s_nb[j];
+        s_kid[(s_d[b + 1] > s_d[a]) ? b : a] = uint16_t(j);
+      }
+    }
+    __syncthreads();
+    // ---- boxes, children before parents (LDS only: a barrier waits for every
+    // outstanding global store, so the records are written afterwards) ------
+    for (int pass = 1; nan; pass++) {
+      int changed = 0;
+      for (int j = t; j < m - 1; j += kTreeLeaves) {
+        if (!s_inb[j] || s_round[j] != 0)
+          continue;
+        const int a = s_na[j], b = s_nb[j];
+        const bool l_el = (a == j);
+        const bool r_el = (j + 1 == b);
+        int lc = -1, rc = -1;
+        if (!l_el) {
+          lc = s_kid[j];
+          const int rd = s_round[lc];
+          if (rd == 0 || rd >= pass)
+            continue;
+        }
+        if (!r_el) {
+          rc = s_kid[j + 1];
+          const int rd = s_round[rc];
+          if (rd == 0 || rd >= pass)
+            continue;
+        }
+        const Box lb = l_el ? s_bt[m + j] : s_bt[1 + lc];
+        const Box rb = r_el ? s_bt[m + j + 1] : s_bt[1 + rc];
+        s_bt[1 + j] = box_union(lb, rb);
+        s_round[j] = uint8_t(pass);
+        changed = 1;
+      }
+      if (!__syncthreads_or(changed))
+        break;
+    }
+    // ---- records of the completed nodes ------------------------------------------
+    // children's parent links by split (consecutive positions), node and
+    // traversal records by node number (consecutive records)
+    auto node_scene = [&](int j, int64_t& P, int64_t& n, uint32_t& seg) {
+      P = 0;
+      n = N;
+      seg = 0;
+      if constexpr (SEG) {
+        seg = s_el_scene[j];
+        P = int64_t(sc.prim_off[seg]);
+        n = int64_t(sc.prim_off[seg + 1]) - P;
+      }
+    };
+    for (int j = t; j < m - 1; j += kTreeLeaves) {
+      if (!s_inb[j])
+        continue;
+      const int a = s_na[j], b = s_nb[j];
+      const int64_t L = s_el_l[a];
+      const int64_t R = s_el_r[b];
+      const int64_t g = s_el_r[j];
+      int64_t P, n;
+      uint32_t seg;
+      node_scene(j, P, n, seg);
+      const int64_t k = (int(s_d[b + 1]) > int(s_d[a])) ? R - P : L - P;
+      const bool lleaf = (a == j) && s_el_l[j] == s_el_r[j];
+      const bool rleaf = (j + 1 == b) && s_el_l[j + 1] == s_el_r[j + 1];
+      const int64_t lg = g - P;
+      const uint32_t up = uint32_t(k) | ((R - L + 1) > kTreelet ? kLinkGlobal : 0u);
+      uint32_t* parent = parent_all + (2 * P - int64_t(seg));
+      parent[lleaf ? (n - 1) + lg : lg] = up | (rleaf ? kLinkSibLeaf : 0u);
+      parent[rleaf ? (n - 1) + lg + 1 : lg + 1] = up | kLinkRight | (lleaf ? kLinkSibLeaf : 0u);
+    }
+    for (int e = t; e < m; e += kTreeLeaves) {
+      const uint32_t jj = s_kid[e];
+      if (jj == 0xffffu)
+        continue; // no node is numbered e
+      const int j = int(jj);
+      const int a = s_na[j], b = s_nb[j];
+      const bool l_el = (a == j);
+      const bool r_el = (j + 1 == b);
+      Box lb, rb, bx;
+      if (nan) {
+        lb = l_el ? s_bt[m + j] : s_bt[1 + s_kid[j]];
+        rb = r_el ? s_bt[m + j + 1] : s_bt[1 + s_kid[j + 1]];
+        bx = s_bt[1 + j];
+      } else {
+        lb = bt_query(s_bt, m, a, j);
+        rb = bt_query(s_bt, m, j + 1, b);
+        bx = box_union(lb, rb);
+      }
+      const int64_t L = s_el_l[a];
+      const int64_t R = s_el_r[b];
+      const int64_t g = s_el_r[j];
+      int64_t P, n;
+      uint32_t seg;
+      node_scene(j, P, n, seg);
+      const int64_t nb = P - int64_t(seg);
+      const int64_t k = (int(s_d[b + 1]) > int(s_d[a])) ? R - P : L - P;
+      const bool lleaf = l_el && s_el_l[j] == s_el_r[j];
+      const bool rleaf = r_el && s_el_l[j + 1] == s_el_r[j + 1];
+      const int64_t lg = g - P;
+      const uint32_t lvv = lleaf ? s_el_val[j] : uint32_t(lg);
+      const uint32_t rvv = rleaf ? s_el_val[j + 1] : uint32_t(lg + 1);
+      store_wide(&wide_all[nb + k], lb, rb, lvv, rvv);
+      store_node(&nodes_all[nb + k], bx, lvv, rvv);
+      other_end_all[nb + k] = uint32_t((k == L - P) ? R - P : L - P);
+    }
+    // ---- listed subtrees: elements and nodes whose parent is not in here ------
+    auto parent_out = [&](int a, int b) {
+      const int da = s_d[a], db = s_d[b + 1];
+      if (da < 0 && db < 0)
+        return false; // the scene's root
+      const int ps = (db > da) ? b : a - 1;
+      return ps < 0 || ps > m - 2 || !s_inb[ps];
+    };
+    for (int e = t; e < m; e += kTreeLeaves)
+      if (parent_out(e, e))
+        s_esc[e] = uint16_t(e + 1); // an element stands alone
+    __syncthreads();
+    for (int j = t; j < m - 1; j += kTreeLeaves)
+      if (s_inb[j] && parent_out(s_na[j], s_nb[j]))
+        s_esc[s_na[j]] = uint16_t(kTreeSlots + 1 + j); // a completed node
+    __syncthreads();
+    // position order of the listed subtrees: exclusive count of list starts
+    const int e0 = 2 * t, e1 = 2 * t + 1;
+    const uint32_t f0 = (e0 < m && s_esc[e0]) ? 1u : 0u;
+    const uint32_t f1 = (e1 < m && s_esc[e1]) ? 1u : 0u;
+    uint32_t incl = f0 + f1;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off, kWave);
+      if (lane >= off)
+        incl += y;
+    }
+    if (lane == kWave - 1)
+      s_wsum[w] = incl;
+    __syncthreads();
+    uint32_t base = incl - (f0 + f1);
+    uint32_t total = 0;
+#pragma unroll
+    for (int q = 0; q < kTreeLeaves / kWave; q++) {
+      const uint32_t v = s_wsum[q];
+      base += (q < w) ? v : 0u;
+      total += v;
+    }
+    TreeEntry* out = &seg_entries[size_t(lv.seg_base[level] + blk) * kTreeSegCap];
+    for (int h = 0; h < 2; h++) {
+      const int e = h == 0 ? e0 : e1;
+      if (e >= m || !s_esc[e])
+        continue;
+      const uint32_t o = base + (h == 1 ? f0 : 0u);
+      if (o >= uint32_t(kTreeSegCap))
+        continue; // cannot happen: two per tree level at most
+      const uint32_t tag = s_esc[e];
+      int a, b;
+      Box bx;
+      if (tag > uint32_t(kTreeSlots)) {
+        const int j = int(tag) - kTreeSlots - 1;
+        a = s_na[j];
+        b = s_nb[j];
+        bx = nan ? s_bt[1 + j] : bt_query(s_bt, m, a, b);
+      } else {
+        a = b = e;
+        bx = s_bt[m + e];
+      }
+      TreeEntry* en = &out[o];
+      coherent_store_u32(&en->l, s_el_l[a]);
+      coherent_store_u32(&en->r, s_el_r[b]);
+      coherent_store_u32(&en->d, (uint32_t(int(s_d[a])) & 0xffffu) | (uint32_t(int(s_d[b + 1])) << 16));
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        coherent_store_f32(&en->box[q], bx.lo[q]);
+        coherent_store_f32(&en->box[3 + q], bx.hi[q]);
+      }
+    }
+    if (t == 0)
+      coherent_store_u32(&seg_count[lv.seg_base[level] + blk],
+                         total < uint32_t(kTreeSegCap) ? total : uint32_t(kTreeSegCap));
+    if (FIRST || level + 1 >= lv.levels)
+      return; // level 0 hands over to the next kernel; the last level completes every node
+    // ---- hand over: the last of this group's workgroups goes on ---------------
+    vmem_drain();
+    __syncthreads();
+    const uint32_t grp = blk / kTreeFan;
+    const uint32_t members = (lv.nblocks[level] - grp * kTreeFan) < uint32_t(kTreeFan)
+                               ? lv.nblocks[level] - grp * kTreeFan
+                               : uint32_t(kTreeFan);
+    if (t == 0) {
+      uint32_t* tk = &tickets[lv.ticket_base[level] + grp];
+      const uint32_t old = atomic_add_agent(tk, 1u);
+      const bool last = old + 1u == members;
+      if (last)
+        coherent_store_u32(tk, 0u); // reset for the next build
+      s_last = last ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last)
+      return;
+    level++;
+    blk = grp;
+    load_group();
+  }
+}
+
 // Rebuilds the treelet list of an existing tree (refit): every child that is
 // not a top node but whose parent is roots a treelet; its leaf range is
 // [c, other_end[c]] (either order) for an internal node, [k, k] for a leaf.
@@ -974,6 +1569,93 @@ launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off, u
   hipLaunchKernelGGL(k_fit_top, dim3(gb), dim3(kBlock), 0, s, prims, kind, sc,
                      static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, flags,
                      static_cast<WideNode*>(wide));
+  return hipGetLastError();
+}
+
+// Level sizes of k_tree for n positions, and the scratch it needs.
+static TreeLevels
+tree_levels(uint64_t n)
+{
+  TreeLevels lv{};
+  uint32_t nb = uint32_t((n + kTreeLeaves - 1) / kTreeLeaves);
+  uint32_t segs = 0, tickets = 0;
+  lv.levels = 0;
+  for (;;) {
+    lv.nblocks[lv.levels] = nb;
+    lv.seg_base[lv.levels] = segs;
+    lv.ticket_base[lv.levels] = tickets;
+    segs += nb;
+    lv.levels++;
+    if (nb == 1 || lv.levels == 16)
+      break;
+    const uint32_t groups = (nb + kTreeFan - 1) / kTreeFan;
+    tickets += groups;
+    nb = groups;
+  }
+  return lv;
+}
+
+uint64_t
+tree_scratch_bytes(uint64_t n)
+{
+  const TreeLevels lv = tree_levels(n);
+  const uint64_t segs = uint64_t(lv.seg_base[lv.levels - 1]) + 1;
+  // segment counts, then the entries
+  const uint64_t head = (segs * 4 + 255) / 256 * 256;
+  return head + segs * kTreeSegCap * sizeof(TreeEntry);
+}
+
+uint64_t
+tree_ticket_words(uint64_t n)
+{
+  const TreeLevels lv = tree_levels(n);
+  return uint64_t(lv.ticket_base[lv.levels - 1]) + 1;
+}
+
+template<typename KeyT, typename CodeT, bool SEG>
+static void
+tree_launch(const void* sorted, const uint32_t* ids, const Scenes& sc, const float* prims,
+            int kind, void* nodes, uint32_t* parent, uint32_t* leafprim, uint32_t* flags,
+            uint32_t* other_end, void* wide, void* scratch, uint32_t* tk, hipStream_t s)
+{
+  const uint64_t n = uint64_t(sc.n);
+  const TreeLevels lv = tree_levels(n);
+  const uint64_t segs = uint64_t(lv.seg_base[lv.levels - 1]) + 1;
+  uint32_t* counts = static_cast<uint32_t*>(scratch);
+  const uint64_t head = (segs * 4 + 255) / 256 * 256;
+  TreeEntry* entries = reinterpret_cast<TreeEntry*>(static_cast<char*>(scratch) + head);
+  hipLaunchKernelGGL((k_tree<KeyT, CodeT, SEG, true>), dim3(lv.nblocks[0]), dim3(kTreeLeaves), 0,
+                     s, static_cast<const KeyT*>(sorted), ids, sc, prims, kind,
+                     static_cast<Node*>(nodes), static_cast<WideNode*>(wide), parent, leafprim,
+                     flags, other_end, lv, counts, entries, tk);
+  if (lv.levels > 1)
+    hipLaunchKernelGGL((k_tree<KeyT, CodeT, SEG, false>), dim3(lv.nblocks[1]), dim3(kTreeLeaves),
+                       0, s, static_cast<const KeyT*>(sorted), ids, sc, prims, kind,
+                       static_cast<Node*>(nodes), static_cast<WideNode*>(wide), parent, leafprim,
+                       flags, other_end, lv, counts, entries, tk);
+}
+
+hipError_t
+launch_tree(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
+            const uint64_t* prim_off, uint32_t nseg, const float* prims, int kind, void* nodes,
+            uint32_t* parent, uint32_t* leafprim, uint32_t* flags, uint32_t* other_end,
+            void* wide, void* scratch, uint32_t* tickets, hipStream_t s)
+{
+  const Scenes sc = make_scenes(prim_off, nseg, n);
+  if (tree_levels(n).nblocks[tree_levels(n).levels - 1] != 1)
+    return hipErrorInvalidValue; // more levels than the launch descriptor holds
+  if (prim_off) {
+    if (code_bits != 30)
+      return hipErrorInvalidValue;
+    tree_launch<uint64_t, uint32_t, true>(sorted, ids, sc, prims, kind, nodes, parent, leafprim,
+                                          flags, other_end, wide, scratch, tickets, s);
+  } else if (code_bits == 30) {
+    tree_launch<uint32_t, uint32_t, false>(sorted, ids, sc, prims, kind, nodes, parent, leafprim,
+                                           flags, other_end, wide, scratch, tickets, s);
+  } else {
+    tree_launch<uint64_t, uint64_t, false>(sorted, ids, sc, prims, kind, nodes, parent, leafprim,
+                                           flags, other_end, wide, scratch, tickets, s);
+  }
   return hipGetLastError();
 }
 

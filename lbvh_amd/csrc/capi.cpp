@@ -395,7 +395,9 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
   if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
     return st;
-  if ((st = ensure(c, c->treelets, 8 * n)) != LBVH_OK)
+  if ((st = ensure(c, c->tree, tree_scratch_bytes(n))) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->tree_tickets, 4 * tree_ticket_words(n), true)) != LBVH_OK)
     return st;
   const uint64_t counts_words = sort_counts_words(n, cfg);
   if ((st = ensure(c, c->counts, 4 * counts_words)) != LBVH_OK)
@@ -465,16 +467,13 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   void* sorted = in_b ? c->keys_b.p : c->keys_a.p;
   uint32_t* sorted_ids = with_ids ? static_cast<uint32_t*>(in_b ? c->ids_b.p : c->ids_a.p)
                                   : nullptr;
+  // tree and fit in one kernel (the fit phase is part of the tree phase)
+  const uint32_t* leaf_ids = opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr;
   if (e == hipSuccess)
-    e = launch_karras(opts.code_bits, sorted,
-                      opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr, n,
-                      nullptr, 1, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end,
-                      c->treelets.p, small + kSmallTreeletCount, b->d_wide, c->stream);
+    e = launch_tree(opts.code_bits, sorted, leaf_ids, n, nullptr, 1, d_prims, kind, b->d_nodes,
+                    b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end, b->d_wide, c->tree.p,
+                    static_cast<uint32_t*>(c->tree_tickets.p), c->stream);
   record(c, 3);
-  if (e == hipSuccess)
-    e = launch_fit(d_prims, kind, n, nullptr, 1, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
-                   c->treelets.p, small + kSmallTreeletCount, small + kSmallTreeletQueue,
-                   b->d_wide, c->stream);
   record(c, 4, true);
   if (e == hipSuccess && opts.keep_codes) {
     e = hipMemcpyAsync(b->d_codes, sorted, kw * n, hipMemcpyDeviceToDevice, c->stream);
@@ -587,7 +586,7 @@ lbvh_ctx_destroy(lbvh_ctx* c)
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = { &c->cent, &c->keys_a, &c->keys_b, &c->ids_a, &c->ids_b, &c->partials,
                      &c->part_hist, &c->small, &c->status, &c->counts, &c->treelets,
-                     &c->io_in, &c->io_out };
+                     &c->io_in, &c->io_out, &c->tree, &c->tree_tickets };
   for (auto* b : bufs)
     dev_free(b->p);
   for (auto& es : c->sets)
@@ -702,7 +701,7 @@ lbvh_ctx_workspace_bytes(const lbvh_ctx* c)
     return 0;
   return c->cent.cap + c->keys_a.cap + c->keys_b.cap + c->ids_a.cap + c->ids_b.cap +
          c->partials.cap + c->part_hist.cap + c->small.cap + c->status.cap + c->counts.cap +
-         c->treelets.cap + c->io_in.cap + c->io_out.cap;
+         c->treelets.cap + c->io_in.cap + c->io_out.cap + c->tree.cap + c->tree_tickets.cap;
 }
 
 lbvh_status

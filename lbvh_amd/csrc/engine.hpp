@@ -102,6 +102,19 @@ hipError_t launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* 
 
 // Rebuilds the treelet list of a built tree (for a refit); *treelet_count
 // must be zero.
+// Bottom-up tree construction and fit in one kernel (levels of workgroups
+// merging in LDS).  Writes node and traversal records, parent links,
+// other_end, leafprim (ids != nullptr and primitive-id leaves) and leaves
+// the fit flags at 0.  `scratch` holds tree_scratch_bytes(n) bytes;
+// `tickets` holds tree_ticket_words(n) words that were zero when first used
+// (the kernel leaves them at zero).
+hipError_t launch_tree(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
+                       const uint64_t* prim_off, uint32_t nseg, const float* prims, int kind,
+                       void* nodes, uint32_t* parent, uint32_t* leafprim, uint32_t* flags,
+                       uint32_t* other_end, void* wide, void* scratch, uint32_t* tickets,
+                       hipStream_t s);
+uint64_t tree_scratch_bytes(uint64_t n);
+uint64_t tree_ticket_words(uint64_t n);
 hipError_t launch_enum_treelets(uint64_t n, const uint64_t* prim_off, uint32_t nseg,
                                 const uint32_t* parent, const uint32_t* other_end,
                                 void* treelets, uint32_t* treelet_count, hipStream_t s);

@@ -67,7 +67,7 @@ struct lbvh_ctx
   std::vector<lbvh_rt::TuneRecord> tune_log;
   // workspace
   lbvh_rt::DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, part_hist, small, status, counts,
-    treelets, io_in, io_out;
+    treelets, io_in, io_out, tree, tree_tickets;
 };
 
 // A built tree: node array, parent links, fit flags, centroid bounds,

@@ -27,25 +27,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 N_PRIMS = 1 << 20
 WIDTH, HEIGHT = 1920, 1080
-SORT_PASS_BYTES_PER_PRIM_63 = 24  # read + write of an 8-byte key and a 4-byte id
 SORT_NAMES = {1: "onesweep", 2: "reduce-then-scan", 3: "hybrid"}
-
-
-def sort_data_passes(info, key_bits):
-    """Passes of the sort over every key and id in HBM: one per digit for
-    onesweep and reduce-then-scan; for the hybrid, one per global digit plus
-    the LDS bucket sort (which reads and writes each key and id once)."""
-    if info.sort_strategy == 3:
-        return info.sort_global_digits + 1
-    d = info.sort_digit_bits
-    return (key_bits + d - 1) // d
-
-
-def build_bytes_per_prim_63(passes):
-    """Algorithmic bytes per primitive of the build with 63-bit codes (u64 key
-    + u32 id): bounds 36, encode 48, sort 8 + 24 per data pass, tree 16,
-    fit 92."""
-    return 36 + 48 + (8 + passes * 24) + 16 + 92
+# Algorithmic bytes per primitive of the build with 63-bit codes (u64 key +
+# u32 id), the canonical accounting whatever sort the tuner picks: bounds 36,
+# encode 48, sort 8 + 8 digit passes x 24, tree 16, fit 92.
+SORT_BYTES_PER_PRIM_63 = 8 + 8 * 24
+BUILD_BYTES_PER_PRIM_63 = 36 + 48 + SORT_BYTES_PER_PRIM_63 + 16 + 92
+# Bytes the closest-hit traversal touches on the benchmark workload (primary
+# + bounce rays on the 1M clustered tree): popped nodes x 32 + box-tested
+# children x 32 + tested triangles x 48, counted by the instrumented CPU
+# traversal of the same tree (tests/test_workloads.py recomputes it).
+TRACE_TOUCHED_BYTES_PER_STEP = 3857317376
 RAY_COMPULSORY_BYTES = 48  # 32-byte ray in, 16-byte hit out
 
 
@@ -263,8 +255,9 @@ def bench_batched(args):
                    "n_prims": n, "n_scenes": nseg, "rays": m},
         "batched": {"build_ms": round(build_ms, 4), "trace_ms": round(trace_ms, 4),
                     "build_phases_ms": {"bounds+encode": round(split[0], 4),
-                                        "sort": round(split[1], 4), "tree": round(split[2], 4),
-                                        "fit": round(split[3], 4)} if len(split) == 4 else split,
+                                        "sort": round(split[1], 4),
+                                        "tree+fit": round(split[2] + split[3], 4)}
+                    if len(split) == 4 else split,
                     "closest_hit_mrays_s": round(m / (trace_ms * 1e-3) / 1e6, 3),
                     "sort": {"digit_bits": info.sort_digit_bits, "tile": info.sort_tile,
                              "strategy": SORT_NAMES.get(info.sort_strategy, "?"),
@@ -527,8 +520,9 @@ def bench_large(args):
         inf = trees[0].info()
         out[bits] = {"build_ms": round(build_ms, 4),
                      "mprims_s": round(n / (build_ms * 1e-3) / 1e6, 2),
-                     "phases_ms": {k: round(x * scale, 4) for k, x in
-                                   zip(("encode (bounds+codes)", "sort", "tree", "fit"), ph)},
+                     "phases_ms": {"encode (bounds+codes)": round(ph[0] * scale, 4),
+                                   "sort": round(ph[1] * scale, 4),
+                                   "tree+fit": round((ph[2] + ph[3]) * scale, 4)},
                      "sort": {"digit_bits": inf.sort_digit_bits, "tile": inf.sort_tile,
                               "strategy": SORT_NAMES.get(inf.sort_strategy, "?"),
                               "global_digits": inf.sort_global_digits}}
@@ -678,33 +672,34 @@ def main():
     mrays = (n_primary + n_bounce) / (trace_ms * 1e-3) / 1e6
 
     digit = sort_info.sort_digit_bits
-    sort_passes = sort_data_passes(sort_info, 60)  # 63-bit codes use 60 key bits
-    sort_pass_ms = phases[1] / sort_passes
-    sort_pass_bytes = N_PRIMS * SORT_PASS_BYTES_PER_PRIM_63
     kernels = {
         "trace (primary+bounce)": trace_ms,
-        "sort pass x%d" % sort_passes: phases[1],
+        "sort": phases[1],
         "bounds+encode": phases[0],
-        "tree": phases[2],
-        "fit": phases[3],
+        "tree+fit": phases[2] + phases[3],
     }
     dominant = max(kernels, key=kernels.get)
-    build_bytes = N_PRIMS * build_bytes_per_prim_63(sort_passes)
+    build_bytes = N_PRIMS * BUILD_BYTES_PER_PRIM_63
     build_frac = build_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-    if dominant.startswith("sort"):
-        ach = sort_pass_bytes / (sort_pass_ms * 1e-3) / 1e9
-        roof = {"kernel": "radix sort (one data pass, u64 key + u32 id)", "bound": "hbm",
+    if dominant == "sort":
+        sb = N_PRIMS * SORT_BYTES_PER_PRIM_63
+        ach = sb / (phases[1] * 1e-3) / 1e9
+        roof = {"kernel": "radix sort (canonical 8 digit passes, u64 key + u32 id)",
+                "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": sb, "avg_launch_ms": round(phases[1], 5)}
+    else:
+        # traversal: the bytes it touches (an upper-bound proxy for its HBM
+        # traffic); the compulsory ray and hit bytes beside them
+        ach = TRACE_TOUCHED_BYTES_PER_STEP / (trace_ms * 1e-3) / 1e9
+        comp = (n_primary + n_bounce) * RAY_COMPULSORY_BYTES / (trace_ms * 1e-3) / 1e9
+        roof = {"kernel": "k_trace (closest hit, reference intersector)", "bound": "hbm",
                 "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": sort_pass_bytes, "avg_launch_ms": round(sort_pass_ms, 5)}
-    else:
-        # traversal: compulsory ray/hit bytes only (a lower bound on its traffic)
-        tb = (n_primary + n_bounce) * RAY_COMPULSORY_BYTES
-        ach = tb / (trace_ms * 1e-3) / 1e9
-        roof = {"kernel": "k_trace (closest hit, reference intersector)", "bound": "hbm",
-                "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
-                "bytes_per_launch": tb / 2, "avg_launch_ms": round(trace_ms / 2, 5)}
+                "bytes": "touched: popped nodes x 32 + tested children x 32 + tested triangles x 48",
+                "bytes_per_launch": TRACE_TOUCHED_BYTES_PER_STEP / 2,
+                "avg_launch_ms": round(trace_ms / 2, 5),
+                "compulsory_gbs": round(comp, 2)}
 
     res = {
         "metric": "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain",
@@ -726,15 +721,14 @@ def main():
                    "bounce_rays": n_bounce},
         "build_ms": round(build_ms, 4),
         "build_phases_ms": {"bounds+encode": round(phases[0], 4), "sort": round(phases[1], 4),
-                            "tree": round(phases[2], 4), "fit": round(phases[3], 4)},
+                            "tree+fit": round(phases[2] + phases[3], 4)},
         "sort": {"digit_bits": digit, "tile": sort_info.sort_tile,
                  "strategy": SORT_NAMES.get(sort_info.sort_strategy, "?"),
-                 "global_digits": sort_info.sort_global_digits,
-                 "data_passes": sort_passes, "picked_by": "autotuner",
+                 "global_digits": sort_info.sort_global_digits, "picked_by": "autotuner",
                  "candidates_ms": {"%s-d%d-t%d" % (r["strategy"], r["digit_bits"], r["tile"]): r["ms"]
                                    for r in tune_log if not r["batch"]}},
         "build_roofline_frac": round(build_frac, 4),
-        "build_algorithmic_bytes_per_prim": build_bytes_per_prim_63(sort_passes),
+        "build_algorithmic_bytes_per_prim": BUILD_BYTES_PER_PRIM_63,
         "closest_hit_mrays_s": round(mrays, 3),
         "trace_ms": round(trace_ms, 4),
         "dominant_kernel": dominant,

@@ -1,6 +1,7 @@
 // gfx950 build kernels: centroid bounds, Morton encode (+ the radix sort's
-// digit histograms), Karras radix tree and the bottom-up box fit.  Each
-// kernel cites the reference code whose results it reproduces bit for bit.
+// digit histograms), the radix tree with its boxes (one pass), and the
+// bottom-up box fit of refits.  Each kernel cites the reference code whose
+// results it reproduces bit for bit.
 #include "device_common.hpp"
 #include "engine.hpp"
 
@@ -268,24 +269,9 @@ k_hist_reduce(const uint32_t* __restrict__ part, uint32_t groups, uint32_t words
     atomicAdd(&hist[w], s);
 }
 
-// ---- Karras radix tree -----------------------------------------------------------
-// divide_node + builder_kernel (lbvh.h:1588-1688, 1713-1731), one internal
-// node per thread:
-//   delta(j, k) = -1 if k is outside [0, n); clz(code_j ^ code_k) if the codes
-//   differ; clz(j ^ k) + bits(code) if they are equal;
-//   d = sign(delta(i, i+1) - delta(i, i-1)); delta_min = delta(i, i-d);
-//   l_max starts at 128 and grows x4 while delta(i, i + l_max d) > delta_min
-//   (the loop exits on k >= n only); binary search for l; split search with
-//   ceil_div; gamma = i + s d + min(d, 0).
-// left = gamma | leaf if min(i, j) == gamma; right = gamma + 1 | leaf if
-// max(i, j) == gamma + 1.  With PRIMITIVE_ID leaves, a leaf stores the id of
-// the primitive at that sorted position.
-//
-// The codes a workgroup's nodes search within +-128 positions are staged in
-// LDS; farther probes (large ranges near the root) read global memory.
-// Besides left/right it writes, for the fit: parent links (see kLink*); the
-// primitive of each leaf position (primitive-id leaves); and clears the fit
-// flags.
+// ---- prefix lengths of sorted codes ----------------------------------------------
+// calc_delta (lbvh.h:1602-1621): delta(j, k) = clz(code_j ^ code_k) when the
+// codes differ, else clz(j ^ k) + bits(code), with scene-local positions.
 
 template<typename CodeT>
 struct CodeTraits;
@@ -312,220 +298,6 @@ key_code(KeyT k)
     return CodeT(k & 0x3fffffffull);
   else
     return CodeT(k);
-}
-
-constexpr int kKarrasHalo = 128;
-constexpr int kKarrasWindow = kBlock + 2 * kKarrasHalo + 64; // + room for scene boundaries
-
-// Codes of one scene (positions base .. base + n - 1 of the sorted array),
-// served from an LDS copy of sorted positions [w0, w0 + kKarrasWindow) when
-// possible.  Positions passed in are scene-local.
-template<typename KeyT, typename CodeT, bool SEG>
-struct CodeWindow
-{
-  const KeyT* __restrict__ g;
-  const KeyT* s;
-  int64_t w0;
-  int64_t base;
-  int64_t n;
-  __device__ CodeT at(int64_t k) const
-  {
-    const int64_t x = base + k;
-    const int64_t o = x - w0;
-    return key_code<KeyT, CodeT, SEG>((o >= 0 && o < kKarrasWindow) ? s[o] : g[x]);
-  }
-  // calc_delta (lbvh.h:1602-1621) over scene-local positions
-  __device__ int delta(int64_t j, CodeT cj, int64_t k) const
-  {
-    if (k < 0 || k >= n)
-      return -1;
-    CodeT ck = at(k);
-    if (cj == ck)
-      return CodeTraits<CodeT>::clz(CodeT(uint64_t(j ^ k))) + CodeTraits<CodeT>::bits;
-    return CodeTraits<CodeT>::clz(cj ^ ck);
-  }
-};
-
-// Largest x in [0, span) with pred(x), for a predicate that holds on a
-// prefix of [1, span) and fails after it (x = 0 when pred(1) fails); span is
-// a power of two.  8-way steps issue 7 independent probes each.
-template<class Pred>
-__device__ __forceinline__ int64_t
-search_prefix(int64_t span, const Pred& pred)
-{
-  int64_t x = 0;
-  while (span >= 8) {
-    const int64_t step = span >> 3;
-    int cnt = 0;
-#pragma unroll
-    for (int q = 1; q < 8; q++)
-      cnt += pred(x + q * step) ? 1 : 0;
-    x += cnt * step;
-    span = step;
-  }
-  while (span > 1) {
-    const int64_t t = span >> 1;
-    if (pred(x + t))
-      x += t;
-    span = t;
-  }
-  return x;
-}
-
-// The same result by halving steps.
-template<class Pred>
-__device__ __forceinline__ int64_t
-search_binary(int64_t span, const Pred& pred)
-{
-  int64_t x = 0;
-  while (span > 1) {
-    const int64_t t = span >> 1;
-    if (pred(x + t))
-      x += t;
-    span = t;
-  }
-  return x;
-}
-
-template<typename KeyT, typename CodeT, bool SEG>
-__global__ __launch_bounds__(kBlock) void
-k_karras(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes sc,
-         Node* __restrict__ nodes, uint32_t* __restrict__ parent,
-         uint32_t* __restrict__ leafprim, uint32_t* __restrict__ flags,
-         uint32_t* __restrict__ other_end, uint2* __restrict__ treelets,
-         uint32_t* __restrict__ treelet_count, WideNode* __restrict__ wide)
-{
-  __shared__ KeyT s_keys[kKarrasWindow];
-  // batches: offsets of the scenes this workgroup's nodes can belong to (a
-  // scene has at least one node, so at most kBlock scenes, + 1 end offset)
-  __shared__ uint64_t s_off[SEG ? kBlock + 1 : 1];
-  __shared__ uint32_t s_first;
-  const int64_t nodes_total = sc.n - (SEG ? int64_t(sc.nseg) : 1);
-  const int64_t g0 = int64_t(blockIdx.x) * kBlock;
-  const int64_t g = g0 + threadIdx.x;
-  uint32_t s0 = 0;
-  uint32_t span = 0; // scenes staged in s_off (entries span + 1)
-  if constexpr (SEG) {
-    if (threadIdx.x == 0)
-      s_first = find_scene(sc.prim_off, sc.nseg, g0, 1, true);
-    __syncthreads();
-    s0 = s_first;
-    span = sc.nseg - s0 < uint32_t(kBlock) ? sc.nseg - s0 : uint32_t(kBlock);
-    for (uint32_t t = threadIdx.x; t <= span; t += kBlock)
-      s_off[t] = sc.prim_off[s0 + t];
-  }
-  // scene of the workgroup's first node sets the window (sorted position of
-  // node g of scene s is g + s)
-  const int64_t w0 = g0 + s0 - kKarrasHalo;
-  for (int t = threadIdx.x; t < kKarrasWindow; t += kBlock) {
-    const int64_t x = w0 + t;
-    s_keys[t] = (x >= 0 && x < sc.n) ? keys[x] : KeyT(0);
-  }
-  __syncthreads();
-  if (g >= nodes_total)
-    return;
-  uint32_t seg = 0;
-  int64_t P = 0, n = sc.n;
-  if constexpr (SEG) {
-    // largest j < span with node base s_off[j] - (s0 + j) <= g
-    uint32_t lo = 0, hi = span;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (int64_t(s_off[mid]) - int64_t(s0 + mid) <= g)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    seg = s0 + lo;
-    P = int64_t(s_off[lo]);
-    n = int64_t(s_off[lo + 1]) - P;
-  }
-  const int64_t node_base = P - seg;
-  const int64_t link_base = 2 * P - seg;
-  const int64_t i = g - node_base;
-  if (n < 2 || i < 0 || i >= n - 1)
-    return; // scenes have at least 2 primitives (checked by the host)
-  const CodeWindow<KeyT, CodeT, SEG> cw{ keys, s_keys, w0, P, n };
-  const CodeT ci = cw.at(i);
-  const int dd = cw.delta(i, ci, i + 1) - cw.delta(i, ci, i - 1);
-  const int64_t d = (dd < 0) ? -1 : 1;
-  const int delta_min = cw.delta(i, ci, i - d);
-  // The reference grows l_max from 128 by x4 while delta(i, i + l_max d)
-  // exceeds delta_min (stopping when i + l_max d >= n), then binary-searches
-  // the largest l < l_max with delta(i, i + l d) > delta_min, then the
-  // largest split s < l with delta(i, i + s d) > delta_node.  For sorted
-  // keys delta(i, k) does not grow as k moves away from i, so each of these
-  // predicates is true on a prefix and false after it: the searches below
-  // find the same boundaries, with independent probes (all l_max candidates
-  // at once, then 7 probes per 8-way step) where the span leaves the LDS
-  // window.
-  auto candidate = [&](int q) {
-    const int64_t span = int64_t(128) << (2 * q);
-    const int64_t k = i + (d > 0 ? span : -span);
-    return (k < n) && (cw.delta(i, ci, k) > delta_min);
-  };
-  int m = 0;
-  if (candidate(0)) { // most nodes stop here, inside the LDS window
-    m = 1;
-#pragma unroll
-    for (int q = 1; q < 13; q++)
-      m += candidate(q) ? 1 : 0; // prefix-true: m = index of the first failing candidate
-  }
-  const int64_t l_max = int64_t(128) << (2 * m);
-  auto pred_l = [&](int64_t x) { return cw.delta(i, ci, i + (d > 0 ? x : -x)) > delta_min; };
-  const int64_t l = (m == 0) ? search_binary(l_max, pred_l) : search_prefix(l_max, pred_l);
-  const int64_t j = i + (d > 0 ? l : -l);
-  const int delta_node = cw.delta(i, ci, j);
-  int64_t span_s = 1;
-  while (span_s < l)
-    span_s <<= 1;
-  auto pred_s = [&](int64_t x) { return cw.delta(i, ci, i + (d > 0 ? x : -x)) > delta_node; };
-  const int64_t s = (span_s <= kKarrasHalo) ? search_binary(span_s, pred_s)
-                                            : search_prefix(span_s, pred_s);
-  const int64_t gamma = (d > 0) ? (i + s) : (i - s - 1);
-  if (gamma < 0 || gamma + 1 > n - 1)
-    return; // unreachable for sorted codes; keeps every index below in range
-  const int64_t mn = (i < j) ? i : j;
-  const int64_t mx = (j > i) ? j : i;
-  const bool lleaf = (mn == gamma);
-  const bool rleaf = (mx == gamma + 1);
-  // a top node: its children hand their boxes up through the top-tree pass
-  const bool top = (mx - mn + 1) > kTreelet;
-  const uint32_t up = uint32_t(i) | (top ? kLinkGlobal : 0u);
-  const uint32_t left_link = up | (rleaf ? kLinkSibLeaf : 0u);
-  const uint32_t right_link = up | kLinkRight | (lleaf ? kLinkSibLeaf : 0u);
-  other_end[node_base + i] = uint32_t(j);
-  if (top) {
-    // children with at most kTreelet leaves root treelets; an entry holds the
-    // leaf range as sorted positions of the whole batch, and in bit 31 of .y
-    // whether its root is a right child (the root is numbered hi for a left
-    // child, lo for a right child)
-    if (gamma - mn + 1 <= kTreelet)
-      treelets[atomicAdd(treelet_count, 1u)] = make_uint2(uint32_t(P + mn), uint32_t(P + gamma));
-    if (mx - gamma <= kTreelet)
-      treelets[atomicAdd(treelet_count, 1u)] =
-        make_uint2(uint32_t(P + gamma + 1), uint32_t(P + mx) | 0x80000000u);
-  } else if (i == 0) {
-    treelets[atomicAdd(treelet_count, 1u)] = make_uint2(uint32_t(P), uint32_t(P + n - 1));
-  }
-  // left / right are written by the fit together with the box (one full
-  // node and one full traversal record per internal node)
-  if (lleaf) {
-    parent[link_base + (n - 1) + gamma] = left_link;
-    if (leafprim)
-      leafprim[P + gamma] = uint32_t(int64_t(ids[P + gamma]) - P);
-  } else {
-    parent[link_base + gamma] = left_link;
-  }
-  if (rleaf) {
-    parent[link_base + (n - 1) + gamma + 1] = right_link;
-    if (leafprim)
-      leafprim[P + gamma + 1] = uint32_t(int64_t(ids[P + gamma + 1]) - P);
-  } else {
-    parent[link_base + gamma + 1] = right_link;
-  }
-  if (top)
-    flags[node_base + i] = 0u; // only top nodes use device-scope visit flags
 }
 
 // ---- bottom-up fit -------------------------------------------------------------
@@ -832,8 +604,8 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
 // workgroups (and XCDs) through coherent stores, a vector-memory drain and
 // the ticket atomic; tickets reset themselves.
 //
-// Outputs match the Karras kernel plus the fit: node and traversal records,
-// parent links (kLink*), other_end, leafprim, and the fit flags left at 0.
+// Outputs: node and traversal records, parent links (kLink*), other_end,
+// leafprim, and the fit flags left at 0 (what a refit starts from).
 
 constexpr int kTreeLeaves = kTreelet;                   // level-0 positions per workgroup
 constexpr int kTreeFan = 8;                             // workgroups merged per level
@@ -933,22 +705,23 @@ struct TreeLevels
 
 constexpr int kTreeLeaf0 = 2048; // leaves of the delta min-tree (>= kTreeSlots + 1)
 static_assert(kTreeLeaf0 >= kTreeSlots + 1, "delta tree size");
+constexpr int kTreeUpThreads = 256; // workgroup of the upper levels (few elements each)
 constexpr int16_t kTreeInf = 0x7fff;
 
 // Largest i <= j with d[i] < x, or -1 (tree: heap-ordered minima of d,
-// leaves at kTreeLeaf0 + i).
+// leaves at leaf0 + i).
 __device__ __forceinline__ int
-tree_left_less(const int16_t* __restrict__ tree, int j, int x)
+tree_left_less(const int16_t* __restrict__ tree, int leaf0, int j, int x)
 {
-  int p = j + kTreeLeaf0;
+  int p = j + leaf0;
   if (tree[p] < x)
     return j;
   while (p > 1) {
     if ((p & 1) && tree[p - 1] < x) {
       p = p - 1;
-      while (p < kTreeLeaf0)
+      while (p < leaf0)
         p = (tree[2 * p + 1] < x) ? 2 * p + 1 : 2 * p;
-      return p - kTreeLeaf0;
+      return p - leaf0;
     }
     p >>= 1;
   }
@@ -957,17 +730,17 @@ tree_left_less(const int16_t* __restrict__ tree, int j, int x)
 
 // Smallest i >= j with d[i] < x, or -1.
 __device__ __forceinline__ int
-tree_right_less(const int16_t* __restrict__ tree, int j, int x)
+tree_right_less(const int16_t* __restrict__ tree, int leaf0, int j, int x)
 {
-  int p = j + kTreeLeaf0;
+  int p = j + leaf0;
   if (tree[p] < x)
     return j;
   while (p > 1) {
     if (!(p & 1) && tree[p + 1] < x) {
       p = p + 1;
-      while (p < kTreeLeaf0)
+      while (p < leaf0)
         p = (tree[2 * p] < x) ? 2 * p : 2 * p + 1;
-      return p - kTreeLeaf0;
+      return p - leaf0;
     }
     p >>= 1;
   }
@@ -1018,7 +791,7 @@ box_has_nan(const Box& b)
 // else the kernel of the levels above (its workgroup b starts as level-1
 // workgroup b).
 template<typename KeyT, typename CodeT, bool SEG, bool FIRST>
-__global__ __launch_bounds__(kTreeLeaves) void
+__global__ __launch_bounds__(FIRST ? kTreeLeaves : kTreeUpThreads) void
 k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes sc,
        const float* __restrict__ prims_all, int kind, Node* __restrict__ nodes_all,
        WideNode* __restrict__ wide_all, uint32_t* __restrict__ parent_all,
@@ -1026,7 +799,9 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
        uint32_t* __restrict__ other_end_all, TreeLevels lv, uint32_t* __restrict__ seg_count,
        TreeEntry* __restrict__ seg_entries, uint32_t* __restrict__ tickets)
 {
-  constexpr int SL = FIRST ? kTreeLeaves : kTreeSlots; // elements per workgroup
+  constexpr int SL = FIRST ? kTreeLeaves : kTreeSlots;      // elements per workgroup
+  constexpr int NT = FIRST ? kTreeLeaves : kTreeUpThreads;  // threads
+  constexpr int EPT = (SL + NT - 1) / NT;                   // elements per thread (lists)
   // elements (level 0: leaves; then listed subtrees), in position order
   __shared__ uint32_t s_el_l[SL];
   __shared__ uint32_t s_el_r[SL];
@@ -1038,7 +813,8 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
   // min tree over d (leaves at kTreeLeaf0): d[e] is the delta left of
   // element e, d[m] the delta right of the last
   __shared__ int16_t s_tree[2 * kTreeLeaf0];
-  int16_t* const s_d = s_tree + kTreeLeaf0;
+  __shared__ int16_t s_dsep[FIRST ? 1 : SL + 1];
+  int16_t* const s_d = FIRST ? s_tree + kTreeLeaf0 : s_dsep; // level 0: the tree's own leaves
   // split j (between elements j and j + 1): its node's element range [a, b]
   __shared__ uint16_t s_na[SL];
   __shared__ uint16_t s_nb[SL];
@@ -1046,7 +822,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
   __shared__ uint8_t s_round[SL];   // pass in which its box was completed (0: not yet)
   __shared__ uint16_t s_kid[SL];    // element-space node number -> split
   __shared__ uint16_t s_esc[SL];    // listed subtree starting at element e (+1; 0: none)
-  __shared__ uint32_t s_wsum[kTreeLeaves / kWave];
+  __shared__ uint32_t s_wsum[NT / kWave];
   __shared__ uint32_t s_pre[kTreeFan + 1];
   __shared__ uint32_t s_last;
 
@@ -1082,7 +858,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
     }
     __syncthreads();
     m = int(s_pre[kTreeFan]);
-    for (int e = t; e < m; e += kTreeLeaves) {
+    for (int e = t; e < m; e += NT) {
       uint32_t c = 0;
       while (c + 1 < uint32_t(kTreeFan) && uint32_t(e) >= s_pre[c + 1])
         c++;
@@ -1152,16 +928,25 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
 
   for (;;) {
     // ---- nodes of this workgroup: ranges from the nearest smaller deltas -------
-    for (int q = m + 1 + t; q < kTreeLeaf0; q += kTreeLeaves)
-      s_tree[kTreeLeaf0 + q] = kTreeInf;
+    // level 0: 2048 leaves, d already in place; above: the next power of two
+    int leaf0 = kTreeLeaf0;
+    if constexpr (!FIRST) {
+      leaf0 = 2;
+      while (leaf0 < m + 1)
+        leaf0 <<= 1;
+      for (int q = t; q <= m; q += NT)
+        s_tree[leaf0 + q] = s_d[q];
+    }
+    for (int q = m + 1 + t; q < leaf0; q += NT)
+      s_tree[leaf0 + q] = kTreeInf;
     __syncthreads();
     // exact NaN semantics need the unions in tree order (passes below)
     int any_nan = 0;
-    for (int e = t; e < m; e += kTreeLeaves)
+    for (int e = t; e < m; e += NT)
       any_nan |= box_has_nan(s_bt[m + e]) ? 1 : 0;
     const bool nan = __syncthreads_or(any_nan) != 0;
-    for (int half = kTreeLeaf0 / 2; half >= 1; half >>= 1) {
-      for (int q = t; q < half; q += kTreeLeaves) {
+    for (int half = leaf0 / 2; half >= 1; half >>= 1) {
+      for (int q = t; q < half; q += NT) {
         const int p = half + q;
         const int16_t a0 = s_tree[2 * p], a1 = s_tree[2 * p + 1];
         s_tree[p] = a0 < a1 ? a0 : a1;
@@ -1170,12 +955,12 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       }
       __syncthreads();
     }
-    for (int j = t; j < m - 1; j += kTreeLeaves) {
+    for (int j = t; j < m - 1; j += NT) {
       const int x = s_d[j + 1];
       uint8_t inb = 0;
       if (x >= 0) {
-        const int i = tree_left_less(s_tree, j, x);
-        const int k = tree_right_less(s_tree, j + 2, x);
+        const int i = tree_left_less(s_tree, leaf0, j, x);
+        const int k = tree_right_less(s_tree, leaf0, j + 2, x);
         if (i >= 0 && k >= 0) {
           inb = 1;
           s_na[j] = uint16_t(i);
@@ -1185,12 +970,12 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       s_inb[j] = inb;
       s_round[j] = 0;
     }
-    for (int e = t; e < m; e += kTreeLeaves) {
+    for (int e = t; e < m; e += NT) {
       s_esc[e] = 0;
       s_kid[e] = 0xffff;
     }
     __syncthreads();
-    for (int j = t; j < m - 1; j += kTreeLeaves) {
+    for (int j = t; j < m - 1; j += NT) {
       if (s_inb[j]) {
         const int a = s_na[j], b = s_nb[j];
         s_kid[(s_d[b + 1] > s_d[a]) ? b : a] = uint16_t(j);
@@ -1201,7 +986,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
     // outstanding global store, so the records are written afterwards) ------
     for (int pass = 1; nan; pass++) {
       int changed = 0;
-      for (int j = t; j < m - 1; j += kTreeLeaves) {
+      for (int j = t; j < m - 1; j += NT) {
         if (!s_inb[j] || s_round[j] != 0)
           continue;
         const int a = s_na[j], b = s_nb[j];
@@ -1242,7 +1027,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
         n = int64_t(sc.prim_off[seg + 1]) - P;
       }
     };
-    for (int j = t; j < m - 1; j += kTreeLeaves) {
+    for (int j = t; j < m - 1; j += NT) {
       if (!s_inb[j])
         continue;
       const int a = s_na[j], b = s_nb[j];
@@ -1261,7 +1046,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       parent[lleaf ? (n - 1) + lg : lg] = up | (rleaf ? kLinkSibLeaf : 0u);
       parent[rleaf ? (n - 1) + lg + 1 : lg + 1] = up | kLinkRight | (lleaf ? kLinkSibLeaf : 0u);
     }
-    for (int e = t; e < m; e += kTreeLeaves) {
+    for (int e = t; e < m; e += NT) {
       const uint32_t jj = s_kid[e];
       if (jj == 0xffffu)
         continue; // no node is numbered e
@@ -1304,19 +1089,23 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       const int ps = (db > da) ? b : a - 1;
       return ps < 0 || ps > m - 2 || !s_inb[ps];
     };
-    for (int e = t; e < m; e += kTreeLeaves)
+    for (int e = t; e < m; e += NT)
       if (parent_out(e, e))
         s_esc[e] = uint16_t(e + 1); // an element stands alone
     __syncthreads();
-    for (int j = t; j < m - 1; j += kTreeLeaves)
+    for (int j = t; j < m - 1; j += NT)
       if (s_inb[j] && parent_out(s_na[j], s_nb[j]))
         s_esc[s_na[j]] = uint16_t(kTreeSlots + 1 + j); // a completed node
     __syncthreads();
     // position order of the listed subtrees: exclusive count of list starts
-    const int e0 = 2 * t, e1 = 2 * t + 1;
-    const uint32_t f0 = (e0 < m && s_esc[e0]) ? 1u : 0u;
-    const uint32_t f1 = (e1 < m && s_esc[e1]) ? 1u : 0u;
-    uint32_t incl = f0 + f1;
+    // (thread t holds elements [t * EPT, t * EPT + EPT))
+    uint32_t mine = 0;
+#pragma unroll
+    for (int h = 0; h < EPT; h++) {
+      const int e = t * EPT + h;
+      mine += (e < m && s_esc[e]) ? 1u : 0u;
+    }
+    uint32_t incl = mine;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
       const uint32_t y = __shfl_up(incl, off, kWave);
@@ -1326,22 +1115,23 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
     if (lane == kWave - 1)
       s_wsum[w] = incl;
     __syncthreads();
-    uint32_t base = incl - (f0 + f1);
+    uint32_t base = incl - mine;
     uint32_t total = 0;
 #pragma unroll
-    for (int q = 0; q < kTreeLeaves / kWave; q++) {
+    for (int q = 0; q < NT / kWave; q++) {
       const uint32_t v = s_wsum[q];
       base += (q < w) ? v : 0u;
       total += v;
     }
     TreeEntry* out = &seg_entries[size_t(lv.seg_base[level] + blk) * kTreeSegCap];
-    for (int h = 0; h < 2; h++) {
-      const int e = h == 0 ? e0 : e1;
+    uint32_t o = base;
+#pragma unroll
+    for (int h = 0; h < EPT; h++) {
+      const int e = t * EPT + h;
       if (e >= m || !s_esc[e])
         continue;
-      const uint32_t o = base + (h == 1 ? f0 : 0u);
       if (o >= uint32_t(kTreeSegCap))
-        continue; // cannot happen: two per tree level at most
+        break; // cannot happen: two per tree level at most
       const uint32_t tag = s_esc[e];
       int a, b;
       Box bx;
@@ -1354,7 +1144,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
         a = b = e;
         bx = s_bt[m + e];
       }
-      TreeEntry* en = &out[o];
+      TreeEntry* en = &out[o++];
       coherent_store_u32(&en->l, s_el_l[a]);
       coherent_store_u32(&en->r, s_el_r[b]);
       coherent_store_u32(&en->d, (uint32_t(int(s_d[a])) & 0xffffu) | (uint32_t(int(s_d[b + 1])) << 16));
@@ -1519,36 +1309,6 @@ make_scenes(const uint64_t* prim_off, uint32_t nseg, uint64_t n)
 }
 
 hipError_t
-launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
-              const uint64_t* prim_off, uint32_t nseg, void* nodes, uint32_t* parent,
-              uint32_t* leafprim, uint32_t* flags, uint32_t* other_end, void* treelets,
-              uint32_t* treelet_count, void* wide, hipStream_t s)
-{
-  const Scenes sc = make_scenes(prim_off, nseg, n);
-  const uint64_t m = n - sc.nseg;
-  const uint32_t g = uint32_t((m + kBlock - 1) / kBlock);
-  uint2* tl = static_cast<uint2*>(treelets);
-  Node* nd = static_cast<Node*>(nodes);
-  WideNode* wd = static_cast<WideNode*>(wide);
-  if (prim_off) {
-    if (code_bits != 30)
-      return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_karras<uint64_t, uint32_t, true>), dim3(g), dim3(kBlock), 0, s,
-                       static_cast<const uint64_t*>(sorted), ids, sc, nd, parent, leafprim, flags,
-                       other_end, tl, treelet_count, wd);
-  } else if (code_bits == 30) {
-    hipLaunchKernelGGL((k_karras<uint32_t, uint32_t, false>), dim3(g), dim3(kBlock), 0, s,
-                       static_cast<const uint32_t*>(sorted), ids, sc, nd, parent, leafprim, flags,
-                       other_end, tl, treelet_count, wd);
-  } else {
-    hipLaunchKernelGGL((k_karras<uint64_t, uint64_t, false>), dim3(g), dim3(kBlock), 0, s,
-                       static_cast<const uint64_t*>(sorted), ids, sc, nd, parent, leafprim, flags,
-                       other_end, tl, treelet_count, wd);
-  }
-  return hipGetLastError();
-}
-
-hipError_t
 launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off, uint32_t nseg,
            void* nodes, const uint32_t* parent, const uint32_t* leafprim, uint32_t* flags,
            const void* treelets, uint32_t* treelet_count, uint32_t* queue, void* wide,
@@ -1629,7 +1389,7 @@ tree_launch(const void* sorted, const uint32_t* ids, const Scenes& sc, const flo
                      static_cast<Node*>(nodes), static_cast<WideNode*>(wide), parent, leafprim,
                      flags, other_end, lv, counts, entries, tk);
   if (lv.levels > 1)
-    hipLaunchKernelGGL((k_tree<KeyT, CodeT, SEG, false>), dim3(lv.nblocks[1]), dim3(kTreeLeaves),
+    hipLaunchKernelGGL((k_tree<KeyT, CodeT, SEG, false>), dim3(lv.nblocks[1]), dim3(kTreeUpThreads),
                        0, s, static_cast<const KeyT*>(sorted), ids, sc, prims, kind,
                        static_cast<Node*>(nodes), static_cast<WideNode*>(wide), parent, leafprim,
                        flags, other_end, lv, counts, entries, tk);

@@ -79,21 +79,14 @@ uint32_t rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles);
 // parent links at 2 prim_off[s] - s.  Batched sorted keys are
 // (scene << 30 | 30-bit code) in u64.
 
-// Karras radix tree: left/right of every internal node; parent links
-// (internal node c at c, the leaf at sorted position k at n-1+k, per scene;
-// bit 31: the parent is a top node, bit 30: right child, bit 29: sibling is
-// a leaf; the index is scene-local); the scene-local primitive of each leaf
-// position when ids != nullptr (leafprim); the other end of each node's leaf
-// range (other_end); the treelet list (uint2 {lo, hi | right-root << 31} as
-// batch positions, at most n entries; treelet_count must be zero); fit flags
-// cleared.
-hipError_t launch_karras(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
-                         const uint64_t* prim_off, uint32_t nseg, void* nodes, uint32_t* parent,
-                         uint32_t* leafprim, uint32_t* flags, uint32_t* other_end, void* treelets,
-                         uint32_t* treelet_count, void* wide, hipStream_t s);
+// Parent links (refits): internal node c at parent[c], the leaf at sorted
+// position k at parent[n - 1 + k], per scene; bit 31: the parent is a top
+// node (more than 1,024 leaves), bit 30: right child, bit 29: the sibling is
+// a leaf; the parent's scene-local index below.
 
-// Bottom-up box fit: treelets in LDS, then the top tree with device-scope
-// flags (flags must be zero and are left zero; *queue must be zero).
+// Bottom-up box fit of an existing tree (refits): treelets from the list in
+// LDS, then the top tree with device-scope flags (flags must be zero and are
+// left zero; *queue must be zero).
 // leafprim == nullptr: the leaf at position k is primitive k of its scene.
 hipError_t launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off,
                       uint32_t nseg, void* nodes, const uint32_t* parent,

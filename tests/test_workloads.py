@@ -87,3 +87,26 @@ def test_serving_stream_mix_and_sizes():
         else:
             assert r["scene"] in made and r["n_tri"] == made[r["scene"]]
     assert st == W.serving_stream(seed=5, count=4000, rate_hz=1000.0)
+
+
+def test_bench_traversal_touched_bytes_constant():
+    """The traversal roofline's byte count: bench.py states the bytes the
+    closest-hit traversal touches on its workload (popped nodes x 32 +
+    box-tested children x 32 + tested triangles x 48, an upper-bound proxy
+    for its memory traffic); the instrumented CPU traversal of the same tree
+    must count the same.  The tree is the oracle's, which the GPU build equals bit for bit."""
+    import importlib.util
+    import os
+    import oracle
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    _, tri = W.scene_clustered(2, 1 << 20)
+    nodes, _, _, _ = oracle.build(tri, 63, oracle.LEAF_PRIMITIVE_ID)
+    rays = W.primary_rays(1920, 1080)
+    p, t, _, c1 = oracle.trace(nodes, tri, rays, oracle.ISECT_REFERENCE, counters=True)
+    bounce, _ = W.bounce_rays(5, tri, rays, p, t)
+    _, _, _, c2 = oracle.trace(nodes, tri, bounce, oracle.ISECT_REFERENCE, counters=True)
+    touched = sum(int(c[0]) * 32 + int(c[1]) * 32 + int(c[2]) * 48 for c in (c1, c2))
+    assert touched == bench.TRACE_TOUCHED_BYTES_PER_STEP

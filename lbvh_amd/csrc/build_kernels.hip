@@ -808,7 +808,8 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
   // element boxes at s_bt[m + e]; below m: the segment tree over them, or
   // (NaN boxes present) node j's box at s_bt[1 + j]
   __shared__ Box s_bt[2 * SL];
-  __shared__ uint32_t s_el_scene[SEG ? SL : 1];
+  // scene of each element (batches above level 0; level 0 reads it from the key)
+  __shared__ uint32_t s_el_scene[(SEG && !FIRST) ? SL : 1];
   __shared__ uint32_t s_el_val[SL]; // child value when the element is one leaf
   // min tree over d (leaves at kTreeLeaf0): d[e] is the delta left of
   // element e, d[m] the delta right of the last
@@ -883,7 +884,8 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       if constexpr (SEG) {
         const uint32_t sg = uint32_t(keys[el] >> 30);
         P = int64_t(sc.prim_off[sg]);
-        s_el_scene[e] = sg;
+        if constexpr (!FIRST)
+          s_el_scene[e] = sg;
       }
       if (el == er)
         s_el_val[e] = (leafprim_all ? uint32_t(int64_t(ids[el]) - P) : uint32_t(int64_t(el) - P)) |
@@ -917,8 +919,6 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       s_el_l[t] = uint32_t(x);
       s_el_r[t] = uint32_t(x);
       s_el_val[t] = prim | kLeafBit;
-      if constexpr (SEG)
-        s_el_scene[t] = uint32_t(keys[x] >> 30);
       s_bt[m + t] = load_prim_box(ts.prims, kind, uint64_t(prim));
     }
     __syncthreads();
@@ -1022,7 +1022,10 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       n = N;
       seg = 0;
       if constexpr (SEG) {
-        seg = s_el_scene[j];
+        if constexpr (FIRST)
+          seg = uint32_t(keys[s_el_l[j]] >> 30);
+        else
+          seg = s_el_scene[j];
         P = int64_t(sc.prim_off[seg]);
         n = int64_t(sc.prim_off[seg + 1]) - P;
       }

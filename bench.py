@@ -39,6 +39,11 @@ BUILD_BYTES_PER_PRIM_63 = 36 + 48 + SORT_BYTES_PER_PRIM_63 + 16 + 92
 # traversal of the same tree (tests/test_workloads.py recomputes it).
 TRACE_TOUCHED_BYTES_PER_STEP = 3857317376
 RAY_COMPULSORY_BYTES = 48  # 32-byte ray in, 16-byte hit out
+# HBM bytes per k_trace dispatch on this workload, averaged over the primary
+# and bounce dispatches: rocprofv3 --pmc FETCH_SIZE (x 2.0 KiB) + WRITE_SIZE
+# (x 1.0 KiB), scaled by a 512 MiB copy kernel
+# (profiles/clustered1m_63bit_hbm_traffic.md: 1008.1 MB read + 219.2 MB written).
+TRACE_HBM_TRAFFIC_PER_LAUNCH = 1227.3e6
 
 
 def log(msg):
@@ -695,7 +700,7 @@ def main():
         comp = (n_primary + n_bounce) * RAY_COMPULSORY_BYTES / (trace_ms * 1e-3) / 1e9
         roof = {"kernel": "k_trace (closest hit, reference intersector)", "bound": "hbm",
                 "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRACE_HBM_TRAFFIC_PER_LAUNCH,
                 "bytes": "touched: popped nodes x 32 + tested children x 32 + tested triangles x 48",
                 "bytes_per_launch": TRACE_TOUCHED_BYTES_PER_STEP / 2,
                 "avg_launch_ms": round(trace_ms / 2, 5),

@@ -2,6 +2,8 @@
 // digit histograms), the radix tree with its boxes (one pass), and the
 // bottom-up box fit of refits.  Each kernel cites the reference code whose
 // results it reproduces bit for bit.
+#include <cstdlib>
+
 #include "device_common.hpp"
 #include "engine.hpp"
 
@@ -9,6 +11,7 @@ namespace lbvh_dev {
 
 constexpr int kBlock = 256;
 constexpr int kCentroidMaxBlocks = 1024;
+constexpr int kCentroidUnroll = 4;
 constexpr int kEncodeMaxBlocks = 256;
 // Largest subtree (in leaves) fitted inside one workgroup's LDS.  Nodes whose
 // range holds more leaves form the "top" tree; their children with at most
@@ -85,19 +88,32 @@ k_centroid_bounds(const float* __restrict__ prims, int kind, uint64_t n,
 
   float lo[3] = { INFINITY, INFINITY, INFINITY };
   float hi[3] = { -INFINITY, -INFINITY, -INFINITY };
-  for (uint64_t i = tid; i < n; i += stride) {
-    Box b = load_prim_box(prims, kind, i);
-    float c[3];
+  // kCentroidUnroll primitives per thread per round, all loads issued first
+  for (uint64_t i0 = tid; i0 < n; i0 += stride * kCentroidUnroll) {
+    Box b[kCentroidUnroll];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      float s = b.lo[k] + b.hi[k];
-      c[k] = s * 0.5f;
-      lo[k] = std_min(lo[k], c[k]);
-      hi[k] = std_max(hi[k], c[k]);
+    for (int u = 0; u < kCentroidUnroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      if (i < n)
+        b[u] = load_prim_box(prims, kind, i);
     }
-    cx[i] = c[0];
-    cy[i] = c[1];
-    cz[i] = c[2];
+#pragma unroll
+    for (int u = 0; u < kCentroidUnroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      if (i >= n)
+        continue;
+      float c[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        float s = b[u].lo[k] + b[u].hi[k];
+        c[k] = s * 0.5f;
+        lo[k] = std_min(lo[k], c[k]);
+        hi[k] = std_max(hi[k], c[k]);
+      }
+      cx[i] = c[0];
+      cy[i] = c[1];
+      cz[i] = c[2];
+    }
   }
   block_reduce_box(lo, hi, s_box);
   if (threadIdx.x == 0) {
@@ -140,7 +156,8 @@ k_centroid_bounds(const float* __restrict__ prims, int kind, uint64_t n,
 // * scale); code = expand(qx) << 2 | expand(qy) << 1 | expand(qz) with the
 // magic-mask expansions of morton_encoder<4>/<8> (lbvh.h:1352-1358,
 // 1377-1385).  The 63-bit variant applies morton_encoder<8> to FP32 centres.
-// Each workgroup also counts the digits of every radix pass in LDS and writes
+// Each workgroup also counts the digits of the radix passes first_pass ..
+// passes - 1 (those whose global histogram the sort reads) in LDS and writes
 // its counts (plain stores) for k_hist_reduce.
 
 __device__ __forceinline__ uint32_t
@@ -192,11 +209,11 @@ __global__ __launch_bounds__(kBlock) void
 k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
          const float* __restrict__ cz, uint64_t n,
          const float* __restrict__ bounds, KeyT* __restrict__ codes,
-         uint32_t* __restrict__ part_hist, int digit_bits, int passes)
+         uint32_t* __restrict__ part_hist, int digit_bits, int first_pass, int passes)
 {
   extern __shared__ uint32_t s_hist[];
   const int radix = 1 << digit_bits;
-  const int words = radix * passes;
+  const int words = radix * (passes - first_pass);
   for (int j = threadIdx.x; j < words; j += kBlock)
     s_hist[j] = 0u;
   const float dom = sizeof(KeyT) == 4 ? 1024.0f : 1048576.0f;
@@ -236,9 +253,9 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
       float qz = dz * scale[2];
       KeyT code = encode<KeyT>(qx, qy, qz);
       codes[i] = code;
-      for (int p = 0; p < passes; p++) {
+      for (int p = first_pass; p < passes; p++) {
         uint32_t d = uint32_t((code >> (p * digit_bits)) & mask);
-        atomicAdd(&s_hist[p * radix + d], 1u);
+        atomicAdd(&s_hist[(p - first_pass) * radix + d], 1u);
       }
     }
   }
@@ -589,7 +606,10 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
 //     the leaves;
 //   level k + 1: the last of each kTreeFan level-k workgroups to finish
 //     (an agent-scope ticket; nobody waits) takes their listed subtrees, in
-//     position order, as its elements.
+//     position order, as its elements.  When they hold more than the
+//     workgroup's slots, it merges them kTreeSubFan lists at a time into
+//     lists of its own (each such merge is an ordinary step of the same
+//     algorithm over a shorter position range) and then merges those.
 // Within a workgroup, with d[e] the delta left of element e, the node split
 // between elements j and j + 1 spans elements [a, b] where d[a] and d[b + 1]
 // are the nearest deltas below d[j + 1] on either side (searched in a min
@@ -608,10 +628,15 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
 // leafprim, and the fit flags left at 0 (what a refit starts from).
 
 constexpr int kTreeLeaves = kTreelet;                   // level-0 positions per workgroup
-constexpr int kTreeFan = 8;                             // workgroups merged per level
+constexpr int kTreeFan = 32;                            // workgroups merged per level
+constexpr int kTreeSubFan = 8;                          // lists per merge that always fits
+constexpr int kTreeSubLists = kTreeFan / kTreeSubFan;   // lists of a split merge
 constexpr int kTreeSegCap = 192;                        // 2 x (tree height bound + 1)
-constexpr int kTreeSlots = kTreeFan * kTreeSegCap;      // elements of one workgroup
+constexpr int kTreeSlots = kTreeSubFan * kTreeSegCap;   // elements of one workgroup
 static_assert(kTreeSlots >= kTreeLeaves, "level-0 slots");
+static_assert(kTreeFan % kTreeSubFan == 0 && kTreeSubLists * kTreeSegCap <= kTreeSlots,
+              "split merges fit");
+static_assert(kTreeFan <= kWave, "one wave scans a group's list counts");
 
 // A listed subtree: leaf range [l, r], delta(l - 1) and delta(r) (packed as
 // int16 pairs), box.
@@ -698,14 +723,16 @@ tree_leaf_value(const TreeScene& ts, const uint32_t* __restrict__ ids, int64_t x
 struct TreeLevels
 {
   int levels;
+  uint32_t slot_cap;        // elements merged at once (<= kTreeSlots), else split
   uint32_t nblocks[16];     // workgroups of each level (the last has 1)
   uint32_t seg_base[16];    // first segment of each level
+  uint32_t tmp_base[16];    // first of each level's split-merge segments (kTreeSubLists each)
   uint32_t ticket_base[16]; // first ticket of each level's groups
 };
 
 constexpr int kTreeLeaf0 = 2048; // leaves of the delta min-tree (>= kTreeSlots + 1)
 static_assert(kTreeLeaf0 >= kTreeSlots + 1, "delta tree size");
-constexpr int kTreeUpThreads = 256; // workgroup of the upper levels (few elements each)
+constexpr int kTreeUpThreads = 1024; // workgroup of the upper levels (one CU each: LDS-bound)
 constexpr int16_t kTreeInf = 0x7fff;
 
 // Largest i <= j with d[i] < x, or -1 (tree: heap-ordered minima of d,
@@ -824,7 +851,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
   __shared__ uint16_t s_kid[SL];    // element-space node number -> split
   __shared__ uint16_t s_esc[SL];    // listed subtree starting at element e (+1; 0: none)
   __shared__ uint32_t s_wsum[NT / kWave];
-  __shared__ uint32_t s_pre[kTreeFan + 1];
+  __shared__ uint32_t s_pre[kWave + 1];
   __shared__ uint32_t s_last;
 
   const int t = threadIdx.x;
@@ -833,38 +860,42 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
   const int64_t N = sc.n;
   int level = FIRST ? 0 : 1;
   uint32_t blk = blockIdx.x;
-  int64_t lo = 0, hi = 0;
   int m = 0;
 
-  // the listed subtrees of level-(level - 1) workgroups [blk * fan, ...), in order
-  auto load_group = [&]() {
-    int64_t span = kTreeLeaves;
-    for (int q = 0; q < level; q++)
-      span *= kTreeFan;
-    lo = int64_t(blk) * span;
-    hi = (N - lo < span) ? N : lo + span;
-    const uint32_t members = (lv.nblocks[level - 1] - blk * kTreeFan) < uint32_t(kTreeFan)
-                               ? lv.nblocks[level - 1] - blk * kTreeFan
-                               : uint32_t(kTreeFan);
-    if (t == 0) {
-      uint32_t acc = 0;
-      s_pre[0] = 0;
-      for (uint32_t c = 0; c < uint32_t(kTreeFan); c++) {
-        const uint32_t cnt =
-          c < members ? coherent_load_u32(&seg_count[lv.seg_base[level - 1] + blk * kTreeFan + c])
-                      : 0u;
-        acc += cnt;
-        s_pre[c + 1] = acc;
+  // Elements from the lists of segments [first, first + nl), in position
+  // order; false (nothing loaded) when they number more than cap.
+  auto load_lists = [&](uint32_t first, uint32_t nl, uint32_t cap) -> bool {
+    if (t < kWave) {
+      const uint32_t cnt = uint32_t(t) < nl ? coherent_load_u32(&seg_count[first + t]) : 0u;
+      uint32_t incl = cnt;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, kWave);
+        if (lane >= off)
+          incl += y;
       }
+      if (uint32_t(t) < nl)
+        s_pre[t + 1] = incl;
+      if (t == 0)
+        s_pre[0] = 0;
     }
     __syncthreads();
-    m = int(s_pre[kTreeFan]);
+    const uint32_t total = s_pre[nl];
+    if (total > cap) {
+      __syncthreads(); // every thread has read s_pre before it is rewritten
+      return false;
+    }
+    m = int(total);
     for (int e = t; e < m; e += NT) {
-      uint32_t c = 0;
-      while (c + 1 < uint32_t(kTreeFan) && uint32_t(e) >= s_pre[c + 1])
-        c++;
-      const uint32_t sgi = lv.seg_base[level - 1] + blk * kTreeFan + c;
-      const TreeEntry* en = &seg_entries[size_t(sgi) * kTreeSegCap + (uint32_t(e) - s_pre[c])];
+      uint32_t c = 0, chi = nl; // the last list starting at or before e
+      while (chi - c > 1) {
+        const uint32_t mid = (c + chi) >> 1;
+        if (s_pre[mid] <= uint32_t(e))
+          c = mid;
+        else
+          chi = mid;
+      }
+      const TreeEntry* en = &seg_entries[size_t(first + c) * kTreeSegCap + (uint32_t(e) - s_pre[c])];
       const uint32_t el = coherent_load_u32(&en->l);
       const uint32_t er = coherent_load_u32(&en->r);
       const uint32_t d = coherent_load_u32(&en->d);
@@ -892,12 +923,34 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
                       kLeafBit;
     }
     __syncthreads();
+    return true;
+  };
+
+  // Group blk of level `level`: the lists of its level-(level - 1) members,
+  // all at once when they fit, else the first kTreeSubFan of them (a split
+  // merge: sub counts the merges done so far).
+  uint32_t out_seg = 0, sub = 0, nsub = 0, grp_first = 0, grp_members = 0;
+  auto begin_group = [&]() {
+    grp_first = lv.seg_base[level - 1] + blk * kTreeFan;
+    grp_members = (lv.nblocks[level - 1] - blk * kTreeFan) < uint32_t(kTreeFan)
+                    ? lv.nblocks[level - 1] - blk * kTreeFan
+                    : uint32_t(kTreeFan);
+    if (load_lists(grp_first, grp_members, lv.slot_cap)) {
+      nsub = 0;
+      out_seg = lv.seg_base[level] + blk;
+      return;
+    }
+    nsub = (grp_members + kTreeSubFan - 1) / kTreeSubFan;
+    sub = 0;
+    load_lists(grp_first, grp_members < uint32_t(kTreeSubFan) ? grp_members : uint32_t(kTreeSubFan),
+               uint32_t(kTreeSlots));
+    out_seg = lv.tmp_base[level] + blk * kTreeSubLists;
   };
 
   if constexpr (FIRST) {
     // ---- level 0 elements: the leaves -------------------------------------------
-    lo = int64_t(blk) * kTreeLeaves;
-    hi = (N - lo < kTreeLeaves) ? N : lo + kTreeLeaves;
+    const int64_t lo = int64_t(blk) * kTreeLeaves;
+    const int64_t hi = (N - lo < kTreeLeaves) ? N : lo + kTreeLeaves;
     m = int(hi - lo);
     s_d[t] = int16_t(adj_delta<KeyT, CodeT, SEG>(keys, sc.prim_off, N, lo - 1 + t));
     if (t == 0)
@@ -923,7 +976,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
     }
     __syncthreads();
   } else {
-    load_group();
+    begin_group();
   }
 
   for (;;) {
@@ -1126,7 +1179,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       base += (q < w) ? v : 0u;
       total += v;
     }
-    TreeEntry* out = &seg_entries[size_t(lv.seg_base[level] + blk) * kTreeSegCap];
+    TreeEntry* out = &seg_entries[size_t(FIRST ? blk : out_seg) * kTreeSegCap];
     uint32_t o = base;
 #pragma unroll
     for (int h = 0; h < EPT; h++) {
@@ -1158,10 +1211,30 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       }
     }
     if (t == 0)
-      coherent_store_u32(&seg_count[lv.seg_base[level] + blk],
+      coherent_store_u32(&seg_count[FIRST ? blk : out_seg],
                          total < uint32_t(kTreeSegCap) ? total : uint32_t(kTreeSegCap));
-    if (FIRST || level + 1 >= lv.levels)
-      return; // level 0 hands over to the next kernel; the last level completes every node
+    if constexpr (FIRST)
+      return; // level 0 hands over to the next kernel
+    if (nsub) {
+      // ---- split merge: the next kTreeSubFan lists, then this workgroup's own ----
+      vmem_drain();
+      __syncthreads();
+      const uint32_t tmp0 = lv.tmp_base[level] + blk * kTreeSubLists;
+      if (++sub < nsub) {
+        const uint32_t rest = grp_members - sub * kTreeSubFan;
+        load_lists(grp_first + sub * kTreeSubFan,
+                   rest < uint32_t(kTreeSubFan) ? rest : uint32_t(kTreeSubFan),
+                   uint32_t(kTreeSlots));
+        out_seg = tmp0 + sub;
+      } else {
+        load_lists(tmp0, nsub, uint32_t(kTreeSlots));
+        out_seg = lv.seg_base[level] + blk;
+        nsub = 0;
+      }
+      continue;
+    }
+    if (level + 1 >= lv.levels)
+      return; // the last level completes every node
     // ---- hand over: the last of this group's workgroups goes on ---------------
     vmem_drain();
     __syncthreads();
@@ -1182,7 +1255,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       return;
     level++;
     blk = grp;
-    load_group();
+    begin_group();
   }
 }
 
@@ -1279,25 +1352,30 @@ launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx, floa
 hipError_t
 launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, uint64_t n,
               const float* bounds, void* codes, uint32_t* part_hist, uint32_t* hist,
-              int digit_bits, int passes, hipStream_t s)
+              int digit_bits, int first_pass, int passes, hipStream_t s)
 {
-  const uint32_t words = uint32_t(1u << digit_bits) * uint32_t(passes);
+  if (first_pass < 0 || first_pass >= passes)
+    return hipErrorInvalidValue;
+  const uint32_t words = uint32_t(1u << digit_bits) * uint32_t(passes - first_pass);
   if (words > uint32_t(kHistMaxWords))
     return hipErrorInvalidValue;
   size_t lds = size_t(words) * sizeof(uint32_t);
   uint32_t g = encode_groups(n);
   if (code_bits == 30)
     hipLaunchKernelGGL(k_encode<uint32_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
-                       bounds, static_cast<uint32_t*>(codes), part_hist, digit_bits, passes);
+                       bounds, static_cast<uint32_t*>(codes), part_hist, digit_bits, first_pass,
+                       passes);
   else
     hipLaunchKernelGGL(k_encode<uint64_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
-                       bounds, static_cast<uint64_t*>(codes), part_hist, digit_bits, passes);
+                       bounds, static_cast<uint64_t*>(codes), part_hist, digit_bits, first_pass,
+                       passes);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
   hipLaunchKernelGGL(k_hist_reduce,
                      dim3((words + kBlock - 1) / kBlock, (g + kHistSlice - 1) / kHistSlice),
-                     dim3(kBlock), 0, s, part_hist, g, words, hist);
+                     dim3(kBlock), 0, s, part_hist, g, words,
+                     hist + size_t(first_pass) * (size_t(1) << digit_bits));
   return hipGetLastError();
 }
 
@@ -1335,7 +1413,22 @@ launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off, u
   return hipGetLastError();
 }
 
-// Level sizes of k_tree for n positions, and the scratch it needs.
+// Elements a tree workgroup merges at once before it splits the merge
+// (kTreeSlots; LBVH_TREE_GROUP_SLOTS lowers it, so that tests reach the
+// split merges with ordinary inputs).
+static uint32_t
+tree_slot_cap()
+{
+  const char* v = getenv("LBVH_TREE_GROUP_SLOTS");
+  if (!v || !*v)
+    return uint32_t(kTreeSlots);
+  const long x = strtol(v, nullptr, 10);
+  return (x < 1 || x > kTreeSlots) ? uint32_t(kTreeSlots) : uint32_t(x);
+}
+
+// Level sizes of k_tree for n positions, and the scratch it needs: the
+// segments of every level, then kTreeSubLists split-merge segments per
+// workgroup of each level above 0.
 static TreeLevels
 tree_levels(uint64_t n)
 {
@@ -1355,6 +1448,12 @@ tree_levels(uint64_t n)
     tickets += groups;
     nb = groups;
   }
+  for (int l = 1; l < lv.levels; l++) {
+    lv.tmp_base[l] = segs;
+    segs += lv.nblocks[l] * kTreeSubLists;
+  }
+  lv.tmp_base[0] = segs; // total segments
+  lv.slot_cap = uint32_t(kTreeSlots);
   return lv;
 }
 
@@ -1362,7 +1461,7 @@ uint64_t
 tree_scratch_bytes(uint64_t n)
 {
   const TreeLevels lv = tree_levels(n);
-  const uint64_t segs = uint64_t(lv.seg_base[lv.levels - 1]) + 1;
+  const uint64_t segs = lv.tmp_base[0];
   // segment counts, then the entries
   const uint64_t head = (segs * 4 + 255) / 256 * 256;
   return head + segs * kTreeSegCap * sizeof(TreeEntry);
@@ -1382,8 +1481,9 @@ tree_launch(const void* sorted, const uint32_t* ids, const Scenes& sc, const flo
             uint32_t* other_end, void* wide, void* scratch, uint32_t* tk, hipStream_t s)
 {
   const uint64_t n = uint64_t(sc.n);
-  const TreeLevels lv = tree_levels(n);
-  const uint64_t segs = uint64_t(lv.seg_base[lv.levels - 1]) + 1;
+  TreeLevels lv = tree_levels(n);
+  lv.slot_cap = tree_slot_cap();
+  const uint64_t segs = lv.tmp_base[0];
   uint32_t* counts = static_cast<uint32_t*>(scratch);
   const uint64_t head = (segs * 4 + 255) / 256 * 256;
   TreeEntry* entries = reinterpret_cast<TreeEntry*>(static_cast<char*>(scratch) + head);

@@ -455,7 +455,8 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   if (e == hipSuccess)
     e = launch_encode(opts.code_bits, cent, cent + n, cent + 2 * n, n, b->d_bounds,
                       c->keys_a.p, static_cast<uint32_t*>(c->part_hist.p), ws.hist,
-                      cfg.digit_bits, passes, c->stream);
+                      cfg.digit_bits, cfg.strategy == 2 ? passes - cfg.global_digits : 0, passes,
+                      c->stream);
   record(c, 1);
   bool in_b = false;
   if (e == hipSuccess)

@@ -250,6 +250,18 @@ def test_batch_of_many_scenes_matches_single_builds(ctx):
     b.destroy()
 
 
+def test_batch_split_tree_merges_equal_oracle(ctx, mixed, monkeypatch):
+    """The mixed batch with every upper-level tree group forced through the
+    split merges (LBVH_TREE_GROUP_SLOTS=1): every scene equals the oracle."""
+    monkeypatch.setenv("LBVH_TREE_GROUP_SLOTS", "1")
+    tri, off = mixed
+    b = ctx.build_batch(tri, off, leaf_order=1, keep_codes=True)
+    for s in range(b.n_scenes):
+        on, ob, osc, oids = oracle.build(_scene(tri, off, s), 30, 1)
+        assert b.nodes(s).tobytes() == on.tobytes(), "scene %d nodes" % s
+    b.destroy()
+
+
 @pytest.mark.parametrize("sizes", [[4, 5, 6], [2, 3], [3000, 7000, 2, 50000], [1025, 2]],
                          ids=["3-tiny", "2-tiny", "4-mixed", "2-edge"])
 @pytest.mark.parametrize("strategy", [2, 3], ids=["rts", "hybrid"])

@@ -76,6 +76,30 @@ def test_build_matches_oracle_ragged(ctx, n, bits, leaf):
     assert np.array_equal(ids, oids)
 
 
+@pytest.mark.parametrize("slots", ["1", "200"])
+@pytest.mark.parametrize("n", [9000, 70001, 300001])
+@pytest.mark.parametrize("bits,leaf", [(30, 0), (63, 1)], ids=["b30_ref", "b63_pid"])
+def test_split_tree_merges_match_oracle(ctx, monkeypatch, slots, n, bits, leaf):
+    """The upper tree levels merge a group's lists 8 at a time (then the
+    partial lists) when they hold more elements than a workgroup's slots.
+    LBVH_TREE_GROUP_SLOTS lowers that limit so ordinary scenes take the
+    split path: 9 level-0 groups in one level-1 group (n = 9000), a partial
+    last group (70001), ten level-1 groups under one top group (300001)."""
+    monkeypatch.setenv("LBVH_TREE_GROUP_SLOTS", slots)
+    v = W.cluster_vertices(2000 + n, n)
+    tri = W.make_triangles(v)
+    bvh = ctx.build_triangles(tri, code_bits=bits, leaf_order=leaf, keep_codes=True)
+    on, ob, osc, oids = oracle.build(tri, bits, leaf)
+    assert bvh.nodes().tobytes() == on.tobytes()
+    sc, ids, bounds = bvh.codes()
+    assert np.array_equal(sc, osc)
+    assert np.array_equal(ids, oids)
+    # the refit starts from the flags and links the tree kernels leave
+    moved = W.make_triangles(W.perturb(7, v, 1e-4))
+    ctx.refit_triangles(bvh, moved)
+    assert bvh.nodes().tobytes() == oracle.fit(on, moved).tobytes()
+
+
 SORT_CONFIGS = [(1, 8, 1024), (1, 8, 2048), (1, 8, 4096), (1, 10, 2048), (1, 10, 4096),
                 (2, 8, 2048), (2, 8, 4096), (2, 10, 2048), (2, 10, 4096),
                 (3, 8, 2048), (3, 8, 4096), (3, 10, 4096)]

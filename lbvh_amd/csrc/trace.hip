@@ -174,51 +174,6 @@ better(float t, uint32_t id, float best_t, uint32_t best_id)
   return (t < best_t) || (t == best_t && id < best_id && t != INFINITY);
 }
 
-// The traversal stack: capacity 128 with pushes dropped when full, as the
-// reference's.  Its top kStackLds entries live in an LDS ring per lane; older
-// entries spill to a private array (only trees deeper than the ring touch
-// it).  Entries: (node index, tmin bits).
-#ifndef LBVH_TRACE_RING
-#define LBVH_TRACE_RING 0
-#endif
-constexpr int kStackLds = LBVH_TRACE_RING;
-
-struct Stack
-{
-  uint2* ring;        // this lane's kStackLds LDS slots (stride kTraceBlock)
-  uint2 spill[kStack];
-  uint32_t sp = 0;    // entries on the stack
-  uint32_t base = 0;  // lowest entry index held in the ring
-  __device__ void push(uint32_t node, float t)
-  {
-    if (sp >= kStack)
-      return;
-    const uint2 e = make_uint2(node, __float_as_uint(t));
-    if constexpr (kStackLds == 0) {
-      spill[sp++] = e;
-    } else {
-      if (sp - base == kStackLds) {
-        spill[base] = ring[(base % kStackLds) * kTraceBlock];
-        base++;
-      }
-      ring[(sp % kStackLds) * kTraceBlock] = e;
-      sp++;
-    }
-  }
-  __device__ uint2 pop()
-  {
-    sp--;
-    if constexpr (kStackLds == 0) {
-      return spill[sp];
-    } else {
-      if (sp >= base)
-        return ring[(sp % kStackLds) * kTraceBlock];
-      base = sp;
-      return spill[sp];
-    }
-  }
-};
-
 // Ray layouts: kLayoutSingle, one tree; kLayoutBatch, a scene batch (ray i
 // belongs to the scene s with ray_off[s] <= i < ray_off[s + 1], whose nodes
 // start at prim_off[s] - s and whose triangles start at prim_off[s]; ids are
@@ -251,7 +206,6 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
         uint64_t nrays, uint32_t* __restrict__ prim_out, float* __restrict__ t_out,
         float* __restrict__ uv_out)
 {
-  __shared__ uint2 s_ring[(kStackLds > 0 ? kStackLds : 1) * kTraceBlock];
   // batches and jobs: the ray offsets of the scenes / jobs this workgroup's
   // rays fall in, staged once (s_rb) when they are at most kTraceBlock
   constexpr bool kStage = LAYOUT != kLayoutSingle;
@@ -340,8 +294,34 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     pos_dir[k] = r.d[k] > 0.0f;
   }
 
-  Stack st;
-  st.ring = s_ring + threadIdx.x;
+  // The traversal stack: capacity 128 with pushes dropped when full, as the
+  // reference's.  Entries: (node index, tmin bits).  Entries 0 .. sp - 2 are
+  // in a private array (the only thing in private memory) and the top one is
+  // kept in registers: a pop hands it over at once and leaves the register
+  // stale (stale: the top is still spill[sp - 1]).  The reload is issued
+  // after the popped node's fetch, so the two latencies overlap; a push onto
+  // a stale top needs no reload at all.
+  uint2 spill[kStack];
+  uint2 top = make_uint2(0u, 0u);
+  uint32_t sp = 0;
+  bool stale = false;
+  auto push = [&](uint32_t node, float tn) {
+    if (sp >= kStack)
+      return;
+    if (sp > 0 && !stale)
+      spill[sp - 1] = top;
+    stale = false;
+    top = make_uint2(node, __float_as_uint(tn));
+    sp++;
+  };
+  auto pop = [&]() {
+    if (stale)
+      top = spill[sp - 1];
+    const uint2 e = top;
+    sp--;
+    stale = sp > 0;
+    return e;
+  };
 
   uint32_t best = 0xffffffffu;
   float best_t = INFINITY, best_u = 0.0f, best_v = 0.0f;
@@ -354,9 +334,9 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
   bool have = true;
   for (;;) {
     if (!have) {
-      if (st.sp == 0)
+      if (sp == 0)
         break;
-      const uint2 e = st.pop();
+      const uint2 e = pop();
       if (best_t < __uint_as_float(e.y))
         continue;
       ni = e.x;
@@ -366,6 +346,10 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
       continue; // only possible for a corrupt node array
     const float4* wp = reinterpret_cast<const float4*>(&nodes[ni]);
     const float4 w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
+    if (stale) { // reload the top while the node is in flight
+      top = spill[sp - 1];
+      stale = false;
+    }
     const uint32_t lr[2] = { __float_as_uint(w3.x), __float_as_uint(w3.y) };
     const float bx[2][6] = { { w0.x, w0.y, w0.z, w0.w, w1.x, w1.y },
                              { w1.z, w1.w, w2.x, w2.y, w2.z, w2.w } };
@@ -411,7 +395,7 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     float next_t;
     if (lh && rh) {
       const bool left_near = tn2[0] < tn2[1];
-      st.push(left_near ? lr[1] : lr[0], left_near ? tn2[1] : tn2[0]);
+      push(left_near ? lr[1] : lr[0], left_near ? tn2[1] : tn2[0]);
       next_n = left_near ? lr[0] : lr[1];
       next_t = left_near ? tn2[0] : tn2[1];
     } else if (lh || rh) {
@@ -422,7 +406,7 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     }
     // push-then-pop of next_n: its push needs room; its pop is then pruned
     // when a closer hit is already known
-    if (st.sp < kStack && !(best_t < next_t)) {
+    if (sp < kStack && !(best_t < next_t)) {
       ni = next_n;
       have = true;
     }

@@ -12,7 +12,7 @@ namespace lbvh_dev {
 constexpr int kBlock = 256;
 constexpr int kCentroidMaxBlocks = 1024;
 constexpr int kCentroidUnroll = 4;
-constexpr int kEncodeMaxBlocks = 256;
+constexpr int kEncodeMaxBlocks = 1024;
 // Largest subtree (in leaves) fitted inside one workgroup's LDS.  Nodes whose
 // range holds more leaves form the "top" tree; their children with at most
 // kTreelet leaves root the treelets.
@@ -204,12 +204,17 @@ encode<uint64_t>(float qx, float qy, float qz)
 constexpr int kHistMaxWords = 8192; // passes * radix, 32 KiB of LDS
 constexpr int kEncodeUnroll = 4;
 
+// counts != nullptr: workgroup g encodes the chunk [g * chunk, (g + 1) *
+// chunk) and also writes its counts of pass first_pass to
+// counts[d * gridDim.x + g] -- the chunk counts of the reduce-then-scan
+// sort's first pass, which then needs no count pass of its own.
 template<typename KeyT>
 __global__ __launch_bounds__(kBlock) void
 k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
          const float* __restrict__ cz, uint64_t n,
          const float* __restrict__ bounds, KeyT* __restrict__ codes,
-         uint32_t* __restrict__ part_hist, int digit_bits, int first_pass, int passes)
+         uint32_t* __restrict__ part_hist, int digit_bits, int first_pass, int passes,
+         uint32_t* __restrict__ counts, uint64_t chunk)
 {
   extern __shared__ uint32_t s_hist[];
   const int radix = 1 << digit_bits;
@@ -226,15 +231,23 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
     scale[k] = dom / den;
   }
   __syncthreads();
-  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  // grid-stride over [0, n), or this workgroup's chunk
+  uint64_t first = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  uint64_t limit = n;
+  uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  if (counts) {
+    const uint64_t b = uint64_t(blockIdx.x) * chunk;
+    first = b + threadIdx.x;
+    limit = (b + chunk < n) ? b + chunk : n;
+    stride = kBlock;
+  }
   const KeyT mask = KeyT(radix - 1);
-  for (uint64_t i0 = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i0 < n;
-       i0 += stride * kEncodeUnroll) {
+  for (uint64_t i0 = first; i0 < limit; i0 += stride * kEncodeUnroll) {
     float x[kEncodeUnroll], y[kEncodeUnroll], z[kEncodeUnroll];
 #pragma unroll
     for (int u = 0; u < kEncodeUnroll; u++) {
       uint64_t i = i0 + u * stride;
-      if (i < n) {
+      if (i < limit) {
         x[u] = cx[i];
         y[u] = cy[i];
         z[u] = cz[i];
@@ -243,7 +256,7 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
 #pragma unroll
     for (int u = 0; u < kEncodeUnroll; u++) {
       uint64_t i = i0 + u * stride;
-      if (i >= n)
+      if (i >= limit)
         continue;
       float dx = x[u] - bmin[0];
       float dy = y[u] - bmin[1];
@@ -263,6 +276,9 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
   uint32_t* out = part_hist + size_t(blockIdx.x) * words;
   for (int j = threadIdx.x; j < words; j += kBlock)
     out[j] = s_hist[j];
+  if (counts)
+    for (int d = threadIdx.x; d < radix; d += kBlock)
+      counts[size_t(d) * gridDim.x + blockIdx.x] = s_hist[d];
 }
 
 // Sums the per-workgroup digit counts into the (zeroed) histogram:
@@ -1352,7 +1368,8 @@ launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx, floa
 hipError_t
 launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, uint64_t n,
               const float* bounds, void* codes, uint32_t* part_hist, uint32_t* hist,
-              int digit_bits, int first_pass, int passes, hipStream_t s)
+              int digit_bits, int first_pass, int passes, uint32_t* counts, uint32_t groups,
+              uint64_t chunk, hipStream_t s)
 {
   if (first_pass < 0 || first_pass >= passes)
     return hipErrorInvalidValue;
@@ -1361,14 +1378,19 @@ launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, 
     return hipErrorInvalidValue;
   size_t lds = size_t(words) * sizeof(uint32_t);
   uint32_t g = encode_groups(n);
+  if (counts) {
+    if (groups == 0 || chunk == 0 || uint64_t(groups) * chunk < n)
+      return hipErrorInvalidValue;
+    g = groups;
+  }
   if (code_bits == 30)
     hipLaunchKernelGGL(k_encode<uint32_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
                        bounds, static_cast<uint32_t*>(codes), part_hist, digit_bits, first_pass,
-                       passes);
+                       passes, counts, chunk);
   else
     hipLaunchKernelGGL(k_encode<uint64_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
                        bounds, static_cast<uint64_t*>(codes), part_hist, digit_bits, first_pass,
-                       passes);
+                       passes, counts, chunk);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;

@@ -7,6 +7,7 @@
 // builds) or borrows the caller's device pointer (*_device builds).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -389,8 +390,12 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   }
   if ((st = ensure(c, c->partials, 24ull * centroid_partials_needed(n))) != LBVH_OK)
     return st;
-  if ((st = ensure(c, c->part_hist, 4ull * encode_groups(n) * uint64_t(passes) * radix)) !=
-      LBVH_OK)
+  // reduce-then-scan and hybrid sorts: the encode counts the first pass's
+  // chunks itself (one encode workgroup per chunk)
+  const bool chunked = cfg.strategy == 1 || cfg.strategy == 2;
+  const uint32_t rts_g = chunked ? rts_groups(n, cfg, nullptr) : 0u;
+  const uint64_t hist_rows = std::max<uint64_t>(encode_groups(n), rts_g);
+  if ((st = ensure(c, c->part_hist, 4ull * hist_rows * uint64_t(passes) * radix)) != LBVH_OK)
     return st;
   const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
   if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
@@ -444,6 +449,7 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   ws.err = small + kSmallErr;
   ws.counts = static_cast<uint32_t*>(c->counts.p);
   ws.counts_words = c->counts.cap / 4;
+  ws.first_counts = chunked;
 
   begin_timed(c, 1);
   record(c, 0);
@@ -456,7 +462,8 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
     e = launch_encode(opts.code_bits, cent, cent + n, cent + 2 * n, n, b->d_bounds,
                       c->keys_a.p, static_cast<uint32_t*>(c->part_hist.p), ws.hist,
                       cfg.digit_bits, cfg.strategy == 2 ? passes - cfg.global_digits : 0, passes,
-                      c->stream);
+                      chunked ? ws.counts : nullptr, rts_g,
+                      chunked ? rts_chunk_keys(n, cfg) : 0, c->stream);
   record(c, 1);
   bool in_b = false;
   if (e == hipSuccess)

@@ -40,6 +40,7 @@ struct SortWorkspace
   uint32_t* err = nullptr;     // bounded-wait failure flag
   uint32_t* counts = nullptr;  // reduce-then-scan chunk counts [radix * groups]
   uint64_t counts_words = 0;
+  bool first_counts = false;   // the first reduce-then-scan pass's counts are in place
 };
 
 // centroid of every primitive box (SoA) and the centroid bounds; also zeroes
@@ -54,10 +55,13 @@ uint32_t centroid_partials_needed(uint64_t n);
 // passes first_pass .. passes - 1 (per-workgroup counts in
 // part_hist[encode_groups(n)][(passes - first_pass) << digit_bits], summed
 // into hist[p << digit_bits ...] for pass p).
+// counts != nullptr: `groups` workgroups encode chunks of `chunk` keys and
+// also write the chunk counts of pass first_pass (counts[d * groups + g]),
+// the reduce-then-scan layout; part_hist then holds `groups` rows.
 hipError_t launch_encode(int code_bits, const float* cx, const float* cy, const float* cz,
                          uint64_t n, const float* bounds, void* codes, uint32_t* part_hist,
                          uint32_t* hist, int digit_bits, int first_pass, int passes,
-                         hipStream_t s);
+                         uint32_t* counts, uint32_t groups, uint64_t chunk, hipStream_t s);
 uint32_t encode_groups(uint64_t n);
 
 // LSD onesweep radix sort of (key, id) pairs in place across ping-pong
@@ -74,6 +78,8 @@ uint64_t sort_status_words(uint64_t n, const SortConfig& cfg);
 uint64_t sort_counts_words(uint64_t n, const SortConfig& cfg);
 // Number of chunks of the reduce-then-scan strategy (and tiles per chunk).
 uint32_t rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles);
+// Keys per chunk of the reduce-then-scan strategy.
+uint64_t rts_chunk_keys(uint64_t n, const SortConfig& cfg);
 
 // Scene batches: prim_off == nullptr is one scene of n primitives; otherwise
 // scene s holds primitives [prim_off[s], prim_off[s + 1]) (n = prim_off[nseg],

@@ -1214,6 +1214,14 @@ rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles)
   return uint32_t((tiles + ct - 1) / ct);
 }
 
+uint64_t
+rts_chunk_keys(uint64_t n, const SortConfig& cfg)
+{
+  uint32_t ct = 1;
+  rts_groups(n, cfg, &ct);
+  return uint64_t(ct) * kSortBlock * uint64_t(cfg.items);
+}
+
 template<typename KeyT, bool VALS, int RBITS, int ITEMS>
 static hipError_t
 run_rts(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb,
@@ -1233,8 +1241,9 @@ run_rts(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb
   bool cur_b = false;
   for (int p = 0; p < passes; p++) {
     const int shift = p * RBITS;
-    hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(groups), dim3(kSortBlock), 0, s, src,
-                       uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts);
+    if (p > 0 || !ws.first_counts)
+      hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(groups), dim3(kSortBlock), 0, s, src,
+                         uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts);
     hipLaunchKernelGGL((k_rts_scan<RBITS>), dim3(RADIX), dim3(kSortBlock), 0, s, ws.counts,
                        groups, ws.hist + size_t(p) * RADIX, nullptr);
     hipLaunchKernelGGL((k_rts_downsweep<KeyT, VALS, RBITS, ITEMS>), dim3(groups),
@@ -1280,8 +1289,9 @@ run_hybrid(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t*
   uint32_t* vdst = vb;
   for (int p = passes - gd; p < passes; p++) {
     const int shift = p * RBITS;
-    hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(groups), dim3(kSortBlock), 0, s, src,
-                       uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts);
+    if (p > passes - gd || !ws.first_counts)
+      hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(groups), dim3(kSortBlock), 0, s, src,
+                         uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts);
     hipLaunchKernelGGL((k_rts_scan<RBITS>), dim3(RADIX), dim3(kSortBlock), 0, s, ws.counts,
                        groups, ws.hist + size_t(p) * RADIX,
                        p == passes - gd ? big_count : nullptr);

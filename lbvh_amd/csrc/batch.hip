@@ -188,21 +188,33 @@ expand10_seg(uint32_t v)
 }
 
 // Morton codes with morton_domain<4> (D = 1024) against each scene's bounds,
-// keyed (scene << 30 | code); LDS digit histograms of every radix pass, one
-// row per workgroup in part_hist.
+// keyed (scene << 30 | code); LDS digit histograms of the radix passes
+// first_pass .. passes - 1, one row per workgroup in part_hist.  counts !=
+// nullptr: workgroup g encodes the chunk [g * chunk, (g + 1) * chunk) and
+// also writes its pass-first_pass counts to counts[d * gridDim.x + g] (the
+// reduce-then-scan sort's first chunk counts).
 __global__ __launch_bounds__(kSegBlock) void
 k_scene_encode(const float* __restrict__ cx, const float* __restrict__ cy,
                const float* __restrict__ cz, uint64_t n, const uint64_t* __restrict__ off,
                uint32_t nseg, const int32_t* __restrict__ bkeys, uint64_t* __restrict__ keys,
-               uint32_t* __restrict__ part_hist, int digit_bits, int passes)
+               uint32_t* __restrict__ part_hist, int digit_bits, int first_pass, int passes,
+               uint32_t* __restrict__ counts, uint64_t chunk)
 {
   extern __shared__ uint32_t s_hist[];
   __shared__ uint32_t s_first;
   const int radix = 1 << digit_bits;
-  const int words = radix * passes;
+  const int words = radix * (passes - first_pass);
   for (int j = threadIdx.x; j < words; j += kSegBlock)
     s_hist[j] = 0u;
-  const BlockRange r = block_range(n);
+  BlockRange r;
+  if (counts) {
+    r.begin = uint64_t(blockIdx.x) * chunk;
+    if (r.begin > n)
+      r.begin = n;
+    r.end = (r.begin + chunk < n) ? r.begin + chunk : n;
+  } else {
+    r = block_range(n);
+  }
   if (threadIdx.x == 0)
     s_first = r.begin < r.end ? find_scene(off, nseg, int64_t(r.begin), 1, false) : 0u;
   __syncthreads();
@@ -247,14 +259,17 @@ k_scene_encode(const float* __restrict__ cx, const float* __restrict__ cy,
                             expand10_seg(uint32_t(qz));
       const uint64_t key = (uint64_t(s) << 30) | uint64_t(code);
       keys[i] = key;
-      for (int p = 0; p < passes; p++)
-        atomicAdd(&s_hist[p * radix + int((key >> (p * digit_bits)) & mask)], 1u);
+      for (int p = first_pass; p < passes; p++)
+        atomicAdd(&s_hist[(p - first_pass) * radix + int((key >> (p * digit_bits)) & mask)], 1u);
     }
   }
   __syncthreads();
   uint32_t* out = part_hist + size_t(blockIdx.x) * words;
   for (int j = threadIdx.x; j < words; j += kSegBlock)
     out[j] = s_hist[j];
+  if (counts)
+    for (int d = threadIdx.x; d < radix; d += kSegBlock)
+      counts[size_t(d) * gridDim.x + blockIdx.x] = s_hist[d];
 }
 
 // hist[w] += sum over workgroups of part[g][w] (a slice of 16 per block row).
@@ -314,19 +329,29 @@ hipError_t
 launch_scene_encode(const float* cx, const float* cy, const float* cz, uint64_t n,
                     const uint64_t* prim_off, uint32_t nseg, const int32_t* bounds_key,
                     uint64_t* keys, uint32_t* part_hist, uint32_t* hist, int digit_bits,
-                    int passes, hipStream_t s)
+                    int first_pass, int passes, uint32_t* counts, uint32_t groups,
+                    uint64_t chunk, hipStream_t s)
 {
-  const uint32_t words = uint32_t(1u << digit_bits) * uint32_t(passes);
+  if (first_pass < 0 || first_pass >= passes)
+    return hipErrorInvalidValue;
+  const uint32_t words = uint32_t(1u << digit_bits) * uint32_t(passes - first_pass);
   if (words > uint32_t(kSegHistMaxWords))
     return hipErrorInvalidValue;
-  const uint32_t g = scene_encode_groups(n);
+  uint32_t g = scene_encode_groups(n);
+  if (counts) {
+    if (groups == 0 || chunk == 0 || uint64_t(groups) * chunk < n)
+      return hipErrorInvalidValue;
+    g = groups;
+  }
   hipLaunchKernelGGL(k_scene_encode, dim3(g), dim3(kSegBlock), size_t(words) * 4, s, cx, cy, cz,
-                     n, prim_off, nseg, bounds_key, keys, part_hist, digit_bits, passes);
+                     n, prim_off, nseg, bounds_key, keys, part_hist, digit_bits, first_pass,
+                     passes, counts, chunk);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
   hipLaunchKernelGGL(k_scene_hist_reduce, dim3((words + kSegBlock - 1) / kSegBlock, (g + 15) / 16),
-                     dim3(kSegBlock), 0, s, part_hist, g, words, hist);
+                     dim3(kSegBlock), 0, s, part_hist, g, words,
+                     hist + size_t(first_pass) * (size_t(1) << digit_bits));
   return hipGetLastError();
 }
 

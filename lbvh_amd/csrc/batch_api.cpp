@@ -7,6 +7,7 @@
 // and replayed.  Clearing inside a captured sequence uses fill kernels only.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -122,9 +123,12 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
   const uint64_t n = b->n;
   const uint64_t m = n - b->nseg; // internal nodes
   uint64_t hist_words = 0;
+  uint64_t hist_rows = scene_encode_groups(n);
   b->small_words = 0;
   b->counts_words = 0;
   for (const SortConfig& cfg : cfgs) {
+    // reduce-then-scan and hybrid: one encode workgroup per sort chunk
+    hist_rows = std::max<uint64_t>(hist_rows, rts_groups(n, cfg, nullptr));
     const uint32_t radix = 1u << cfg.digit_bits;
     const uint32_t words = uint32_t(sort_passes(b->key_bits, cfg.digit_bits)) * radix;
     if (words > hist_words)
@@ -154,7 +158,7 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
     { reinterpret_cast<void**>(&b->d_keys_b), 8 * n },
     { reinterpret_cast<void**>(&b->d_ids_a), b->with_ids ? 4 * n : 0 },
     { reinterpret_cast<void**>(&b->d_ids_b), b->with_ids ? 4 * n : 0 },
-    { reinterpret_cast<void**>(&b->d_part_hist), 4ull * scene_encode_groups(n) * hist_words },
+    { reinterpret_cast<void**>(&b->d_part_hist), 4ull * hist_rows * hist_words },
     { reinterpret_cast<void**>(&b->d_small), 4ull * b->small_words },
     { reinterpret_cast<void**>(&b->d_counts), 4 * b->counts_words },
     { &b->d_treelets, 8 * n },
@@ -192,6 +196,11 @@ queue_build(lbvh_ctx* c, lbvh_batch* b, bool timed)
   ws.err = small + kSmallErr;
   ws.counts = b->d_counts;
   ws.counts_words = b->counts_words;
+  // reduce-then-scan and the hybrid (what batches sort with): the encode
+  // writes the first pass's chunk counts
+  const bool chunked = b->cfg.strategy == 1 || b->cfg.strategy == 2;
+  ws.first_counts = chunked;
+  const int first_pass = b->cfg.strategy == 2 ? b->passes - b->cfg.global_digits : 0;
   if (timed)
     record(c, 0);
   hipError_t e = launch_fill_u32(small + kSmallTickets, b->small_words - kSmallTickets, 0u, s);
@@ -203,7 +212,10 @@ queue_build(lbvh_ctx* c, lbvh_batch* b, bool timed)
   if (e == hipSuccess)
     e = launch_scene_encode(b->d_cent, b->d_cent + n, b->d_cent + 2 * n, n, b->d_off, b->nseg,
                             b->d_bounds_key, b->d_keys_a, b->d_part_hist, ws.hist,
-                            b->cfg.digit_bits, b->passes, s);
+                            b->cfg.digit_bits, first_pass, b->passes,
+                            chunked ? b->d_counts : nullptr,
+                            chunked ? rts_groups(n, b->cfg, nullptr) : 0u,
+                            chunked ? rts_chunk_keys(n, b->cfg) : 0, s);
   if (timed)
     record(c, 1);
   bool in_b = false;

@@ -157,13 +157,15 @@ hipError_t launch_scene_bounds(const float* prims, int kind, uint64_t n, const u
                                uint32_t nseg, float* cx, float* cy, float* cz, int32_t* bounds_key,
                                hipStream_t s);
 // Batched 30-bit codes keyed (scene << 30 | code) and the digit histograms
-// of `passes` radix passes (per-workgroup rows in part_hist
-// [scene_encode_groups(n)][passes << digit_bits], summed into hist, which
-// must be zero).
+// of radix passes first_pass .. passes - 1 (per-workgroup rows in part_hist,
+// summed into hist[p << digit_bits ...], which must be zero).  counts !=
+// nullptr: `groups` workgroups encode chunks of `chunk` keys and write the
+// first pass's chunk counts (counts[d * groups + g]) as launch_encode does.
 hipError_t launch_scene_encode(const float* cx, const float* cy, const float* cz, uint64_t n,
                                const uint64_t* prim_off, uint32_t nseg, const int32_t* bounds_key,
                                uint64_t* keys, uint32_t* part_hist, uint32_t* hist,
-                               int digit_bits, int passes, hipStream_t s);
+                               int digit_bits, int first_pass, int passes, uint32_t* counts,
+                               uint32_t groups, uint64_t chunk, hipStream_t s);
 uint32_t scene_encode_groups(uint64_t n);
 
 // Fill `words` 32-bit words with `value`.

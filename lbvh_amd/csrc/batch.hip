@@ -96,8 +96,11 @@ flush_box(int32_t* __restrict__ keys, uint32_t s, const float lo[3], const float
   }
 }
 
+// KIND: kPrimTriangle or kPrimAabb (a template argument, so that a thread's
+// kSegUnroll loads issue together).
+template<int KIND>
 __global__ __launch_bounds__(kSegBlock) void
-k_scene_bounds(const float* __restrict__ prims, int kind, uint64_t n,
+k_scene_bounds(const float* __restrict__ prims, uint64_t n,
                const uint64_t* __restrict__ off, uint32_t nseg, float* __restrict__ cx,
                float* __restrict__ cy, float* __restrict__ cz, int32_t* __restrict__ keys)
 {
@@ -117,12 +120,12 @@ k_scene_bounds(const float* __restrict__ prims, int kind, uint64_t n,
   float rlo[3] = { INFINITY, INFINITY, INFINITY };
   float rhi[3] = { -INFINITY, -INFINITY, -INFINITY };
   for (uint64_t t0 = r.begin; t0 < r.end; t0 += uint64_t(kSegBlock) * kSegUnroll) {
+    // unconditional loads (past the end: the range's last primitive, unused)
     Box b[kSegUnroll];
 #pragma unroll
     for (int u = 0; u < kSegUnroll; u++) {
       const uint64_t i = t0 + uint64_t(u) * kSegBlock + threadIdx.x;
-      if (i < r.end)
-        b[u] = load_prim_box(prims, kind, i);
+      b[u] = load_prim_box(prims, KIND, i < r.end ? i : r.end - 1);
     }
 #pragma unroll
     for (int u = 0; u < kSegUnroll; u++) {
@@ -223,15 +226,15 @@ k_scene_encode(const float* __restrict__ cx, const float* __restrict__ cy,
   float bmin[3] = { 0.0f, 0.0f, 0.0f }, scale[3] = { 0.0f, 0.0f, 0.0f };
   const uint64_t mask = uint64_t(radix - 1);
   for (uint64_t t0 = r.begin; t0 < r.end; t0 += uint64_t(kSegBlock) * kSegUnroll) {
+    // unconditional loads (past the end: the range's last centre, unused)
     float x[kSegUnroll], y[kSegUnroll], z[kSegUnroll];
 #pragma unroll
     for (int u = 0; u < kSegUnroll; u++) {
       const uint64_t i = t0 + uint64_t(u) * kSegBlock + threadIdx.x;
-      if (i < r.end) {
-        x[u] = cx[i];
-        y[u] = cy[i];
-        z[u] = cz[i];
-      }
+      const uint64_t j = i < r.end ? i : r.end - 1;
+      x[u] = cx[j];
+      y[u] = cy[j];
+      z[u] = cz[j];
     }
 #pragma unroll
     for (int u = 0; u < kSegUnroll; u++) {
@@ -320,8 +323,12 @@ launch_scene_bounds(const float* prims, int kind, uint64_t n, const uint64_t* pr
                     hipStream_t s)
 {
   const uint32_t g = scene_encode_groups(n);
-  hipLaunchKernelGGL(k_scene_bounds, dim3(g), dim3(kSegBlock), 0, s, prims, kind, n, prim_off,
-                     nseg, cx, cy, cz, bounds_key);
+  if (kind == kPrimTriangle)
+    hipLaunchKernelGGL(k_scene_bounds<kPrimTriangle>, dim3(g), dim3(kSegBlock), 0, s, prims, n,
+                       prim_off, nseg, cx, cy, cz, bounds_key);
+  else
+    hipLaunchKernelGGL(k_scene_bounds<kPrimAabb>, dim3(g), dim3(kSegBlock), 0, s, prims, n,
+                       prim_off, nseg, cx, cy, cz, bounds_key);
   return hipGetLastError();
 }
 

@@ -71,8 +71,11 @@ block_reduce_box(float lo[3], float hi[3], float (*s_box)[6])
   }
 }
 
+// KIND: kPrimTriangle or kPrimAabb (a template argument, so that the loads
+// of a thread's kCentroidUnroll primitives issue together).
+template<int KIND>
 __global__ __launch_bounds__(kBlock) void
-k_centroid_bounds(const float* __restrict__ prims, int kind, uint64_t n,
+k_centroid_bounds(const float* __restrict__ prims, uint64_t n,
                   float* __restrict__ cx, float* __restrict__ cy,
                   float* __restrict__ cz, float* __restrict__ partials,
                   uint32_t* __restrict__ ticket, float* __restrict__ bounds,
@@ -90,12 +93,13 @@ k_centroid_bounds(const float* __restrict__ prims, int kind, uint64_t n,
   float hi[3] = { -INFINITY, -INFINITY, -INFINITY };
   // kCentroidUnroll primitives per thread per round, all loads issued first
   for (uint64_t i0 = tid; i0 < n; i0 += stride * kCentroidUnroll) {
+    // unconditional loads (indexes past the end read the last primitive and
+    // are not used), so that all of them are in flight together
     Box b[kCentroidUnroll];
 #pragma unroll
     for (int u = 0; u < kCentroidUnroll; u++) {
       const uint64_t i = i0 + u * stride;
-      if (i < n)
-        b[u] = load_prim_box(prims, kind, i);
+      b[u] = load_prim_box(prims, KIND, i < n ? i : n - 1);
     }
 #pragma unroll
     for (int u = 0; u < kCentroidUnroll; u++) {
@@ -243,15 +247,16 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
   }
   const KeyT mask = KeyT(radix - 1);
   for (uint64_t i0 = first; i0 < limit; i0 += stride * kEncodeUnroll) {
+    // unconditional loads (past the end: the last centre, unused), all in
+    // flight together
     float x[kEncodeUnroll], y[kEncodeUnroll], z[kEncodeUnroll];
 #pragma unroll
     for (int u = 0; u < kEncodeUnroll; u++) {
-      uint64_t i = i0 + u * stride;
-      if (i < limit) {
-        x[u] = cx[i];
-        y[u] = cy[i];
-        z[u] = cz[i];
-      }
+      const uint64_t i = i0 + u * stride;
+      const uint64_t j = i < limit ? i : limit - 1;
+      x[u] = cx[j];
+      y[u] = cy[j];
+      z[u] = cz[j];
     }
 #pragma unroll
     for (int u = 0; u < kEncodeUnroll; u++) {
@@ -1360,8 +1365,12 @@ launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx, floa
                        uint32_t* zero_ptr, uint32_t zero_words, hipStream_t s)
 {
   uint32_t g = centroid_partials_needed(n);
-  hipLaunchKernelGGL(k_centroid_bounds, dim3(g), dim3(kBlock), 0, s, prims, kind, n, cx, cy,
-                     cz, partials, ticket, bounds, zero_ptr, zero_words);
+  if (kind == kPrimTriangle)
+    hipLaunchKernelGGL(k_centroid_bounds<kPrimTriangle>, dim3(g), dim3(kBlock), 0, s, prims, n,
+                       cx, cy, cz, partials, ticket, bounds, zero_ptr, zero_words);
+  else
+    hipLaunchKernelGGL(k_centroid_bounds<kPrimAabb>, dim3(g), dim3(kBlock), 0, s, prims, n, cx,
+                       cy, cz, partials, ticket, bounds, zero_ptr, zero_words);
   return hipGetLastError();
 }
 

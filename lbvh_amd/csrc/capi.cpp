@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -421,6 +422,9 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   c->epoch_next += kMaxSortPasses;
 
   auto* b = new (std::nothrow) lbvh_bvh;
+  static std::atomic<uint64_t> next_tree_id{ 1 };
+  if (b)
+    b->id = next_tree_id.fetch_add(1);
   if (!b)
     return fail(c, LBVH_E_OOM, "host allocation failed");
   b->device = c->device;
@@ -594,7 +598,7 @@ lbvh_ctx_destroy(lbvh_ctx* c)
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = { &c->cent, &c->keys_a, &c->keys_b, &c->ids_a, &c->ids_b, &c->partials,
                      &c->part_hist, &c->small, &c->status, &c->counts, &c->treelets,
-                     &c->io_in, &c->io_out, &c->tree, &c->tree_tickets };
+                     &c->io_in, &c->io_out, &c->tree, &c->tree_tickets, &c->treelet_words };
   for (auto* b : bufs)
     dev_free(b->p);
   for (auto& es : c->sets)
@@ -709,7 +713,8 @@ lbvh_ctx_workspace_bytes(const lbvh_ctx* c)
     return 0;
   return c->cent.cap + c->keys_a.cap + c->keys_b.cap + c->ids_a.cap + c->ids_b.cap +
          c->partials.cap + c->part_hist.cap + c->small.cap + c->status.cap + c->counts.cap +
-         c->treelets.cap + c->io_in.cap + c->io_out.cap + c->tree.cap + c->tree_tickets.cap;
+         c->treelets.cap + c->io_in.cap + c->io_out.cap + c->tree.cap + c->tree_tickets.cap +
+         c->treelet_words.cap;
 }
 
 lbvh_status
@@ -789,24 +794,33 @@ lbvh_build_aabbs(lbvh_ctx* c, const float* boxes, uint64_t n, const lbvh_build_o
   return build_host(c, boxes, 1, n, opts, out);
 }
 
-// Queues a refit of `b` over its current primitives.
+// Queues a refit of `b` over its current primitives.  The treelet list
+// depends on the topology only, so a context keeps the list of the tree it
+// last refitted and a refit of the same tree reuses it.
 static lbvh_status
 queue_refit(lbvh_ctx* c, lbvh_bvh* b)
 {
   lbvh_status st;
+  const void* kept = c->treelets.p;
   if ((st = ensure(c, c->treelets, 8 * b->n)) != LBVH_OK)
     return st;
-  if ((st = ensure(c, c->small, 4ull * (kSmallHist + 256), true)) != LBVH_OK)
+  if (c->treelets.p != kept)
+    c->treelets_of = 0; // reallocated: nothing kept
+  if ((st = ensure(c, c->treelet_words, 8, true)) != LBVH_OK)
     return st;
-  uint32_t* small = static_cast<uint32_t*>(c->small.p);
+  uint32_t* tw = static_cast<uint32_t*>(c->treelet_words.p); // count, pull queue
+  const bool enumerate = c->treelets_of != b->id;
   begin_timed(c, 3);
   record(c, 0);
-  HIP_TRY(c, launch_fill_u32(small + kSmallTreeletCount, 2, 0u, c->stream));
-  HIP_TRY(c, launch_enum_treelets(b->n, nullptr, 1, b->d_parent, b->d_other_end, c->treelets.p,
-                                  small + kSmallTreeletCount, c->stream));
-  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, nullptr, 1, b->d_nodes, b->d_parent, b->d_leafprim, b->d_flags,
-                        c->treelets.p, small + kSmallTreeletCount, small + kSmallTreeletQueue,
-                        b->d_wide, c->stream));
+  HIP_TRY(c, launch_fill_u32(enumerate ? tw : tw + 1, enumerate ? 2 : 1, 0u, c->stream));
+  if (enumerate) {
+    c->treelets_of = 0;
+    HIP_TRY(c, launch_enum_treelets(b->n, nullptr, 1, b->d_parent, b->d_other_end, c->treelets.p,
+                                    tw, c->stream));
+    c->treelets_of = b->id;
+  }
+  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, nullptr, 1, b->d_nodes, b->d_parent, b->d_leafprim,
+                        b->d_flags, c->treelets.p, tw, tw + 1, b->d_wide, c->stream));
   record(c, 1, true);
   return LBVH_OK;
 }

@@ -68,6 +68,10 @@ struct lbvh_ctx
   // workspace
   lbvh_rt::DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, part_hist, small, status, counts,
     treelets, io_in, io_out, tree, tree_tickets;
+  // the treelet list in `treelets` belongs to the tree with this id (0: none);
+  // its count and the fit's pull queue live in treelet_words
+  uint64_t treelets_of = 0;
+  lbvh_rt::DevBuf treelet_words;
 };
 
 // A built tree: node array, parent links, fit flags, centroid bounds,
@@ -92,6 +96,7 @@ struct lbvh_bvh
   void* d_wide = nullptr;          // traversal records (64 B per internal node)
   uint64_t bytes = 0;
   lbvh_eng::SortConfig sort;       // the sort configuration the build used
+  uint64_t id = 0;                 // unique per tree in the process (refit caches)
 };
 
 namespace lbvh_rt {

@@ -275,6 +275,25 @@ def test_refit_equals_oracle_fit(ctx, leaf):
     assert bvh.nodes().tobytes() == before.tobytes()
 
 
+def test_refits_alternating_between_trees(ctx):
+    """A context keeps the treelet list of the tree it last refitted: refits
+    of A, B, A again and B again each equal the oracle's fit."""
+    va, ta = W.scene_uniform(5, 150000, 5e-4)
+    vb, tb = W.scene_clustered(6, 90000)
+    a = ctx.build_triangles(ta, code_bits=63, leaf_order=1)
+    b = ctx.build_triangles(tb, code_bits=30, leaf_order=0)
+    na, nb = a.nodes(), b.nodes()
+    for k in range(2):
+        ma = W.make_triangles(W.perturb(40 + k, va, 1e-4))
+        ctx.refit_triangles(a, ma)
+        assert a.nodes().tobytes() == oracle.fit(na, ma).tobytes(), "A, round %d" % k
+        mb = W.make_triangles(W.perturb(50 + k, vb.reshape(-1, 9), 1e-4))
+        ctx.refit_triangles(b, mb)
+        assert b.nodes().tobytes() == oracle.fit(nb, mb).tobytes(), "B, round %d" % k
+    a.destroy()
+    b.destroy()
+
+
 def test_device_path_equals_host_path(ctx):
     from lbvh_amd import hip
     v, tri = W.scene_uniform(1, 65536, 0.01)

@@ -648,8 +648,8 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
 // Outputs: node and traversal records, parent links (kLink*), other_end,
 // leafprim, and the fit flags left at 0 (what a refit starts from).
 
-constexpr int kTreeLeaves = kTreelet;                   // level-0 positions per workgroup
-constexpr int kTreeFan = 32;                            // workgroups merged per level
+constexpr int kTreeLeaves = 512;                        // level-0 positions per workgroup
+constexpr int kTreeFan = 64;                            // workgroups merged per level
 constexpr int kTreeSubFan = 8;                          // lists per merge that always fits
 constexpr int kTreeSubLists = kTreeFan / kTreeSubFan;   // lists of a split merge
 constexpr int kTreeSegCap = 192;                        // 2 x (tree height bound + 1)
@@ -753,6 +753,8 @@ struct TreeLevels
 
 constexpr int kTreeLeaf0 = 2048; // leaves of the delta min-tree (>= kTreeSlots + 1)
 static_assert(kTreeLeaf0 >= kTreeSlots + 1, "delta tree size");
+constexpr int kTreeLeaf0First = 1024; // the same at level 0 (>= kTreeLeaves + 1)
+static_assert(kTreeLeaf0First >= kTreeLeaves + 1, "level-0 delta tree size");
 constexpr int kTreeUpThreads = 1024; // workgroup of the upper levels (one CU each: LDS-bound)
 constexpr int16_t kTreeInf = 0x7fff;
 
@@ -835,7 +837,7 @@ box_has_nan(const Box& b)
   return x;
 }
 
-// FIRST: the level-0 kernel (leaves, 1,024 elements: two workgroups fit a CU);
+// FIRST: the level-0 kernel (leaves, 512 elements, ~40 KB of LDS: four workgroups fit a CU);
 // else the kernel of the levels above (its workgroup b starts as level-1
 // workgroup b).
 template<typename KeyT, typename CodeT, bool SEG, bool FIRST>
@@ -861,9 +863,9 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
   __shared__ uint32_t s_el_val[SL]; // child value when the element is one leaf
   // min tree over d (leaves at kTreeLeaf0): d[e] is the delta left of
   // element e, d[m] the delta right of the last
-  __shared__ int16_t s_tree[2 * kTreeLeaf0];
+  __shared__ int16_t s_tree[2 * (FIRST ? kTreeLeaf0First : kTreeLeaf0)];
   __shared__ int16_t s_dsep[FIRST ? 1 : SL + 1];
-  int16_t* const s_d = FIRST ? s_tree + kTreeLeaf0 : s_dsep; // level 0: the tree's own leaves
+  int16_t* const s_d = FIRST ? s_tree + kTreeLeaf0First : s_dsep; // level 0: the tree's own leaves
   // split j (between elements j and j + 1): its node's element range [a, b]
   __shared__ uint16_t s_na[SL];
   __shared__ uint16_t s_nb[SL];
@@ -1002,8 +1004,8 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
 
   for (;;) {
     // ---- nodes of this workgroup: ranges from the nearest smaller deltas -------
-    // level 0: 2048 leaves, d already in place; above: the next power of two
-    int leaf0 = kTreeLeaf0;
+    // level 0: kTreeLeaf0First leaves, d already in place; above: the next power of two
+    int leaf0 = kTreeLeaf0First;
     if constexpr (!FIRST) {
       leaf0 = 2;
       while (leaf0 < m + 1)

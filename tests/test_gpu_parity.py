@@ -83,8 +83,9 @@ def test_split_tree_merges_match_oracle(ctx, monkeypatch, slots, n, bits, leaf):
     """The upper tree levels merge a group's lists 8 at a time (then the
     partial lists) when they hold more elements than a workgroup's slots.
     LBVH_TREE_GROUP_SLOTS lowers that limit so ordinary scenes take the
-    split path: 9 level-0 groups in one level-1 group (n = 9000), a partial
-    last group (70001), ten level-1 groups under one top group (300001)."""
+    split path (512-position level-0 workgroups, 64 per group): 18 level-0
+    workgroups in one level-1 group (n = 9000), a partial last group
+    (70001), ten level-1 workgroups under one top group (300001)."""
     monkeypatch.setenv("LBVH_TREE_GROUP_SLOTS", slots)
     v = W.cluster_vertices(2000 + n, n)
     tri = W.make_triangles(v)

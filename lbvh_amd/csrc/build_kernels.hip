@@ -161,8 +161,8 @@ k_centroid_bounds(const float* __restrict__ prims, uint64_t n,
 // magic-mask expansions of morton_encoder<4>/<8> (lbvh.h:1352-1358,
 // 1377-1385).  The 63-bit variant applies morton_encoder<8> to FP32 centres.
 // Each workgroup also counts the digits of the radix passes first_pass ..
-// passes - 1 (those whose global histogram the sort reads) in LDS and writes
-// its counts (plain stores) for k_hist_reduce.
+// passes - 1 (those whose global histogram the sort reads) in LDS and adds
+// its nonzero counts to the global histogram with atomics.
 
 __device__ __forceinline__ uint32_t
 expand10(uint32_t n)
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void
 k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
          const float* __restrict__ cz, uint64_t n,
          const float* __restrict__ bounds, KeyT* __restrict__ codes,
-         uint32_t* __restrict__ part_hist, int digit_bits, int first_pass, int passes,
+         uint32_t* __restrict__ hist, int digit_bits, int first_pass, int passes,
          uint32_t* __restrict__ counts, uint64_t chunk)
 {
   extern __shared__ uint32_t s_hist[];
@@ -278,33 +278,16 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
     }
   }
   __syncthreads();
-  uint32_t* out = part_hist + size_t(blockIdx.x) * words;
-  for (int j = threadIdx.x; j < words; j += kBlock)
-    out[j] = s_hist[j];
+  // the workgroup's counts go straight into the global histogram (zeroed
+  // by k_centroid_bounds): no per-workgroup rows and no reduction kernel
+  for (int j = threadIdx.x; j < words; j += kBlock) {
+    const uint32_t v = s_hist[j];
+    if (v)
+      atomicAdd(&hist[j], v);
+  }
   if (counts)
     for (int d = threadIdx.x; d < radix; d += kBlock)
       counts[size_t(d) * gridDim.x + blockIdx.x] = s_hist[d];
-}
-
-// Sums the per-workgroup digit counts into the (zeroed) histogram:
-// hist[w] += sum of part[g][w] over a slice of kHistSlice workgroups.
-constexpr uint32_t kHistSlice = 16;
-
-__global__ __launch_bounds__(kBlock) void
-k_hist_reduce(const uint32_t* __restrict__ part, uint32_t groups, uint32_t words,
-              uint32_t* __restrict__ hist)
-{
-  const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
-  if (w >= words)
-    return;
-  const uint32_t g0 = blockIdx.y * kHistSlice;
-  uint32_t s = 0;
-#pragma unroll
-  for (uint32_t g = 0; g < kHistSlice; g++)
-    if (g0 + g < groups)
-      s += part[size_t(g0 + g) * words + w];
-  if (s)
-    atomicAdd(&hist[w], s);
 }
 
 // ---- prefix lengths of sorted codes ----------------------------------------------
@@ -1378,7 +1361,7 @@ launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx, floa
 
 hipError_t
 launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, uint64_t n,
-              const float* bounds, void* codes, uint32_t* part_hist, uint32_t* hist,
+              const float* bounds, void* codes, uint32_t* hist,
               int digit_bits, int first_pass, int passes, uint32_t* counts, uint32_t groups,
               uint64_t chunk, hipStream_t s)
 {
@@ -1394,21 +1377,15 @@ launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, 
       return hipErrorInvalidValue;
     g = groups;
   }
+  uint32_t* h = hist + size_t(first_pass) * (size_t(1) << digit_bits);
   if (code_bits == 30)
     hipLaunchKernelGGL(k_encode<uint32_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
-                       bounds, static_cast<uint32_t*>(codes), part_hist, digit_bits, first_pass,
+                       bounds, static_cast<uint32_t*>(codes), h, digit_bits, first_pass,
                        passes, counts, chunk);
   else
     hipLaunchKernelGGL(k_encode<uint64_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
-                       bounds, static_cast<uint64_t*>(codes), part_hist, digit_bits, first_pass,
+                       bounds, static_cast<uint64_t*>(codes), h, digit_bits, first_pass,
                        passes, counts, chunk);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess)
-    return e;
-  hipLaunchKernelGGL(k_hist_reduce,
-                     dim3((words + kBlock - 1) / kBlock, (g + kHistSlice - 1) / kHistSlice),
-                     dim3(kBlock), 0, s, part_hist, g, words,
-                     hist + size_t(first_pass) * (size_t(1) << digit_bits));
   return hipGetLastError();
 }
 

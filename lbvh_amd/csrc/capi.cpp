@@ -395,9 +395,7 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   // chunks itself (one encode workgroup per chunk)
   const bool chunked = cfg.strategy == 1 || cfg.strategy == 2;
   const uint32_t rts_g = chunked ? rts_groups(n, cfg, nullptr) : 0u;
-  const uint64_t hist_rows = std::max<uint64_t>(encode_groups(n), rts_g);
-  if ((st = ensure(c, c->part_hist, 4ull * hist_rows * uint64_t(passes) * radix)) != LBVH_OK)
-    return st;
+
   const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
   if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
     return st;
@@ -464,7 +462,7 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
                                         c->stream);
   if (e == hipSuccess)
     e = launch_encode(opts.code_bits, cent, cent + n, cent + 2 * n, n, b->d_bounds,
-                      c->keys_a.p, static_cast<uint32_t*>(c->part_hist.p), ws.hist,
+                      c->keys_a.p, ws.hist,
                       cfg.digit_bits, cfg.strategy == 2 ? passes - cfg.global_digits : 0, passes,
                       chunked ? ws.counts : nullptr, rts_g,
                       chunked ? rts_chunk_keys(n, cfg) : 0, c->stream);
@@ -597,7 +595,7 @@ lbvh_ctx_destroy(lbvh_ctx* c)
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = { &c->cent, &c->keys_a, &c->keys_b, &c->ids_a, &c->ids_b, &c->partials,
-                     &c->part_hist, &c->small, &c->status, &c->counts, &c->treelets,
+                     &c->small, &c->status, &c->counts, &c->treelets,
                      &c->io_in, &c->io_out, &c->tree, &c->tree_tickets, &c->treelet_words };
   for (auto* b : bufs)
     dev_free(b->p);
@@ -712,7 +710,7 @@ lbvh_ctx_workspace_bytes(const lbvh_ctx* c)
   if (!c)
     return 0;
   return c->cent.cap + c->keys_a.cap + c->keys_b.cap + c->ids_a.cap + c->ids_b.cap +
-         c->partials.cap + c->part_hist.cap + c->small.cap + c->status.cap + c->counts.cap +
+         c->partials.cap + c->small.cap + c->status.cap + c->counts.cap +
          c->treelets.cap + c->io_in.cap + c->io_out.cap + c->tree.cap + c->tree_tickets.cap +
          c->treelet_words.cap;
 }

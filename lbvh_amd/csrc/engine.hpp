@@ -52,16 +52,14 @@ hipError_t launch_centroid_bounds(const float* prims, int kind, uint64_t n, floa
 uint32_t centroid_partials_needed(uint64_t n);
 
 // Morton codes (u32 for 30-bit, u64 for 63-bit) and the digit histograms of
-// passes first_pass .. passes - 1 (per-workgroup counts in
-// part_hist[encode_groups(n)][(passes - first_pass) << digit_bits], summed
-// into hist[p << digit_bits ...] for pass p).
-// counts != nullptr: `groups` workgroups encode chunks of `chunk` keys and
-// also write the chunk counts of pass first_pass (counts[d * groups + g]),
-// the reduce-then-scan layout; part_hist then holds `groups` rows.
+// passes first_pass .. passes - 1, added into hist[p << digit_bits ...] (zero
+// before).  counts != nullptr: `groups` workgroups encode chunks of `chunk`
+// keys and also write the chunk counts of pass first_pass (counts[d * groups
+// + g]), the reduce-then-scan layout.
 hipError_t launch_encode(int code_bits, const float* cx, const float* cy, const float* cz,
-                         uint64_t n, const float* bounds, void* codes, uint32_t* part_hist,
-                         uint32_t* hist, int digit_bits, int first_pass, int passes,
-                         uint32_t* counts, uint32_t groups, uint64_t chunk, hipStream_t s);
+                         uint64_t n, const float* bounds, void* codes, uint32_t* hist,
+                         int digit_bits, int first_pass, int passes, uint32_t* counts,
+                         uint32_t groups, uint64_t chunk, hipStream_t s);
 uint32_t encode_groups(uint64_t n);
 
 // LSD onesweep radix sort of (key, id) pairs in place across ping-pong

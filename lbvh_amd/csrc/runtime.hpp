@@ -66,7 +66,7 @@ struct lbvh_ctx
   std::map<uint64_t, lbvh_eng::SortConfig> tuned; // by tune_key()
   std::vector<lbvh_rt::TuneRecord> tune_log;
   // workspace
-  lbvh_rt::DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, part_hist, small, status, counts,
+  lbvh_rt::DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, small, status, counts,
     treelets, io_in, io_out, tree, tree_tickets;
   // the treelet list in `treelets` belongs to the tree with this id (0: none);
   // its count and the fit's pull queue live in treelet_words

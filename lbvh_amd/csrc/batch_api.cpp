@@ -251,8 +251,8 @@ queue_refit(lbvh_ctx* c, lbvh_batch* b)
                              b->d_treelets, small + kSmallTreeletCount, s);
   if (e == hipSuccess)
     e = launch_fit(b->d_prims, 0, b->n, b->d_off, b->nseg, b->d_nodes, b->d_parent,
-                   b->d_leafprim, b->d_flags, b->d_treelets, small + kSmallTreeletCount,
-                   small + kSmallTreeletQueue, b->d_wide, s);
+                   b->d_other_end, b->d_leafprim, b->d_flags, b->d_treelets,
+                   small + kSmallTreeletCount, small + kSmallTreeletQueue, b->d_wide, s);
   return e;
 }
 

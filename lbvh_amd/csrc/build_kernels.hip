@@ -321,6 +321,46 @@ key_code(KeyT k)
     return CodeT(k);
 }
 
+// Union of element boxes [a, b] left to right, from a bottom-up segment
+// tree over m elements (leaves at bt[m + e]; bt[p] = union(bt[2p],
+// bt[2p + 1])).  std::min / std::max with the left operand kept on ties
+// are associative on non-NaN values, so any order-preserving grouping gives
+// the bits of the reference's union of its two children, recursively.
+__device__ __forceinline__ Box
+bt_query(const Box* __restrict__ bt, int m, int a, int b)
+{
+  int l = a + m, r = b + m + 1;
+  Box rl, rr;
+  bool hl = false, hr = false;
+  while (l < r) {
+    if (l & 1) {
+      rl = hl ? box_union(rl, bt[l]) : bt[l];
+      hl = true;
+      l++;
+    }
+    if (r & 1) {
+      r--;
+      rr = hr ? box_union(bt[r], rr) : bt[r];
+      hr = true;
+    }
+    l >>= 1;
+    r >>= 1;
+  }
+  if (hl && hr)
+    return box_union(rl, rr);
+  return hl ? rl : rr;
+}
+
+__device__ __forceinline__ bool
+box_has_nan(const Box& b)
+{
+  bool x = false;
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+    x = x || (b.lo[q] != b.lo[q]) || (b.hi[q] != b.hi[q]);
+  return x;
+}
+
 // ---- bottom-up fit -------------------------------------------------------------
 // builder::fit_boxes (lbvh.h:1827-1866) computes every internal box as
 // union(left box, right box), left operand first, where a leaf's box is the
@@ -422,17 +462,28 @@ treelet_scene(const Scenes& sc, uint2 e)
   return TreeletScene{ glo - v.P, ghi - v.P, v.n, v.P, v.nb, v.lb };
 }
 
+// One workgroup per treelet (pulled from the list): the leaf boxes go to LDS
+// as the leaves of a segment tree, and every internal node of the treelet --
+// node indices [lo, hi - 1] when its root is a right child (numbered lo),
+// [lo + 1, hi] when a left child (numbered hi) -- takes its children's boxes
+// as the unions over their leaf ranges [L, g] and [g + 1, R] (std::min /
+// std::max keeping the left operand are associative on non-NaN values, so
+// the bits equal the reference's recursion).  A treelet whose leaf boxes hold
+// a NaN is fitted in tree order instead, in passes, each node once both
+// children are done.  The node's range comes from other_end; its split g
+// from its children (node g or leaf g on the left, node g + 1 or leaf g + 1
+// on the right).
 __global__ __launch_bounds__(kTreelet) void
 k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __restrict__ nodes_all,
-               const uint32_t* __restrict__ parent_all, const uint32_t* __restrict__ leafprim_all,
-               const uint2* __restrict__ treelets, const uint32_t* __restrict__ count_ptr,
-               uint32_t* __restrict__ queue, WideNode* __restrict__ wide_all)
+               const uint32_t* __restrict__ other_end_all,
+               const uint32_t* __restrict__ leafprim_all, const uint2* __restrict__ treelets,
+               const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ queue,
+               WideNode* __restrict__ wide_all)
 {
-  __shared__ uint32_t s_flag[kTreelet];
-  __shared__ uint32_t s_link[kTreelet]; // link of internal node lo + t
-  __shared__ Box s_leaf[kTreelet];      // box of the leaf at position lo + t
-  __shared__ Box s_box[kTreelet];       // box of internal node lo + t
-  __shared__ uint32_t s_leafv[kTreelet]; // child value of the leaf at position lo + t
+  // leaf boxes at s_bt[m + e]; below m the segment tree, or (NaN) node
+  // base + j's box at s_bt[1 + j]
+  __shared__ Box s_bt[2 * kTreelet];
+  __shared__ uint8_t s_done[kTreelet];
   __shared__ uint32_t s_item;
   const int t = threadIdx.x;
   const uint32_t count = *count_ptr;
@@ -443,60 +494,85 @@ k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* _
     const uint32_t item = s_item;
     if (item >= count)
       return; // uniform across the workgroup
-    const TreeletScene ts = treelet_scene(sc, treelets[item]);
+    const uint2 en = treelets[item];
+    const TreeletScene ts = treelet_scene(sc, en);
     const int64_t lo = ts.lo, hi = ts.hi, n = ts.n;
-    const int64_t m = hi - lo + 1;
+    const int m = int(hi - lo + 1);
     if (lo >= 0 && m >= 2 && m <= kTreelet && hi < n) {
       const float* prims = prims_all + prim_stride(kind) * ts.P;
       Node* nodes = nodes_all + ts.nb;
-      const uint32_t* parent = parent_all + ts.lb;
+      const uint32_t* other_end = other_end_all + ts.nb;
       const uint32_t* leafprim = leafprim_all ? leafprim_all + ts.P : nullptr;
       WideNode* wide = wide_all + ts.nb;
-      const int64_t k = lo + t;
-      uint32_t link = 0u;
-      if (t < m) {
-        s_leaf[t] = leaf_box(prims, kind, leafprim, k);
-        s_leafv[t] = child_value(k, true, leafprim);
-        link = parent[(n - 1) + k];
-        if (k < n - 1)
-          s_link[t] = parent[k];
+      // the root is numbered lo when it is a right child (and for a
+      // whole-tree treelet, whose root is node 0)
+      const bool root_lo = (en.y & 0x80000000u) || (lo == 0 && hi == n - 1);
+      const int64_t base = root_lo ? lo : lo + 1;
+      bool bad = false;
+      if (t < m)
+        bad = box_has_nan(s_bt[m + t] = leaf_box(prims, kind, leafprim, lo + t));
+      const bool nan = __syncthreads_or(bad ? 1 : 0) != 0;
+      // this thread's internal node k = base + t (t < m - 1): range, children, split
+      const int64_t k = base + t;
+      bool mine = false;
+      int64_t L = 0, R = 0, g = 0;
+      uint32_t lv = 0, rv = 0;
+      if (t < m - 1 && k < n - 1) {
+        const int64_t oe = other_end[k];
+        L = k < oe ? k : oe;
+        R = k < oe ? oe : k;
+        const uint2 ch = *reinterpret_cast<const uint2*>(&nodes[k].left);
+        lv = ch.x;
+        rv = ch.y;
+        g = !(lv & kLeafBit) ? int64_t(lv) : (!(rv & kLeafBit) ? int64_t(rv) - 1 : L);
+        // ranges and children come from the tree kernel; never false for a valid tree
+        mine = L >= lo && R <= hi && L <= g && g < R;
+        s_done[t] = 0;
       }
-      s_flag[t] = 0u;
       __syncthreads();
-      if (t < m) {
-        Box mine = s_leaf[t];
-        int64_t cur = k;
-        uint32_t cur_v = s_leafv[t]; // child value of cur
+      if (!nan) {
+        // the segment tree over the m leaf boxes
+        int leaf0 = 1;
+        while (leaf0 < m)
+          leaf0 <<= 1;
+        for (int half = leaf0 / 2; half >= 1; half >>= 1) {
+          for (int q = t; q < half; q += kTreelet) {
+            const int p = half + q;
+            if (p < m)
+              s_bt[p] = box_union(s_bt[2 * p], s_bt[2 * p + 1]);
+          }
+          __syncthreads();
+        }
+        if (mine) {
+          const Box lb = bt_query(s_bt, m, int(L - lo), int(g - lo));
+          const Box rb = bt_query(s_bt, m, int(g + 1 - lo), int(R - lo));
+          store_node(&nodes[k], box_union(lb, rb), lv, rv);
+          store_wide(&wide[k], lb, rb, lv, rv);
+        }
+      } else {
+        // children before parents: node box of base + j at s_bt[1 + j]
         for (;;) {
-          if (link & kLinkGlobal)
-            break; // cur is the treelet root
-          const int64_t p = int64_t(link & kLinkIndex);
-          const bool right = (link & kLinkRight) != 0;
-          const int64_t sib = right ? cur - 1 : cur + 1;
-          // links come from the tree kernel; never taken for a valid tree
-          if (p < lo || p > hi || p >= n - 1 || sib < lo || sib > hi)
+          int changed = 0;
+          if (mine && !s_done[t]) {
+            const bool l_leaf = (lv & kLeafBit) != 0, r_leaf = (rv & kLeafBit) != 0;
+            const int lj = int(g - base), rj = int(g + 1 - base);
+            const bool l_ok = l_leaf || (lj >= 0 && lj < m - 1 && s_done[lj]);
+            const bool r_ok = r_leaf || (rj >= 0 && rj < m - 1 && s_done[rj]);
+            if (l_ok && r_ok) {
+              const Box lb = l_leaf ? s_bt[m + int(g - lo)] : s_bt[1 + lj];
+              const Box rb = r_leaf ? s_bt[m + int(g + 1 - lo)] : s_bt[1 + rj];
+              const Box b = box_union(lb, rb);
+              s_bt[1 + t] = b;
+              store_node(&nodes[k], b, lv, rv);
+              store_wide(&wide[k], lb, rb, lv, rv);
+              changed = 1;
+            }
+          }
+          __syncthreads();
+          if (changed)
+            s_done[t] = 1;
+          if (!__syncthreads_or(changed))
             break;
-          const uint32_t old = __hip_atomic_fetch_add(&s_flag[p - lo], 1u, __ATOMIC_ACQ_REL,
-                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-          if (old == 0u)
-            break; // first arrival: the sibling's thread completes p
-          const bool sib_leaf = (link & kLinkSibLeaf) != 0;
-          const Box ob = sib_leaf ? s_leaf[sib - lo] : s_box[sib - lo];
-          const uint32_t sib_v = sib_leaf ? s_leafv[sib - lo] : uint32_t(sib);
-          const Box& lb = right ? ob : mine;
-          const Box& rb = right ? mine : ob;
-          const uint32_t lv = right ? sib_v : cur_v;
-          const uint32_t rv = right ? cur_v : sib_v;
-          const Box b = box_union(lb, rb);
-          store_node(&nodes[p], b, lv, rv);
-          store_wide(&wide[p], lb, rb, lv, rv);
-          s_box[p - lo] = b;
-          if (p == 0)
-            break; // the whole tree was one treelet
-          mine = b;
-          cur = p;
-          cur_v = uint32_t(p);
-          link = s_link[p - lo];
         }
       }
     }
@@ -778,46 +854,6 @@ tree_right_less(const int16_t* __restrict__ tree, int leaf0, int j, int x)
     p >>= 1;
   }
   return -1;
-}
-
-// Union of element boxes [a, b] left to right, from a bottom-up segment
-// tree over m elements (leaves at bt[m + e]; bt[p] = union(bt[2p],
-// bt[2p + 1])).  std::min / std::max with the left operand kept on ties
-// are associative on non-NaN values, so any order-preserving grouping gives
-// the bits of the reference's union of its two children, recursively.
-__device__ __forceinline__ Box
-bt_query(const Box* __restrict__ bt, int m, int a, int b)
-{
-  int l = a + m, r = b + m + 1;
-  Box rl, rr;
-  bool hl = false, hr = false;
-  while (l < r) {
-    if (l & 1) {
-      rl = hl ? box_union(rl, bt[l]) : bt[l];
-      hl = true;
-      l++;
-    }
-    if (r & 1) {
-      r--;
-      rr = hr ? box_union(bt[r], rr) : bt[r];
-      hr = true;
-    }
-    l >>= 1;
-    r >>= 1;
-  }
-  if (hl && hr)
-    return box_union(rl, rr);
-  return hl ? rl : rr;
-}
-
-__device__ __forceinline__ bool
-box_has_nan(const Box& b)
-{
-  bool x = false;
-#pragma unroll
-  for (int q = 0; q < 3; q++)
-    x = x || (b.lo[q] != b.lo[q]) || (b.hi[q] != b.hi[q]);
-  return x;
 }
 
 // FIRST: the level-0 kernel (leaves, 512 elements, ~40 KB of LDS: four workgroups fit a CU);
@@ -1401,17 +1437,17 @@ make_scenes(const uint64_t* prim_off, uint32_t nseg, uint64_t n)
 
 hipError_t
 launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off, uint32_t nseg,
-           void* nodes, const uint32_t* parent, const uint32_t* leafprim, uint32_t* flags,
-           const void* treelets, uint32_t* treelet_count, uint32_t* queue, void* wide,
-           hipStream_t s)
+           void* nodes, const uint32_t* parent, const uint32_t* other_end,
+           const uint32_t* leafprim, uint32_t* flags, const void* treelets,
+           uint32_t* treelet_count, uint32_t* queue, void* wide, hipStream_t s)
 {
   const Scenes sc = make_scenes(prim_off, nseg, n);
   const uint2* tl = static_cast<const uint2*>(treelets);
   // typical trees hold about 2 n / kTreelet treelets (plus one per small
   // scene); the loops are persistent, so any count is covered
-  const uint32_t ga = grid_for(2 * n + uint64_t(sc.nseg) * kTreelet, kTreelet, 512);
+  const uint32_t ga = grid_for(2 * n + uint64_t(sc.nseg) * kTreelet, kTreelet, 768);
   hipLaunchKernelGGL(k_fit_treelets, dim3(ga), dim3(kTreelet), 0, s, prims, kind, sc,
-                     static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, queue,
+                     static_cast<Node*>(nodes), other_end, leafprim, tl, treelet_count, queue,
                      static_cast<WideNode*>(wide));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)

@@ -817,8 +817,9 @@ queue_refit(lbvh_ctx* c, lbvh_bvh* b)
                                     tw, c->stream));
     c->treelets_of = b->id;
   }
-  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, nullptr, 1, b->d_nodes, b->d_parent, b->d_leafprim,
-                        b->d_flags, c->treelets.p, tw, tw + 1, b->d_wide, c->stream));
+  HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, nullptr, 1, b->d_nodes, b->d_parent, b->d_other_end,
+                        b->d_leafprim, b->d_flags, c->treelets.p, tw, tw + 1, b->d_wide,
+                        c->stream));
   record(c, 1, true);
   return LBVH_OK;
 }

@@ -96,8 +96,9 @@ uint64_t rts_chunk_keys(uint64_t n, const SortConfig& cfg);
 // leafprim == nullptr: the leaf at position k is primitive k of its scene.
 hipError_t launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off,
                       uint32_t nseg, void* nodes, const uint32_t* parent,
-                      const uint32_t* leafprim, uint32_t* flags, const void* treelets,
-                      uint32_t* treelet_count, uint32_t* queue, void* wide, hipStream_t s);
+                      const uint32_t* other_end, const uint32_t* leafprim, uint32_t* flags,
+                      const void* treelets, uint32_t* treelet_count, uint32_t* queue, void* wide,
+                      hipStream_t s);
 
 // Rebuilds the treelet list of a built tree (for a refit); *treelet_count
 // must be zero.

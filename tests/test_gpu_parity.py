@@ -295,6 +295,23 @@ def test_refits_alternating_between_trees(ctx):
     b.destroy()
 
 
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+def test_refit_with_nan_vertices_equals_oracle_fit(ctx, leaf):
+    """A refit whose new vertices hold NaNs (the topology is kept, so no
+    code is computed from them): treelets with a NaN box are fitted in tree
+    order, and every box equals the oracle's fit bit for bit."""
+    v, tri = W.scene_uniform(8, 120000, 5e-4)
+    bvh = ctx.build_triangles(tri, code_bits=30, leaf_order=leaf)
+    before = bvh.nodes()
+    v2 = W.perturb(9, v, 1e-4)
+    rng = np.random.default_rng(10)
+    bad = rng.choice(v2.shape[0], 40, replace=False)
+    v2[bad, rng.integers(0, 9, 40)] = np.float32(np.nan)
+    tri2 = W.make_triangles(v2)
+    ctx.refit_triangles(bvh, tri2)
+    assert bvh.nodes().tobytes() == oracle.fit(before, tri2).tobytes()
+
+
 def test_device_path_equals_host_path(ctx):
     from lbvh_amd import hip
     v, tri = W.scene_uniform(1, 65536, 0.01)

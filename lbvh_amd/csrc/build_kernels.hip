@@ -364,17 +364,16 @@ box_has_nan(const Box& b)
 // ---- bottom-up fit -------------------------------------------------------------
 // builder::fit_boxes (lbvh.h:1827-1866) computes every internal box as
 // union(left box, right box), left operand first, where a leaf's box is the
-// converter applied to the primitive its index names.  Here the second of
-// the two child computations to finish computes the parent's box.  In the
-// Karras layout the children of a node with split g are g and g + 1 (leaf
-// positions or node indices), so the parent link's side and sibling-is-leaf
-// bits locate the sibling without reading the node.
+// converter applied to the primitive its index names.  In the Karras
+// layout the children of a node with split g are g and g + 1 (leaf
+// positions or node indices), so in the top tree the parent link's side and
+// sibling-is-leaf bits locate the sibling without reading the node, and the
+// second of the two child computations to finish computes the parent's box.
 //
 // Pass 1 (k_fit_treelets): each treelet -- a subtree of at most kTreelet
-// leaves under a top node -- is fitted by one workgroup entirely in LDS: the
-// leaves' boxes and the internal nodes' links are staged once, then one
-// thread per leaf climbs with LDS visit flags.  Workgroups pull treelets
-// from the list the tree kernel wrote.
+// leaves under a top node -- is fitted by one workgroup in LDS from a
+// segment tree over its leaf boxes.  Workgroups pull treelets from the list
+// k_enum_treelets wrote.
 // Pass 2 (k_fit_top): one thread per treelet climbs from the treelet root
 // through the top nodes with device-scope visit flags and device-coherent
 // box stores/loads (the top tree is shallow).  Flags are cleared by their

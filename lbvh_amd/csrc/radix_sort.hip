@@ -1205,7 +1205,13 @@ rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles)
 {
   const uint64_t tile = uint64_t(kSortBlock) * cfg.items;
   const uint64_t tiles = (n + tile - 1) / tile;
-  uint64_t target = cfg.groups > 0 ? uint64_t(cfg.groups) : 512;
+  // default: chunks of about two tiles, 512 to 2,048 of them (more chunks,
+  // more workgroups in flight for the memory-bound downsweep; fewer, a
+  // smaller count matrix)
+  uint64_t target = tiles / 2;
+  target = target < 512 ? 512 : (target > 2048 ? 2048 : target);
+  if (cfg.groups > 0)
+    target = uint64_t(cfg.groups);
   uint64_t ct = (tiles + target - 1) / target;
   if (ct < 1)
     ct = 1;

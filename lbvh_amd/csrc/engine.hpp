@@ -16,7 +16,7 @@ struct SortConfig
   int strategy = 0;   // 0: onesweep (decoupled look-back), 1: reduce-then-scan,
                       // 2: hybrid (top digits by reduce-then-scan, then every
                       //    run of equal top bits sorted by the full key in LDS)
-  int groups = 0;     // reduce-then-scan: target number of chunks (0 = 512)
+  int groups = 0;     // reduce-then-scan: target number of chunks (0 = about two tiles each, 512-2,048)
   int global_digits = 0; // hybrid: digits sorted by global passes (the top ones)
 };
 

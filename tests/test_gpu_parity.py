@@ -295,6 +295,20 @@ def test_refits_alternating_between_trees(ctx):
     b.destroy()
 
 
+@pytest.mark.parametrize("n", [2, 3, 5, 1023, 1024, 1025, 2049, 5000])
+@pytest.mark.parametrize("bits,leaf", [(30, 0), (63, 1)], ids=["b30_ref", "b63_pid"])
+def test_refit_small_and_treelet_edge_sizes(ctx, n, bits, leaf):
+    """Refits of trees that are one treelet (n <= 1,024: the root is node 0)
+    or a few treelets under a short top tree equal the oracle's fit."""
+    v = W.cluster_vertices(3000 + n, n)
+    bvh = ctx.build_triangles(W.make_triangles(v), code_bits=bits, leaf_order=leaf)
+    before = bvh.nodes()
+    moved = W.make_triangles(W.perturb(11, v, 1e-4))
+    ctx.refit_triangles(bvh, moved)
+    assert bvh.nodes().tobytes() == oracle.fit(before, moved).tobytes()
+    bvh.destroy()
+
+
 @pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
 def test_refit_with_nan_vertices_equals_oracle_fit(ctx, leaf):
     """A refit whose new vertices hold NaNs (the topology is kept, so no

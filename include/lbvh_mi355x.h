@@ -192,6 +192,28 @@ lbvh_status lbvh_bvh_download_codes(lbvh_ctx* ctx, const lbvh_bvh* bvh,
                                     float* bounds6);
 void lbvh_bvh_destroy(lbvh_bvh* bvh);
 
+/* -- builder<double> --------------------------------------------------------
+ * Replaces lbvh::builder<double>::operator()(primitives, count, converter)
+ * (lbvh.h:342-345, 549-558, 1793-1822 with scalar_type = double) for any
+ * primitive whose converter the caller evaluated: boxes48[i] =
+ * converter(primitives[i]) as lbvh::aabb<double> {min xyz, max xyz}
+ * (lbvh.h:205-212).  Codes are u64 (morton_domain<8>, lbvh.h:1322-1326) and
+ * all box arithmetic is FP64.  Output: n - 1 lbvh::node<double> records of
+ * 64 bytes {aabb<double> box; u64 left; u64 right}, bit 63 of left/right
+ * marks a leaf, root = node 0 (lbvh.h:220-255).  opts->leaf_order applies;
+ * code_bits and the sort fields are ignored (the code width follows the
+ * scalar type, as in the reference).  n in [2, 2^31 - 1].  The device
+ * variant borrows d_boxes48 (16-byte aligned) and writes d_nodes_out, both
+ * device memory, and returns after queuing; the host variant copies in and
+ * out and returns when the nodes are in nodes_out. */
+lbvh_status lbvh_build_aabbs_f64(lbvh_ctx* ctx, const double* boxes48,
+                                 uint64_t n, const lbvh_build_opts* opts,
+                                 void* nodes_out);
+lbvh_status lbvh_build_aabbs_f64_device(lbvh_ctx* ctx,
+                                        const double* d_boxes48, uint64_t n,
+                                        const lbvh_build_opts* opts,
+                                        void* d_nodes_out);
+
 /* -- traversal -------------------------------------------------------------
  * Replaces lbvh::traverser<float, triangle<float>,
  * triangle_intersection<float>>::operator()(ray, triangle_intersector)

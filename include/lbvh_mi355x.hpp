@@ -20,8 +20,11 @@
 //   lbvh::gpu::device_tree: a device-resident tree for batched closest-hit
 //     of triangle scenes (lbvh_trace_closest), refit and node download.
 //
-// Only scalar_type = float is served by the engine; builder<double> does not
-// compile (the engine builds FP32 trees).
+//   lbvh::builder<double, Sched>::operator()(prims, count, converter): the
+//     converter is evaluated on the host through the caller's scheduler and
+//     the FP64 boxes are built on the device (lbvh_build_aabbs_f64: u64
+//     codes, FP64 arithmetic, 64-byte node<double>), byte-identical to the
+//     reference's builder<double>.
 #pragma once
 
 #include "lbvh_mi355x.h"
@@ -157,6 +160,7 @@ struct alignas(sizeof(T) * 8) node final
 };
 
 static_assert(sizeof(node<float>) == 32, "node<float> must match the engine's node layout");
+static_assert(sizeof(node<double>) == 64, "node<double> must match the engine's FP64 node layout");
 
 template<typename T>
 class bvh final
@@ -392,9 +396,28 @@ class triangle_aabb_converter;
 template<typename scalar_type, typename task_scheduler = default_scheduler>
 class builder final
 {
-  static_assert(std::is_same<scalar_type, float>::value,
-                "the MI355X engine builds FP32 trees: use lbvh::builder<float>");
+  static_assert(std::is_same<scalar_type, float>::value ||
+                  std::is_same<scalar_type, double>::value,
+                "lbvh::builder serves float and double");
   task_scheduler scheduler;
+
+  // boxes[i] = converter(primitives[i]), split over the caller's scheduler
+  // as the reference divides its loops (lbvh.h:1282-1293).
+  template<typename primitive, typename aabb_converter>
+  std::vector<aabb<scalar_type>> convert(const primitive* primitives, size_type count,
+                                         const aabb_converter& converter)
+  {
+    std::vector<aabb<scalar_type>> boxes(count);
+    auto task = [&](const work_division& div) {
+      size_type chunk = count / div.max;
+      size_type b = chunk * div.idx;
+      size_type e = (div.idx + 1 == div.max) ? count : b + chunk;
+      for (size_type i = b; i < e; i++)
+        boxes[i] = converter(primitives[i]);
+    };
+    scheduler(task);
+    return boxes;
+  }
 
 public:
   using bvh_type = bvh<scalar_type>;
@@ -415,31 +438,34 @@ public:
     lbvh_build_opts_init(&o);
     o.code_bits = S.opts.code_bits;
     o.leaf_order = S.opts.leaf_order;
-    lbvh_bvh* h = nullptr;
     if (count < 2)
       throw std::invalid_argument("lbvh::builder: a tree needs at least 2 primitives");
-    if (std::is_same<primitive, triangle<float>>::value &&
-        std::is_same<aabb_converter, triangle_aabb_converter<float>>::value) {
-      S.check(lbvh_build_triangles(c, reinterpret_cast<const float*>(primitives), count, &o, &h),
-              "lbvh_build_triangles");
+    if constexpr (std::is_same<scalar_type, double>::value) {
+      // code width follows the scalar type: u64 codes, as the reference
+      auto boxes = convert(primitives, count, converter);
+      node_vec nodes(count - 1);
+      S.check(lbvh_build_aabbs_f64(c, reinterpret_cast<const double*>(boxes.data()), count, &o,
+                                   nodes.data()),
+              "lbvh_build_aabbs_f64");
+      return bvh_type(std::move(nodes));
     } else {
-      std::vector<aabb<float>> boxes(count);
-      auto task = [&](const work_division& div) {
-        size_type chunk = count / div.max;
-        size_type b = chunk * div.idx;
-        size_type e = (div.idx + 1 == div.max) ? count : b + chunk;
-        for (size_type i = b; i < e; i++)
-          boxes[i] = converter(primitives[i]);
-      };
-      scheduler(task);
-      S.check(lbvh_build_aabbs(c, reinterpret_cast<const float*>(boxes.data()), count, &o, &h),
-              "lbvh_build_aabbs");
+      lbvh_bvh* h = nullptr;
+      if (std::is_same<primitive, triangle<float>>::value &&
+          std::is_same<aabb_converter, triangle_aabb_converter<float>>::value) {
+        S.check(lbvh_build_triangles(c, reinterpret_cast<const float*>(primitives), count, &o, &h),
+                "lbvh_build_triangles");
+      } else {
+        auto boxes = convert(primitives, count, converter);
+        S.check(lbvh_build_aabbs(c, reinterpret_cast<const float*>(boxes.data()), count, &o, &h),
+                "lbvh_build_aabbs");
+      }
+      node_vec nodes(count - 1);
+      lbvh_status st =
+        lbvh_bvh_download_nodes(c, h, nodes.data(), nodes.size() * sizeof(node_type));
+      lbvh_bvh_destroy(h);
+      S.check(st, "lbvh_bvh_download_nodes");
+      return bvh_type(std::move(nodes));
     }
-    node_vec nodes(count - 1);
-    lbvh_status st = lbvh_bvh_download_nodes(c, h, nodes.data(), nodes.size() * sizeof(node_type));
-    lbvh_bvh_destroy(h);
-    S.check(st, "lbvh_bvh_download_nodes");
-    return bvh_type(std::move(nodes));
   }
 };
 

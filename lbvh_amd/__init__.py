@@ -12,6 +12,8 @@ Host layouts (the reference's own, lbvh.h):
   boxes     float32[n, 6]   lbvh::aabb<float>
   rays      float32[m, 8]   lbvh::ray<float> {pos, dir, tmin, tmax}
   nodes     NODE_DTYPE[n-1] lbvh::node<float> {box[6], left, right}
+  boxes64   float64[n, 6]   lbvh::aabb<double> (builder<double> input)
+  nodes64   NODE64_DTYPE[n-1] lbvh::node<double> {box[6], u64 left, u64 right}
 """
 import ctypes
 import os
@@ -29,6 +31,9 @@ LEAF_BIT = 0x80000000
 MISS = 0xFFFFFFFF
 
 NODE_DTYPE = np.dtype([("box", np.float32, (6,)), ("left", np.uint32), ("right", np.uint32)])
+# lbvh::node<double> (builder<double>): leaf flag in bit 63
+NODE64_DTYPE = np.dtype([("box", np.float64, (6,)), ("left", np.uint64), ("right", np.uint64)])
+LEAF_BIT64 = 1 << 63
 
 STATUS = {0: "LBVH_OK", 1: "LBVH_E_INVALID", 2: "LBVH_E_OOM", 3: "LBVH_E_HIP", 4: "LBVH_E_BUSY",
           5: "LBVH_E_NODEVICE", 6: "LBVH_E_TIMEOUT"}
@@ -130,6 +135,8 @@ SIGNATURES = {
     "lbvh_build_triangles_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts),
                                                    ctypes.POINTER(VP)]),
     "lbvh_build_aabbs": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), ctypes.POINTER(VP)]),
+    "lbvh_build_aabbs_f64": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), VP]),
+    "lbvh_build_aabbs_f64_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), VP]),
     "lbvh_build_aabbs_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts),
                                                ctypes.POINTER(VP)]),
     "lbvh_refit_triangles": (ctypes.c_int, [VP, VP, VP]),
@@ -614,6 +621,35 @@ class Context:
         self._check(getattr(lib(), fn)(self.handle, _hptr(a), n, ctypes.byref(opts),
                                        ctypes.byref(out)))
         return Bvh(self, out, n, code_bits, leaf_order)
+
+    def build_aabbs_f64(self, boxes, leaf_order=LEAF_REFERENCE, out=None):
+        """lbvh::builder<double> over converter boxes (float64 [n, 6]).
+        Host input: returns NODE64_DTYPE[n - 1].  Device input (a
+        DeviceArray or a torch float64 tensor): writes the node records into
+        `out` (device memory of at least 64 (n - 1) bytes; allocated like
+        the input when None) and returns it without synchronizing."""
+        opts = _make_opts(63, leaf_order, False)
+        if _is_device(boxes):
+            n = _numel(boxes) // 6
+            if out is None:
+                if _is_torch(boxes):
+                    import torch
+                    out = torch.empty(max(n - 1, 1) * 8, dtype=torch.int64, device=boxes.device)
+                else:
+                    from .hip import DeviceArray
+                    out = DeviceArray((max(n - 1, 1),), NODE64_DTYPE)
+            nbytes = out.numel() * out.element_size() if _is_torch(out) else out.nbytes
+            if nbytes < 64 * max(n - 1, 0):
+                raise ValueError("out holds fewer than 64 (n - 1) bytes")
+            self._check(lib().lbvh_build_aabbs_f64_device(self.handle, _dptr(boxes), n,
+                                                          ctypes.byref(opts), _dptr(out)))
+            return out
+        a = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        n = a.shape[0]
+        nodes = np.zeros(max(n - 1, 1), dtype=NODE64_DTYPE)
+        self._check(lib().lbvh_build_aabbs_f64(self.handle, _hptr(a), n, ctypes.byref(opts),
+                                               _hptr(nodes)))
+        return nodes[:n - 1]
 
     def build_batch(self, tri, offsets, leaf_order=LEAF_REFERENCE, keep_codes=False,
                     sort_digit_bits=0, sort_tile=0, sort_strategy=SORT_AUTO):

@@ -1,0 +1,168 @@
+// C ABI of lbvh::builder<double> (include/lbvh_mi355x.h): FP64 boxes in,
+// node<double> records out, on the context's stream and workspace.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "engine.hpp"
+#include "lbvh_mi355x.h"
+#include "runtime.hpp"
+
+using namespace lbvh_eng;
+using namespace lbvh_rt;
+
+namespace {
+
+constexpr uint64_t kMaxF64Prims = (uint64_t(1) << 31) - 1;
+
+// Queues the whole FP64 build of n boxes at d_boxes into d_nodes.
+lbvh_status
+queue_f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build_opts* opts_in,
+                void* d_nodes)
+{
+  lbvh_build_opts opts;
+  lbvh_build_opts_init(&opts);
+  if (opts_in)
+    opts = *opts_in;
+  if (opts.leaf_order > 1)
+    return fail(c, LBVH_E_INVALID, "unknown leaf order");
+  if (n < 2)
+    return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
+  if (n > kMaxF64Prims)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 primitives per FP64 tree");
+  if ((reinterpret_cast<uintptr_t>(d_boxes) & 15) != 0 ||
+      (reinterpret_cast<uintptr_t>(d_nodes) & 7) != 0)
+    return fail(c, LBVH_E_INVALID, "boxes must be 16-byte and nodes 8-byte aligned");
+  const int key_bits = 60;
+  const SortConfig cfg = default_sort();
+  if (!valid_sort_config(cfg, key_bits))
+    return fail(c, LBVH_E_INVALID, "unsupported sort configuration");
+  const int passes = sort_passes(key_bits, cfg.digit_bits);
+  const uint32_t radix = 1u << cfg.digit_bits;
+  const bool with_ids = opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID;
+
+  lbvh_status st;
+  if ((st = ensure(c, c->keys_a, 8 * n)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->keys_b, 8 * n)) != LBVH_OK)
+    return st;
+  if (with_ids) {
+    if ((st = ensure(c, c->ids_a, 4 * n)) != LBVH_OK)
+      return st;
+    if ((st = ensure(c, c->ids_b, 4 * n)) != LBVH_OK)
+      return st;
+  }
+  // bounds partials, then the final bounds
+  if ((st = ensure(c, c->partials, 48ull * (f64_bounds_blocks(n) + 1))) != LBVH_OK)
+    return st;
+  const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
+  if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
+    return st;
+  // parent links [2n - 1] and fit flags [n - 1]
+  if ((st = ensure(c, c->tree, 4 * ((2 * n - 1) + (n - 1)))) != LBVH_OK)
+    return st;
+  const uint64_t counts_words = sort_counts_words(n, cfg);
+  if ((st = ensure(c, c->counts, 4 * counts_words)) != LBVH_OK)
+    return st;
+  const uint64_t sw = sort_status_words(n, cfg);
+  if (c->status.cap < sw * 8) {
+    if ((st = ensure(c, c->status, sw * 8, true)) != LBVH_OK)
+      return st;
+  }
+  if (c->epoch_next + kMaxSortPasses >= (1u << 24)) {
+    HIP_TRY(c, launch_fill_u32(static_cast<uint32_t*>(c->status.p), c->status.cap / 4, 0u,
+                               c->stream));
+    c->epoch_next = 1;
+  }
+  const uint32_t epoch = c->epoch_next;
+  c->epoch_next += kMaxSortPasses;
+
+  uint32_t* small = static_cast<uint32_t*>(c->small.p);
+  SortWorkspace ws;
+  ws.status = static_cast<uint64_t*>(c->status.p);
+  ws.status_words = sw;
+  ws.tickets = small + kSmallTickets;
+  ws.hist = small + kSmallHist;
+  ws.err = small + kSmallErr;
+  ws.counts = static_cast<uint32_t*>(c->counts.p);
+  ws.counts_words = c->counts.cap / 4;
+  ws.first_counts = false;
+
+  double* partials = static_cast<double*>(c->partials.p);
+  double* bounds = partials + 6 * size_t(f64_bounds_blocks(n));
+  uint32_t* parent = static_cast<uint32_t*>(c->tree.p);
+  uint32_t* flags = parent + (2 * n - 1);
+
+  begin_timed(c, 1);
+  record(c, 0);
+  hipError_t e = launch_fill_u32(small + kSmallTickets, small_words - kSmallTickets, 0u, c->stream);
+  if (e == hipSuccess)
+    e = launch_f64_bounds(d_boxes, n, partials, bounds, c->stream);
+  if (e == hipSuccess)
+    e = launch_f64_encode(d_boxes, n, bounds, static_cast<uint64_t*>(c->keys_a.p), ws.hist,
+                          cfg.digit_bits, passes, c->stream);
+  record(c, 1);
+  bool in_b = false;
+  if (e == hipSuccess)
+    e = launch_sort(key_bits, 8, n, c->keys_a.p, c->keys_b.p,
+                    with_ids ? static_cast<uint32_t*>(c->ids_a.p) : nullptr,
+                    with_ids ? static_cast<uint32_t*>(c->ids_b.p) : nullptr, cfg, ws, epoch,
+                    &in_b, c->stream);
+  record(c, 2);
+  const uint64_t* sorted = static_cast<const uint64_t*>(in_b ? c->keys_b.p : c->keys_a.p);
+  const uint32_t* ids =
+    with_ids ? static_cast<const uint32_t*>(in_b ? c->ids_b.p : c->ids_a.p) : nullptr;
+  if (e == hipSuccess)
+    e = launch_f64_tree(sorted, ids, d_boxes, n, d_nodes, parent, flags, c->stream);
+  record(c, 3);
+  record(c, 4, true);
+  if (e != hipSuccess)
+    return hip_fail(c, e, "FP64 build launch");
+  return LBVH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+lbvh_status
+lbvh_build_aabbs_f64_device(lbvh_ctx* c, const double* d_boxes, uint64_t n,
+                            const lbvh_build_opts* opts, void* d_nodes)
+{
+  if (!c || !d_boxes || !d_nodes)
+    return fail(c, LBVH_E_INVALID, "null argument");
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  return queue_f64_build(c, d_boxes, n, opts, d_nodes);
+}
+
+lbvh_status
+lbvh_build_aabbs_f64(lbvh_ctx* c, const double* boxes, uint64_t n, const lbvh_build_opts* opts,
+                     void* nodes_out)
+{
+  if (!c || !boxes || !nodes_out)
+    return fail(c, LBVH_E_INVALID, "null argument");
+  if (n < 2)
+    return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
+  if (n > kMaxF64Prims)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 primitives per FP64 tree");
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  const uint64_t in_bytes = 48 * n, out_bytes = 64 * (n - 1);
+  if ((st = ensure(c, c->io_in, in_bytes)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->io_out, out_bytes)) != LBVH_OK)
+    return st;
+  HIP_TRY(c, hipMemcpyAsync(c->io_in.p, boxes, in_bytes, hipMemcpyHostToDevice, c->stream));
+  if ((st = queue_f64_build(c, static_cast<const double*>(c->io_in.p), n, opts, c->io_out.p)) !=
+      LBVH_OK) {
+    (void)hipStreamSynchronize(c->stream);
+    return st;
+  }
+  HIP_TRY(c, hipMemcpyAsync(nodes_out, c->io_out.p, out_bytes, hipMemcpyDeviceToHost, c->stream));
+  return sync_and_check(c);
+}
+
+} // extern "C"

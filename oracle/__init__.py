@@ -30,6 +30,10 @@ ISECT_WATERTIGHT = 2
 
 NODE_DTYPE = np.dtype([("box", np.float32, (6,)), ("left", np.uint32), ("right", np.uint32)])
 assert NODE_DTYPE.itemsize == 32
+# lbvh::node<double>: 64 bytes, leaf flag in bit 63.
+NODE64_DTYPE = np.dtype([("box", np.float64, (6,)), ("left", np.uint64), ("right", np.uint64)])
+assert NODE64_DTYPE.itemsize == 64
+LEAF_BIT64 = np.uint64(1 << 63)
 
 _orc = None
 _ref = {}
@@ -59,6 +63,13 @@ def orc():
         L.orc_brute_closest.argtypes = [F32P, ctypes.c_uint64, F32P, ctypes.c_uint64, ctypes.c_int,
                                         ctypes.c_int, U32P, F32P]
         L.orc_t_fp64.argtypes = [F32P, F32P, U32P, ctypes.c_uint64, F64P]
+        L.orc_triangle_aabb_f64.argtypes = [F64P, ctypes.c_uint64, F64P]
+        L.orc_morton_codes_f64.argtypes = [F64P, ctypes.c_uint64, F64P, U64P]
+        L.orc_karras_f64.argtypes = [U64P, ctypes.c_uint64, VP]
+        L.orc_karras_f64.restype = ctypes.c_int
+        L.orc_fit_f64.argtypes = [VP, ctypes.c_uint64, F64P]
+        L.orc_build_f64.argtypes = [F64P, ctypes.c_uint64, ctypes.c_int, VP, U64P, U32P]
+        L.orc_build_f64.restype = ctypes.c_int
         L.orc_fnv1a64.argtypes = [VP, ctypes.c_uint64]
         L.orc_fnv1a64.restype = ctypes.c_uint64
         _orc = L
@@ -84,6 +95,10 @@ def ref(fast=False):
         L.ref_trace.argtypes = [VP, ctypes.c_uint64, F32P, F32P, ctypes.c_uint64, ctypes.c_int,
                                 ctypes.c_int, U32P, F32P, F32P]
         L.ref_trace.restype = ctypes.c_int
+        L.ref_build_f64_triangles.argtypes = [F64P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, VP]
+        L.ref_build_f64_triangles.restype = ctypes.c_int
+        L.ref_build_f64_aabbs.argtypes = [F64P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, VP]
+        L.ref_build_f64_aabbs.restype = ctypes.c_int
         L.ref_hardware_concurrency.restype = ctypes.c_int
         _ref[key] = L
     return _ref[key]
@@ -166,6 +181,59 @@ def build(tri, code_bits=30, leaf_order=LEAF_REFERENCE):
     return nodes, bounds, sc, ids
 
 
+# ---- builder<double> -------------------------------------------------------
+
+def _f64(a, w):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1, w)
+
+
+def triangle_aabb_f64(tri96):
+    """triangle_aabb_converter<double> over triangle<double> records [n, 12]."""
+    tri96 = _f64(tri96, 12)
+    out = np.empty((tri96.shape[0], 6), dtype=np.float64)
+    orc().orc_triangle_aabb_f64(_p(tri96, F64P), tri96.shape[0], _p(out, F64P))
+    return out
+
+
+def morton_codes_f64(boxes48):
+    """Returns (codes u64 in primitive order, centroid bounds[6] f64)."""
+    boxes48 = _f64(boxes48, 6)
+    codes = np.empty(boxes48.shape[0], dtype=np.uint64)
+    bounds = np.empty(6, dtype=np.float64)
+    orc().orc_morton_codes_f64(_p(boxes48, F64P), boxes48.shape[0], _p(bounds, F64P),
+                               _p(codes, U64P))
+    return codes, bounds
+
+
+def karras_f64(sorted_codes):
+    sc = np.ascontiguousarray(sorted_codes, dtype=np.uint64)
+    nodes = np.zeros(sc.shape[0] - 1, dtype=NODE64_DTYPE)
+    if orc().orc_karras_f64(_p(sc, U64P), sc.shape[0], nodes.ctypes.data) != 0:
+        raise ValueError("orc_karras_f64 needs n >= 2")
+    return nodes
+
+
+def fit_f64(nodes, boxes48):
+    nodes = np.array(nodes, dtype=NODE64_DTYPE, copy=True)
+    boxes48 = _f64(boxes48, 6)
+    orc().orc_fit_f64(nodes.ctypes.data, boxes48.shape[0], _p(boxes48, F64P))
+    return nodes
+
+
+def build_f64(boxes48, leaf_order=LEAF_REFERENCE):
+    """builder<double> over converter boxes [n, 6].  Returns (nodes, sorted_codes, ids)."""
+    boxes48 = _f64(boxes48, 6)
+    n = boxes48.shape[0]
+    nodes = np.zeros(max(n - 1, 0), dtype=NODE64_DTYPE)
+    sc = np.empty(n, dtype=np.uint64)
+    ids = np.empty(n, dtype=np.uint32)
+    rc = orc().orc_build_f64(_p(boxes48, F64P), n, leaf_order, nodes.ctypes.data, _p(sc, U64P),
+                             _p(ids, U32P))
+    if rc != 0:
+        raise ValueError("orc_build_f64 failed (n < 2)")
+    return nodes, sc, ids
+
+
 def trace(nodes, tri, rays, mode=ISECT_REFERENCE, threads=None, counters=False):
     """Returns (prim u32, t f32, uv f32[m,2]) [, counters(popped, boxes, tests)]."""
     tri = _tri(tri)
@@ -229,6 +297,20 @@ def ref_build(tri, code_bits=30, leaf_order=LEAF_REFERENCE, threads=None, phases
     if phases:
         return nodes, sc, bounds, ph
     return nodes, sc, bounds
+
+
+def ref_build_f64(prims, kind="triangles", leaf_order=LEAF_REFERENCE, threads=None):
+    """lbvh::builder<double> as shipped over triangle<double> [n, 12] or
+    aabb<double> [n, 6] records.  Returns the node<double> array."""
+    w = 12 if kind == "triangles" else 6
+    prims = _f64(prims, w)
+    n = prims.shape[0]
+    nodes = np.zeros(n - 1, dtype=NODE64_DTYPE)
+    th = threads or ref().ref_hardware_concurrency() or 1
+    fn = ref().ref_build_f64_triangles if kind == "triangles" else ref().ref_build_f64_aabbs
+    if fn(_p(prims, F64P), n, leaf_order, th, nodes.ctypes.data) != 0:
+        raise ValueError("ref_build_f64 failed")
+    return nodes
 
 
 def ref_codes(tri, code_bits=30, threads=None):

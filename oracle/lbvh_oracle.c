@@ -417,6 +417,163 @@ orc_build(const float* tris, uint64_t n, int code_bits, int leaf_order,
   return 0;
 }
 
+/* ---- builder<double> (lbvh.h:549-558, 1796-1822 with scalar_type = double):
+ *      u64 codes (morton_domain<8>, morton_encoder<8>), u64 node indices with
+ *      the leaf flag in bit 63, FP64 boxes.  Input is the converter's boxes
+ *      (boxes48[i] = converter(primitives[i])). -------------------------- */
+
+static inline double
+std_min_d(double a, double b)
+{
+  return (b < a) ? b : a;
+}
+static inline double
+std_max_d(double a, double b)
+{
+  return (a < b) ? b : a;
+}
+
+/* triangle_aabb_converter<double> (lbvh.h:2001-2012) over triangle<double>
+ * records of 12 doubles {p0, e0, e1, normal}. */
+void
+orc_triangle_aabb_f64(const double* tri96, uint64_t n, double* boxes48)
+{
+  for (uint64_t i = 0; i < n; i++) {
+    const double* t = tri96 + 12 * i;
+    double* o = boxes48 + 6 * i;
+    for (int k = 0; k < 3; k++) {
+      double p0 = t[k], p1 = t[k] - t[3 + k], p2 = t[k] + t[6 + k];
+      o[k] = std_min_d(std_min_d(p0, p1), p2);
+      o[3 + k] = std_max_d(std_max_d(p0, p1), p2);
+    }
+  }
+}
+
+/* centroid_bounds_kernel (lbvh.h:1282-1293) and morton_curve_kernel
+ * (lbvh.h:1432-1473) in FP64: centre = (min + max) * 0.5 (lbvh.h:1164-1167),
+ * scale = 2^20 / (extent + 1), q = (c - min) * scale truncated to u64. */
+void
+orc_morton_codes_f64(const double* boxes48, uint64_t n, double* bounds6,
+                     uint64_t* codes)
+{
+  double lo[3] = { INFINITY, INFINITY, INFINITY };
+  double hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+  for (uint64_t i = 0; i < n; i++) {
+    const double* b = boxes48 + 6 * i;
+    for (int k = 0; k < 3; k++) {
+      double c = (b[k] + b[3 + k]) * 0.5;
+      lo[k] = std_min_d(lo[k], c);
+      hi[k] = std_max_d(hi[k], c);
+    }
+  }
+  double scale[3];
+  for (int k = 0; k < 3; k++)
+    scale[k] = 1048576.0 / ((hi[k] - lo[k]) + 1.0);
+  for (uint64_t i = 0; i < n; i++) {
+    const double* b = boxes48 + 6 * i;
+    uint64_t q[3];
+    for (int k = 0; k < 3; k++) {
+      double c = (b[k] + b[3 + k]) * 0.5;
+      q[k] = (uint64_t)((c - lo[k]) * scale[k]);
+    }
+    codes[i] = (expand20(q[0]) << 2) | (expand20(q[1]) << 1) | expand20(q[2]);
+  }
+  if (bounds6) {
+    for (int k = 0; k < 3; k++) {
+      bounds6[k] = lo[k];
+      bounds6[3 + k] = hi[k];
+    }
+  }
+}
+
+/* builder_kernel<uint64_t, double> (lbvh.h:1713-1731) over sorted u64 codes. */
+int
+orc_karras_f64(const uint64_t* sorted_codes, uint64_t n, orc_node64* nodes)
+{
+  if (n < 2)
+    return -1;
+  orc_table t = { sorted_codes, (int64_t)n, 1 };
+  for (int64_t i = 0; i < (int64_t)n - 1; i++) {
+    int64_t j, g;
+    divide_node(&t, i, &j, &g);
+    int64_t mn = (i < j) ? i : j;
+    int64_t mx = (j > i) ? j : i;
+    uint64_t lm = (mn == g) ? ORC_LEAF_BIT64 : 0u;
+    uint64_t rm = (mx == g + 1) ? ORC_LEAF_BIT64 : 0u;
+    nodes[i].left = (uint64_t)g | lm;
+    nodes[i].right = (uint64_t)(g + 1) | rm;
+  }
+  return 0;
+}
+
+/* fit_boxes (lbvh.h:1827-1866) in FP64 over precomputed boxes. */
+void
+orc_fit_f64(orc_node64* nodes, uint64_t n, const double* boxes48)
+{
+  uint64_t m = n - 1;
+  uint64_t* idx = (uint64_t*)malloc(m * sizeof(uint64_t));
+  uint64_t cnt = 0;
+  idx[cnt++] = 0;
+  for (uint64_t i = 0; i < cnt; i++) {
+    uint64_t j = idx[i];
+    if (!(nodes[j].left & ORC_LEAF_BIT64))
+      idx[cnt++] = nodes[j].left;
+    if (!(nodes[j].right & ORC_LEAF_BIT64))
+      idx[cnt++] = nodes[j].right;
+  }
+  for (uint64_t i = cnt; i > 0; i--) {
+    orc_node64* nd = &nodes[idx[i - 1]];
+    const double *a, *b;
+    if (nd->left & ORC_LEAF_BIT64)
+      a = boxes48 + 6 * (nd->left & ~ORC_LEAF_BIT64);
+    else
+      a = nodes[nd->left].min;
+    if (nd->right & ORC_LEAF_BIT64)
+      b = boxes48 + 6 * (nd->right & ~ORC_LEAF_BIT64);
+    else
+      b = nodes[nd->right].min;
+    double box[6];
+    for (int k = 0; k < 3; k++) {
+      box[k] = std_min_d(a[k], b[k]);
+      box[3 + k] = std_max_d(a[3 + k], b[3 + k]);
+    }
+    memcpy(nd->min, box, sizeof(box));
+  }
+  free(idx);
+}
+
+int
+orc_build_f64(const double* boxes48, uint64_t n, int leaf_order,
+              orc_node64* nodes, uint64_t* sorted_codes_out, uint32_t* ids_out)
+{
+  if (n < 2 || n >= (1ull << 32))
+    return -1;
+  uint64_t* codes = (uint64_t*)malloc(n * 8);
+  uint64_t* sorted = (uint64_t*)malloc(n * 8);
+  uint32_t* ids = (uint32_t*)malloc(n * sizeof(uint32_t));
+  orc_morton_codes_f64(boxes48, n, NULL, codes);
+  orc_sort_pairs(63, codes, n, sorted, ids);
+  memset(nodes, 0, (n - 1) * sizeof(orc_node64));
+  orc_karras_f64(sorted, n, nodes);
+  if (leaf_order == ORC_LEAF_PRIMITIVE_ID) {
+    for (uint64_t i = 0; i < n - 1; i++) {
+      if (nodes[i].left & ORC_LEAF_BIT64)
+        nodes[i].left = ids[nodes[i].left & ~ORC_LEAF_BIT64] | ORC_LEAF_BIT64;
+      if (nodes[i].right & ORC_LEAF_BIT64)
+        nodes[i].right = ids[nodes[i].right & ~ORC_LEAF_BIT64] | ORC_LEAF_BIT64;
+    }
+  }
+  orc_fit_f64(nodes, n, boxes48);
+  if (sorted_codes_out)
+    memcpy(sorted_codes_out, sorted, n * 8);
+  if (ids_out)
+    memcpy(ids_out, ids, n * sizeof(uint32_t));
+  free(codes);
+  free(sorted);
+  free(ids);
+  return 0;
+}
+
 /* ---- A12: triangle_intersector (lbvh.h:2055-2075) ------------------------ */
 
 static inline int

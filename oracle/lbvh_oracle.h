@@ -21,6 +21,11 @@
  *   orc_fit                golden fixtures; subtree min/max closed form;
  *                          volume invariant of lbvh_test.cpp:458-518
  *   orc_trace (modes 0,1)  golden fixtures; brute-force closest hit
+ *   orc_*_f64 (builder<double>)
+ *                          golden fixtures from the compiled lbvh.h
+ *                          (builder<double> over triangle<double> and over
+ *                          plain boxes); numpy min/max of centres; brute-force
+ *                          radix tree; subtree min/max closed form
  *   orc_trace (mode 2)     brute-force closest hit with the same test; FP64
  *                          plane-distance error bound.  No reference counterpart
  *                          exists for the watertight test: its hit set is pinned
@@ -78,6 +83,29 @@ void orc_fit(orc_node* nodes, uint64_t n, const float* tris,
 int orc_build(const float* tris, uint64_t n, int code_bits, int leaf_order,
               orc_node* nodes, orc_aabb* bounds_out, void* sorted_codes_out,
               uint32_t* ids_out);
+
+/* lbvh::node<double> (lbvh.h:220-255 with scalar_type = double): 64 bytes,
+ * leaf flag in bit 63. */
+typedef struct
+{
+  double min[3];
+  double max[3];
+  uint64_t left;
+  uint64_t right;
+} orc_node64;
+
+#define ORC_LEAF_BIT64 0x8000000000000000ull
+
+/* builder<double> over converter boxes (6 doubles each). */
+void orc_triangle_aabb_f64(const double* tri96, uint64_t n, double* boxes48);
+void orc_morton_codes_f64(const double* boxes48, uint64_t n, double* bounds6,
+                          uint64_t* codes);
+int orc_karras_f64(const uint64_t* sorted_codes, uint64_t n,
+                   orc_node64* nodes);
+void orc_fit_f64(orc_node64* nodes, uint64_t n, const double* boxes48);
+int orc_build_f64(const double* boxes48, uint64_t n, int leaf_order,
+                  orc_node64* nodes, uint64_t* sorted_codes_out,
+                  uint32_t* ids_out);
 void orc_trace(const orc_node* nodes, uint64_t n, const float* tris,
                const float* rays, uint64_t nrays, int mode, int nthreads,
                uint32_t* prim, float* t, float* uv, uint64_t* counters);

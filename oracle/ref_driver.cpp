@@ -20,6 +20,9 @@
 //   * PRIMITIVE_ID leaves: the primitives are first permuted into stable
 //     (code, id) order and built with the same reference pipeline; a leaf at
 //     sorted position k then names primitive perm[k].
+//   * builder<double> (lbvh.h:549-558): as shipped, over triangle<double>
+//     with triangle_aabb_converter<double> and over plain aabb<double> with
+//     an identity converter; PRIMITIVE_ID leaves as above.
 //   * Traversal: lbvh::traverser with triangle_intersector (lbvh.h:1874-1942,
 //     2055-2075); the lowest-id tie-break wraps the primitive and its
 //     intersection record, leaving the traverser unmodified.
@@ -257,6 +260,56 @@ struct lowid_hit
   bool operator<(float d) const noexcept { return h.distance < d; }
 };
 
+// builder<double>: the converter of plain FP64 boxes (the generic
+// primitive/converter path with P = aabb<double>).
+struct box_cvt_f64
+{
+  lbvh::aabb<double> operator()(const lbvh::aabb<double>& b) const noexcept { return b; }
+};
+
+using node64_t = lbvh::node<double>;
+static_assert(sizeof(node64_t) == 64, "node<double> layout");
+static_assert(sizeof(lbvh::triangle<double>) == 96, "triangle<double> layout");
+
+// lbvh::builder<double> as shipped (lbvh.h:1796-1822); PRIMITIVE_ID leaves
+// as for FP32: build over the primitives in stable (code, id) order, then
+// name the primitive behind each leaf.
+template<typename P, typename C>
+std::vector<node64_t>
+build_f64(const P* prims, uint64_t n, int leaf_order, unsigned th, const C& cvt)
+{
+  std::vector<P> permuted;
+  std::vector<uint32_t> perm;
+  const P* src = prims;
+  if (leaf_order == 1) {
+    sched_t sched(th);
+    lbvh::detail::morton_curve_builder<double, sched_t> cb(sched);
+    auto curve = cb(prims, n, cvt);
+    perm.resize(n);
+    for (uint64_t i = 0; i < n; i++)
+      perm[i] = uint32_t(i);
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](uint32_t a, uint32_t b) { return curve[a].code < curve[b].code; });
+    permuted.resize(n);
+    for (uint64_t i = 0; i < n; i++)
+      permuted[i] = prims[perm[i]];
+    src = permuted.data();
+  }
+  lbvh::builder<double, sched_t> b{ sched_t(th) };
+  auto bvh = b(src, n, cvt);
+  std::vector<node64_t> nodes(bvh.begin(), bvh.end());
+  if (leaf_order == 1) {
+    const uint64_t bit = uint64_t(1) << 63;
+    for (auto& nd : nodes) {
+      if (nd.left_is_leaf())
+        nd.left = perm[nd.left_leaf_index()] | bit;
+      if (nd.right_is_leaf())
+        nd.right = perm[nd.right_leaf_index()] | bit;
+    }
+  }
+  return nodes;
+}
+
 } // namespace
 
 extern "C" {
@@ -426,6 +479,37 @@ ref_trace(const void* nodes_in, uint64_t n, const float* tri48,
   work(0);
   for (auto& x : pool)
     x.join();
+  return 0;
+}
+
+// builder<double> over triangle<double> records (12 doubles each: p0, e0,
+// e1, normal) with triangle_aabb_converter<double>.  Returns 0 on success.
+int
+ref_build_f64_triangles(const double* tri96, uint64_t n, int leaf_order, int threads,
+                        void* nodes_out)
+{
+  if (n < 2)
+    return -1;
+  std::vector<lbvh::triangle<double>> prims(n);
+  std::memcpy(static_cast<void*>(prims.data()), tri96, n * 96);
+  auto nodes = build_f64(prims.data(), n, leaf_order, threads > 0 ? unsigned(threads) : 1u,
+                         lbvh::triangle_aabb_converter<double>());
+  std::memcpy(nodes_out, static_cast<const void*>(nodes.data()), nodes.size() * 64);
+  return 0;
+}
+
+// builder<double> over plain boxes (6 doubles each).
+int
+ref_build_f64_aabbs(const double* boxes48, uint64_t n, int leaf_order, int threads,
+                    void* nodes_out)
+{
+  if (n < 2)
+    return -1;
+  std::vector<lbvh::aabb<double>> prims(n);
+  std::memcpy(static_cast<void*>(prims.data()), boxes48, n * 48);
+  auto nodes = build_f64(prims.data(), n, leaf_order, threads > 0 ? unsigned(threads) : 1u,
+                         box_cvt_f64());
+  std::memcpy(nodes_out, static_cast<const void*>(nodes.data()), nodes.size() * 64);
   return 0;
 }
 

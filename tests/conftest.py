@@ -47,6 +47,20 @@ def golden_cases():
     return out
 
 
+def f64_golden_cases():
+    """builder<double> fixtures: (id, kind, prims, {leaf_order: node words u64[n-1, 8]})."""
+    data = np.load(os.path.join(GOLDEN, "f64.npz"))
+    cases = {}
+    for key in data.files:
+        kind, name, field = key.split("/", 2)
+        cases.setdefault((kind, name), {})[field] = data[key]
+    out = []
+    for (kind, name), rec in sorted(cases.items()):
+        out.append((kind + ":" + name, kind, rec["prims"],
+                    {0: rec["l0_nodes"], 1: rec["l1_nodes"]}))
+    return out
+
+
 @pytest.fixture(scope="session")
 def golden():
     return dict(golden_cases())

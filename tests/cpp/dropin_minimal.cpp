@@ -6,7 +6,8 @@
 // Usage: dropin_minimal <out.bin>
 // Writes: u64 n, n triangles (48 B each), (n-1) nodes (32 B each),
 //         u64 m, m rays (32 B), m u32 prim, m f32 t (host traverser),
-//         then the 2 nodes of the 3-sphere tree.
+//         then the 2 nodes of the 3-sphere tree, then u64 n, n FP64
+//         triangles (96 B each) and their builder<double> nodes (64 B each).
 // Exits non-zero if the device-tree closest hits differ from the host ones.
 #include <lbvh_mi355x.hpp>
 
@@ -127,6 +128,53 @@ main(int argc, char** argv)
     return 1;
   }
 
+  // builder<double>: FP64 triangles of the same vertices, u64 codes and
+  // 64-byte nodes; the host traverser runs over the double tree unchanged
+  std::vector<lbvh::triangle<double>> tris64;
+  tris64.reserve(n);
+  for (std::size_t i = 0; i < n; i++) {
+    lcg h{ 777 + i };
+    lbvh::vec3<double> c{ h.next(), h.next(), h.next() };
+    lbvh::vec3<double> v[3];
+    for (auto& p : v)
+      p = { c.x + (h.next() - 0.5) * 0.02, c.y + (h.next() - 0.5) * 0.02,
+            c.z + (h.next() - 0.5) * 0.02 };
+    tris64.emplace_back(v[0], v[1], v[2]);
+  }
+  lbvh::builder<double> builder64;
+  auto bvh64 = builder64(tris64.data(), tris64.size(), lbvh::triangle_aabb_converter<double>());
+  if (bvh64.size() != n - 1) {
+    std::fprintf(stderr, "builder<double>: %zu nodes\n", std::size_t(bvh64.size()));
+    return 1;
+  }
+  lbvh::traverser<double, lbvh::triangle<double>, lbvh::triangle_intersection<double>> tr64(
+    bvh64, tris64.data());
+  lbvh::triangle_intersector<double> isect64;
+  std::size_t hits64 = 0;
+  for (std::size_t i = 0; i < 256; i++) {
+    lbvh::ray<double> r;
+    r.pos = { 0.5, 0.5, -1.0 };
+    r.dir = lbvh::math::normalize(
+      lbvh::vec3<double>{ double(i % 16) / 15.0 - 0.5, double(i / 16) / 15.0 - 0.5, 1.0 });
+    auto h = tr64(r, isect64);
+    // brute force over every triangle, first hit wins ties as in the traverser
+    lbvh::triangle_intersection<double> best;
+    std::uint64_t bi = ~std::uint64_t(0);
+    for (std::size_t k = 0; k < n; k++) {
+      auto info = isect64(tris64[k], r);
+      if (info < best) {
+        best = info;
+        bi = k;
+      }
+    }
+    if ((h ? h.info.distance : best.distance) != best.distance || (!h && bi != ~std::uint64_t(0))) {
+      std::fprintf(stderr, "builder<double> ray %zu: tree %g, brute force %g\n", i,
+                   h ? h.info.distance : -1.0, best.distance);
+      return 1;
+    }
+    hits64 += h ? 1 : 0;
+  }
+
   FILE* f = std::fopen(argv[1], "wb");
   if (!f)
     return 2;
@@ -139,9 +187,14 @@ main(int argc, char** argv)
   std::fwrite(hp.data(), 4, m, f);
   std::fwrite(ht.data(), 4, m, f);
   std::fwrite(sbvh.data(), 32, 2, f);
+  std::fwrite(&nn, 8, 1, f);
+  std::fwrite(tris64.data(), 96, n, f);
+  std::fwrite(bvh64.data(), 64, n - 1, f);
   std::fclose(f);
-  std::printf("dropin ok: %zu triangles, %zu rays, %zu hits\n", n, m,
+  std::printf("dropin ok: %zu triangles, %zu rays, %zu hits; builder<double>: %zu of 256 hits\n",
+              n, m,
               std::size_t(std::count_if(hp.begin(), hp.end(),
-                                        [](std::uint32_t p) { return p != 0xffffffffu; })));
+                                        [](std::uint32_t p) { return p != 0xffffffffu; })),
+              hits64);
   return 0;
 }

@@ -9,6 +9,11 @@ data only: inputs (triangles, rays) and the reference's outputs.
   adversarial.npz  duplicate codes, tight clusters, a flat axis, 1e6-scale
                    and negative coordinates, repeated blocks
   meshes.npz       the teapot and monkey meshes shipped with the reference
+  f64.npz          lbvh::builder<double>: triangle<double> scenes (uniform
+                   sizes, duplicates, clusters, a flat axis, 1e6-scale
+                   coordinates, the teapot) and plain aabb<double> inputs
+                   (wide-range, point and nested boxes); node<double> arrays
+                   for both leaf orders
   hashes.json      FNV-1a-64 hashes of inputs and outputs for the full
                    65,536-triangle, 1,048,576-triangle and 16,777,216-triangle
                    workloads
@@ -147,6 +152,55 @@ def mesh_cases():
     return out
 
 
+def tri96(v):
+    """triangle<double> records {p0, e0 = a - b, e1 = c - a, normal} from
+    vertex triples, computed in FP64."""
+    v = np.asarray(v, dtype=np.float64).reshape(-1, 9)
+    a, b, c = v[:, 0:3], v[:, 3:6], v[:, 6:9]
+    e0 = a - b
+    e1 = c - a
+    nrm = np.stack([e0[:, 1] * e1[:, 2] - e0[:, 2] * e1[:, 1],
+                    e0[:, 2] * e1[:, 0] - e0[:, 0] * e1[:, 2],
+                    e0[:, 0] * e1[:, 1] - e0[:, 1] * e1[:, 0]], axis=1)
+    return np.ascontiguousarray(np.concatenate([a, e0, e1, nrm], axis=1))
+
+
+def f64_record(name, prims, kind):
+    rec = {"prims": prims}
+    for lo in (0, 1):
+        rec["l%d_nodes" % lo] = oracle.ref_build_f64(prims, kind, lo).view(np.uint64).reshape(-1, 8)
+    return {kind + "/" + name + "/" + k: v for k, v in rec.items()}
+
+
+def f64_cases():
+    out = {}
+    rng = np.random.default_rng(31)
+    for n in (2, 3, 5, 17, 64, 1000, 2048):
+        out.update(f64_record("n%d" % n, tri96(rng.random((n, 9))), "triangles"))
+    out.update(f64_record("all_equal", tri96(np.tile(rng.random((1, 9)), (300, 1))), "triangles"))
+    a = rng.uniform(-1e-9, 1e-9, size=(400, 9)) + 0.1
+    b = rng.uniform(-1e-9, 1e-9, size=(400, 9)) + 0.9
+    out.update(f64_record("two_clusters", tri96(np.concatenate([a, b])), "triangles"))
+    v = rng.random((1000, 9))
+    v[:, 2::3] = 0.5
+    out.update(f64_record("flat_z", tri96(v), "triangles"))
+    c = rng.uniform(-1e6, 1e6, size=(1500, 3))
+    v = np.tile(c, (1, 3)) + rng.uniform(-1e3, 1e3, size=(1500, 9))
+    out.update(f64_record("big_negative", tri96(v), "triangles"))
+    out.update(f64_record("dup_blocks", tri96(np.repeat(rng.random((48, 9)), 24, axis=0)), "triangles"))
+    teapot = load_obj(os.path.join(REF_MODELS, "teapot.obj")).astype(np.float64)
+    out.update(f64_record("teapot", tri96(teapot), "triangles"))
+    # plain boxes through the generic converter path
+    lo = rng.uniform(-1e4, 1e4, size=(1500, 3))
+    ext = rng.exponential(10.0, size=(1500, 3))
+    out.update(f64_record("wide", np.concatenate([lo, lo + ext], axis=1), "aabbs"))
+    p = rng.random((700, 3))
+    out.update(f64_record("points", np.concatenate([p, p], axis=1), "aabbs"))
+    r = np.linspace(1.0, 2.0, 200)[:, None] * np.ones((1, 3))
+    out.update(f64_record("nested", np.concatenate([-r, r], axis=1), "aabbs"))
+    return out
+
+
 def config_hashes():
     H = {}
     h = oracle.fnv1a64
@@ -197,6 +251,10 @@ def main():
     build_oracle.build_all()
     if not oracle.ref_available():
         raise SystemExit("reference driver not available: fixtures need the reference sources")
+    np.savez_compressed(os.path.join(HERE, "f64.npz"), **f64_cases())
+    print("f64 done", flush=True)
+    if "--f64-only" in sys.argv:
+        return
     np.savez_compressed(os.path.join(HERE, "small.npz"), **small_cases())
     print("small done", flush=True)
     np.savez_compressed(os.path.join(HERE, "adversarial.npz"), **adversarial_cases())

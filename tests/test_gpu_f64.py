@@ -1,0 +1,165 @@
+"""GPU parity of lbvh::builder<double> (lbvh_build_aabbs_f64 and its device
+variant) through the C ABI: node<double> arrays bit-exact against the golden
+vectors written by the real builder<double> and against the oracle on
+seeded inputs (ties, clusters, flat axes, 1e6-scale and tiny extents, both
+leaf orders), device-resident input, alternation with FP32 builds on one
+context, and argument errors."""
+import numpy as np
+import pytest
+
+import lbvh_amd
+import oracle
+from conftest import f64_golden_cases
+
+pytestmark = pytest.mark.gpu
+
+CASES = f64_golden_cases()
+IDS = [c[0] for c in CASES]
+
+
+def _boxes(kind, prims):
+    return oracle.triangle_aabb_f64(prims) if kind == "triangles" else prims
+
+
+def _scene(dist, n, seed):
+    rng = np.random.default_rng(seed)
+    if dist == "uniform":
+        lo = rng.random((n, 3))
+        ext = rng.random((n, 3)) * 0.01
+    elif dist == "clustered":
+        centres = rng.normal(size=(32, 3)) * 10.0
+        lo = centres[rng.integers(0, 32, size=n)] + rng.normal(size=(n, 3)) * 0.05
+        ext = rng.random((n, 3)) * 1e-3
+    elif dist == "duplicates":
+        base = rng.random((max(n // 16, 1), 3))
+        lo = np.repeat(base, 16, axis=0)[:n]
+        if lo.shape[0] < n:
+            lo = np.concatenate([lo, base[: n - lo.shape[0]]])
+        ext = np.full((n, 3), 1e-3)
+    elif dist == "flat":
+        lo = rng.random((n, 3))
+        lo[:, 1] = -2.5
+        ext = rng.random((n, 3)) * 0.01
+        ext[:, 1] = 0.0
+    elif dist == "huge":
+        lo = rng.uniform(-1e6, 1e6, size=(n, 3))
+        ext = rng.random((n, 3)) * 1e3
+    else:  # tiny: extents far below one Morton cell of FP32 codes
+        lo = 0.5 + rng.random((n, 3)) * 1e-9
+        ext = rng.random((n, 3)) * 1e-12
+    return np.ascontiguousarray(np.concatenate([lo, lo + ext], axis=1))
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+def test_f64_build_matches_golden(ctx, case, leaf):
+    _, kind, prims, want = case
+    got = ctx.build_aabbs_f64(_boxes(kind, prims), leaf)
+    assert np.array_equal(got.view(np.uint64).reshape(-1, 8), want[leaf])
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 255, 256, 257, 1000, 4097, 65537, 300001])
+@pytest.mark.parametrize("dist", ["uniform", "clustered", "duplicates", "flat", "huge", "tiny"])
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+def test_f64_build_matches_oracle(ctx, n, dist, leaf):
+    print("f64 build n=%d dist=%s leaf=%d" % (n, dist, leaf), flush=True)
+    boxes = _scene(dist, n, 1000 + n)
+    want, _, _ = oracle.build_f64(boxes, leaf)
+    got = ctx.build_aabbs_f64(boxes, leaf)
+    assert got.tobytes() == want.tobytes()
+
+
+def test_f64_large_matches_oracle(ctx):
+    n = 4_000_037
+    print("f64 build n=%d clustered" % n, flush=True)
+    boxes = _scene("clustered", n, 7)
+    want, _, _ = oracle.build_f64(boxes, 1)
+    got = ctx.build_aabbs_f64(boxes, 1)
+    assert got.tobytes() == want.tobytes()
+
+
+def test_f64_device_input_matches_oracle(ctx):
+    from lbvh_amd.hip import DeviceArray
+    n = 2_000_003
+    print("f64 device build n=%d" % n, flush=True)
+    boxes = _scene("uniform", n, 9)
+    want, _, _ = oracle.build_f64(boxes, 0)
+    d = DeviceArray.from_host(boxes)
+    out = ctx.build_aabbs_f64(d, 0)
+    ctx.synchronize()
+    assert out.numpy()[: n - 1].tobytes() == want.tobytes()
+    # a second build into the same output buffer
+    boxes2 = _scene("huge", n, 10)
+    d.copy_from(boxes2)
+    ctx.build_aabbs_f64(d, 1, out=out)
+    ctx.synchronize()
+    want2, _, _ = oracle.build_f64(boxes2, 1)
+    assert out.numpy()[: n - 1].tobytes() == want2.tobytes()
+    d.close()
+    out.close()
+
+
+F64_TORCH = r"""
+import sys, numpy as np
+import torch  # first: the engine then shares torch's HIP runtime
+sys.path.insert(0, %r)
+import lbvh_amd, oracle
+rng = np.random.default_rng(4)
+lo = rng.random((300001, 3))
+boxes = np.concatenate([lo, lo + rng.random((300001, 3)) * 1e-3], axis=1)
+ctx = lbvh_amd.Context(0)
+ctx.set_stream(torch.cuda.current_stream())
+out = ctx.build_aabbs_f64(torch.from_numpy(boxes).cuda(), 1)
+torch.cuda.synchronize()
+want, _, _ = oracle.build_f64(boxes, 1)
+assert out.cpu().numpy().view(oracle.NODE64_DTYPE)[:300000].tobytes() == want.tobytes()
+print("f64 torch ok")
+"""
+
+
+def test_f64_torch_tensor_input():
+    """A torch float64 CUDA tensor as device input (fresh process, torch
+    imported first: one HIP runtime per process)."""
+    import subprocess
+    import sys
+    from conftest import ROOT
+    r = subprocess.run([sys.executable, "-c", F64_TORCH % ROOT], capture_output=True, text=True,
+                       timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "f64 torch ok" in r.stdout
+
+
+def test_f64_alternates_with_fp32_builds(ctx):
+    """FP64 and FP32 builds share the context workspace; each stays exact
+    whatever ran before it, larger or smaller."""
+    from lbvh_amd import workload as W
+    tri = W.make_triangles(W.cluster_vertices(5, 120000))
+    want32, _, _, _ = oracle.build(tri, 63, 1)
+    for n in (200001, 5003, 90001):
+        boxes = _scene("clustered", n, n)
+        want, _, _ = oracle.build_f64(boxes, 1)
+        assert ctx.build_aabbs_f64(boxes, 1).tobytes() == want.tobytes()
+        b32 = ctx.build_triangles(tri, code_bits=63, leaf_order=1)
+        assert b32.nodes().tobytes() == want32.tobytes()
+        b32.destroy()
+
+
+def test_f64_rejects_bad_arguments(ctx):
+    with pytest.raises(lbvh_amd.LbvhError) as e:
+        ctx.build_aabbs_f64(np.zeros((1, 6)))
+    assert e.value.status == 1
+    with pytest.raises(lbvh_amd.LbvhError):
+        ctx.build_aabbs_f64(np.zeros((0, 6)))
+    with pytest.raises(lbvh_amd.LbvhError):
+        ctx.build_aabbs_f64(np.zeros((10, 6)), leaf_order=7)
+    # boxes 8 bytes past a 16-byte boundary: refused before any launch
+    from lbvh_amd.hip import DeviceArray
+    d = DeviceArray.from_host(np.zeros(6 * 100 + 1))
+    with pytest.raises(lbvh_amd.LbvhError) as e:
+        ctx.build_aabbs_f64(d.rows(1, 601), 0)
+    assert e.value.status == 1
+    d.close()
+    # the context stays usable
+    boxes = _scene("uniform", 1000, 3)
+    want, _, _ = oracle.build_f64(boxes, 0)
+    assert ctx.build_aabbs_f64(boxes, 0).tobytes() == want.tobytes()

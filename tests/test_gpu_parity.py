@@ -418,6 +418,17 @@ def test_cpp_dropin_conformance(tmp_path):
     op, ot, _ = oracle.trace(on, tri, rays, 0)
     assert np.array_equal(hp, op)
     assert np.array_equal(ht.view(np.uint32), ot.view(np.uint32))
+    # builder<double> over triangle<double> (its traversal is checked against
+    # brute force inside the program)
+    off += 4 * m + 64
+    n64 = int(np.frombuffer(raw[off:off + 8], np.uint64)[0])
+    off += 8
+    tri64 = np.frombuffer(raw[off:off + 96 * n64], np.float64).reshape(n64, 12)
+    off += 96 * n64
+    nodes64 = np.frombuffer(raw[off:off + 64 * (n64 - 1)], oracle.NODE64_DTYPE)
+    assert off + 64 * (n64 - 1) == len(raw)
+    want, _, _ = oracle.build_f64(oracle.triangle_aabb_f64(tri64), 0)
+    assert nodes64.tobytes() == want.tobytes()
 
 
 def test_aabb_build_equals_triangle_build(ctx):

@@ -200,12 +200,13 @@ void lbvh_bvh_destroy(lbvh_bvh* bvh);
  * (lbvh.h:205-212).  Codes are u64 (morton_domain<8>, lbvh.h:1322-1326) and
  * all box arithmetic is FP64.  Output: n - 1 lbvh::node<double> records of
  * 64 bytes {aabb<double> box; u64 left; u64 right}, bit 63 of left/right
- * marks a leaf, root = node 0 (lbvh.h:220-255).  opts->leaf_order applies;
- * code_bits and the sort fields are ignored (the code width follows the
- * scalar type, as in the reference).  n in [2, 2^31 - 1].  The device
- * variant borrows d_boxes48 (16-byte aligned) and writes d_nodes_out, both
- * device memory, and returns after queuing; the host variant copies in and
- * out and returns when the nodes are in nodes_out. */
+ * marks a leaf, root = node 0 (lbvh.h:220-255).  opts->leaf_order and the
+ * sort fields apply; code_bits is ignored (the code width follows the
+ * scalar type, as in the reference); the sort is tuned per size as for
+ * 63-bit FP32 builds.  n in [2, 2^31 - 1].  The device variant borrows
+ * d_boxes48 and writes d_nodes_out, both device memory aligned to 16 bytes,
+ * and returns after queuing; the host variant copies in and out and
+ * returns when the nodes are in nodes_out. */
 lbvh_status lbvh_build_aabbs_f64(lbvh_ctx* ctx, const double* boxes48,
                                  uint64_t n, const lbvh_build_opts* opts,
                                  void* nodes_out);

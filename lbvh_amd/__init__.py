@@ -622,13 +622,14 @@ class Context:
                                        ctypes.byref(out)))
         return Bvh(self, out, n, code_bits, leaf_order)
 
-    def build_aabbs_f64(self, boxes, leaf_order=LEAF_REFERENCE, out=None):
+    def build_aabbs_f64(self, boxes, leaf_order=LEAF_REFERENCE, out=None, sort_digit_bits=0,
+                        sort_tile=0, sort_strategy=SORT_AUTO):
         """lbvh::builder<double> over converter boxes (float64 [n, 6]).
         Host input: returns NODE64_DTYPE[n - 1].  Device input (a
         DeviceArray or a torch float64 tensor): writes the node records into
         `out` (device memory of at least 64 (n - 1) bytes; allocated like
         the input when None) and returns it without synchronizing."""
-        opts = _make_opts(63, leaf_order, False)
+        opts = _make_opts(63, leaf_order, False, sort_digit_bits, sort_tile, sort_strategy)
         if _is_device(boxes):
             n = _numel(boxes) // 6
             if out is None:

@@ -15,29 +15,66 @@ namespace {
 
 constexpr uint64_t kMaxF64Prims = (uint64_t(1) << 31) - 1;
 
-// Queues the whole FP64 build of n boxes at d_boxes into d_nodes.
+lbvh_status queue_f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n,
+                            const lbvh_build_opts& opts, void* d_nodes, const SortConfig& cfg);
+
+// The sort configuration of an FP64 build: explicit options, else the
+// context's tuned choice for 60-bit keys of this size (shared with 63-bit
+// FP32 builds, which sort the same keys), tuning now on this very build when
+// there is none yet.
 lbvh_status
-queue_f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build_opts* opts_in,
-                void* d_nodes)
+choose_sort_f64(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build_opts& opts,
+                void* d_nodes, SortConfig* cfg)
 {
-  lbvh_build_opts opts;
-  lbvh_build_opts_init(&opts);
-  if (opts_in)
-    opts = *opts_in;
-  if (opts.leaf_order > 1)
-    return fail(c, LBVH_E_INVALID, "unknown leaf order");
-  if (n < 2)
-    return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
-  if (n > kMaxF64Prims)
-    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 primitives per FP64 tree");
-  if ((reinterpret_cast<uintptr_t>(d_boxes) & 15) != 0 ||
-      (reinterpret_cast<uintptr_t>(d_nodes) & 7) != 0)
-    return fail(c, LBVH_E_INVALID, "boxes must be 16-byte and nodes 8-byte aligned");
   const int key_bits = 60;
-  const SortConfig cfg = default_sort();
+  if (sort_overridden(opts)) {
+    *cfg = sort_from_opts(opts, key_bits, n);
+    return LBVH_OK;
+  }
+  *cfg = default_sort();
+  if (!c->autotune || n < kTuneMinPrims)
+    return LBVH_OK;
+  const uint64_t key = tune_key(false, key_bits, n);
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) {
+    *cfg = it->second;
+    return LBVH_OK;
+  }
+  float best = -1.0f;
+  size_t first = c->tune_log.size();
+  for (const SortConfig& cand : sort_candidates(false, key_bits, n)) {
+    float ms = -1.0f;
+    for (int rep = 0; rep < 2; rep++) {
+      lbvh_status st;
+      ms = time_call(c, [&] { return queue_f64_build(c, d_boxes, n, opts, d_nodes, cand); }, &st);
+      if (st != LBVH_OK)
+        return st;
+      if ((st = sync_and_check(c)) != LBVH_OK)
+        return st;
+    }
+    c->tune_log.push_back(TuneRecord{ 0, key_bits, ceil_log2_u64(n), cand, ms, false });
+    if (ms >= 0.0f && (best < 0.0f || ms < best)) {
+      best = ms;
+      *cfg = cand;
+    }
+  }
+  for (size_t i = first; i < c->tune_log.size(); i++)
+    c->tune_log[i].picked = same_sort(c->tune_log[i].cfg, *cfg);
+  c->tuned[key] = *cfg;
+  return LBVH_OK;
+}
+
+// Queues the whole FP64 build of n boxes at d_boxes into d_nodes with the
+// given sort configuration.
+lbvh_status
+queue_f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build_opts& opts,
+                void* d_nodes, const SortConfig& cfg)
+{
+  const int key_bits = 60;
   if (!valid_sort_config(cfg, key_bits))
-    return fail(c, LBVH_E_INVALID, "unsupported sort configuration");
+    return fail(c, LBVH_E_INVALID, "unsupported sort digit width / tile");
   const int passes = sort_passes(key_bits, cfg.digit_bits);
+  const int first_pass = cfg.strategy == 2 ? passes - cfg.global_digits : 0;
   const uint32_t radix = 1u << cfg.digit_bits;
   const bool with_ids = opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID;
 
@@ -58,8 +95,7 @@ queue_f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build
   const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
   if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
     return st;
-  // parent links [2n - 1] and fit flags [n - 1]
-  if ((st = ensure(c, c->tree, 4 * ((2 * n - 1) + (n - 1)))) != LBVH_OK)
+  if ((st = ensure(c, c->tree, f64_tree_scratch_bytes(n))) != LBVH_OK)
     return st;
   const uint64_t counts_words = sort_counts_words(n, cfg);
   if ((st = ensure(c, c->counts, 4 * counts_words)) != LBVH_OK)
@@ -90,8 +126,6 @@ queue_f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build
 
   double* partials = static_cast<double*>(c->partials.p);
   double* bounds = partials + 6 * size_t(f64_bounds_blocks(n));
-  uint32_t* parent = static_cast<uint32_t*>(c->tree.p);
-  uint32_t* flags = parent + (2 * n - 1);
 
   begin_timed(c, 1);
   record(c, 0);
@@ -100,7 +134,7 @@ queue_f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build
     e = launch_f64_bounds(d_boxes, n, partials, bounds, c->stream);
   if (e == hipSuccess)
     e = launch_f64_encode(d_boxes, n, bounds, static_cast<uint64_t*>(c->keys_a.p), ws.hist,
-                          cfg.digit_bits, passes, c->stream);
+                          cfg.digit_bits, first_pass, passes, c->stream);
   record(c, 1);
   bool in_b = false;
   if (e == hipSuccess)
@@ -113,12 +147,37 @@ queue_f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build
   const uint32_t* ids =
     with_ids ? static_cast<const uint32_t*>(in_b ? c->ids_b.p : c->ids_a.p) : nullptr;
   if (e == hipSuccess)
-    e = launch_f64_tree(sorted, ids, d_boxes, n, d_nodes, parent, flags, c->stream);
+    e = launch_f64_tree(sorted, ids, d_boxes, n, d_nodes, c->tree.p, c->stream);
   record(c, 3);
   record(c, 4, true);
   if (e != hipSuccess)
     return hip_fail(c, e, "FP64 build launch");
   return LBVH_OK;
+}
+
+// Validates the arguments, picks the sort and queues the build.
+lbvh_status
+f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build_opts* opts_in,
+          void* d_nodes)
+{
+  lbvh_build_opts opts;
+  lbvh_build_opts_init(&opts);
+  if (opts_in)
+    opts = *opts_in;
+  if (opts.leaf_order > 1)
+    return fail(c, LBVH_E_INVALID, "unknown leaf order");
+  if (n < 2)
+    return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
+  if (n > kMaxF64Prims)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 primitives per FP64 tree");
+  if ((reinterpret_cast<uintptr_t>(d_boxes) & 15) != 0 ||
+      (reinterpret_cast<uintptr_t>(d_nodes) & 15) != 0)
+    return fail(c, LBVH_E_INVALID, "boxes and nodes must be 16-byte aligned");
+  SortConfig cfg;
+  lbvh_status st = choose_sort_f64(c, d_boxes, n, opts, d_nodes, &cfg);
+  if (st != LBVH_OK)
+    return st;
+  return queue_f64_build(c, d_boxes, n, opts, d_nodes, cfg);
 }
 
 } // namespace
@@ -134,7 +193,7 @@ lbvh_build_aabbs_f64_device(lbvh_ctx* c, const double* d_boxes, uint64_t n,
   lbvh_status st = set_device(c);
   if (st != LBVH_OK)
     return st;
-  return queue_f64_build(c, d_boxes, n, opts, d_nodes);
+  return f64_build(c, d_boxes, n, opts, d_nodes);
 }
 
 lbvh_status
@@ -156,7 +215,7 @@ lbvh_build_aabbs_f64(lbvh_ctx* c, const double* boxes, uint64_t n, const lbvh_bu
   if ((st = ensure(c, c->io_out, out_bytes)) != LBVH_OK)
     return st;
   HIP_TRY(c, hipMemcpyAsync(c->io_in.p, boxes, in_bytes, hipMemcpyHostToDevice, c->stream));
-  if ((st = queue_f64_build(c, static_cast<const double*>(c->io_in.p), n, opts, c->io_out.p)) !=
+  if ((st = f64_build(c, static_cast<const double*>(c->io_in.p), n, opts, c->io_out.p)) !=
       LBVH_OK) {
     (void)hipStreamSynchronize(c->stream);
     return st;

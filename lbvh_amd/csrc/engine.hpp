@@ -169,19 +169,21 @@ uint32_t scene_encode_groups(uint64_t n);
 
 // builder<double>: centroid bounds of FP64 boxes (6 doubles each, 16-byte
 // aligned) via f64_bounds_blocks(n) partials of 6 doubles; u64 codes with the
-// digit histograms of `passes` passes added to hist (zero before); and the
-// tree over sorted codes: node<double> records (64 B), parent links
-// [2n - 1] and fit flags [n - 1] (cleared by the launch).  ids != nullptr:
-// leaves name ids[position] instead of the position.
+// digit histograms of passes first_pass .. passes - 1 added to hist[p <<
+// digit_bits ...] (zero before); and the tree over sorted
+// codes into node<double> records (64 B) with f64_tree_scratch_bytes(n) of
+// scratch (parent links, range ends, fit flags cleared by the launch,
+// staging boxes).  ids != nullptr: leaves name ids[position] instead of the
+// position.
 uint32_t f64_bounds_blocks(uint64_t n);
 hipError_t launch_f64_bounds(const double* boxes, uint64_t n, double* partials, double* bounds,
                              hipStream_t s);
 hipError_t launch_f64_encode(const double* boxes, uint64_t n, const double* bounds,
-                             uint64_t* codes, uint32_t* hist, int digit_bits, int passes,
-                             hipStream_t s);
+                             uint64_t* codes, uint32_t* hist, int digit_bits, int first_pass,
+                             int passes, hipStream_t s);
+uint64_t f64_tree_scratch_bytes(uint64_t n);
 hipError_t launch_f64_tree(const uint64_t* sorted, const uint32_t* ids, const double* boxes,
-                           uint64_t n, void* nodes, uint32_t* parent, uint32_t* flags,
-                           hipStream_t s);
+                           uint64_t n, void* nodes, void* scratch, hipStream_t s);
 
 // Fill `words` 32-bit words with `value`.
 hipError_t launch_fill_u32(uint32_t* p, uint64_t words, uint32_t value, hipStream_t s);

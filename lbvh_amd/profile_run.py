@@ -8,6 +8,9 @@
                                             # codes, 3 refits after a perturbation
   python3 -m lbvh_amd.profile_run batched   # 1,024 scenes log-uniform [64, 262144]
                                             # (seed 4): 3 rebuilds + 3 batched traces
+  python3 -m lbvh_amd.profile_run f64       # builder<double>: 16,777,216 uniform
+                                            # FP64 boxes (seed 1), 5 builds each with
+                                            # reference and primitive-id leaves
   python3 -m lbvh_amd.profile_run copy      # calibration: an elementwise kernel that
                                             # reads 512 MiB and writes 512 MiB (torch),
                                             # 5 times
@@ -96,6 +99,27 @@ def batched():
     ctx.close()
 
 
+def f64():
+    import numpy as np
+    import lbvh_amd
+    from lbvh_amd import hip
+    n = 1 << 24
+    rng = np.random.default_rng(1)
+    lo = rng.random((n, 3))
+    boxes = np.concatenate([lo, lo + rng.random((n, 3)) * 5e-4], axis=1)
+    del lo
+    ctx = lbvh_amd.Context(0)
+    d = hip.DeviceArray.from_host(boxes)
+    del boxes
+    out = hip.DeviceArray((n - 1,), lbvh_amd.NODE64_DTYPE)
+    for leaf in (0, 1):
+        for i in range(5):
+            ctx.build_aabbs_f64(d, leaf, out=out)
+            ctx.synchronize()
+            _log("f64 build leaf order %d, %d" % (leaf, i))
+    ctx.close()
+
+
 def copy():
     import torch
     a = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
@@ -109,5 +133,5 @@ def copy():
 if __name__ == "__main__":
     mode = sys.argv[1] if len(sys.argv) > 1 else "single"
     t0 = time.time()
-    {"single": single, "large": large, "batched": batched, "copy": copy}[mode]()
+    {"single": single, "large": large, "batched": batched, "f64": f64, "copy": copy}[mode]()
     _log("%s done in %.1f s" % (mode, time.time() - t0))

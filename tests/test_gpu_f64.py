@@ -69,6 +69,30 @@ def test_f64_build_matches_oracle(ctx, n, dist, leaf):
     assert got.tobytes() == want.tobytes()
 
 
+@pytest.mark.parametrize("strategy,digits,tile", [(1, 8, 0), (2, 8, 0), (2, 10, 0), (3, 8, 0),
+                                                  (3, 10, 4096)])
+def test_f64_every_sort_configuration_matches_oracle(ctx, strategy, digits, tile):
+    """Each sort strategy (onesweep, reduce-then-scan, hybrid) and digit
+    width gives the same tree: the sort is stable by primitive id."""
+    print("f64 sort strategy=%d digits=%d tile=%d" % (strategy, digits, tile), flush=True)
+    boxes = _scene("duplicates", 250007, 77)
+    want, _, _ = oracle.build_f64(boxes, 1)
+    got = ctx.build_aabbs_f64(boxes, 1, sort_strategy=strategy, sort_digit_bits=digits,
+                              sort_tile=tile)
+    assert got.tobytes() == want.tobytes()
+
+
+def test_f64_tuner_times_candidates_on_a_fresh_context():
+    c = lbvh_amd.Context(0)
+    boxes = _scene("clustered", 150001, 78)
+    want, _, _ = oracle.build_f64(boxes, 0)
+    assert c.build_aabbs_f64(boxes, 0).tobytes() == want.tobytes()
+    log = c.tune_log()
+    assert len(log) >= 4 and sum(r["picked"] for r in log) >= 1
+    assert all(r["key_bits"] == 60 for r in log)
+    c.close()
+
+
 def test_f64_large_matches_oracle(ctx):
     n = 4_000_037
     print("f64 build n=%d clustered" % n, flush=True)

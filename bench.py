@@ -59,13 +59,14 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU-baseline sample")
-    ap.add_argument("--workload", choices=["single", "batched", "serving", "large"],
+    ap.add_argument("--workload", choices=["single", "batched", "serving", "large", "f64"],
                     default="single",
                     help="single: the 1M-triangle scene (default); batched: 4,096 uneven "
                          "scenes built in one segmented pass against the plain per-scene path; "
                          "serving: an open-loop request stream through the request server "
                          "against the plain path one request at a time; large: 16,777,216 "
-                         "triangles, per-phase 30/63-bit builds and a refit")
+                         "triangles, per-phase 30/63-bit builds and a refit; f64: "
+                         "builder<double> over the 1M scene in FP64, and 16,777,216 boxes")
     ap.add_argument("--scenes", type=int, default=4096, help="batched: number of scenes")
     ap.add_argument("--requests", type=int, default=160, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=1000.0,
@@ -574,6 +575,136 @@ def bench_large(args):
     print(json.dumps(res), flush=True)
 
 
+# Algorithmic bytes per primitive of the FP64 build (builder<double>, 60-bit
+# keys, reference leaves): bounds 48, encode 48 + 8, sort 8 + 8 digit passes
+# x 16 (key only), tree 8 + 16 + 4 + 8 (codes, left/right, range end,
+# parent links), fit 48 + 48 + 8 (leaf box, node box, links).
+BUILD_BYTES_PER_PRIM_F64 = 48 + 56 + (8 + 8 * 16) + 36 + 104
+
+
+def _tri96(v):
+    """triangle<double> records {p0, e0 = a - b, e1 = c - a, normal} of FP64
+    vertex triples."""
+    import numpy as np
+    v = np.asarray(v, dtype=np.float64).reshape(-1, 9)
+    a, b, c = v[:, 0:3], v[:, 3:6], v[:, 6:9]
+    e0 = a - b
+    e1 = c - a
+    return np.ascontiguousarray(np.concatenate([a, e0, e1, np.cross(e0, e1)], axis=1))
+
+
+def _boxes48(tri96):
+    """triangle_aabb_converter<double>: min / max of p0, p0 - e0, p0 + e1."""
+    import numpy as np
+    p0 = tri96[:, 0:3]
+    p1 = p0 - tri96[:, 3:6]
+    p2 = p0 + tri96[:, 6:9]
+    return np.ascontiguousarray(np.concatenate(
+        [np.minimum(np.minimum(p0, p1), p2), np.maximum(np.maximum(p0, p1), p2)], axis=1))
+
+
+def bench_f64(args):
+    """builder<double>: the 1,048,576-triangle clustered scene of the default
+    workload as triangle<double> (vertices widened to FP64), converter boxes
+    built on the device with u64 codes into node<double>; then 16,777,216
+    uniform FP64 boxes.  Reference leaf order (builder<double>'s bytes).
+    Inputs resident in HBM; steps = timed builds per size."""
+    import numpy as np
+
+    import lbvh_amd
+    from lbvh_amd import hip
+    from lbvh_amd import workload as W
+
+    log("bench: FP64 scenes (1,048,576 clustered triangles, seed 2; 16,777,216 uniform boxes)")
+    v, _ = W.scene_clustered(2, N_PRIMS)
+    tri96 = _tri96(v)
+    del v
+    rng = np.random.default_rng(1)
+    lo = rng.random((1 << 24, 3))
+    big = np.concatenate([lo, lo + rng.random((1 << 24, 3)) * 5e-4], axis=1)
+    del lo
+    ctx = lbvh_amd.Context(0)
+    out = {}
+    for name, boxes in (("1m", _boxes48(tri96)), ("16m", big)):
+        n = boxes.shape[0]
+        d = hip.DeviceArray.from_host(boxes)
+        nodes = hip.DeviceArray((n - 1,), lbvh_amd.NODE64_DTYPE)
+        ctx.build_aabbs_f64(d, 0, out=nodes)  # tunes the sort for this size
+        for _ in range(args.warmup):
+            ctx.build_aabbs_f64(d, 0, out=nodes)
+        ctx.synchronize()
+        ctx.set_timing(2)
+        for _ in range(args.steps):
+            ctx.build_aabbs_f64(d, 0, out=nodes)
+        ctx.synchronize()
+        tot, calls = ctx.phase_stats(1)
+        ctx.set_timing(1)
+        for _ in range(3):
+            ctx.build_aabbs_f64(d, 0, out=nodes)
+        ctx.synchronize()
+        ph, _ = ctx.phase_stats(1)
+        ctx.set_timing(0)
+        build_ms = sum(tot) / calls
+        scale = build_ms / sum(ph)
+        out[name] = {"n_prims": n, "build_ms": round(build_ms, 4),
+                     "mprims_s": round(n / (build_ms * 1e-3) / 1e6, 2),
+                     "phases_ms": {"encode (bounds+codes)": round(ph[0] * scale, 4),
+                                   "sort": round(ph[1] * scale, 4),
+                                   "tree+fit": round((ph[2] + ph[3]) * scale, 4)}}
+        d.close()
+        nodes.close()
+        log("bench: FP64 build of %d boxes %.3f ms" % (n, build_ms))
+    ctx.close()
+    del big
+    one = out["1m"]
+    achieved = BUILD_BYTES_PER_PRIM_F64 * N_PRIMS / (one["build_ms"] * 1e-3) / 1e9
+    res = {
+        "metric": "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain",
+        "value": one["mprims_s"],
+        "unit": "Mprims/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": one["build_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "builder<double>: 1,048,576 clustered triangles (seed 2) as "
+                               "triangle<double>, converter boxes in FP64, reference leaf order; "
+                               "also 16,777,216 uniform FP64 boxes (seed 1)",
+                   "n_prims": N_PRIMS},
+        "roofline": {"bound": "hbm", "scope": "whole build (all kernels of one build)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_prim": BUILD_BYTES_PER_PRIM_F64},
+        "build_1m": one,
+        "build_16m": out["16m"],
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        fast = oracle.ref_available(fast=True)
+        if fast or oracle.ref_available():
+            threads = oracle.ref(fast).ref_hardware_concurrency() or 1
+            times = []
+            t_end = time.time() + args.cpu_seconds * 0.6
+            while time.time() < t_end or len(times) < 2:
+                t0 = time.perf_counter()
+                oracle.ref_build_f64(tri96, "triangles", 0, threads=threads, fast=fast)
+                times.append(time.perf_counter() - t0)
+                if len(times) >= 6:
+                    break
+            build_s = sorted(times)[len(times) // 2]
+            res["cpu_baseline"] = {
+                "value": round(N_PRIMS / build_s / 1e6, 3), "unit": "Mprims/s",
+                "cores": int(threads), "kind": "reference",
+                "flags": "-O3 -ffast-math" if fast else "-O2 -ffp-contract=off",
+                "sample": "%d builds of the 1,048,576-triangle scene with lbvh::builder<double> "
+                          "(median %.1f ms)" % (len(times), build_s * 1e3)}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -587,6 +718,8 @@ def main():
         return bench_serving(args)
     if args.workload == "large":
         return bench_large(args)
+    if args.workload == "f64":
+        return bench_f64(args)
     import numpy as np
 
     import lbvh_amd

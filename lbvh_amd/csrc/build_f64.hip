@@ -27,6 +27,8 @@ namespace lbvh_dev {
 
 constexpr int kF64Block = 256;
 constexpr uint32_t kF64BoundsBlocks = 1024;
+constexpr uint32_t kF64EncodeBlocks = 512;
+constexpr int kF64Unroll = 4;
 constexpr uint64_t kLeafBit64 = 0x8000000000000000ull;
 
 // libstdc++ std::min / std::max (the vector min/max behind union_of).
@@ -117,13 +119,25 @@ k_f64_bounds(const double* __restrict__ boxes, uint64_t n, double* __restrict__ 
     hi[k] = -__builtin_inf();
   }
   const uint64_t stride = uint64_t(gridDim.x) * kF64Block;
-  for (uint64_t i = uint64_t(blockIdx.x) * kF64Block + threadIdx.x; i < n; i += stride) {
-    double c[3];
-    centre64(load_box64(boxes, i), c);
+  for (uint64_t i0 = uint64_t(blockIdx.x) * kF64Block + threadIdx.x; i0 < n;
+       i0 += stride * kF64Unroll) {
+    // all loads in flight together (past the end: the last box again, which
+    // changes no minimum or maximum)
+    Box64 b[kF64Unroll];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      lo[k] = std_min_d(lo[k], c[k]);
-      hi[k] = std_max_d(hi[k], c[k]);
+    for (int u = 0; u < kF64Unroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      b[u] = load_box64(boxes, i < n ? i : n - 1);
+    }
+#pragma unroll
+    for (int u = 0; u < kF64Unroll; u++) {
+      double c[3];
+      centre64(b[u], c);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        lo[k] = std_min_d(lo[k], c[k]);
+        hi[k] = std_max_d(hi[k], c[k]);
+      }
     }
   }
   block_bounds(lo, hi, s_part);
@@ -190,19 +204,34 @@ k_f64_encode(const double* __restrict__ boxes, uint64_t n, const double* __restr
   __syncthreads();
   const uint64_t mask = uint64_t(radix - 1);
   const uint64_t stride = uint64_t(gridDim.x) * kF64Block;
-  for (uint64_t i = uint64_t(blockIdx.x) * kF64Block + threadIdx.x; i < n; i += stride) {
-    double c[3];
-    centre64(load_box64(boxes, i), c);
-    uint64_t q[3];
+  for (uint64_t i0 = uint64_t(blockIdx.x) * kF64Block + threadIdx.x; i0 < n;
+       i0 += stride * kF64Unroll) {
+    Box64 b[kF64Unroll];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      double d = c[k] - bmin[k];
-      q[k] = uint64_t(d * scale[k]);
+    for (int u = 0; u < kF64Unroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      b[u] = load_box64(boxes, i < n ? i : n - 1);
     }
-    const uint64_t code = (expand20_d(q[0]) << 2) | (expand20_d(q[1]) << 1) | expand20_d(q[2]);
-    codes[i] = code;
-    for (int p = first_pass; p < passes; p++)
-      atomicAdd(&s_hist[(p - first_pass) * radix + int((code >> (p * digit_bits)) & mask)], 1u);
+#pragma unroll
+    for (int u = 0; u < kF64Unroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      if (i >= n)
+        continue;
+      double c[3];
+      centre64(b[u], c);
+      uint64_t q[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        double d = c[k] - bmin[k];
+        q[k] = uint64_t(d * scale[k]);
+      }
+      const uint64_t code =
+        (expand20_d(q[0]) << 2) | (expand20_d(q[1]) << 1) | expand20_d(q[2]);
+      codes[i] = code;
+      for (int p = first_pass; p < passes; p++)
+        atomicAdd(&s_hist[(p - first_pass) * radix + int((code >> (p * digit_bits)) & mask)],
+                  1u);
+    }
   }
   __syncthreads();
   for (int j = threadIdx.x; j < words; j += kF64Block) {
@@ -462,10 +491,13 @@ k_f64_fit_top(const double* __restrict__ boxes, uint64_t n, const uint32_t* __re
     for (;;) {
       const uint32_t p = link & 0x7fffffffu;
       const bool right = (link >> 31) != 0;
+      // the sibling and the next parent link do not depend on the arrival
+      // count: their loads complete under the drain below
+      const uint64_t sib = node_words[size_t(p) * 8 + (right ? 6 : 7)];
+      const uint32_t next = parent[p];
       vmem_drain();
       if (atomic_add_agent(&flags[p], 1u) == 0u)
         break;
-      const uint64_t sib = node_words[size_t(p) * 8 + (right ? 6 : 7)];
       Box64 ob;
       if (sib & kLeafBit64) {
         ob = load_box64(boxes, sib & ~kLeafBit64);
@@ -487,7 +519,7 @@ k_f64_fit_top(const double* __restrict__ boxes, uint64_t n, const uint32_t* __re
         coherent_store_u64(st + c, (uint64_t)__double_as_longlong(box.lo[c]));
         coherent_store_u64(st + 3 + c, (uint64_t)__double_as_longlong(box.hi[c]));
       }
-      link = parent[p];
+      link = next;
     }
   }
 }
@@ -501,7 +533,7 @@ using namespace lbvh_dev;
 uint32_t
 f64_bounds_blocks(uint64_t n)
 {
-  uint64_t b = (n + kF64Block - 1) / kF64Block;
+  uint64_t b = (n + kF64Block * kF64Unroll - 1) / (kF64Block * kF64Unroll);
   return uint32_t(b < kF64BoundsBlocks ? (b ? b : 1) : kF64BoundsBlocks);
 }
 
@@ -522,8 +554,9 @@ launch_f64_encode(const double* boxes, uint64_t n, const double* bounds, uint64_
   const size_t lds = (size_t(4) << digit_bits) * size_t(passes - first_pass);
   if (first_pass < 0 || first_pass >= passes || lds > 64 * 1024)
     return hipErrorInvalidValue;
-  uint64_t b = (n + kF64Block - 1) / kF64Block;
-  const uint32_t g = uint32_t(b < 2048 ? (b ? b : 1) : 2048);
+  // few workgroups: each flushes up to passes x radix histogram words
+  uint64_t b = (n + kF64Block * kF64Unroll - 1) / (kF64Block * kF64Unroll);
+  const uint32_t g = uint32_t(b < kF64EncodeBlocks ? (b ? b : 1) : kF64EncodeBlocks);
   // the kernel counts pass p at (p - first_pass) * radix of its pointer
   hipLaunchKernelGGL(k_f64_encode, dim3(g), dim3(kF64Block), lds, s, boxes, n, bounds, codes,
                      hist + size_t(first_pass) * (size_t(1) << digit_bits), digit_bits,

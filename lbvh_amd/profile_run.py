@@ -8,9 +8,9 @@
                                             # codes, 3 refits after a perturbation
   python3 -m lbvh_amd.profile_run batched   # 1,024 scenes log-uniform [64, 262144]
                                             # (seed 4): 3 rebuilds + 3 batched traces
-  python3 -m lbvh_amd.profile_run f64       # builder<double>: 16,777,216 uniform
-                                            # FP64 boxes (seed 1), 5 builds each with
-                                            # reference and primitive-id leaves
+  python3 -m lbvh_amd.profile_run f64 [n]   # builder<double>: n (default 16,777,216)
+                                            # uniform FP64 boxes (seed 1), 5 builds each
+                                            # with reference and primitive-id leaves
   python3 -m lbvh_amd.profile_run copy      # calibration: an elementwise kernel that
                                             # reads 512 MiB and writes 512 MiB (torch),
                                             # 5 times
@@ -103,7 +103,7 @@ def f64():
     import numpy as np
     import lbvh_amd
     from lbvh_amd import hip
-    n = 1 << 24
+    n = int(sys.argv[2]) if len(sys.argv) > 2 else 1 << 24
     rng = np.random.default_rng(1)
     lo = rng.random((n, 3))
     boxes = np.concatenate([lo, lo + rng.random((n, 3)) * 5e-4], axis=1)

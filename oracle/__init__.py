@@ -299,15 +299,16 @@ def ref_build(tri, code_bits=30, leaf_order=LEAF_REFERENCE, threads=None, phases
     return nodes, sc, bounds
 
 
-def ref_build_f64(prims, kind="triangles", leaf_order=LEAF_REFERENCE, threads=None):
+def ref_build_f64(prims, kind="triangles", leaf_order=LEAF_REFERENCE, threads=None, fast=False):
     """lbvh::builder<double> as shipped over triangle<double> [n, 12] or
     aabb<double> [n, 6] records.  Returns the node<double> array."""
     w = 12 if kind == "triangles" else 6
     prims = _f64(prims, w)
     n = prims.shape[0]
     nodes = np.zeros(n - 1, dtype=NODE64_DTYPE)
-    th = threads or ref().ref_hardware_concurrency() or 1
-    fn = ref().ref_build_f64_triangles if kind == "triangles" else ref().ref_build_f64_aabbs
+    th = threads or ref(fast).ref_hardware_concurrency() or 1
+    L = ref(fast)
+    fn = L.ref_build_f64_triangles if kind == "triangles" else L.ref_build_f64_aabbs
     if fn(_p(prims, F64P), n, leaf_order, th, nodes.ctypes.data) != 0:
         raise ValueError("ref_build_f64 failed")
     return nodes

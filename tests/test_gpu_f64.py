@@ -58,7 +58,8 @@ def test_f64_build_matches_golden(ctx, case, leaf):
     assert np.array_equal(got.view(np.uint64).reshape(-1, 8), want[leaf])
 
 
-@pytest.mark.parametrize("n", [2, 3, 4, 255, 256, 257, 1000, 4097, 65537, 300001])
+@pytest.mark.parametrize("n", [2, 3, 4, 255, 256, 257, 511, 512, 513, 1000, 1025, 2049, 4097, 65537,
+                               300001])
 @pytest.mark.parametrize("dist", ["uniform", "clustered", "duplicates", "flat", "huge", "tiny"])
 @pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
 def test_f64_build_matches_oracle(ctx, n, dist, leaf):

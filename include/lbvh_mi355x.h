@@ -201,9 +201,9 @@ void lbvh_bvh_destroy(lbvh_bvh* bvh);
  * all box arithmetic is FP64.  Output: n - 1 lbvh::node<double> records of
  * 64 bytes {aabb<double> box; u64 left; u64 right}, bit 63 of left/right
  * marks a leaf, root = node 0 (lbvh.h:220-255).  opts->leaf_order and the
- * sort fields apply; code_bits is ignored (the code width follows the
- * scalar type, as in the reference); the sort is tuned per size as for
- * 63-bit FP32 builds.  n in [2, 2^31 - 1].  The device variant borrows
+ * sort fields apply; code_bits and keep_codes are ignored (the code width
+ * follows the scalar type, as in the reference); the sort is tuned per size
+ * as for 63-bit FP32 builds.  n in [2, 2^31 - 1].  The device variant borrows
  * d_boxes48 and writes d_nodes_out, both device memory aligned to 16 bytes,
  * and returns after queuing; the host variant copies in and out and
  * returns when the nodes are in nodes_out. */

@@ -7,7 +7,8 @@
 // Writes: u64 n, n triangles (48 B each), (n-1) nodes (32 B each),
 //         u64 m, m rays (32 B), m u32 prim, m f32 t (host traverser),
 //         then the 2 nodes of the 3-sphere tree, then u64 n, n FP64
-//         triangles (96 B each) and their builder<double> nodes (64 B each).
+//         triangles (96 B each) and their builder<double> nodes (64 B each)
+//         with reference leaves, then with primitive-id leaves.
 // Exits non-zero if the device-tree closest hits differ from the host ones.
 #include <lbvh_mi355x.hpp>
 
@@ -175,6 +176,11 @@ main(int argc, char** argv)
     hits64 += h ? 1 : 0;
   }
 
+  // the same FP64 scene with primitive-id leaves through the session options
+  lbvh::gpu::default_options().leaf_order = LBVH_LEAF_PRIMITIVE_ID;
+  auto bvh64_pid = builder64(tris64.data(), tris64.size(), lbvh::triangle_aabb_converter<double>());
+  lbvh::gpu::default_options().leaf_order = LBVH_LEAF_REFERENCE;
+
   FILE* f = std::fopen(argv[1], "wb");
   if (!f)
     return 2;
@@ -190,6 +196,7 @@ main(int argc, char** argv)
   std::fwrite(&nn, 8, 1, f);
   std::fwrite(tris64.data(), 96, n, f);
   std::fwrite(bvh64.data(), 64, n - 1, f);
+  std::fwrite(bvh64_pid.data(), 64, n - 1, f);
   std::fclose(f);
   std::printf("dropin ok: %zu triangles, %zu rays, %zu hits; builder<double>: %zu of 256 hits\n",
               n, m,

@@ -426,9 +426,14 @@ def test_cpp_dropin_conformance(tmp_path):
     tri64 = np.frombuffer(raw[off:off + 96 * n64], np.float64).reshape(n64, 12)
     off += 96 * n64
     nodes64 = np.frombuffer(raw[off:off + 64 * (n64 - 1)], oracle.NODE64_DTYPE)
+    off += 64 * (n64 - 1)
+    nodes64_pid = np.frombuffer(raw[off:off + 64 * (n64 - 1)], oracle.NODE64_DTYPE)
     assert off + 64 * (n64 - 1) == len(raw)
-    want, _, _ = oracle.build_f64(oracle.triangle_aabb_f64(tri64), 0)
+    boxes64 = oracle.triangle_aabb_f64(tri64)
+    want, _, _ = oracle.build_f64(boxes64, 0)
     assert nodes64.tobytes() == want.tobytes()
+    want_pid, _, _ = oracle.build_f64(boxes64, 1)
+    assert nodes64_pid.tobytes() == want_pid.tobytes()
 
 
 def test_aabb_build_equals_triangle_build(ctx):

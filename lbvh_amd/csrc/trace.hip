@@ -17,6 +17,9 @@ using lbvh_eng::TraceJob;
 
 constexpr int kTraceBlock = 256;
 constexpr int kStack = 128;
+// Stack entries below this depth live in LDS (16 KB per workgroup keeps
+// eight workgroups per CU); deeper ones in the private array.
+constexpr int kStackLds = 8;
 // Dispatches of fewer rays use the PF variant of k_trace (256 CUs x 32 waves
 // x 64 lanes: fewer rays leave the device partly idle).
 constexpr uint64_t kTracePrefetchBelow = 256ull * 32ull * 64ull;
@@ -303,12 +306,23 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
 
   // The traversal stack: capacity 128 with pushes dropped when full, as the
   // reference's.  Entries: (node index, tmin bits).  Entries 0 .. sp - 2 are
-  // in a private array (the only thing in private memory) and the top one is
-  // kept in registers: a pop hands it over at once and leaves the register
-  // stale (stale: the top is still spill[sp - 1]).  The reload is issued
-  // after the popped node's fetch, so the two latencies overlap; a push onto
-  // a stale top needs no reload at all.
-  uint2 spill[kStack];
+  // in LDS (the first kStackLds, one column per thread) and a private array
+  // (the rest, the only thing in private memory); the top one is kept in
+  // registers: a pop hands it over at once and leaves the register stale
+  // (stale: the top is still entry sp - 1).  The reload is issued after the
+  // popped node's fetch, so the two latencies overlap; a push onto a stale
+  // top needs no reload at all.
+  __shared__ uint2 s_stk[kStackLds * kTraceBlock];
+  uint2 spill[kStack - kStackLds];
+  auto stk_store = [&](uint32_t e, uint2 v) {
+    if (e < uint32_t(kStackLds))
+      s_stk[e * kTraceBlock + threadIdx.x] = v;
+    else
+      spill[e - kStackLds] = v;
+  };
+  auto stk_load = [&](uint32_t e) -> uint2 {
+    return e < uint32_t(kStackLds) ? s_stk[e * kTraceBlock + threadIdx.x] : spill[e - kStackLds];
+  };
   uint2 top = make_uint2(0u, 0u);
   uint32_t sp = 0;
   bool stale = false;
@@ -319,7 +333,7 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     if (sp >= kStack)
       return;
     if (sp > 0 && !stale)
-      spill[sp - 1] = top;
+      stk_store(sp - 1, top);
     stale = false;
     top = make_uint2(node, __float_as_uint(tn));
     sp++;
@@ -336,7 +350,7 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
   };
   auto pop = [&]() {
     if (stale)
-      top = spill[sp - 1];
+      top = stk_load(sp - 1);
     const uint2 e = top;
     sp--;
     stale = sp > 0;
@@ -385,7 +399,7 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
       w3 = wp[3];
     }
     if (stale) { // reload the top while the node is in flight
-      top = spill[sp - 1];
+      top = stk_load(sp - 1);
       stale = false;
     }
     const uint32_t lr[2] = { __float_as_uint(w3.x), __float_as_uint(w3.y) };

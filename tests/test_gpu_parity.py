@@ -63,6 +63,33 @@ def test_trace_matches_golden(ctx, case, leaf, mode):
     bvh.destroy()
 
 
+@pytest.mark.parametrize("nrays", [4096, 600000], ids=["small_dispatch", "full_dispatch"])
+@pytest.mark.parametrize("mode", [0, 1], ids=["first", "lowid"])
+def test_trace_deep_stacks_match_oracle(ctx, nrays, mode):
+    """20,000 large, mutually overlapping triangles: most rays hit both
+    children at many levels, so traversal stacks run far past the entries
+    kept in registers and LDS into the private array (hundreds of pops per
+    ray).  Both traversal variants (small and full dispatches) against the
+    oracle, bit for bit."""
+    print("deep-stack trace nrays=%d mode=%d" % (nrays, mode), flush=True)
+    rng = np.random.default_rng(3)
+    n = 20000
+    c = rng.random((n, 3)) * 0.2 + 0.4
+    v = np.concatenate([c + rng.normal(size=(n, 3)) * 0.3 for _ in range(3)], axis=1)
+    tri = W.make_triangles(v.astype(np.float32))
+    rays = W.box_rays(5, np.full(3, -0.5, np.float32), np.full(3, 1.5, np.float32), nrays)
+    bvh = ctx.build_triangles(tri, code_bits=63, leaf_order=1)
+    nodes, _, _, _ = oracle.build(tri, 63, 1)
+    assert bvh.nodes().tobytes() == nodes.tobytes()
+    p, t, uv = ctx.trace(bvh, rays, mode)
+    op, ot, ouv = oracle.trace(nodes, tri, rays, mode)
+    assert np.array_equal(p, op)
+    assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+    hit = p != 0xFFFFFFFF
+    assert np.array_equal(uv[hit].view(np.uint32), ouv[hit].view(np.uint32))
+    bvh.destroy()
+
+
 @pytest.mark.parametrize("n", [2, 3, 5, 33, 4097, 70001])
 @pytest.mark.parametrize("bits,leaf", VARIANTS, ids=VIDS)
 def test_build_matches_oracle_ragged(ctx, n, bits, leaf):

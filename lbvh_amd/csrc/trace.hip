@@ -20,9 +20,6 @@ constexpr int kStack = 128;
 // Stack entries below this depth live in LDS (16 KB per workgroup keeps
 // eight workgroups per CU); deeper ones in the private array.
 constexpr int kStackLds = 8;
-// Dispatches of fewer rays use the PF variant of k_trace (256 CUs x 32 waves
-// x 64 lanes: fewer rays leave the device partly idle).
-constexpr uint64_t kTracePrefetchBelow = 256ull * 32ull * 64ull;
 
 struct Ray
 {
@@ -206,8 +203,9 @@ job_of(const TraceJob* __restrict__ jobs, uint32_t njobs, uint64_t i)
 
 // PF: the record of a pushed child is fetched at the push (beside the near
 // child's fetch) and kept with the stack top, so popping that entry needs no
-// fetch.  It costs 16 registers, so it is used for dispatches too small to
-// fill the device, whose time is the latency of their longest rays.
+// fetch.  It costs 16 registers.  Since the stack's first entries moved to
+// LDS the plain form is faster at every dispatch size measured (16K to 2M
+// rays), so only PF = false is launched.
 template<int MODE, int LAYOUT, bool PF>
 __global__ __launch_bounds__(kTraceBlock) void
 k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
@@ -503,14 +501,8 @@ trace_dispatch(int mode, const void* nodes, const float* tris, uint64_t n, const
     return hipSuccess;
   const uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
   const WideNode* nd = static_cast<const WideNode*>(nodes);
-  // a dispatch that cannot fill the device is bound by its longest rays:
-  // the variant that fetches pushed records early
-  const bool pf = nrays < kTracePrefetchBelow;
 #define LBVH_TM(M)                                                                                \
-  (pf ? trace_mode<M, true>(g, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, nrays, prim,  \
-                            t, uv)                                                               \
-      : trace_mode<M, false>(g, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, nrays, prim, \
-                             t, uv))
+  trace_mode<M, false>(g, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, nrays, prim, t, uv)
   switch (mode) {
     case 0:
       LBVH_TM(0);

@@ -106,6 +106,8 @@ typedef struct lbvh_tune_record
   uint32_t sort_strategy;
   uint32_t picked;   /* the configuration kept for this bucket */
   float ms;          /* device time of the whole build with it */
+  uint32_t scalar_bits; /* 32: builder<float> builds, 64: builder<double>
+                           builds (tuned in buckets of their own) */
 } lbvh_tune_record;
 
 /* -- status / options ---------------------------------------------------- */

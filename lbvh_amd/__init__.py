@@ -65,7 +65,8 @@ class TuneRecord(ctypes.Structure):
     _fields_ = [("batch", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
                 ("log2_n", ctypes.c_uint32), ("sort_digit_bits", ctypes.c_uint32),
                 ("sort_tile", ctypes.c_uint32), ("sort_strategy", ctypes.c_uint32),
-                ("picked", ctypes.c_uint32), ("ms", ctypes.c_float)]
+                ("picked", ctypes.c_uint32), ("ms", ctypes.c_float),
+                ("scalar_bits", ctypes.c_uint32)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -229,6 +230,51 @@ def _numel(x):
     return int(x.numel())
 
 
+def _dev_dtype(x):
+    """The numpy dtype of a device array (DeviceArray or torch tensor)."""
+    if _is_torch(x):
+        try:
+            return np.dtype(str(x.dtype).replace("torch.", ""))
+        except TypeError:
+            return None
+    return np.dtype(x.dtype)
+
+
+def _dev_nbytes(x):
+    if _is_torch(x):
+        return int(x.numel()) * int(x.element_size())
+    return int(x.nbytes)
+
+
+def _check_dev(x, what, dtypes, count=None, at_least=None, width=1):
+    """Rejects a device array whose dtype is not one of `dtypes`, whose
+    element count is not a multiple of `width`, or that holds other than
+    `count` (or fewer than `at_least`) elements, before any C call: the
+    engine reads and writes device memory by the counts it is given."""
+    dt = _dev_dtype(x)
+    if dt is None or all(dt != np.dtype(d) for d in dtypes):
+        raise ValueError("%s: device dtype %s, expected %s"
+                         % (what, dt, " or ".join(np.dtype(d).name for d in dtypes)))
+    n = _numel(x)
+    if n % width:
+        raise ValueError("%s: %d elements is not a multiple of %d" % (what, n, width))
+    if count is not None and n != count:
+        raise ValueError("%s: %d elements, expected %d" % (what, n, count))
+    if at_least is not None and n < at_least:
+        raise ValueError("%s: %d elements, expected at least %d" % (what, n, at_least))
+    return n // width
+
+
+def _check_dev_out(out, m, with_uv):
+    """Device outputs (prim, t, uv) of m rays: u32 / f32 / 2 x f32 each."""
+    prim, t, uv = out
+    _check_dev(prim, "prim", (np.uint32, np.int32), at_least=m)
+    _check_dev(t, "t", (np.float32,), at_least=m)
+    if uv is not None:
+        _check_dev(uv, "uv", (np.float32,), at_least=2 * m)
+    return prim, t, uv
+
+
 def _dptr(x):
     return ctypes.c_void_p(int(x.data_ptr()))
 
@@ -348,6 +394,7 @@ class Batch:
         if tri is not None:
             if not _is_device(tri):
                 raise ValueError("rebuild takes device triangles")
+            _check_dev(tri, "triangles", (np.float32,), count=12 * self.n)
             p = _dptr(tri)
             self._keep = tri
         self.ctx._check(lib().lbvh_batch_rebuild(self.ctx.handle, self.handle, p, int(graph)))
@@ -358,6 +405,7 @@ class Batch:
         if tri is not None:
             if not _is_device(tri):
                 raise ValueError("refit takes device triangles")
+            _check_dev(tri, "triangles", (np.float32,), count=12 * self.n)
             p = _dptr(tri)
             self._keep = tri
         self.ctx._check(lib().lbvh_batch_refit(self.ctx.handle, self.handle, p, int(graph)))
@@ -369,14 +417,16 @@ class Batch:
         if _is_device(rays):
             if not _is_device(ray_offsets):
                 raise ValueError("device rays need device ray offsets")
-            m = _numel(rays) // 8
+            m = _check_dev(rays, "rays", (np.float32,), width=8)
+            _check_dev(ray_offsets, "ray offsets", (np.uint64, np.int64),
+                       count=self.n_scenes + 1)
             if out is None:
                 from .hip import DeviceArray
                 prim = DeviceArray(m, np.uint32)
                 t = DeviceArray(m, np.float32)
                 uv = DeviceArray((m, 2), np.float32) if with_uv else None
             else:
-                prim, t, uv = out
+                prim, t, uv = _check_dev_out(out, m, with_uv)
             self.ctx._check(lib().lbvh_batch_trace_closest_device(
                 self.ctx.handle, self.handle, _dptr(rays), _dptr(ray_offsets), m, mode,
                 _dptr(prim), _dptr(t), _dptr(uv) if uv is not None else None))
@@ -437,14 +487,16 @@ class Server:
         req.scene = scene
         keep = []
         if _is_device(rays):
-            m = _numel(rays) // 8
-            if tri is not None and not _is_device(tri):
-                raise ValueError("a device request needs device triangles")
+            m = _check_dev(rays, "rays", (np.float32,), width=8)
+            if tri is not None:
+                if not _is_device(tri):
+                    raise ValueError("a device request needs device triangles")
+                _check_dev(tri, "triangles", (np.float32,), width=12)
             if out is None:
                 from .hip import DeviceArray
                 out = (DeviceArray(m, np.uint32), DeviceArray(m, np.float32),
                        DeviceArray((m, 2), np.float32) if with_uv else None)
-            prim, t, uv = out
+            prim, t, uv = _check_dev_out(out, m, True)
             req.flags = REQUEST_DEVICE
             req.rays32 = _dptr(rays).value if m else None
             req.prim = _dptr(prim).value
@@ -592,7 +644,8 @@ class Context:
         return [{"batch": bool(r.batch), "key_bits": r.key_bits, "log2_n": r.log2_n,
                  "digit_bits": r.sort_digit_bits, "tile": r.sort_tile,
                  "strategy": SORT_NAMES.get(r.sort_strategy, "?"),
-                 "picked": bool(r.picked), "ms": round(r.ms, 4)} for r in buf[:n]]
+                 "picked": bool(r.picked), "ms": round(r.ms, 4),
+                 "scalar_bits": r.scalar_bits} for r in buf[:n]]
 
     def workspace_bytes(self):
         return int(lib().lbvh_ctx_workspace_bytes(self.handle))
@@ -612,7 +665,7 @@ class Context:
         opts = _make_opts(code_bits, leaf_order, keep_codes, sdb, stile, sstrat)
         out = ctypes.c_void_p()
         if _is_device(prims):
-            n = _numel(prims) // width
+            n = _check_dev(prims, "primitives", (np.float32,), width=width)
             self._check(getattr(lib(), fn + "_device")(self.handle, _dptr(prims), n,
                                                        ctypes.byref(opts), ctypes.byref(out)))
             return Bvh(self, out, n, code_bits, leaf_order, keep=prims)
@@ -631,7 +684,7 @@ class Context:
         the input when None) and returns it without synchronizing."""
         opts = _make_opts(63, leaf_order, False, sort_digit_bits, sort_tile, sort_strategy)
         if _is_device(boxes):
-            n = _numel(boxes) // 6
+            n = _check_dev(boxes, "boxes", (np.float64,), width=6)
             if out is None:
                 if _is_torch(boxes):
                     import torch
@@ -639,7 +692,7 @@ class Context:
                 else:
                     from .hip import DeviceArray
                     out = DeviceArray((max(n - 1, 1),), NODE64_DTYPE)
-            nbytes = out.numel() * out.element_size() if _is_torch(out) else out.nbytes
+            nbytes = _dev_nbytes(out)
             if nbytes < 64 * max(n - 1, 0):
                 raise ValueError("out holds fewer than 64 (n - 1) bytes")
             self._check(lib().lbvh_build_aabbs_f64_device(self.handle, _dptr(boxes), n,
@@ -661,6 +714,7 @@ class Context:
         nseg = off.shape[0] - 1
         out = ctypes.c_void_p()
         if _is_device(tri):
+            _check_dev(tri, "triangles", (np.float32,), count=12 * int(off[-1]))
             self._check(lib().lbvh_build_batch_device(self.handle, _dptr(tri), _hptr(off), nseg,
                                                       ctypes.byref(opts), ctypes.byref(out)))
             return Batch(self, out, off, leaf_order, keep=tri)
@@ -673,6 +727,7 @@ class Context:
 
     def refit_triangles(self, bvh, tri):
         if _is_device(tri):
+            _check_dev(tri, "triangles", (np.float32,), count=12 * bvh.n)
             self._check(lib().lbvh_refit_triangles_device(self.handle, bvh.handle, _dptr(tri)))
             bvh._keep = tri
             return
@@ -688,14 +743,14 @@ class Context:
         device outputs, queued on the context stream (synchronize before
         reading)."""
         if _is_device(rays):
-            m = _numel(rays) // 8
+            m = _check_dev(rays, "rays", (np.float32,), width=8)
             if out is None:
                 from .hip import DeviceArray
                 prim = DeviceArray(m, np.uint32)
                 t = DeviceArray(m, np.float32)
                 uv = DeviceArray((m, 2), np.float32) if with_uv else None
             else:
-                prim, t, uv = out
+                prim, t, uv = _check_dev_out(out, m, with_uv)
             self._check(lib().lbvh_trace_closest_device(self.handle, bvh.handle, _dptr(rays), m,
                                                         mode, _dptr(prim), _dptr(t),
                                                         _dptr(uv) if uv is not None else None))

@@ -437,6 +437,8 @@ lbvh_build_batch_device(lbvh_ctx* c, const float* d_tri, const uint64_t* prim_of
   if (!c || !out || !d_tri)
     return fail(c, LBVH_E_INVALID, "null argument");
   *out = nullptr;
+  if (misaligned(d_tri, 16))
+    return fail(c, LBVH_E_INVALID, "device triangles must be 16-byte aligned");
   lbvh_status st = check_offsets(c, prim_off, n_scenes);
   if (st != LBVH_OK)
     return st;
@@ -489,6 +491,8 @@ update_prims(lbvh_ctx* c, lbvh_batch* b, const float* d_tri)
 {
   if (!d_tri || d_tri == b->d_prims)
     return LBVH_OK;
+  if (misaligned(d_tri, 16))
+    return fail(c, LBVH_E_INVALID, "device triangles must be 16-byte aligned");
   if (b->owns_prims) {
     HIP_TRY(c, hipMemcpyAsync(b->d_prims, d_tri, b->n * 48, hipMemcpyDeviceToDevice, c->stream));
   } else {
@@ -615,6 +619,10 @@ lbvh_batch_trace_closest_device(lbvh_ctx* c, const lbvh_batch* b, const float* d
     return fail(c, LBVH_E_INVALID, "null argument");
   if (mode < 0 || mode > 2)
     return fail(c, LBVH_E_INVALID, "unknown intersector mode");
+  if (misaligned(d_rays, 16) || misaligned(d_ray_off, 8) ||
+      trace_outputs_misaligned(d_prim, d_t, d_uv))
+    return fail(c, LBVH_E_INVALID, "device rays must be 16-byte aligned, ray offsets and uv "
+                                   "8-byte, prim and t 4-byte");
   lbvh_status st = set_device(c);
   if (st != LBVH_OK)
     return st;

@@ -211,9 +211,10 @@ ceil_log2_u64(uint64_t x)
 }
 
 uint64_t
-tune_key(bool batch, int key_bits, uint64_t n)
+tune_key(bool batch, int key_bits, uint64_t n, int scalar_bits)
 {
-  return (uint64_t(batch) << 40) | (uint64_t(key_bits) << 32) | uint64_t(ceil_log2_u64(n));
+  return (uint64_t(scalar_bits == 64) << 48) | (uint64_t(batch) << 40) |
+         (uint64_t(key_bits) << 32) | uint64_t(ceil_log2_u64(n));
 }
 
 bool
@@ -722,6 +723,8 @@ lbvh_build_triangles_device(lbvh_ctx* c, const float* d_tri, uint64_t n,
   if (!c || !out || (!d_tri && n))
     return fail(c, LBVH_E_INVALID, "null argument");
   *out = nullptr;
+  if (misaligned(d_tri, 16))
+    return fail(c, LBVH_E_INVALID, "device triangles must be 16-byte aligned");
   lbvh_status st = set_device(c);
   if (st != LBVH_OK)
     return st;
@@ -735,6 +738,8 @@ lbvh_build_aabbs_device(lbvh_ctx* c, const float* d_boxes, uint64_t n,
   if (!c || !out || (!d_boxes && n))
     return fail(c, LBVH_E_INVALID, "null argument");
   *out = nullptr;
+  if (misaligned(d_boxes, 8))
+    return fail(c, LBVH_E_INVALID, "device boxes must be 8-byte aligned");
   lbvh_status st = set_device(c);
   if (st != LBVH_OK)
     return st;
@@ -831,6 +836,8 @@ lbvh_refit_triangles_device(lbvh_ctx* c, lbvh_bvh* b, const float* d_tri)
     return fail(c, LBVH_E_INVALID, "null argument");
   if (b->prim_kind != 0)
     return fail(c, LBVH_E_INVALID, "refit_triangles on a tree built from boxes");
+  if (misaligned(d_tri, 16))
+    return fail(c, LBVH_E_INVALID, "device triangles must be 16-byte aligned");
   lbvh_status st = set_device(c);
   if (st != LBVH_OK)
     return st;
@@ -912,6 +919,7 @@ lbvh_ctx_tune_log(const lbvh_ctx* c, lbvh_tune_record* out, uint32_t cap)
     out[i].sort_strategy = uint32_t(r.cfg.strategy + 1);
     out[i].picked = r.picked ? 1u : 0u;
     out[i].ms = r.ms;
+    out[i].scalar_bits = uint32_t(r.scalar_bits);
   }
   return n;
 }
@@ -968,6 +976,9 @@ lbvh_trace_closest_device(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, u
     return fail(c, LBVH_E_INVALID, "device traversal needs a triangle tree");
   if (mode < 0 || mode > 2)
     return fail(c, LBVH_E_INVALID, "unknown intersector mode");
+  if (misaligned(d_rays, 16) || trace_outputs_misaligned(d_prim, d_t, d_uv))
+    return fail(c, LBVH_E_INVALID,
+                "device rays must be 16-byte aligned, prim and t 4-byte, uv 8-byte");
   lbvh_status st = set_device(c);
   if (st != LBVH_OK)
     return st;

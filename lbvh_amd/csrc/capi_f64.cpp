@@ -19,9 +19,9 @@ lbvh_status queue_f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n,
                             const lbvh_build_opts& opts, void* d_nodes, const SortConfig& cfg);
 
 // The sort configuration of an FP64 build: explicit options, else the
-// context's tuned choice for 60-bit keys of this size (shared with 63-bit
-// FP32 builds, which sort the same keys), tuning now on this very build when
-// there is none yet.
+// context's tuned choice for FP64 builds of this size (a bucket of their
+// own, apart from 63-bit FP32 builds of the same key width), tuning now on
+// this very build when there is none yet.
 lbvh_status
 choose_sort_f64(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build_opts& opts,
                 void* d_nodes, SortConfig* cfg)
@@ -34,7 +34,7 @@ choose_sort_f64(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build
   *cfg = default_sort();
   if (!c->autotune || n < kTuneMinPrims)
     return LBVH_OK;
-  const uint64_t key = tune_key(false, key_bits, n);
+  const uint64_t key = tune_key(false, key_bits, n, 64);
   auto it = c->tuned.find(key);
   if (it != c->tuned.end()) {
     *cfg = it->second;
@@ -52,7 +52,7 @@ choose_sort_f64(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build
       if ((st = sync_and_check(c)) != LBVH_OK)
         return st;
     }
-    c->tune_log.push_back(TuneRecord{ 0, key_bits, ceil_log2_u64(n), cand, ms, false });
+    c->tune_log.push_back(TuneRecord{ 0, key_bits, ceil_log2_u64(n), cand, ms, false, 64 });
     if (ms >= 0.0f && (best < 0.0f || ms < best)) {
       best = ms;
       *cfg = cand;

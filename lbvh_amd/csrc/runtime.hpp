@@ -48,6 +48,7 @@ struct TuneRecord
   SortConfig cfg;
   float ms; // device time of the whole build with this configuration
   bool picked;
+  int scalar_bits = 32; // 32: builder<float> builds, 64: builder<double> builds
 };
 
 } // namespace lbvh_rt
@@ -102,6 +103,20 @@ struct lbvh_bvh
 namespace lbvh_rt {
 
 lbvh_status fail(lbvh_ctx* c, lbvh_status s, const std::string& msg);
+// Device pointers the kernels read or write with vector loads and stores:
+// triangles and rays need 16-byte alignment, boxes, uv pairs and 64-bit
+// offsets 8, ids and distances 4.
+inline bool
+misaligned(const void* p, uintptr_t align)
+{
+  return (reinterpret_cast<uintptr_t>(p) & (align - 1)) != 0;
+}
+// Checks the device outputs of a traversal (prim, t, uv may be null).
+inline bool
+trace_outputs_misaligned(const void* prim, const void* t, const void* uv)
+{
+  return misaligned(prim, 4) || misaligned(t, 4) || misaligned(uv, 8);
+}
 lbvh_status hip_fail(lbvh_ctx* c, hipError_t e, const char* what);
 lbvh_status ensure(lbvh_ctx* c, DevBuf& b, uint64_t bytes, bool zero_new = false);
 void dev_free(void* p);
@@ -124,7 +139,10 @@ bool same_sort(const SortConfig& a, const SortConfig& b);
 // LDS sort.
 int hybrid_global_digits(int key_bits, int digit_bits, uint64_t n);
 constexpr uint64_t kHybridMeanBucket = 64;
-uint64_t tune_key(bool batch, int key_bits, uint64_t n);
+// FP64 builds have buckets of their own: their pipeline around the sort
+// differs (no encode-written chunk counts), so a pick timed on one kind of
+// build does not carry over to the other.
+uint64_t tune_key(bool batch, int key_bits, uint64_t n, int scalar_bits = 32);
 int ceil_log2_u64(uint64_t x);
 // Device milliseconds of the work `queue` puts on the context stream between
 // its begin_timed()/record() calls (per-call timing forced on meanwhile).

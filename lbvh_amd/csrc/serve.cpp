@@ -442,6 +442,21 @@ start(lbvh_server* s, Req* q, Scene* sc)
     if (q->refit) {
       if (sc->has_trace_ev)
         (void)hipStreamWaitEvent(st, sc->ev_trace, 0);
+      if (!dev && !sc->owns_tri) {
+        // host data for a scene whose triangles are borrowed device memory:
+        // borrowed memory is only read, so the scene takes a buffer of its own
+        float* own = nullptr;
+        if (hipMalloc(&own, ntri * 48) != hipSuccess) {
+          (void)hipGetLastError();
+          return bad(LBVH_E_OOM, "scene allocation failed");
+        }
+        sc->d_tri = own;
+        sc->owns_tri = true;
+        std::lock_guard<std::mutex> g(s->mu);
+        sc->bytes += ntri * 48;
+        s->mem_used += ntri * 48;
+        s->mem_peak = std::max(s->mem_peak, s->mem_used);
+      }
       if (dev && !sc->owns_tri) {
         sc->d_tri = const_cast<float*>(tri_src); // borrow the new triangles
       } else if (hipMemcpyAsync(sc->d_tri, tri_src, ntri * 48,
@@ -759,6 +774,10 @@ lbvh_server_submit(lbvh_server* s, const lbvh_request* r, uint64_t* ticket, uint
   if (r->n_rays && (!r->rays32 || !r->prim || !r->t))
     return LBVH_E_INVALID;
   if (r->scene == 0 && (!r->tri48 || r->n_tri < 2 || r->n_tri >= (uint64_t(1) << 29)))
+    return LBVH_E_INVALID;
+  if ((r->flags & LBVH_REQUEST_DEVICE) &&
+      (misaligned(r->tri48, 16) || misaligned(r->rays32, 16) ||
+       trace_outputs_misaligned(r->prim, r->t, r->uv)))
     return LBVH_E_INVALID;
   auto* q = new (std::nothrow) Req;
   if (!q)

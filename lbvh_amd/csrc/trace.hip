@@ -201,12 +201,7 @@ job_of(const TraceJob* __restrict__ jobs, uint32_t njobs, uint64_t i)
   return lo;
 }
 
-// PF: the record of a pushed child is fetched at the push (beside the near
-// child's fetch) and kept with the stack top, so popping that entry needs no
-// fetch.  It costs 16 registers.  Since the stack's first entries moved to
-// LDS the plain form is faster at every dispatch size measured (16K to 2M
-// rays), so only PF = false is launched.
-template<int MODE, int LAYOUT, bool PF>
+template<int MODE, int LAYOUT>
 __global__ __launch_bounds__(kTraceBlock) void
 k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
         const uint64_t* __restrict__ prim_off, const uint64_t* __restrict__ ray_off,
@@ -324,9 +319,6 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
   uint2 top = make_uint2(0u, 0u);
   uint32_t sp = 0;
   bool stale = false;
-  // PF: the top entry's record, valid when top_rec
-  float4 tr0 = make_float4(0.f, 0.f, 0.f, 0.f), tr1 = tr0, tr2 = tr0, tr3 = tr0;
-  bool top_rec = false, popped_rec = false;
   auto push = [&](uint32_t node, float tn) {
     if (sp >= kStack)
       return;
@@ -335,16 +327,6 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     stale = false;
     top = make_uint2(node, __float_as_uint(tn));
     sp++;
-    if constexpr (PF) {
-      top_rec = node + 1 < nprims; // a valid child index (always, for a valid tree)
-      if (top_rec) {
-        const float4* q = reinterpret_cast<const float4*>(&nodes[node]);
-        tr0 = q[0];
-        tr1 = q[1];
-        tr2 = q[2];
-        tr3 = q[3];
-      }
-    }
   };
   auto pop = [&]() {
     if (stale)
@@ -352,10 +334,6 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     const uint2 e = top;
     sp--;
     stale = sp > 0;
-    if constexpr (PF) {
-      popped_rec = top_rec;
-      top_rec = false; // the next top's record was not fetched
-    }
     return e;
   };
 
@@ -369,7 +347,6 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
   uint32_t ni = 0u;
   bool have = true;
   for (;;) {
-    bool fetched = false; // PF: the record of ni is in tr0..tr3
     if (!have) {
       if (sp == 0)
         break;
@@ -377,25 +354,12 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
       if (best_t < __uint_as_float(e.y))
         continue;
       ni = e.x;
-      if constexpr (PF)
-        fetched = popped_rec;
     }
     have = false;
     if (ni + 1 >= nprims)
       continue; // only possible for a corrupt node array
-    float4 w0, w1, w2, w3;
-    if (PF && fetched) {
-      w0 = tr0;
-      w1 = tr1;
-      w2 = tr2;
-      w3 = tr3;
-    } else {
-      const float4* wp = reinterpret_cast<const float4*>(&nodes[ni]);
-      w0 = wp[0];
-      w1 = wp[1];
-      w2 = wp[2];
-      w3 = wp[3];
-    }
+    const float4* wp = reinterpret_cast<const float4*>(&nodes[ni]);
+    const float4 w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
     if (stale) { // reload the top while the node is in flight
       top = stk_load(sp - 1);
       stale = false;
@@ -473,22 +437,22 @@ namespace lbvh_eng {
 
 using namespace lbvh_dev;
 
-template<int MODE, bool PF>
+template<int MODE>
 static void
 trace_mode(uint32_t g, hipStream_t s, const WideNode* nd, const float* tris, uint64_t n,
            const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg, const TraceJob* jobs,
            const float* rays, uint64_t nrays, uint32_t* prim, float* t, float* uv)
 {
   if (jobs)
-    hipLaunchKernelGGL((k_trace<MODE, kLayoutJobs, PF>), dim3(g), dim3(kTraceBlock), 0, s,
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutJobs>), dim3(g), dim3(kTraceBlock), 0, s,
                        nullptr, nullptr, 0u, nullptr, nullptr, nseg, jobs, nullptr, nrays, nullptr,
                        nullptr, nullptr);
   else if (prim_off)
-    hipLaunchKernelGGL((k_trace<MODE, kLayoutBatch, PF>), dim3(g), dim3(kTraceBlock), 0, s, nd,
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutBatch>), dim3(g), dim3(kTraceBlock), 0, s, nd,
                        tris, uint32_t(n), prim_off, ray_off, nseg, nullptr, rays, nrays, prim, t,
                        uv);
   else
-    hipLaunchKernelGGL((k_trace<MODE, kLayoutSingle, PF>), dim3(g), dim3(kTraceBlock), 0, s, nd,
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutSingle>), dim3(g), dim3(kTraceBlock), 0, s, nd,
                        tris, uint32_t(n), nullptr, nullptr, 1u, nullptr, rays, nrays, prim, t, uv);
 }
 
@@ -502,7 +466,7 @@ trace_dispatch(int mode, const void* nodes, const float* tris, uint64_t n, const
   const uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
   const WideNode* nd = static_cast<const WideNode*>(nodes);
 #define LBVH_TM(M)                                                                                \
-  trace_mode<M, false>(g, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, nrays, prim, t, uv)
+  trace_mode<M>(g, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, nrays, prim, t, uv)
   switch (mode) {
     case 0:
       LBVH_TM(0);

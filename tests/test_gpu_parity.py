@@ -63,14 +63,14 @@ def test_trace_matches_golden(ctx, case, leaf, mode):
     bvh.destroy()
 
 
-@pytest.mark.parametrize("nrays", [4096, 600000], ids=["small_dispatch", "full_dispatch"])
+@pytest.mark.parametrize("nrays", [4096, 600000], ids=["rays4096", "rays600000"])
 @pytest.mark.parametrize("mode", [0, 1], ids=["first", "lowid"])
 def test_trace_deep_stacks_match_oracle(ctx, nrays, mode):
     """20,000 large, mutually overlapping triangles: most rays hit both
     children at many levels, so traversal stacks run far past the entries
     kept in registers and LDS into the private array (hundreds of pops per
-    ray).  Both traversal variants (small and full dispatches) against the
-    oracle, bit for bit."""
+    ray).  A dispatch that leaves most of the device idle and one that fills
+    it, against the oracle, bit for bit."""
     print("deep-stack trace nrays=%d mode=%d" % (nrays, mode), flush=True)
     rng = np.random.default_rng(3)
     n = 20000

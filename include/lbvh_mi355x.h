@@ -189,6 +189,30 @@ lbvh_status lbvh_bvh_download_nodes(lbvh_ctx* ctx, const lbvh_bvh* bvh,
  * sorted order (space_filling_curve entries after sort, lbvh.h:1104-1134) and
  * the centroid bounds (6 floats).  Needs opts.keep_codes; any pointer may be
  * NULL. */
+/* -- allocation-free build --------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: lbvh_build_triangles_device's build with no allocation
+ * of any kind.  Every scratch buffer (centroids, keys, ids, sort counts, tree
+ * segments and tickets, and the tree records the nodes are built beside) is
+ * carved from one fixed device array that each device holds for the life of
+ * the process (40 MiB, set up when a context is first created on the
+ * device).  Inputs: d_tri48, n device triangles (16-byte aligned), n in
+ * [2, LBVH_NOALLOC_MAX_PRIMS].  Output: the (n - 1) lbvh::node<float> records
+ * of the tree in d_nodes_out (device memory, 32-byte aligned, (n - 1) x 32
+ * bytes), byte-identical to lbvh_build_triangles_device's node array for the
+ * same options.  keep_codes must be 0; the onesweep sort is rejected; with
+ * the sort fields at 0 the context's tuned choice for the size is used when
+ * one exists (no tuning runs), else a fixed default.  Larger n is rejected
+ * with LBVH_E_INVALID.  Returns after queuing on the context stream.  Every
+ * call on a device shares that device's scratch, so calls that overlap on one
+ * device (contexts with different streams) race; calls on one stream do not. */
+#define LBVH_NOALLOC_MAX_PRIMS 262144u
+lbvh_status lbvh_build_triangles_noalloc(lbvh_ctx* ctx, const float* d_tri48,
+                                         uint64_t n,
+                                         const lbvh_build_opts* opts,
+                                         void* d_nodes_out);
+
 lbvh_status lbvh_bvh_download_codes(lbvh_ctx* ctx, const lbvh_bvh* bvh,
                                     void* codes, uint32_t* ids,
                                     float* bounds6);

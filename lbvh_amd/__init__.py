@@ -141,6 +141,7 @@ SIGNATURES = {
     "lbvh_build_aabbs_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts),
                                                ctypes.POINTER(VP)]),
     "lbvh_refit_triangles": (ctypes.c_int, [VP, VP, VP]),
+    "lbvh_build_triangles_noalloc": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), VP]),
     "lbvh_refit_triangles_device": (ctypes.c_int, [VP, VP, VP]),
     "lbvh_bvh_get_info": (ctypes.c_int, [VP, ctypes.POINTER(BvhInfo)]),
     "lbvh_bvh_download_nodes": (ctypes.c_int, [VP, VP, VP, U64]),

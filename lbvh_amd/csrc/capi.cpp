@@ -614,6 +614,10 @@ lbvh_ctx_set_stream(lbvh_ctx* c, void* stream)
 {
   if (!c)
     return LBVH_E_INVALID;
+  // set-up of the allocation-free build on this device; if it fails here,
+  // that entry point retries on its first call and reports the error
+  if (noalloc_init(c) != LBVH_OK)
+    c->last_error.clear();
   c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
   return LBVH_OK;
 }

@@ -128,6 +128,9 @@ lbvh_status sync_and_check(lbvh_ctx* c);
 
 // ---- sort autotuner ----------------------------------------------------------
 // The default configuration, the candidates timed by the tuner, and whether
+// Finds this device's copy of the allocation-free build's scratch array (the
+// device must be current; once per device).
+lbvh_status noalloc_init(lbvh_ctx* c);
 // build options override the choice (then the tuner is not consulted).
 SortConfig default_sort();
 std::vector<SortConfig> sort_candidates(bool batch, int key_bits, uint64_t n);

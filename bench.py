@@ -1,22 +1,32 @@
 """LBVH build + closest-hit benchmark on one MI355X.
 
-Workload: the skewed single-scene case — 1,048,576 FP32 triangles whose
-centres come from 64 Gaussian clusters (seed 2, sigma 0.02, clamped to [0,1],
-edge offsets U[-0.002, 0.002]^3), 63-bit Morton codes, primitive-id leaves;
-1920x1080 pinhole primary rays from (0.5, 0.5, -1) looking +z (60 degree
-vertical FOV, tmin 1e-4, tmax 1e30) plus one cosine-weighted diffuse bounce
-ray per primary hit.  Inputs are generated once on the host and are resident
-in HBM before the timed region.
+Default workload (the headline): the skewed single-scene case -- 1,048,576
+FP32 triangles whose centres come from 64 Gaussian clusters (seed 2, sigma
+0.02, clamped to [0,1], edge offsets U[-0.002, 0.002]^3), 63-bit Morton
+codes, primitive-id leaves; 1920x1080 pinhole primary rays from
+(0.5, 0.5, -1) looking +z (60 degree vertical FOV, tmin 1e-4, tmax 1e30)
+plus one cosine-weighted diffuse bounce ray per primary hit.  Inputs are
+generated once on the host and are resident in HBM before the timed region.
 
 One step = build the BVH (bounds, encode, sort, tree, fit) + closest-hit of
 the primary rays + closest-hit of the bounce rays, all on one stream.
 
 Prints one JSON line.  `value` is the build throughput in Mprims/s; the line
 also carries closest-hit Mrays/s, the per-phase split, the roofline of the
-dominant kernel, and the reference CPU path timed on this host.
+dominant kernel, the error of the hit distance against FP64, the reference
+CPU path timed on this host, and the two throughput / latency comparisons
+of the metric: 4,096 scenes built and traced as one segmented batch against
+the plain per-scene path, and an open-loop request stream through the
+request server against the plain path one request at a time.
+
+Other workloads (--workload): config0 (65,536 triangles, 30-bit codes, the
+reference leaf order, beside the reference CPU path), batched, serving,
+large (16,777,216 triangles), f64 (builder<double>), hbm-copy (a copy of
+known size, the calibration of the rocprofv3 byte counters).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,12 +37,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 N_PRIMS = 1 << 20
 WIDTH, HEIGHT = 1920, 1080
+MISS = 0xFFFFFFFF
 SORT_NAMES = {1: "onesweep", 2: "reduce-then-scan", 3: "hybrid"}
+METRIC = "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain"
 # Algorithmic bytes per primitive of the build with 63-bit codes (u64 key +
 # u32 id), the canonical accounting whatever sort the tuner picks: bounds 36,
 # encode 48, sort 8 + 8 digit passes x 24, tree 16, fit 92.
 SORT_BYTES_PER_PRIM_63 = 8 + 8 * 24
 BUILD_BYTES_PER_PRIM_63 = 36 + 48 + SORT_BYTES_PER_PRIM_63 + 16 + 92
+# With 30-bit codes (u32 key + u32 id): bounds 36, encode 44, sort 4 + 4
+# digit passes x 16, tree 12, fit 92.
+BUILD_BYTES_PER_PRIM_30 = 36 + 44 + (4 + 4 * 16) + 12 + 92
 # Bytes the closest-hit traversal touches on the benchmark workload (primary
 # + bounce rays on the 1M clustered tree): popped nodes x 32 + box-tested
 # children x 32 + tested triangles x 48, counted by the instrumented CPU
@@ -45,6 +60,10 @@ RAY_COMPULSORY_BYTES = 48  # 32-byte ray in, 16-byte hit out
 # (profiles/clustered1m_63bit_hbm_traffic.md: primary 633.4 + 35.5 MB, bounce
 # 910.7 + 4.2 MB; mean 772.1 MB read + 19.9 MB written).
 TRACE_HBM_TRAFFIC_PER_LAUNCH = 791.9e6
+# Offered load of the serving comparison in the default line (requests/s):
+# above the plain path's capacity, so both paths run saturated.
+HEADLINE_SERVING_RATE = 4000.0
+HEADLINE_SERVING_REQUESTS = 160
 
 
 def log(msg):
@@ -57,21 +76,69 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="single: skip the batched and served comparisons; batched: skip the "
+                         "plain per-scene path (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU-baseline sample")
-    ap.add_argument("--workload", choices=["single", "batched", "serving", "large", "f64"],
+    ap.add_argument("--workload",
+                    choices=["single", "config0", "batched", "serving", "large", "f64",
+                             "hbm-copy"],
                     default="single",
-                    help="single: the 1M-triangle scene (default); batched: 4,096 uneven "
-                         "scenes built in one segmented pass against the plain per-scene path; "
-                         "serving: an open-loop request stream through the request server "
-                         "against the plain path one request at a time; large: 16,777,216 "
-                         "triangles, per-phase 30/63-bit builds and a refit; f64: "
-                         "builder<double> over the 1M scene in FP64, and 16,777,216 boxes")
+                    help="single: the 1M-triangle scene (default); config0: 65,536 uniform "
+                         "triangles, 30-bit codes, reference leaf order, 256x256 rays, beside the "
+                         "reference CPU path; batched: 4,096 uneven scenes built in one "
+                         "segmented pass against the plain per-scene path; serving: an open-loop "
+                         "request stream through the request server against the plain path one "
+                         "request at a time; large: 16,777,216 triangles, per-phase 30/63-bit "
+                         "builds and a refit; f64: builder<double> over the 1M scene in FP64, and "
+                         "16,777,216 boxes; hbm-copy: 5 copies of 512 MiB (counter calibration)")
     ap.add_argument("--scenes", type=int, default=4096, help="batched: number of scenes")
     ap.add_argument("--requests", type=int, default=160, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=1000.0,
                     help="serving: offered load, requests per second (Poisson)")
     return ap.parse_args()
+
+
+def host_info():
+    """CPU model, usable cores and SMT state of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    tpc = None
+    try:
+        with open("/sys/devices/system/cpu/cpu0/topology/thread_siblings_list") as f:
+            sib = f.read().strip()
+        n = 0
+        for part in sib.split(","):
+            a, _, b = part.partition("-")
+            n += (int(b) - int(a) + 1) if b else 1
+        tpc = n
+    except (OSError, ValueError):
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "usable_cores": usable, "logical_cpus": os.cpu_count(),
+            "smt_active": smt, "threads_per_core": tpc}
+
+
+def _ref_threads(fast):
+    import oracle
+    return oracle.ref(fast).ref_hardware_concurrency() or os.cpu_count() or 1
 
 
 def cpu_baseline(tri, rays, bounce, budget_s):
@@ -86,16 +153,15 @@ def cpu_baseline(tri, rays, bounce, budget_s):
         kind = "port"
     threads = os.cpu_count() or 1
     if kind == "reference":
-        threads = oracle.ref(fast).ref_hardware_concurrency() or threads
+        threads = _ref_threads(fast)
     t_end = time.time() + budget_s * 0.6
     times = []
-    nodes = None
     while time.time() < t_end or len(times) < 2:
         t0 = time.perf_counter()
         if kind == "reference":
-            nodes, _, _ = oracle.ref_build(tri, 63, 0, threads=threads, fast=fast)
+            oracle.ref_build(tri, 63, 0, threads=threads, fast=fast)
         else:
-            nodes, _, _, _ = oracle.build(tri, 63, 0)
+            oracle.build(tri, 63, 0)
         times.append(time.perf_counter() - t0)
         if len(times) >= 6:
             break
@@ -113,7 +179,7 @@ def cpu_baseline(tri, rays, bounce, budget_s):
     else:
         oracle.trace(pid_nodes, tri, sample, 0, threads=threads)
     trace_s = time.perf_counter() - t0
-    return {
+    out = {
         "value": round(N_PRIMS / build_s / 1e6, 3),
         "unit": "Mprims/s",
         "cores": int(threads),
@@ -125,6 +191,8 @@ def cpu_baseline(tri, rays, bounce, budget_s):
         "build_ms": round(build_s * 1e3, 2),
         "closest_hit_mrays_s": round(len(sample) / trace_s / 1e6, 4),
     }
+    out.update(host_info())
+    return out
 
 
 def percentile(xs, q):
@@ -135,7 +203,42 @@ def percentile(xs, q):
     return xs[k]
 
 
-def bench_batched(args):
+def _p(xs, q, nd=4):
+    v = percentile(xs, q)
+    return None if v is None else round(v, nd)
+
+
+def t_error_vs_fp64(tri, rays, prim, t):
+    """Error of the FP32 hit distances against the plane distance in FP64:
+    t64 = dot(n, p0 - o) / dot(n, d) with n = e0 x e1, all in FP64, on the
+    stored triangle (p0, e0, e1) of the hit primitive.  Relative error and
+    error in units of the FP32 spacing at t64, max and mean over the hits."""
+    import numpy as np
+    hit = prim != MISS
+    if not hit.any():
+        return {"hits": 0}
+    tr = tri[prim[hit].astype(np.int64)].astype(np.float64)
+    r = rays[hit].astype(np.float64)
+    n = np.cross(tr[:, 3:6], tr[:, 6:9])
+    c = tr[:, 0:3] - r[:, 0:3]
+    t64 = (n * c).sum(axis=1) / (n * r[:, 3:6]).sum(axis=1)
+    t32 = t[hit].astype(np.float64)
+    err = np.abs(t32 - t64)
+    rel = err / np.abs(t64)
+    ulp = err / np.spacing(np.abs(t64).astype(np.float32)).astype(np.float64)
+    return {"hits": int(hit.sum()), "max_rel": float("%.4g" % rel.max()),
+            "mean_rel": float("%.4g" % rel.mean()), "max_ulp": float("%.4g" % ulp.max()),
+            "mean_ulp": float("%.4g" % ulp.mean())}
+
+
+# ---- batched vs plain ---------------------------------------------------------
+
+BATCHED_DESC = ("%d scenes (seed 4), triangle counts log-uniform in [64, 262144], uniform scenes "
+                "in their own unit cubes, 30-bit codes, primitive-id leaves; 1,024 box-surface "
+                "rays per scene")
+
+
+def batched_compare(nseg, steps, warmup, plain_path=True):
     """4,096 independent scenes (seed 4) with triangle counts log-uniform in
     [64, 262144], each a uniform scene in its own unit cube (offsets
     U[-0.01, 0.01]^3), 30-bit codes, primitive-id leaves; 1,024 rays per scene
@@ -144,16 +247,20 @@ def bench_batched(args):
     Batched step: one segmented rebuild of every scene (replayed from a HIP
     graph) + one closest-hit launch over every scene's rays.  Plain path: the
     same scenes built and traced one call at a time through the single-scene
-    entry points (a fresh tree per scene, as the plain API returns), once with
-    a host synchronization per scene (per-request latency) and once queued
-    back to back.  Both produce identical hits (checked)."""
+    entry points (a fresh tree per scene, as the plain API returns), once
+    queued back to back and once with a host synchronization per scene.
+
+    Latency: every scene is requested at once.  A batched step completes all
+    of them together, so each scene's latency is the step's host wall time
+    (distribution over the timed steps); served one at a time, scene s
+    completes after every scene before it (distribution over the scenes).
+    Both produce identical hits (checked)."""
     import numpy as np
 
     import lbvh_amd
     from lbvh_amd import hip
     from lbvh_amd import workload as W
 
-    nseg = args.scenes
     log("bench: generating %d scenes (seed 4, log-uniform [64, 262144])" % nseg)
     sizes = W.batched_scene_sizes(4, nseg, 64, 262144)
     tri, off = W.batched_scenes(4, sizes=sizes)
@@ -170,7 +277,6 @@ def bench_batched(args):
     mode = lbvh_amd.ISECT_REFERENCE
     del tri
 
-    # ---- batched ----------------------------------------------------------
     t0 = time.perf_counter()
     b = ctx.build_batch(d_tri, off, leaf_order=PID)
     ctx.synchronize()
@@ -180,27 +286,35 @@ def bench_batched(args):
     log("bench: batch built (first call incl. allocation and tuning: %.1f ms); sort %d-bit "
         "digits, tile %d, %d passes" % (first_build_s * 1e3, info.sort_digit_bits, info.sort_tile,
                                         info.sort_passes))
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         b.rebuild(graph=True)
         b.trace(d_rays, d_roff, mode, out=out)
     ctx.synchronize()
+    # device time of back-to-back steps
     ctx.set_timing(2)
     hip.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         b.rebuild(graph=True)
         b.trace(d_rays, d_roff, mode, out=out)
     hip.synchronize()
-    wall_b = (time.perf_counter() - t0) / args.steps
+    wall_b = (time.perf_counter() - t0) / steps
     ctx.synchronize()
     bt, _ = ctx.phase_stats(1)
     tt, _ = ctx.phase_stats(2)
     ctx.set_timing(0)
-    build_ms = sum(bt) / args.steps
-    trace_ms = sum(tt) / args.steps
+    build_ms = sum(bt) / steps
+    trace_ms = sum(tt) / steps
+    # latency: each step submitted alone and waited for
+    lat_b = []
+    for _ in range(steps):
+        ts = time.perf_counter()
+        b.rebuild(graph=True)
+        b.trace(d_rays, d_roff, mode, out=out)
+        ctx.synchronize()
+        lat_b.append((time.perf_counter() - ts) * 1e3)
     batch_prim = out[0].numpy()
     batch_t = out[1].numpy()
-    # phase split of one eager rebuild
     ctx.set_timing(1)
     b.rebuild()
     ctx.synchronize()
@@ -208,10 +322,29 @@ def bench_batched(args):
     ctx.set_timing(0)
     log("bench: batched step %.3f ms (build %.3f ms, trace %.3f ms)" % (wall_b * 1e3, build_ms,
                                                                       trace_ms))
+    res = {"workload": BATCHED_DESC % nseg, "scenes": nseg, "prims": n, "rays": m,
+           "steps": steps,
+           "step_ms": round(wall_b * 1e3, 4), "build_ms": round(build_ms, 4),
+           "trace_ms": round(trace_ms, 4),
+           "build_mprims_s": round(n / (build_ms * 1e-3) / 1e6, 3),
+           "closest_hit_mrays_s": round(m / (trace_ms * 1e-3) / 1e6, 3),
+           "build_phases_ms": ({"bounds+encode": round(split[0], 4), "sort": round(split[1], 4),
+                                "tree+fit": round(split[2] + split[3], 4)}
+                               if len(split) == 4 else split),
+           "sort": {"digit_bits": info.sort_digit_bits, "tile": info.sort_tile,
+                    "strategy": SORT_NAMES.get(info.sort_strategy, "?"),
+                    "digit_passes": info.sort_passes, "picked_by": "autotuner",
+                    "candidates_ms": {"%s-d%d-t%d" % (r["strategy"], r["digit_bits"],
+                                                      r["tile"]): r["ms"]
+                                      for r in tune_log if r["batch"]}},
+           "first_build_ms": round(first_build_s * 1e3, 2)}
+    if not plain_path:
+        b.destroy()
+        ctx.close()
+        return res
 
-    # ---- plain, one request at a time ---------------------------------------
     def plain(sync_each):
-        lat, trees = [], []
+        done, svc, trees = [], [], []
         hip.synchronize()
         t0 = time.perf_counter()
         for s in range(nseg):
@@ -223,70 +356,73 @@ def bench_batched(args):
                       out=(out[0].rows(r0, r1), out[1].rows(r0, r1), out[2].rows(r0, r1)))
             if sync_each:
                 ctx.synchronize()
-                lat.append((time.perf_counter() - ts) * 1e3)
+                now = time.perf_counter()
+                svc.append((now - ts) * 1e3)
+                done.append((now - t0) * 1e3)
             trees.append(bv)
         ctx.synchronize()
         wall = time.perf_counter() - t0
         for bv in trees:
             bv.destroy()
-        return wall, lat
+        return wall, done, svc
 
-    plain(False)  # warm the workspace
-    wall_async, _ = plain(False)
-    wall_sync, lat = plain(True)
+    plain(False)  # warm the workspace and the tuner
+    wall_async, _, _ = plain(False)
+    wall_sync, done, svc = plain(True)
     same = bool(np.array_equal(out[0].numpy(), batch_prim) and
                 np.array_equal(out[1].numpy().view(np.uint32), batch_t.view(np.uint32)))
     log("bench: plain %.1f ms queued, %.1f ms one at a time; hits identical: %s"
         % (wall_async * 1e3, wall_sync * 1e3, same))
     b.destroy()
     ctx.close()
+    step_ms = wall_b * 1e3
+    res.update({
+        "plain_queued_ms": round(wall_async * 1e3, 3),
+        "plain_one_at_a_time_ms": round(wall_sync * 1e3, 3),
+        "throughput_vs_plain_queued": round(wall_async * 1e3 / step_ms, 3),
+        "throughput_vs_plain_one_at_a_time": round(wall_sync * 1e3 / step_ms, 3),
+        "latency_ms": {"all scenes requested at once": True,
+                       "batched": {"p50": _p(lat_b, 50), "p99": _p(lat_b, 99),
+                                   "samples": len(lat_b)},
+                       "plain": {"p50": _p(done, 50), "p99": _p(done, 99),
+                                 "samples": len(done)},
+                       "plain_per_scene_service": {"p50": _p(svc, 50), "p99": _p(svc, 99)}},
+        "hits_identical": same,
+    })
+    return res
 
-    step_ms = build_ms + trace_ms
+
+def bench_batched(args):
+    r = batched_compare(args.scenes, args.steps, args.warmup, plain_path=not args.no_extras)
     res = {
-        "metric": "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain",
-        "value": round(n / (build_ms * 1e-3) / 1e6, 3),
+        "metric": METRIC,
+        "value": r["build_mprims_s"],
         "unit": "Mprims/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall_b * 1e3, 4),
+        "ms_per_step": r["step_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batched: %d scenes (seed 4), triangle counts log-uniform in "
-                               "[64, 262144], uniform scenes in their own unit cubes, 30-bit "
-                               "codes, primitive-id leaves; 1,024 box-surface rays per scene"
-                               % nseg,
-                   "n_prims": n, "n_scenes": nseg, "rays": m},
-        "batched": {"build_ms": round(build_ms, 4), "trace_ms": round(trace_ms, 4),
-                    "build_phases_ms": {"bounds+encode": round(split[0], 4),
-                                        "sort": round(split[1], 4),
-                                        "tree+fit": round(split[2] + split[3], 4)}
-                    if len(split) == 4 else split,
-                    "closest_hit_mrays_s": round(m / (trace_ms * 1e-3) / 1e6, 3),
-                    "sort": {"digit_bits": info.sort_digit_bits, "tile": info.sort_tile,
-                             "strategy": SORT_NAMES.get(info.sort_strategy, "?"),
-                             "digit_passes": info.sort_passes, "picked_by": "autotuner",
-                             "candidates_ms": {"%s-d%d-t%d" % (r["strategy"], r["digit_bits"],
-                                                               r["tile"]): r["ms"]
-                                               for r in tune_log if r["batch"]}},
-                    "first_build_ms": round(first_build_s * 1e3, 2),
-                    "latency_ms_p50": round(wall_b * 1e3, 4),
-                    "latency_ms_p99": round(wall_b * 1e3, 4)},
-        "plain": {"queued_ms": round(wall_async * 1e3, 3),
-                  "one_at_a_time_ms": round(wall_sync * 1e3, 3),
-                  "latency_ms_p50": round(percentile(lat, 50), 4),
-                  "latency_ms_p99": round(percentile(lat, 99), 4)},
-        "speedup_vs_plain_queued": round(wall_async * 1e3 / step_ms, 3),
-        "speedup_vs_plain_one_at_a_time": round(wall_sync * 1e3 / step_ms, 3),
-        "hits_identical": same,
+        "config": {"workload": "batched: " + r["workload"], "n_prims": r["prims"],
+                   "n_scenes": r["scenes"], "rays": r["rays"]},
+        "batched": r,
     }
     print(json.dumps(res), flush=True)
 
 
-def bench_serving(args):
+# ---- served vs plain ------------------------------------------------------------
+
+SERVING_DESC = ("%d requests, Poisson arrivals at %.0f req/s (seed 5); new scenes log-uniform "
+                "[1024, 4194304] clustered triangles, 30%% reuse, 10%% refits; rays log-uniform "
+                "[4096, 2097152], half primary, half bounce; 63-bit codes, primitive-id leaves; "
+                "request data resident in HBM")
+
+
+def served_compare(count, rate):
     """Open-loop Poisson request stream (seed 5) at a fixed offered load.
     New scenes: triangle counts log-uniform in [1,024, 4,194,304], clustered
     as the 1M-triangle scene (64 Gaussian clusters, sigma 0.02, offsets
@@ -312,44 +448,29 @@ def bench_serving(args):
     from lbvh_amd import hip
     from lbvh_amd import workload as W
 
-    stream = W.serving_stream(seed=5, count=args.requests, rate_hz=args.rate)
+    stream = W.serving_stream(seed=5, count=count, rate_hz=rate)
     log("bench: %d requests (%d new scenes, %d traces, %d refits), offered load %.0f req/s"
         % (len(stream), sum(r["kind"] == "new" for r in stream),
            sum(r["kind"] == "trace" for r in stream), sum(r["kind"] == "refit" for r in stream),
-           args.rate))
+           rate))
     PID = lbvh_amd.LEAF_PRIMITIVE_ID
     setup = lbvh_amd.Context(0)
     setup.set_autotune(False)
-    verts = {}
-    reqs = []  # (tri or None, rays)
     trees = {}
-    for i, r in enumerate(stream):
+
+    def hits(i, r, tri, prim_rays):
         k = r["scene"]
-        tri = None
         if r["kind"] == "new":
-            verts[k] = W.cluster_vertices((5 << 32) | k, r["n_tri"])
-            tri = W.make_triangles(verts[k])
             trees[k] = setup.build_triangles(tri, code_bits=63, leaf_order=PID)
         elif r["kind"] == "refit":
-            verts[k] = W.perturb((5 << 32) | (1 << 20) | i, verts[k], 1e-4)
-            tri = W.make_triangles(verts[k])
             setup.refit_triangles(trees[k], tri)
-        m = r["n_rays"]
-        half = m // 2
-        w = int(np.ceil(np.sqrt(max(half, 1) * 16.0 / 9.0)))
-        h = max(1, (half + w - 1) // w)
-        prim_rays = W.primary_rays(w, h)[:half]
-        hp, ht, _ = setup.trace(trees[k], prim_rays, 0, with_uv=False)
-        cur_tri = W.make_triangles(verts[k])
-        bounce, _ = W.bounce_rays((5 << 32) | (2 << 20) | i, cur_tri, prim_rays, hp, ht)
-        need = m - half
-        if bounce.shape[0] == 0:
-            extra = W.primary_rays(w, h + 1)
-            bounce = extra[np.arange(need) % extra.shape[0]]
-        else:
-            bounce = bounce[np.arange(need) % bounce.shape[0]]
+        p, t, _ = setup.trace(trees[k], prim_rays, 0, with_uv=False)
+        return p, t
+
+    reqs = []  # (device triangles or None, device rays)
+    for i, r, tri, rays in W.serving_requests(stream, hits):
         reqs.append((hip.DeviceArray.from_host(tri) if tri is not None else None,
-                     hip.DeviceArray.from_host(np.concatenate([prim_rays, bounce]))))
+                     hip.DeviceArray.from_host(rays)))
     for b in trees.values():
         b.destroy()
     setup.close()
@@ -362,11 +483,16 @@ def bench_serving(args):
     out_p = [outputs(r[1].shape[0]) for r in reqs]
     log("bench: %d rays in all; streams ready" % total_rays)
 
-    # ---- served ------------------------------------------------------------
-    srv = lbvh_amd.Server(0, code_bits=63, leaf_order=PID, isect_mode=0)
+    # both paths start with workspaces sized for the largest scene of the
+    # stream: the server reserves its lanes, the plain context builds it once
+    largest = max(range(len(stream)), key=lambda i: stream[i]["n_tri"]
+                  if stream[i]["kind"] == "new" else 0)
+    # ---- served ----------------------------------------------------------
+    srv = lbvh_amd.Server(0, code_bits=63, leaf_order=PID, isect_mode=0,
+                          reserve_prims=stream[largest]["n_tri"])
     # warm the lanes and pools outside the measured stream
     warm = outputs(reqs[0][1].shape[0])
-    wt, ws = srv.submit(reqs[0][1], 0, reqs[0][0], out=warm)
+    wt, _ = srv.submit(reqs[0][1], 0, reqs[0][0], out=warm)
     srv.wait(wt)
     tickets = queue.Queue()
     ids = {}
@@ -385,17 +511,15 @@ def bench_serving(args):
                 tk, _ = srv.submit(rays, ids[r["scene"]], tri, out=out_s[i])
             else:
                 tk, _ = srv.submit(rays, ids[r["scene"]], out=out_s[i])
-            tickets.put((i, tk, t0))
+            tickets.put((i, tk))
 
     th = threading.Thread(target=submitter)
     th.start()
-    served = [None] * len(stream)
-    lat_s, first_sub, last_done, t0_sub = [], None, 0.0, None
+    lat_s, first_sub, last_done = [], None, 0.0
     coalesced = []
     for _ in stream:
-        i, tk, t0 = tickets.get()
-        p, t, uv, info = srv.wait(tk)
-        served[i] = True
+        i, tk = tickets.get()
+        _, _, _, info = srv.wait(tk)
         lat_s.append((info["done_us"] - info["submit_us"]) * 1e-3)
         coalesced.append(info["coalesced_with"])
         first_sub = info["submit_us"] if first_sub is None else min(first_sub, info["submit_us"])
@@ -407,10 +531,10 @@ def bench_serving(args):
     log("bench: served makespan %.1f ms, p50 %.2f ms, p99 %.2f ms, %d launches"
         % (served_span_s * 1e3, percentile(lat_s, 50), percentile(lat_s, 99), sst["launches"]))
 
-    # ---- plain, one request at a time ----------------------------------------
+    # ---- plain, one request at a time ---------------------------------------
     ctx = lbvh_amd.Context(0)
     ctx.set_autotune(False)
-    w0 = ctx.build_triangles(reqs[0][0], code_bits=63, leaf_order=PID)
+    w0 = ctx.build_triangles(reqs[largest][0], code_bits=63, leaf_order=PID)
     ctx.trace(w0, reqs[0][1], 0, out=warm)
     ctx.synchronize()
     w0.destroy()
@@ -443,28 +567,11 @@ def bench_serving(args):
     ctx.close()
     log("bench: plain makespan %.1f ms, p50 %.2f ms, p99 %.2f ms; identical: %s"
         % (plain_span_s * 1e3, percentile(lat_p, 50), percentile(lat_p, 99), same))
-
     served_mrays = total_rays / served_span_s / 1e6
     plain_mrays = total_rays / plain_span_s / 1e6
-    res = {
-        "metric": "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain",
-        "value": round(served_mrays, 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": len(stream),
-        "warmup": 1,
-        "ms_per_step": round(served_span_s * 1e3 / len(stream), 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "serving: %d requests, Poisson arrivals at %.0f req/s (seed 5); "
-                               "new scenes log-uniform [1024, 4194304] clustered triangles, 30%% "
-                               "reuse, 10%% refits; rays log-uniform [4096, 2097152], half "
-                               "primary, half bounce; 63-bit codes, primitive-id leaves; "
-                               "request data resident in HBM" % (len(stream), args.rate),
-                   "requests": len(stream), "offered_load_rps": args.rate, "rays": total_rays},
+    return {
+        "workload": SERVING_DESC % (len(stream), rate),
+        "requests": len(stream), "offered_load_rps": rate, "rays": total_rays,
         "served": {"makespan_ms": round(served_span_s * 1e3, 3),
                    "mrays_s": round(served_mrays, 3),
                    "latency_ms_p50": round(percentile(lat_s, 50), 3),
@@ -481,6 +588,27 @@ def bench_serving(args):
                   "latency_ms_p99": round(percentile(lat_p, 99), 3)},
         "throughput_vs_plain": round(served_mrays / plain_mrays, 3),
         "results_identical": bool(same),
+    }
+
+
+def bench_serving(args):
+    r = served_compare(args.requests, args.rate)
+    res = {
+        "metric": METRIC,
+        "value": r["served"]["mrays_s"],
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": r["requests"],
+        "warmup": 1,
+        "ms_per_step": round(r["served"]["makespan_ms"] / r["requests"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "serving: " + r["workload"], "requests": r["requests"],
+                   "offered_load_rps": r["offered_load_rps"], "rays": r["rays"]},
+        "serving": r,
     }
     print(json.dumps(res), flush=True)
 
@@ -551,7 +679,7 @@ def bench_large(args):
         log("bench: %d-bit build %.3f ms" % (bits, build_ms))
     ctx.close()
     res = {
-        "metric": "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain",
+        "metric": METRIC,
         "value": out[30]["mprims_s"],
         "unit": "Mprims/s",
         "n_gpus": 1,
@@ -659,7 +787,7 @@ def bench_f64(args):
     one = out["1m"]
     achieved = BUILD_BYTES_PER_PRIM_F64 * N_PRIMS / (one["build_ms"] * 1e-3) / 1e9
     res = {
-        "metric": "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain",
+        "metric": METRIC,
         "value": one["mprims_s"],
         "unit": "Mprims/s",
         "n_gpus": 1,
@@ -705,21 +833,152 @@ def bench_f64(args):
     print(json.dumps(res), flush=True)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
-              "(scenes shard across GPUs only by assigning whole scenes)" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "batched":
-        return bench_batched(args)
-    if args.workload == "serving":
-        return bench_serving(args)
-    if args.workload == "large":
-        return bench_large(args)
-    if args.workload == "f64":
-        return bench_f64(args)
+
+# ---- config 0: the reference's own CPU-runnable case ----------------------------
+
+def bench_config0(args):
+    """65,536 uniform triangles (seed 1, offsets U[-0.01, 0.01]^3), 30-bit
+    codes, the reference leaf order (lbvh::builder<float>'s exact tree: a leaf
+    index is the primitive's Morton-order position); 256x256 pinhole primary
+    rays from (0.5, 0.5, -1), closest hit with tmin 1e-4, tmax 1e30.  The GPU
+    build and trace (inputs resident in HBM) beside the real lbvh.h on this
+    host's cores: its builder and its traverser over the same tree and rays.
+    On this tree a ray visits most leaves (the leaf index is the sorted
+    position, not the primitive), so the CPU traverser is near brute force."""
+    import numpy as np
+
+    import lbvh_amd
+    from lbvh_amd import hip
+    from lbvh_amd import workload as W
+
+    n = 65536
+    _, tri = W.scene_uniform(1, n, 0.01)
+    rays = W.primary_rays(256, 256)
+    m = rays.shape[0]
+    ctx = lbvh_amd.Context(0)
+    d_tri = hip.DeviceArray.from_host(tri)
+    d_rays = hip.DeviceArray.from_host(rays)
+    out = (hip.DeviceArray(m, np.uint32), hip.DeviceArray(m, np.float32),
+           hip.DeviceArray((m, 2), np.float32))
+    REF = lbvh_amd.LEAF_REFERENCE
+    for _ in range(args.warmup):
+        b = ctx.build_triangles(d_tri, code_bits=30, leaf_order=REF)
+        ctx.trace(b, d_rays, 0, out=out)
+        ctx.synchronize()
+        b.destroy()
+    trees = []
+    ctx.set_timing(2)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b = ctx.build_triangles(d_tri, code_bits=30, leaf_order=REF)
+        ctx.trace(b, d_rays, 0, out=out)
+        trees.append(b)
+    hip.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.synchronize()
+    bt, nb = ctx.phase_stats(1)
+    tt, nt = ctx.phase_stats(2)
+    ctx.set_timing(0)
+    nodes = trees[-1].nodes()
+    for b in trees:
+        b.destroy()
+    gp, gt = out[0].numpy(), out[1].numpy()
+    ctx.close()
+    build_ms = sum(bt) / args.steps
+    trace_ms = sum(tt) / args.steps
+    res = {
+        "metric": METRIC,
+        "value": round(n / (build_ms * 1e-3) / 1e6, 3),
+        "unit": "Mprims/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "config 0: 65,536 uniform FP32 triangles (seed 1), 30-bit codes, "
+                               "reference leaf order; 256x256 primary rays", "n_prims": n,
+                   "code_bits": 30, "rays": m},
+        "build_ms": round(build_ms, 4),
+        "build_roofline_frac": round(BUILD_BYTES_PER_PRIM_30 * n / (build_ms * 1e-3) / 1e9
+                                     / HBM_PEAK_GBS, 5),
+        "closest_hit_mrays_s": round(m / (trace_ms * 1e-3) / 1e6, 3),
+        "trace_ms": round(trace_ms, 4),
+        "hits": int((gp != MISS).sum()),
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        fast = oracle.ref_available(fast=True)
+        if fast or oracle.ref_available():
+            threads = _ref_threads(fast)
+            times = []
+            for _ in range(6):
+                t0 = time.perf_counter()
+                rn, _, _ = oracle.ref_build(tri, 30, 0, threads=threads, fast=fast)
+                times.append(time.perf_counter() - t0)
+            build_s = sorted(times)[len(times) // 2]
+            # the traverser on a first sample sizes the bounded run
+            probe = rays[::64]
+            t0 = time.perf_counter()
+            oracle.ref_trace(rn, tri, probe, 0, threads=threads, fast=fast)
+            per_ray = (time.perf_counter() - t0) / probe.shape[0]
+            step = 1
+            while per_ray * m / step > args.cpu_seconds * 0.8 and step < 64:
+                step *= 2
+            sample = rays[::step]
+            t0 = time.perf_counter()
+            rp, rt, _ = oracle.ref_trace(rn, tri, sample, 0, threads=threads, fast=fast)
+            trace_s = time.perf_counter() - t0
+            res["cpu_baseline"] = {
+                "value": round(n / build_s / 1e6, 3), "unit": "Mprims/s", "cores": int(threads),
+                "kind": "reference",
+                "flags": "-O3 -ffast-math" if fast else "-O2 -ffp-contract=off",
+                "sample": "6 builds of the 65,536-triangle scene (30-bit codes, reference leaf "
+                          "order, median %.2f ms); closest hit of %d rays (every %d%s of the "
+                          "256x256) on that tree in %.2f s"
+                          % (build_s * 1e3, sample.shape[0], step,
+                             {1: "st", 2: "nd", 3: "rd"}.get(step, "th"), trace_s),
+                "build_ms": round(build_s * 1e3, 3),
+                "closest_hit_mrays_s": round(sample.shape[0] / trace_s / 1e6, 5),
+                "same_nodes_as_gpu": bool(rn.tobytes() == nodes.tobytes()),
+                "same_hits_as_gpu": bool(np.array_equal(rp, gp[::step]) and
+                                         np.array_equal(rt.view(np.uint32),
+                                                        gt[::step].view(np.uint32))),
+            }
+            res["cpu_baseline"].update(host_info())
+    print(json.dumps(res), flush=True)
+
+
+# ---- counter calibration -------------------------------------------------------
+
+def bench_hbm_copy(args):
+    """Five device-to-device copies of 512 MiB (the runtime's copy kernel
+    reads 512 MiB and writes 512 MiB per dispatch): the known-size kernel
+    that scales rocprofv3's FETCH_SIZE and WRITE_SIZE counters to bytes."""
+    import numpy as np
+
+    from lbvh_amd import hip
+    nbytes = 512 << 20
+    a = hip.DeviceArray(nbytes // 4, np.float32)
+    b = hip.DeviceArray(nbytes // 4, np.float32)
+    hip.check(hip.hip().hipMemset(a.ptr, 1, nbytes), "hipMemset")
+    for i in range(5):
+        t0 = time.perf_counter()
+        b.copy_from(a)
+        hip.synchronize()
+        log("bench: copy %d: %d bytes read, %d written in %.3f ms"
+            % (i, nbytes, nbytes, (time.perf_counter() - t0) * 1e3))
+    a.close()
+    b.close()
+
+
+# ---- the headline --------------------------------------------------------------
+
+def bench_single(args):
     import numpy as np
 
     import lbvh_amd
@@ -741,15 +1000,32 @@ def main():
     p, t, _ = ctx.trace(bvh, d_rays, lbvh_amd.ISECT_REFERENCE)
     ctx.synchronize()
     sort_info = bvh.info()
-    tune_log = ctx.tune_log()
-    bvh.destroy()
     hp = p.numpy()
     ht = t.numpy()
     bounce, _ = W.bounce_rays(5, tri, rays, hp, ht)
     d_bounce = hip.DeviceArray.from_host(bounce)
     n_primary, n_bounce = rays.shape[0], bounce.shape[0]
-    log("bench: %d primary rays, %d hits -> %d bounce rays" % (n_primary, int((hp != 0xFFFFFFFF).sum()),
+    log("bench: %d primary rays, %d hits -> %d bounce rays" % (n_primary, int((hp != MISS).sum()),
                                                              n_bounce))
+    # hit-distance error against FP64, both ray sets, reference and
+    # watertight intersectors (outside the timed region)
+    t_err = {}
+    for name, mode in (("reference", lbvh_amd.ISECT_REFERENCE),
+                       ("watertight", lbvh_amd.ISECT_WATERTIGHT)):
+        e = {}
+        for rname, rs, drs in (("primary", rays, d_rays), ("bounce", bounce, d_bounce)):
+            pp, tt, _ = ctx.trace(bvh, drs, mode, with_uv=False)
+            ctx.synchronize()
+            e[rname] = t_error_vs_fp64(tri, rs, pp.numpy(), tt.numpy())
+        t_err[name] = e
+    # the first traversal of each size bucket tunes the traversal
+    ctx.trace(bvh, d_bounce, lbvh_amd.ISECT_REFERENCE, with_uv=False)
+    ctx.synchronize()
+    tune_log = ctx.tune_log()
+    trace_log = [r for r in tune_log if r["kind"] == "traversal" and r["log2_n"] in
+                 (int(math.ceil(math.log2(n_primary))), int(math.ceil(math.log2(n_bounce))))
+                 and r["key_bits"] == 0]
+    bvh.destroy()
     out_p = (hip.DeviceArray(n_primary, np.uint32), hip.DeviceArray(n_primary, np.float32),
              hip.DeviceArray((n_primary, 2), np.float32))
     out_b = (hip.DeviceArray(n_bounce, np.uint32), hip.DeviceArray(n_bounce, np.float32),
@@ -794,12 +1070,19 @@ def main():
     split = []
     ctx.set_timing(1)
     for i in range(5):
-        split.append(ctx.build_triangles(d_tri, code_bits=63, leaf_order=lbvh_amd.LEAF_PRIMITIVE_ID))
+        split.append(ctx.build_triangles(d_tri, code_bits=63,
+                                         leaf_order=lbvh_amd.LEAF_PRIMITIVE_ID))
     ctx.synchronize()
     split_ph, ns = ctx.phase_stats(1)
     ctx.set_timing(0)
     for b in split:
         b.destroy()
+    ctx.close()
+    d_tri.close()
+    d_rays.close()
+    d_bounce.close()
+    for a in out_p + out_b:
+        a.close()
 
     K = args.steps
     build_ms = sum(build_tot) / K
@@ -841,7 +1124,7 @@ def main():
                 "compulsory_gbs": round(comp, 2)}
 
     res = {
-        "metric": "build Mprims/s + closest-hit Mrays/s @1M tris; p99 latency batched vs plain",
+        "metric": METRIC,
         "value": round(mprims, 3),
         "unit": "Mprims/s",
         "n_gpus": 1,
@@ -865,19 +1148,47 @@ def main():
                  "strategy": SORT_NAMES.get(sort_info.sort_strategy, "?"),
                  "global_digits": sort_info.sort_global_digits, "picked_by": "autotuner",
                  "candidates_ms": {"%s-d%d-t%d" % (r["strategy"], r["digit_bits"], r["tile"]): r["ms"]
-                                   for r in tune_log if not r["batch"]}},
+                                   for r in tune_log if not r["batch"] and r["kind"] == "sort"}},
+        "traversal_tuning": {
+            "2^%d rays" % lg: {"picked": next(("%s, %d waves/SIMD"
+                                               % ("key order" if r["reorder"] else "as given",
+                                                  r["waves_per_simd"])
+                                               for r in trace_log if r["log2_n"] == lg
+                                               and r["picked"]), None),
+                               "candidates_ms": {"%s-w%d" % ("sorted" if r["reorder"] else "given",
+                                                             r["waves_per_simd"]): r["ms"]
+                                                 for r in trace_log if r["log2_n"] == lg}}
+            for lg in sorted(set(r["log2_n"] for r in trace_log))},
         "build_roofline_frac": round(build_frac, 4),
         "build_algorithmic_bytes_per_prim": BUILD_BYTES_PER_PRIM_63,
         "closest_hit_mrays_s": round(mrays, 3),
         "trace_ms": round(trace_ms, 4),
+        "t_error_vs_fp64": t_err,
         "dominant_kernel": dominant,
         "roofline": roof,
     }
+    if not args.no_extras:
+        log("bench: batched vs plain (4,096 scenes)")
+        res["batched"] = batched_compare(4096, 10, 3)
+        log("bench: served vs plain (%d requests at %.0f req/s)"
+            % (HEADLINE_SERVING_REQUESTS, HEADLINE_SERVING_RATE))
+        res["serving"] = served_compare(HEADLINE_SERVING_REQUESTS, HEADLINE_SERVING_RATE)
     if not args.no_cpu_baseline:
         log("bench: timing the reference CPU path (bounded sample)")
         res["cpu_baseline"] = cpu_baseline(tri, rays, bounce, args.cpu_seconds)
     print(json.dumps(res), flush=True)
-    ctx.close()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
+              "(scenes shard across GPUs only by assigning whole scenes)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    return {"single": bench_single, "config0": bench_config0, "batched": bench_batched,
+            "serving": bench_serving, "large": bench_large, "f64": bench_f64,
+            "hbm-copy": bench_hbm_copy}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -95,19 +95,24 @@ typedef struct lbvh_bvh_info
   uint32_t sort_global_digits; /* hybrid: top digits sorted by global passes */
 } lbvh_bvh_info;
 
-/* One configuration timed by the context's sort autotuner. */
+/* One configuration timed by the context's autotuner: the sort of a build
+ * (kind 0) or a closest-hit traversal (kind 1). */
 typedef struct lbvh_tune_record
 {
   uint32_t batch;    /* 1: a scene batch, 0: a single-scene build */
-  uint32_t key_bits; /* sorted key width */
-  uint32_t log2_n;   /* size bucket: ceil(log2(primitives)) */
-  uint32_t sort_digit_bits;
+  uint32_t key_bits; /* sorted key width (builds) */
+  uint32_t log2_n;   /* size bucket: ceil(log2(primitives)), or of rays */
+  uint32_t sort_digit_bits; /* builds: the sort configuration */
   uint32_t sort_tile;
   uint32_t sort_strategy;
   uint32_t picked;   /* the configuration kept for this bucket */
-  float ms;          /* device time of the whole build with it */
+  float ms;          /* device time of the whole build (traversal) with it */
   uint32_t scalar_bits; /* 32: builder<float> builds, 64: builder<double>
                            builds (tuned in buckets of their own) */
+  uint32_t kind;     /* 0: a build's sort, 1: a traversal */
+  uint32_t trace_reorder;        /* traversals: rays put in key order first */
+  uint32_t trace_waves_per_simd; /* traversals: waves sharing a SIMD (8: as
+                                    many as the registers allow) */
 } lbvh_tune_record;
 
 /* -- status / options ---------------------------------------------------- */
@@ -138,12 +143,17 @@ lbvh_status lbvh_ctx_phase_stats(lbvh_ctx* ctx, int kind, float* sum_ms,
                                  uint32_t cap, uint32_t* count,
                                  uint32_t* calls);
 const char* lbvh_ctx_last_error(const lbvh_ctx* ctx);
-/* Sort autotuner (on by default).  A build whose options leave
- * sort_digit_bits, sort_tile and sort_strategy at 0 uses the configuration
- * tuned for its key width and size bucket; the first such build of a bucket
- * times every candidate configuration on its own data (each twice, on the
- * device) and keeps the fastest.  Results are identical for every
- * configuration; only the speed differs.  Disabled: a fixed default. */
+/* Autotuner (on by default).  A build whose options leave sort_digit_bits,
+ * sort_tile and sort_strategy at 0 uses the sort configuration tuned for its
+ * key width and size bucket; the first such build of a bucket times every
+ * candidate configuration on its own data (each twice, on the device) and
+ * keeps the fastest.  lbvh_trace_closest[_device] does the same per
+ * intersector mode and ray-count bucket (from 16,384 rays) over the
+ * traversal's configurations: rays traced as given or first put in the order
+ * of a key (direction octant and a coarse position along the ray, so that a
+ * wave's rays visit the same nodes), and the waves sharing a SIMD (8, 6 or
+ * 4).  Results are identical for every configuration; only the speed
+ * differs.  Disabled: fixed defaults (rays as given, 8 waves). */
 lbvh_status lbvh_ctx_set_autotune(lbvh_ctx* ctx, int enable);
 /* Copies up to cap records of every configuration timed so far; returns the
  * total number of records. */
@@ -179,16 +189,6 @@ lbvh_status lbvh_refit_triangles(lbvh_ctx* ctx, lbvh_bvh* bvh,
 lbvh_status lbvh_refit_triangles_device(lbvh_ctx* ctx, lbvh_bvh* bvh,
                                         const float* d_tri48);
 
-/* -- the built tree --------------------------------------------------------
- * Replaces lbvh::bvh<float> (lbvh.h:259-299): nodes stay resident on the
- * device; download yields the reference's node array. */
-lbvh_status lbvh_bvh_get_info(const lbvh_bvh* bvh, lbvh_bvh_info* out);
-lbvh_status lbvh_bvh_download_nodes(lbvh_ctx* ctx, const lbvh_bvh* bvh,
-                                    void* dst, uint64_t cap_bytes);
-/* Sorted Morton codes (u32 for 30-bit, u64 for 63-bit), the primitive ids in
- * sorted order (space_filling_curve entries after sort, lbvh.h:1104-1134) and
- * the centroid bounds (6 floats).  Needs opts.keep_codes; any pointer may be
- * NULL. */
 /* -- allocation-free build --------------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory -- code under HIP graph capture, tight serving loops, harnesses that
@@ -213,6 +213,16 @@ lbvh_status lbvh_build_triangles_noalloc(lbvh_ctx* ctx, const float* d_tri48,
                                          const lbvh_build_opts* opts,
                                          void* d_nodes_out);
 
+/* -- the built tree --------------------------------------------------------
+ * Replaces lbvh::bvh<float> (lbvh.h:259-299): nodes stay resident on the
+ * device; download yields the reference's node array. */
+lbvh_status lbvh_bvh_get_info(const lbvh_bvh* bvh, lbvh_bvh_info* out);
+lbvh_status lbvh_bvh_download_nodes(lbvh_ctx* ctx, const lbvh_bvh* bvh,
+                                    void* dst, uint64_t cap_bytes);
+/* Sorted Morton codes (u32 for 30-bit, u64 for 63-bit), the primitive ids in
+ * sorted order (space_filling_curve entries after sort, lbvh.h:1104-1134) and
+ * the centroid bounds (6 floats).  Needs opts.keep_codes; any pointer may be
+ * NULL. */
 lbvh_status lbvh_bvh_download_codes(lbvh_ctx* ctx, const lbvh_bvh* bvh,
                                     void* codes, uint32_t* ids,
                                     float* bounds6);
@@ -364,7 +374,11 @@ typedef struct lbvh_server_opts
   uint32_t code_bits;        /* 30 or 63 (63) */
   uint32_t leaf_order;       /* (LBVH_LEAF_PRIMITIVE_ID) */
   uint32_t isect_mode;       /* LBVH_ISECT_* (REFERENCE) */
-  uint32_t reserved[5];
+  uint32_t reserve_prims;    /* lanes' build workspaces sized at creation for
+                                scenes of this many triangles (0: none); a
+                                larger scene grows a lane's workspace to the
+                                next power of two, which waits for the device */
+  uint32_t reserved[4];
 } lbvh_server_opts;
 
 /* lbvh_request.flags */

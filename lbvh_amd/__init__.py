@@ -66,7 +66,8 @@ class TuneRecord(ctypes.Structure):
                 ("log2_n", ctypes.c_uint32), ("sort_digit_bits", ctypes.c_uint32),
                 ("sort_tile", ctypes.c_uint32), ("sort_strategy", ctypes.c_uint32),
                 ("picked", ctypes.c_uint32), ("ms", ctypes.c_float),
-                ("scalar_bits", ctypes.c_uint32)]
+                ("scalar_bits", ctypes.c_uint32), ("kind", ctypes.c_uint32),
+                ("trace_reorder", ctypes.c_uint32), ("trace_waves_per_simd", ctypes.c_uint32)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -82,7 +83,7 @@ class ServerOpts(ctypes.Structure):
     _fields_ = [("memory_cap_bytes", ctypes.c_uint64), ("lanes", ctypes.c_uint32),
                 ("max_inflight", ctypes.c_uint32), ("code_bits", ctypes.c_uint32),
                 ("leaf_order", ctypes.c_uint32), ("isect_mode", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 5)]
+                ("reserve_prims", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
 
 
 class Request(ctypes.Structure):
@@ -140,8 +141,8 @@ SIGNATURES = {
     "lbvh_build_aabbs_f64_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), VP]),
     "lbvh_build_aabbs_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts),
                                                ctypes.POINTER(VP)]),
-    "lbvh_refit_triangles": (ctypes.c_int, [VP, VP, VP]),
     "lbvh_build_triangles_noalloc": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), VP]),
+    "lbvh_refit_triangles": (ctypes.c_int, [VP, VP, VP]),
     "lbvh_refit_triangles_device": (ctypes.c_int, [VP, VP, VP]),
     "lbvh_bvh_get_info": (ctypes.c_int, [VP, ctypes.POINTER(BvhInfo)]),
     "lbvh_bvh_download_nodes": (ctypes.c_int, [VP, VP, VP, U64]),
@@ -463,12 +464,13 @@ class Server:
     returns at once; wait() / poll() collect a request's (prim, t, uv, info)."""
 
     def __init__(self, device=0, memory_cap_bytes=None, lanes=None, max_inflight=None,
-                 code_bits=None, leaf_order=None, isect_mode=None):
+                 code_bits=None, leaf_order=None, isect_mode=None, reserve_prims=None):
         o = ServerOpts()
         lib().lbvh_server_opts_init(ctypes.byref(o))
         for k, v in (("memory_cap_bytes", memory_cap_bytes), ("lanes", lanes),
                      ("max_inflight", max_inflight), ("code_bits", code_bits),
-                     ("leaf_order", leaf_order), ("isect_mode", isect_mode)):
+                     ("leaf_order", leaf_order), ("isect_mode", isect_mode),
+                     ("reserve_prims", reserve_prims)):
             if v is not None:
                 setattr(o, k, v)
         h = ctypes.c_void_p()
@@ -642,11 +644,13 @@ class Context:
         n = lib().lbvh_ctx_tune_log(self.handle, None, 0)
         buf = (TuneRecord * max(n, 1))()
         lib().lbvh_ctx_tune_log(self.handle, buf, n)
-        return [{"batch": bool(r.batch), "key_bits": r.key_bits, "log2_n": r.log2_n,
+        return [{"kind": "traversal" if r.kind == 1 else "sort",
+                 "batch": bool(r.batch), "key_bits": r.key_bits, "log2_n": r.log2_n,
                  "digit_bits": r.sort_digit_bits, "tile": r.sort_tile,
-                 "strategy": SORT_NAMES.get(r.sort_strategy, "?"),
+                 "strategy": SORT_NAMES.get(r.sort_strategy, "-"),
                  "picked": bool(r.picked), "ms": round(r.ms, 4),
-                 "scalar_bits": r.scalar_bits} for r in buf[:n]]
+                 "scalar_bits": r.scalar_bits, "reorder": bool(r.trace_reorder),
+                 "waves_per_simd": r.trace_waves_per_simd} for r in buf[:n]]
 
     def workspace_bytes(self):
         return int(lib().lbvh_ctx_workspace_bytes(self.handle))

@@ -207,13 +207,17 @@ encode<uint64_t>(float qx, float qy, float qz)
 
 constexpr int kHistMaxWords = 8192; // passes * radix, 32 KiB of LDS
 constexpr int kEncodeUnroll = 4;
+// 1,024 threads: one encode workgroup per sort chunk leaves a single
+// workgroup on most CUs at 1M keys, so it brings 16 waves to hide the load
+// latency instead of 4.
+constexpr int kEncodeBlock = 1024;
 
 // counts != nullptr: workgroup g encodes the chunk [g * chunk, (g + 1) *
 // chunk) and also writes its counts of pass first_pass to
 // counts[d * gridDim.x + g] -- the chunk counts of the reduce-then-scan
 // sort's first pass, which then needs no count pass of its own.
 template<typename KeyT>
-__global__ __launch_bounds__(kBlock) void
+__global__ __launch_bounds__(kEncodeBlock) void
 k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
          const float* __restrict__ cz, uint64_t n,
          const float* __restrict__ bounds, KeyT* __restrict__ codes,
@@ -223,7 +227,7 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
   extern __shared__ uint32_t s_hist[];
   const int radix = 1 << digit_bits;
   const int words = radix * (passes - first_pass);
-  for (int j = threadIdx.x; j < words; j += kBlock)
+  for (int j = threadIdx.x; j < words; j += kEncodeBlock)
     s_hist[j] = 0u;
   const float dom = sizeof(KeyT) == 4 ? 1024.0f : 1048576.0f;
   float bmin[3], scale[3];
@@ -236,14 +240,14 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
   }
   __syncthreads();
   // grid-stride over [0, n), or this workgroup's chunk
-  uint64_t first = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  uint64_t first = uint64_t(blockIdx.x) * kEncodeBlock + threadIdx.x;
   uint64_t limit = n;
-  uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  uint64_t stride = uint64_t(gridDim.x) * kEncodeBlock;
   if (counts) {
     const uint64_t b = uint64_t(blockIdx.x) * chunk;
     first = b + threadIdx.x;
     limit = (b + chunk < n) ? b + chunk : n;
-    stride = kBlock;
+    stride = kEncodeBlock;
   }
   const KeyT mask = KeyT(radix - 1);
   for (uint64_t i0 = first; i0 < limit; i0 += stride * kEncodeUnroll) {
@@ -280,13 +284,13 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
   __syncthreads();
   // the workgroup's counts go straight into the global histogram (zeroed
   // by k_centroid_bounds): no per-workgroup rows and no reduction kernel
-  for (int j = threadIdx.x; j < words; j += kBlock) {
+  for (int j = threadIdx.x; j < words; j += kEncodeBlock) {
     const uint32_t v = s_hist[j];
     if (v)
       atomicAdd(&hist[j], v);
   }
   if (counts)
-    for (int d = threadIdx.x; d < radix; d += kBlock)
+    for (int d = threadIdx.x; d < radix; d += kEncodeBlock)
       counts[size_t(d) * gridDim.x + blockIdx.x] = s_hist[d];
 }
 
@@ -1376,7 +1380,7 @@ centroid_partials_needed(uint64_t n)
 uint32_t
 encode_groups(uint64_t n)
 {
-  return grid_for(n, uint64_t(kBlock) * 16, kEncodeMaxBlocks);
+  return grid_for(n, uint64_t(kEncodeBlock) * 4, kEncodeMaxBlocks);
 }
 
 hipError_t
@@ -1414,11 +1418,11 @@ launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, 
   }
   uint32_t* h = hist + size_t(first_pass) * (size_t(1) << digit_bits);
   if (code_bits == 30)
-    hipLaunchKernelGGL(k_encode<uint32_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
+    hipLaunchKernelGGL(k_encode<uint32_t>, dim3(g), dim3(kEncodeBlock), lds, s, cx, cy, cz, n,
                        bounds, static_cast<uint32_t*>(codes), h, digit_bits, first_pass,
                        passes, counts, chunk);
   else
-    hipLaunchKernelGGL(k_encode<uint64_t>, dim3(g), dim3(kBlock), lds, s, cx, cy, cz, n,
+    hipLaunchKernelGGL(k_encode<uint64_t>, dim3(g), dim3(kEncodeBlock), lds, s, cx, cy, cz, n,
                        bounds, static_cast<uint64_t*>(codes), h, digit_bits, first_pass,
                        passes, counts, chunk);
   return hipGetLastError();

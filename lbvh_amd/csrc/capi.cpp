@@ -255,16 +255,26 @@ alloc_bvh(lbvh_ctx* c, lbvh_bvh* b, bool keep_codes)
     { reinterpret_cast<void**>(&b->d_ids), keep_codes ? n * 4 : 0 },
     { reinterpret_cast<void**>(&b->d_leafprim), b->leaf_order == LBVH_LEAF_PRIMITIVE_ID ? n * 4 : 0 },
   };
+  // one allocation for all of them (256-byte aligned parts): an allocation
+  // costs host time on the submitting thread, whatever its size
+  uint64_t total = 0;
+  for (auto& r : req)
+    if (r.bytes)
+      total += (r.bytes + 255) & ~uint64_t(255);
+  hipError_t e = hipMalloc(&b->block, total);
+  if (e != hipSuccess || !b->block) {
+    (void)hipGetLastError();
+    b->block = nullptr;
+    return fail(c, LBVH_E_OOM, "device allocation for the tree failed");
+  }
+  char* at = static_cast<char*>(b->block);
   for (auto& r : req) {
     if (!r.bytes)
       continue;
-    hipError_t e = hipMalloc(r.p, r.bytes);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(c, LBVH_E_OOM, "device allocation for the tree failed");
-    }
-    b->bytes += r.bytes;
+    *r.p = at;
+    at += (r.bytes + 255) & ~uint64_t(255);
   }
+  b->bytes += total;
   return LBVH_OK;
 }
 
@@ -276,15 +286,7 @@ free_bvh(lbvh_bvh* b)
   int cur = 0;
   (void)hipGetDevice(&cur);
   (void)hipSetDevice(b->device);
-  dev_free(b->d_nodes);
-  dev_free(b->d_parent);
-  dev_free(b->d_flags);
-  dev_free(b->d_bounds);
-  dev_free(b->d_codes);
-  dev_free(b->d_ids);
-  dev_free(b->d_leafprim);
-  dev_free(b->d_other_end);
-  dev_free(b->d_wide);
+  dev_free(b->block); // every tree buffer
   if (b->owns_prims)
     dev_free(b->d_prims);
   (void)hipSetDevice(cur);
@@ -294,6 +296,60 @@ free_bvh(lbvh_bvh* b)
 lbvh_status build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
                          const lbvh_build_opts* opts_in, bool owns, lbvh_bvh** out,
                          const SortConfig* forced = nullptr);
+
+lbvh_status
+reserve_build(lbvh_ctx* c, uint64_t n, int code_bits, const SortConfig& cfg, bool with_ids)
+{
+  const int key_bits = code_bits == 30 ? 30 : 60;
+  const int passes = sort_passes(key_bits, cfg.digit_bits);
+  const uint64_t kw = code_bits == 30 ? 4 : 8;
+  lbvh_status st;
+  if ((st = ensure(c, c->cent, 12 * n)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->keys_a, kw * n)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->keys_b, kw * n)) != LBVH_OK)
+    return st;
+  if (with_ids) {
+    if ((st = ensure(c, c->ids_a, 4 * n)) != LBVH_OK)
+      return st;
+    if ((st = ensure(c, c->ids_b, 4 * n)) != LBVH_OK)
+      return st;
+  }
+  if ((st = ensure(c, c->partials, 24ull * centroid_partials_needed(n))) != LBVH_OK)
+    return st;
+  const uint32_t small_words = kSmallHist + uint32_t(passes) * (1u << cfg.digit_bits);
+  if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->tree, tree_scratch_bytes(n))) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->tree_tickets, 4 * tree_ticket_words(n), true)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->counts, 4 * sort_counts_words(n, cfg))) != LBVH_OK)
+    return st;
+  const uint64_t sw = sort_status_words(n, cfg);
+  if (c->status.cap < sw * 8) {
+    if ((st = ensure(c, c->status, sw * 8, true)) != LBVH_OK)
+      return st;
+  }
+  return LBVH_OK;
+}
+
+SortConfig
+untuned_sort(int key_bits, uint64_t n)
+{
+  SortConfig c = default_sort();
+  if (n < kTuneMinPrims)
+    return c;
+  // the hybrid at 10-bit digits and 4,096-key tiles: the tuner's pick at 1M
+  // and 16M 63-bit keys and for batches
+  SortConfig h;
+  h.digit_bits = 10;
+  h.items = 16;
+  h.strategy = 2;
+  h.global_digits = hybrid_global_digits(key_bits, h.digit_bits, n);
+  return h.global_digits <= sort_passes(key_bits, h.digit_bits) - 2 ? h : c;
+}
 
 // The sort configuration of a single-scene build: explicit options, else
 // the context's tuned choice for this key width and size, tuning now (every
@@ -307,7 +363,7 @@ choose_sort(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n, const lbvh_
     *cfg = sort_from_opts(opts, key_bits, n);
     return LBVH_OK;
   }
-  *cfg = default_sort();
+  *cfg = untuned_sort(key_bits, n);
   if (!c->autotune || n < kTuneMinPrims)
     return LBVH_OK;
   const uint64_t key = tune_key(false, key_bits, n);
@@ -377,41 +433,15 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   const size_t kw = opts.code_bits == 30 ? 4 : 8;
   const bool with_ids = opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID || opts.keep_codes;
 
-  lbvh_status st;
-  if ((st = ensure(c, c->cent, 12 * n)) != LBVH_OK)
-    return st;
-  if ((st = ensure(c, c->keys_a, kw * n)) != LBVH_OK)
-    return st;
-  if ((st = ensure(c, c->keys_b, kw * n)) != LBVH_OK)
-    return st;
-  if (with_ids) {
-    if ((st = ensure(c, c->ids_a, 4 * n)) != LBVH_OK)
-      return st;
-    if ((st = ensure(c, c->ids_b, 4 * n)) != LBVH_OK)
-      return st;
-  }
-  if ((st = ensure(c, c->partials, 24ull * centroid_partials_needed(n))) != LBVH_OK)
+  lbvh_status st = reserve_build(c, n, int(opts.code_bits), cfg, with_ids);
+  if (st != LBVH_OK)
     return st;
   // reduce-then-scan and hybrid sorts: the encode counts the first pass's
   // chunks itself (one encode workgroup per chunk)
   const bool chunked = cfg.strategy == 1 || cfg.strategy == 2;
   const uint32_t rts_g = chunked ? rts_groups(n, cfg, nullptr) : 0u;
-
   const uint32_t small_words = kSmallHist + uint32_t(passes) * radix;
-  if ((st = ensure(c, c->small, 4ull * small_words, true)) != LBVH_OK)
-    return st;
-  if ((st = ensure(c, c->tree, tree_scratch_bytes(n))) != LBVH_OK)
-    return st;
-  if ((st = ensure(c, c->tree_tickets, 4 * tree_ticket_words(n), true)) != LBVH_OK)
-    return st;
-  const uint64_t counts_words = sort_counts_words(n, cfg);
-  if ((st = ensure(c, c->counts, 4 * counts_words)) != LBVH_OK)
-    return st;
   const uint64_t sw = sort_status_words(n, cfg);
-  if (c->status.cap < sw * 8) {
-    if ((st = ensure(c, c->status, sw * 8, true)) != LBVH_OK)
-      return st;
-  }
   if (c->epoch_next + kMaxSortPasses >= (1u << 24)) {
     HIP_TRY(c, launch_fill_u32(static_cast<uint32_t*>(c->status.p), c->status.cap / 4, 0u,
                                c->stream));
@@ -584,6 +614,10 @@ lbvh_ctx_create(int device, lbvh_ctx** out)
     return LBVH_E_HIP;
   }
   c->stream = c->own_stream;
+  // set-up of the allocation-free build on this device; if it fails here,
+  // that entry point retries on its first call and reports the error
+  if (noalloc_init(c) != LBVH_OK)
+    c->last_error.clear();
   *out = c;
   return LBVH_OK;
 }
@@ -597,7 +631,8 @@ lbvh_ctx_destroy(lbvh_ctx* c)
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = { &c->cent, &c->keys_a, &c->keys_b, &c->ids_a, &c->ids_b, &c->partials,
                      &c->small, &c->status, &c->counts, &c->treelets,
-                     &c->io_in, &c->io_out, &c->tree, &c->tree_tickets, &c->treelet_words };
+                     &c->io_in, &c->io_out, &c->tree, &c->tree_tickets, &c->treelet_words,
+                     &c->ray_keys, &c->ray_ids, &c->ray_counts, &c->ray_hist };
   for (auto* b : bufs)
     dev_free(b->p);
   for (auto& es : c->sets)
@@ -614,10 +649,6 @@ lbvh_ctx_set_stream(lbvh_ctx* c, void* stream)
 {
   if (!c)
     return LBVH_E_INVALID;
-  // set-up of the allocation-free build on this device; if it fails here,
-  // that entry point retries on its first call and reports the error
-  if (noalloc_init(c) != LBVH_OK)
-    c->last_error.clear();
   c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
   return LBVH_OK;
 }
@@ -717,7 +748,8 @@ lbvh_ctx_workspace_bytes(const lbvh_ctx* c)
   return c->cent.cap + c->keys_a.cap + c->keys_b.cap + c->ids_a.cap + c->ids_b.cap +
          c->partials.cap + c->small.cap + c->status.cap + c->counts.cap +
          c->treelets.cap + c->io_in.cap + c->io_out.cap + c->tree.cap + c->tree_tickets.cap +
-         c->treelet_words.cap;
+         c->treelet_words.cap + c->ray_keys.cap + c->ray_ids.cap + c->ray_counts.cap +
+         c->ray_hist.cap;
 }
 
 lbvh_status
@@ -915,15 +947,19 @@ lbvh_ctx_tune_log(const lbvh_ctx* c, lbvh_tune_record* out, uint32_t cap)
   const uint32_t n = uint32_t(c->tune_log.size());
   for (uint32_t i = 0; out && i < n && i < cap; i++) {
     const TuneRecord& r = c->tune_log[i];
+    const bool sort = r.kind == 0;
     out[i].batch = uint32_t(r.batch);
     out[i].key_bits = uint32_t(r.key_bits);
     out[i].log2_n = uint32_t(r.log2n);
-    out[i].sort_digit_bits = uint32_t(r.cfg.digit_bits);
-    out[i].sort_tile = uint32_t(r.cfg.items * 256);
-    out[i].sort_strategy = uint32_t(r.cfg.strategy + 1);
+    out[i].sort_digit_bits = sort ? uint32_t(r.cfg.digit_bits) : 0u;
+    out[i].sort_tile = sort ? uint32_t(r.cfg.items * 256) : 0u;
+    out[i].sort_strategy = sort ? uint32_t(r.cfg.strategy + 1) : 0u;
     out[i].picked = r.picked ? 1u : 0u;
     out[i].ms = r.ms;
     out[i].scalar_bits = uint32_t(r.scalar_bits);
+    out[i].kind = uint32_t(r.kind);
+    out[i].trace_reorder = uint32_t(r.trace.reorder);
+    out[i].trace_waves_per_simd = uint32_t(r.trace.waves_per_simd);
   }
   return n;
 }
@@ -970,6 +1006,132 @@ lbvh_bvh_destroy(lbvh_bvh* b)
   free_bvh(b);
 }
 
+} // extern "C"
+
+namespace {
+
+// Traversal candidates of the tuner: rays in key order or as given, and
+// waves per SIMD (8: as many as the registers allow).
+std::vector<TraceConfig>
+trace_candidates()
+{
+  std::vector<TraceConfig> v;
+  for (int reorder : { 0, 1 })
+    for (int waves : { 8, 6, 4 }) {
+      TraceConfig t;
+      t.reorder = reorder;
+      t.waves_per_simd = waves;
+      v.push_back(t);
+    }
+  return v;
+}
+
+uint64_t
+trace_tune_key(int mode, uint64_t nrays)
+{
+  return (uint64_t(1) << 56) | (uint64_t(mode) << 40) | uint64_t(ceil_log2_u64(nrays));
+}
+
+// Traversals below this many rays are launch-bound: the default is used.
+constexpr uint64_t kTraceTuneMinRays = 1u << 14;
+
+// Queues one closest-hit traversal with configuration tc (timed as one call).
+lbvh_status
+queue_trace(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, uint64_t nrays, int mode,
+            uint32_t* d_prim, float* d_t, float* d_uv, const TraceConfig& tc)
+{
+  const uint32_t* order = nullptr;
+  lbvh_status st;
+  SortConfig sc;
+  sc.digit_bits = 8;
+  sc.items = 16;
+  sc.strategy = 1; // reduce-then-scan: two passes, no look-back state
+  const bool reorder = tc.reorder && nrays >= 2 && nrays < (uint64_t(1) << 31);
+  if (reorder) {
+    const uint32_t groups = rts_groups(nrays, sc, nullptr);
+    const uint64_t chunk = rts_chunk_keys(nrays, sc);
+    const uint64_t counts_words = sort_counts_words(nrays, sc);
+    if ((st = ensure(c, c->ray_keys, 8 * nrays)) != LBVH_OK ||
+        (st = ensure(c, c->ray_ids, 8 * nrays)) != LBVH_OK ||
+        (st = ensure(c, c->ray_counts, 4 * counts_words)) != LBVH_OK ||
+        (st = ensure(c, c->ray_hist, 4 * 512)) != LBVH_OK)
+      return st;
+    uint32_t* keys = static_cast<uint32_t*>(c->ray_keys.p);
+    uint32_t* ids = static_cast<uint32_t*>(c->ray_ids.p);
+    SortWorkspace ws;
+    ws.hist = static_cast<uint32_t*>(c->ray_hist.p);
+    ws.counts = static_cast<uint32_t*>(c->ray_counts.p);
+    ws.counts_words = c->ray_counts.cap / 4;
+    ws.first_counts = true;
+    begin_timed(c, 2);
+    record(c, 0);
+    HIP_TRY(c, launch_fill_u32(ws.hist, 512, 0u, c->stream));
+    HIP_TRY(c, launch_ray_keys(d_rays, nrays, b->d_bounds, keys, ws.hist, ws.counts, groups, chunk,
+                               c->stream));
+    bool in_b = false;
+    HIP_TRY(c, launch_sort(kRayKeyBits, 4, nrays, keys, keys + nrays, ids, ids + nrays, sc, ws, 0u,
+                           &in_b, c->stream));
+    order = in_b ? ids + nrays : ids;
+  } else {
+    begin_timed(c, 2);
+    record(c, 0);
+  }
+  HIP_TRY(c, launch_trace(mode, b->d_wide, b->d_prims, b->n, nullptr, nullptr, 1, d_rays, nrays,
+                          d_prim, d_t, d_uv, c->stream, order, tc.waves_per_simd));
+  record(c, 1, true);
+  return LBVH_OK;
+}
+
+// The traversal configuration for this intersector mode and batch size:
+// the context's tuned choice, tuning now (every candidate timed on this very
+// call, twice) when there is none yet.  Results are identical for every
+// candidate.
+lbvh_status
+choose_trace(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, uint64_t nrays, int mode,
+             uint32_t* d_prim, float* d_t, float* d_uv, TraceConfig* tc)
+{
+  *tc = TraceConfig();
+  if (!c->autotune || nrays < kTraceTuneMinRays || nrays >= (uint64_t(1) << 31))
+    return LBVH_OK;
+  const uint64_t key = trace_tune_key(mode, nrays);
+  auto it = c->tuned_trace.find(key);
+  if (it != c->tuned_trace.end()) {
+    *tc = it->second;
+    return LBVH_OK;
+  }
+  float best = -1.0f;
+  const size_t first = c->tune_log.size();
+  for (const TraceConfig& cand : trace_candidates()) {
+    float ms = -1.0f;
+    for (int rep = 0; rep < 2; rep++) {
+      lbvh_status st;
+      ms = time_call(c, [&] { return queue_trace(c, b, d_rays, nrays, mode, d_prim, d_t, d_uv,
+                                                 cand); }, &st);
+      if (st != LBVH_OK)
+        return st;
+      if ((st = sync_and_check(c)) != LBVH_OK)
+        return st;
+    }
+    TuneRecord r{ 0, 0, ceil_log2_u64(nrays), SortConfig(), ms, false };
+    r.kind = 1;
+    r.trace = cand;
+    c->tune_log.push_back(r);
+    if (ms >= 0.0f && (best < 0.0f || ms < best)) {
+      best = ms;
+      *tc = cand;
+    }
+  }
+  for (size_t i = first; i < c->tune_log.size(); i++)
+    c->tune_log[i].picked = c->tune_log[i].trace.reorder == tc->reorder &&
+                            c->tune_log[i].trace.waves_per_simd == tc->waves_per_simd;
+  c->tuned_trace[key] = *tc;
+  return LBVH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
 lbvh_status
 lbvh_trace_closest_device(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, uint64_t nrays,
                           int mode, uint32_t* d_prim, float* d_t, float* d_uv)
@@ -986,12 +1148,12 @@ lbvh_trace_closest_device(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, u
   lbvh_status st = set_device(c);
   if (st != LBVH_OK)
     return st;
-  begin_timed(c, 2);
-  record(c, 0);
-  HIP_TRY(c, launch_trace(mode, b->d_wide, b->d_prims, b->n, nullptr, nullptr, 1, d_rays, nrays, d_prim, d_t, d_uv,
-                          c->stream));
-  record(c, 1, true);
-  return LBVH_OK;
+  if (nrays == 0)
+    return LBVH_OK;
+  TraceConfig tc;
+  if ((st = choose_trace(c, b, d_rays, nrays, mode, d_prim, d_t, d_uv, &tc)) != LBVH_OK)
+    return st;
+  return queue_trace(c, b, d_rays, nrays, mode, d_prim, d_t, d_uv, tc);
 }
 
 lbvh_status

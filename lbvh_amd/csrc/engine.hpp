@@ -121,11 +121,32 @@ hipError_t launch_enum_treelets(uint64_t n, const uint64_t* prim_off, uint32_t n
 
 // Closest-hit traversal over the 64-byte traversal records (wide).  For a
 // batch (prim_off != nullptr) ray r belongs to the scene s with ray_off[s] <=
-// r < ray_off[s + 1] and hits report scene-local primitive ids.
+// r < ray_off[s + 1] and hits report scene-local primitive ids.  order (one
+// tree only, may be null): a permutation of the rays; the launch takes them
+// in that order and writes each hit to the ray's own slot.  waves_per_simd
+// (0 or 8: as many as the registers allow) caps the waves sharing a SIMD.
 hipError_t launch_trace(int mode, const void* wide, const float* tris, uint64_t n,
                         const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg,
                         const float* rays, uint64_t nrays, uint32_t* prim, float* t, float* uv,
-                        hipStream_t s);
+                        hipStream_t s, const uint32_t* order = nullptr, int waves_per_simd = 0);
+
+// Traversal configuration chosen by the tuner: whether rays are put in the
+// order of their keys first, and the waves per SIMD.
+struct TraceConfig
+{
+  int reorder = 0;
+  int waves_per_simd = 8;
+};
+
+// Sort keys of a ray batch for the traversal order (kRayKeyBits bits: the
+// direction octant above a coarse Morton code of a point along the ray, over
+// the tree's centroid bounds): `groups` workgroups of `chunk` rays each
+// write their keys, the first 8-bit pass's chunk counts (counts[d * groups +
+// g]) and add both passes' digit counts to hist[0 .. 511] (zero before).
+constexpr int kRayKeyBits = 15;
+hipError_t launch_ray_keys(const float* rays, uint64_t nrays, const float* bounds,
+                           uint32_t* keys, uint32_t* hist, uint32_t* counts, uint32_t groups,
+                           uint64_t chunk, hipStream_t s);
 
 // One tree's share of a coalesced traversal launch: its rays are global rays
 // [ray_begin, next job's ray_begin) (the last job ends at the launch's ray

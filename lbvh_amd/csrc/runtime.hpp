@@ -46,9 +46,11 @@ struct TuneRecord
   int key_bits;
   int log2n;
   SortConfig cfg;
-  float ms; // device time of the whole build with this configuration
+  float ms; // device time of the whole build (or traversal) with this configuration
   bool picked;
   int scalar_bits = 32; // 32: builder<float> builds, 64: builder<double> builds
+  int kind = 0;         // 0: a build's sort, 1: a traversal
+  TraceConfig trace{};  // kind 1: the traversal configuration
 };
 
 } // namespace lbvh_rt
@@ -65,10 +67,14 @@ struct lbvh_ctx
   uint32_t epoch_next = 1;
   bool autotune = true;
   std::map<uint64_t, lbvh_eng::SortConfig> tuned; // by tune_key()
+  std::map<uint64_t, lbvh_eng::TraceConfig> tuned_trace; // by trace_tune_key()
   std::vector<lbvh_rt::TuneRecord> tune_log;
   // workspace
   lbvh_rt::DevBuf cent, keys_a, keys_b, ids_a, ids_b, partials, small, status, counts,
     treelets, io_in, io_out, tree, tree_tickets;
+  // traversal order of a ray batch: keys and ray indices (ping-pong), sort
+  // chunk counts, histograms
+  lbvh_rt::DevBuf ray_keys, ray_ids, ray_counts, ray_hist;
   // the treelet list in `treelets` belongs to the tree with this id (0: none);
   // its count and the fit's pull queue live in treelet_words
   uint64_t treelets_of = 0;
@@ -95,6 +101,7 @@ struct lbvh_bvh
   uint32_t* d_leafprim = nullptr; // primitive of each leaf position (primitive-id leaves)
   uint32_t* d_other_end = nullptr; // other end of each internal node's leaf range
   void* d_wide = nullptr;          // traversal records (64 B per internal node)
+  void* block = nullptr;           // the one allocation holding every buffer above
   uint64_t bytes = 0;
   lbvh_eng::SortConfig sort;       // the sort configuration the build used
   uint64_t id = 0;                 // unique per tree in the process (refit caches)
@@ -121,6 +128,9 @@ lbvh_status hip_fail(lbvh_ctx* c, hipError_t e, const char* what);
 lbvh_status ensure(lbvh_ctx* c, DevBuf& b, uint64_t bytes, bool zero_new = false);
 void dev_free(void* p);
 lbvh_status set_device(lbvh_ctx* c);
+// Finds this device's copy of the allocation-free build's scratch array (the
+// device must be current; once per device).
+lbvh_status noalloc_init(lbvh_ctx* c);
 void begin_timed(lbvh_ctx* c, int kind);
 void record(lbvh_ctx* c, int idx, bool final = false);
 bool valid_sort_config(const SortConfig& c, int key_bits);
@@ -128,11 +138,15 @@ lbvh_status sync_and_check(lbvh_ctx* c);
 
 // ---- sort autotuner ----------------------------------------------------------
 // The default configuration, the candidates timed by the tuner, and whether
-// Finds this device's copy of the allocation-free build's scratch array (the
-// device must be current; once per device).
-lbvh_status noalloc_init(lbvh_ctx* c);
 // build options override the choice (then the tuner is not consulted).
 SortConfig default_sort();
+// The sort a single build uses without tuning (tuner off, or before a
+// bucket is tuned): the hybrid from 65,536 keys, else the default.
+SortConfig untuned_sort(int key_bits, uint64_t n);
+// Grows a context's workspace to what a build of n primitives with this
+// code width and sort configuration needs.
+lbvh_status reserve_build(lbvh_ctx* c, uint64_t n, int code_bits, const SortConfig& cfg,
+                          bool with_ids);
 std::vector<SortConfig> sort_candidates(bool batch, int key_bits, uint64_t n);
 bool sort_overridden(const lbvh_build_opts& o);
 SortConfig sort_from_opts(const lbvh_build_opts& o, int key_bits, uint64_t n);

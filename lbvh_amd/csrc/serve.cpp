@@ -429,6 +429,16 @@ start(lbvh_server* s, Req* q, Scene* sc)
         return bad(LBVH_E_HIP, "triangle upload failed");
     }
     const uint64_t ws0 = lbvh_ctx_workspace_bytes(lane);
+    // a workspace that must grow waits for the device (the old buffers are
+    // freed), so it grows to the next power of two at once
+    uint64_t want = 2;
+    while (want < ntri)
+      want <<= 1;
+    const int key_bits = o.code_bits == 30 ? 30 : 60;
+    lbvh_status rs = reserve_build(lane, want, int(o.code_bits), untuned_sort(key_bits, want),
+                                   o.leaf_order == LBVH_LEAF_PRIMITIVE_ID);
+    if (rs != LBVH_OK)
+      return bad(rs, lbvh_ctx_last_error(lane));
     lbvh_status bs = lbvh_build_triangles_device(lane, sc->d_tri, ntri, &o, &sc->bvh);
     if (bs != LBVH_OK)
       return bad(bs, lbvh_ctx_last_error(lane));
@@ -723,6 +733,19 @@ lbvh_server_create(int device, const lbvh_server_opts* opts, lbvh_server** out)
     // a request's latency must not include tuning runs
     lbvh_ctx_set_autotune(c, 0);
     s->lanes.push_back(c);
+    if (o.reserve_prims >= 2) {
+      const int key_bits = o.code_bits == 30 ? 30 : 60;
+      st = reserve_build(c, o.reserve_prims, int(o.code_bits),
+                         untuned_sort(key_bits, o.reserve_prims),
+                         o.leaf_order == LBVH_LEAF_PRIMITIVE_ID);
+      if (st != LBVH_OK) {
+        for (auto* l : s->lanes)
+          lbvh_ctx_destroy(l);
+        delete s;
+        return st;
+      }
+      s->lane_ws += lbvh_ctx_workspace_bytes(c);
+    }
   }
   if (hipStreamCreateWithFlags(&s->trace_stream, hipStreamNonBlocking) != hipSuccess) {
     for (auto* l : s->lanes)
@@ -735,6 +758,9 @@ lbvh_server_create(int device, const lbvh_server_opts* opts, lbvh_server** out)
   s->mem_cap = o.memory_cap_bytes;
   if (free_b && s->mem_cap > uint64_t(free_b) - (uint64_t(free_b) >> 4))
     s->mem_cap = uint64_t(free_b) - (uint64_t(free_b) >> 4); // keep 1/16 of free memory spare
+  s->mem_used = s->lane_ws; // reserved lane workspaces
+  s->mem_peak = s->mem_used;
+  s->lane_ws = 0;
   s->worker = std::thread(worker_main, s);
   *out = s;
   return LBVH_OK;

@@ -16,6 +16,8 @@ namespace lbvh_dev {
 using lbvh_eng::TraceJob;
 
 constexpr int kTraceBlock = 256;
+// Waves per SIMD the registers allow (the occupancy knob's "no limit").
+constexpr int kTraceMaxWavesPerSimd = 8;
 constexpr int kStack = 128;
 // Stack entries below this depth live in LDS (16 KB per workgroup keeps
 // eight workgroups per CU); deeper ones in the private array.
@@ -201,13 +203,16 @@ job_of(const TraceJob* __restrict__ jobs, uint32_t njobs, uint64_t i)
   return lo;
 }
 
+// order (single layout only, may be null): ray k of the launch is ray
+// order[k] of the input, and its hit is written there; a permutation that
+// puts rays which visit the same nodes into the same wave.
 template<int MODE, int LAYOUT>
 __global__ __launch_bounds__(kTraceBlock) void
 k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
         const uint64_t* __restrict__ prim_off, const uint64_t* __restrict__ ray_off,
         uint32_t nseg, const TraceJob* __restrict__ jobs, const float* __restrict__ rays,
-        uint64_t nrays, uint32_t* __restrict__ prim_out, float* __restrict__ t_out,
-        float* __restrict__ uv_out)
+        const uint32_t* __restrict__ order, uint64_t nrays, uint32_t* __restrict__ prim_out,
+        float* __restrict__ t_out, float* __restrict__ uv_out)
 {
   // batches and jobs: the ray offsets of the scenes / jobs this workgroup's
   // rays fall in, staged once (s_rb) when they are at most kTraceBlock
@@ -273,6 +278,9 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     t_out = jb.t;
     uv_out = jb.uv;
     i -= jb.ray_begin;
+  } else {
+    if (order)
+      i = order[i];
   }
   const float4* rp = reinterpret_cast<const float4*>(rays + 8 * i);
   const float4 r0 = rp[0], r1 = rp[1];
@@ -431,6 +439,75 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     *reinterpret_cast<float2*>(uv_out + 2 * i) = make_float2(best_u, best_v);
 }
 
+// ---- traversal order of a ray batch ---------------------------------------------
+// Key of a ray: its direction octant (3 bits) above the 12-bit Morton code of
+// the point o + d * reach, quantized to 16 cells per axis over the tree's
+// centroid bounds grown by reach on every side, reach being the bounds'
+// largest extent.  Rays with one key start in the same part of the scene and
+// head the same way (for primary rays from one eye: neighbouring pixels), so
+// a wave of them visits much the same nodes in the same order.  Sorted with
+// two 8-bit radix passes; each workgroup encodes one chunk of the
+// reduce-then-scan sort and writes that chunk's first-pass digit counts and
+// its share of both passes' histograms (zero before).
+
+constexpr int kRayKeyBlock = 256;
+
+__device__ __forceinline__ uint32_t
+spread4(uint32_t v)
+{
+  return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4) | ((v & 8u) << 6);
+}
+
+__global__ __launch_bounds__(kRayKeyBlock) void
+k_ray_keys(const float* __restrict__ rays, uint64_t nrays, const float* __restrict__ bounds,
+           uint32_t* __restrict__ keys, uint32_t* __restrict__ hist,
+           uint32_t* __restrict__ counts, uint64_t chunk)
+{
+  __shared__ uint32_t s_hist[2 * 256];
+  for (int j = threadIdx.x; j < 2 * 256; j += kRayKeyBlock)
+    s_hist[j] = 0u;
+  float lo[3], scale[3];
+  float reach = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    reach = fmaxf(reach, bounds[3 + k] - bounds[k]);
+  reach = fmaxf(reach, 1e-20f);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = bounds[k] - reach;
+    scale[k] = 16.0f / (bounds[3 + k] + reach - lo[k]);
+  }
+  __syncthreads();
+  const uint64_t b = uint64_t(blockIdx.x) * chunk;
+  const uint64_t end = (b + chunk < nrays) ? b + chunk : nrays;
+  for (uint64_t i = b + threadIdx.x; i < end; i += kRayKeyBlock) {
+    const float4* rp = reinterpret_cast<const float4*>(rays + 8 * i);
+    const float4 r0 = rp[0], r1 = rp[1];
+    const float o[3] = { r0.x, r0.y, r0.z };
+    const float d[3] = { r0.w, r1.x, r1.y };
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float p = o[k] + d[k] * reach;
+      const float c = fminf(fmaxf((p - lo[k]) * scale[k], 0.0f), 15.0f); // NaN -> 0
+      q[k] = uint32_t(c);
+    }
+    const uint32_t oct = (d[0] < 0.0f ? 4u : 0u) | (d[1] < 0.0f ? 2u : 0u) | (d[2] < 0.0f ? 1u : 0u);
+    const uint32_t key = (oct << 12) | (spread4(q[0]) << 2) | (spread4(q[1]) << 1) | spread4(q[2]);
+    keys[i] = key;
+    atomicAdd(&s_hist[key & 0xffu], 1u);
+    atomicAdd(&s_hist[256 + (key >> 8)], 1u);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < 2 * 256; j += kRayKeyBlock) {
+    const uint32_t v = s_hist[j];
+    if (v)
+      atomicAdd(&hist[j], v);
+  }
+  for (int dd = threadIdx.x; dd < 256; dd += kRayKeyBlock)
+    counts[size_t(dd) * gridDim.x + blockIdx.x] = s_hist[dd];
+}
+
 } // namespace lbvh_dev
 
 namespace lbvh_eng {
@@ -439,34 +516,53 @@ using namespace lbvh_dev;
 
 template<int MODE>
 static void
-trace_mode(uint32_t g, hipStream_t s, const WideNode* nd, const float* tris, uint64_t n,
-           const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg, const TraceJob* jobs,
-           const float* rays, uint64_t nrays, uint32_t* prim, float* t, float* uv)
+trace_mode(uint32_t g, size_t lds, hipStream_t s, const WideNode* nd, const float* tris,
+           uint64_t n, const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg,
+           const TraceJob* jobs, const float* rays, const uint32_t* order, uint64_t nrays,
+           uint32_t* prim, float* t, float* uv)
 {
   if (jobs)
-    hipLaunchKernelGGL((k_trace<MODE, kLayoutJobs>), dim3(g), dim3(kTraceBlock), 0, s,
-                       nullptr, nullptr, 0u, nullptr, nullptr, nseg, jobs, nullptr, nrays, nullptr,
-                       nullptr, nullptr);
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutJobs>), dim3(g), dim3(kTraceBlock), lds, s,
+                       nullptr, nullptr, 0u, nullptr, nullptr, nseg, jobs, nullptr, nullptr,
+                       nrays, nullptr, nullptr, nullptr);
   else if (prim_off)
-    hipLaunchKernelGGL((k_trace<MODE, kLayoutBatch>), dim3(g), dim3(kTraceBlock), 0, s, nd,
-                       tris, uint32_t(n), prim_off, ray_off, nseg, nullptr, rays, nrays, prim, t,
-                       uv);
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutBatch>), dim3(g), dim3(kTraceBlock), lds, s, nd,
+                       tris, uint32_t(n), prim_off, ray_off, nseg, nullptr, rays, nullptr, nrays,
+                       prim, t, uv);
   else
-    hipLaunchKernelGGL((k_trace<MODE, kLayoutSingle>), dim3(g), dim3(kTraceBlock), 0, s, nd,
-                       tris, uint32_t(n), nullptr, nullptr, 1u, nullptr, rays, nrays, prim, t, uv);
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutSingle>), dim3(g), dim3(kTraceBlock), lds, s, nd,
+                       tris, uint32_t(n), nullptr, nullptr, 1u, nullptr, rays, order, nrays, prim,
+                       t, uv);
+}
+
+// LDS a workgroup reserves beyond its own so that at most `waves` waves
+// share a SIMD (a workgroup puts one wave on each of the CU's 4 SIMDs, so
+// waves per SIMD = workgroups per CU); 0 = no limit beyond the registers'.
+static size_t
+occupancy_lds(int waves)
+{
+  if (waves <= 0 || waves >= kTraceMaxWavesPerSimd)
+    return 0;
+  constexpr size_t kLdsPerCu = 160 * 1024;
+  constexpr size_t kOwn = sizeof(uint2) * kStackLds * kTraceBlock + 4096; // stack + small
+  const size_t per_wg = (kLdsPerCu / size_t(waves)) & ~size_t(511);
+  return per_wg > kOwn ? per_wg - kOwn : 0;
 }
 
 static hipError_t
 trace_dispatch(int mode, const void* nodes, const float* tris, uint64_t n, const uint64_t* prim_off,
                const uint64_t* ray_off, uint32_t nseg, const TraceJob* jobs, const float* rays,
-               uint64_t nrays, uint32_t* prim, float* t, float* uv, hipStream_t s)
+               const uint32_t* order, uint64_t nrays, uint32_t* prim, float* t, float* uv,
+               int waves, hipStream_t s)
 {
   if (nrays == 0)
     return hipSuccess;
   const uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
   const WideNode* nd = static_cast<const WideNode*>(nodes);
+  const size_t lds = occupancy_lds(waves);
 #define LBVH_TM(M)                                                                                \
-  trace_mode<M>(g, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, nrays, prim, t, uv)
+  trace_mode<M>(g, lds, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, order, nrays, prim,   \
+                t, uv)
   switch (mode) {
     case 0:
       LBVH_TM(0);
@@ -487,12 +583,15 @@ trace_dispatch(int mode, const void* nodes, const float* tris, uint64_t n, const
 hipError_t
 launch_trace(int mode, const void* nodes, const float* tris, uint64_t n, const uint64_t* prim_off,
              const uint64_t* ray_off, uint32_t nseg, const float* rays, uint64_t nrays,
-             uint32_t* prim, float* t, float* uv, hipStream_t s)
+             uint32_t* prim, float* t, float* uv, hipStream_t s, const uint32_t* order,
+             int waves_per_simd)
 {
   if (prim_off && (!ray_off || nseg == 0))
     return hipErrorInvalidValue;
-  return trace_dispatch(mode, nodes, tris, n, prim_off, ray_off, nseg, nullptr, rays, nrays, prim,
-                        t, uv, s);
+  if (prim_off && order)
+    return hipErrorInvalidValue;
+  return trace_dispatch(mode, nodes, tris, n, prim_off, ray_off, nseg, nullptr, rays, order, nrays,
+                        prim, t, uv, waves_per_simd, s);
 }
 
 hipError_t
@@ -500,8 +599,19 @@ launch_trace_jobs(int mode, const TraceJob* jobs, uint32_t njobs, uint64_t nrays
 {
   if (!jobs || njobs == 0)
     return hipErrorInvalidValue;
-  return trace_dispatch(mode, nullptr, nullptr, 0, nullptr, nullptr, njobs, jobs, nullptr, nrays,
-                        nullptr, nullptr, nullptr, s);
+  return trace_dispatch(mode, nullptr, nullptr, 0, nullptr, nullptr, njobs, jobs, nullptr, nullptr,
+                        nrays, nullptr, nullptr, nullptr, 0, s);
+}
+
+hipError_t
+launch_ray_keys(const float* rays, uint64_t nrays, const float* bounds, uint32_t* keys,
+                uint32_t* hist, uint32_t* counts, uint32_t groups, uint64_t chunk, hipStream_t s)
+{
+  if (nrays == 0 || groups == 0 || chunk == 0 || uint64_t(groups) * chunk < nrays)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ray_keys, dim3(groups), dim3(kRayKeyBlock), 0, s, rays, nrays, bounds,
+                     keys, hist, counts, chunk);
+  return hipGetLastError();
 }
 
 } // namespace lbvh_eng

@@ -110,3 +110,43 @@ def test_bench_traversal_touched_bytes_constant():
     _, _, _, c2 = oracle.trace(nodes, tri, bounce, oracle.ISECT_REFERENCE, counters=True)
     touched = sum(int(c[0]) * 32 + int(c[1]) * 32 + int(c[2]) * 48 for c in (c1, c2))
     assert touched == bench.TRACE_TOUCHED_BYTES_PER_STEP
+
+
+def _load_bench():
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    return bench
+
+
+def test_bench_fp64_distance_error_matches_oracle():
+    """bench.py reports the hit distance's error against FP64 with its own
+    numpy evaluation of the plane distance; it must agree with the oracle's
+    FP64 evaluation of the same formula, and a planted error of k units of
+    the FP32 spacing must read back as k ulp."""
+    import oracle
+    bench = _load_bench()
+    _, tri = W.scene_uniform(1, 65536, 0.01)
+    nodes, _, _, _ = oracle.build(tri, 30, oracle.LEAF_PRIMITIVE_ID)
+    rays = W.primary_rays(128, 128)
+    p, t, _ = oracle.trace(nodes, tri, rays, oracle.ISECT_REFERENCE)
+    hit = p != 0xFFFFFFFF
+    assert hit.sum() > 1000
+    t64 = oracle.t_fp64(tri, rays, p)
+    rel = np.abs(t[hit] - t64[hit]) / np.abs(t64[hit])
+    e = bench.t_error_vs_fp64(tri, rays, p, t)
+    assert e["hits"] == int(hit.sum())
+    assert math.isclose(e["max_rel"], float("%.4g" % rel.max()), rel_tol=1e-3)
+    assert math.isclose(e["mean_rel"], float("%.4g" % rel.mean()), rel_tol=1e-3)
+    # plant: the FP32 value rounded from t64, moved by 3 spacings
+    t_exact = t64.astype(np.float32)
+    moved = t_exact.copy()
+    for _ in range(3):
+        moved = np.nextafter(moved, np.float32(np.inf))
+    e3 = bench.t_error_vs_fp64(tri, rays, p, np.where(hit, moved, t))
+    e0 = bench.t_error_vs_fp64(tri, rays, p, np.where(hit, t_exact, t))
+    assert e0["max_ulp"] <= 0.5 + 1e-6
+    assert 2.5 <= e3["max_ulp"] <= 3.5 + 1e-6 and e3["mean_ulp"] >= 2.5

@@ -146,9 +146,10 @@ const char* lbvh_ctx_last_error(const lbvh_ctx* ctx);
 /* Autotuner (on by default).  A build whose options leave sort_digit_bits,
  * sort_tile and sort_strategy at 0 uses the sort configuration tuned for its
  * key width and size bucket; the first such build of a bucket times every
- * candidate configuration on its own data (each twice, on the device) and
- * keeps the fastest.  lbvh_trace_closest[_device] does the same per
- * intersector mode and ray-count bucket (from 16,384 rays) over the
+ * candidate configuration on its own data (the faster of two device-timed
+ * runs each) and keeps the fastest.  lbvh_trace_closest[_device] does the
+ * same per intersector mode and ray-count bucket (from 16,384 rays; one
+ * untimed run, then the fastest of three per candidate) over the
  * traversal's configurations: rays traced as given or first put in the order
  * of a key (direction octant and a coarse position along the ray, so that a
  * wave's rays visit the same nodes), and the waves sharing a SIMD (8, 6 or
@@ -375,9 +376,10 @@ typedef struct lbvh_server_opts
   uint32_t leaf_order;       /* (LBVH_LEAF_PRIMITIVE_ID) */
   uint32_t isect_mode;       /* LBVH_ISECT_* (REFERENCE) */
   uint32_t reserve_prims;    /* lanes' build workspaces sized at creation for
-                                scenes of this many triangles (0: none); a
-                                larger scene grows a lane's workspace to the
-                                next power of two, which waits for the device */
+                                scenes of up to this many triangles, rounded
+                                up to a power of two (0: none); a larger scene
+                                grows a lane's workspace to the next power of
+                                two, which waits for the device */
   uint32_t reserved[4];
 } lbvh_server_opts;
 

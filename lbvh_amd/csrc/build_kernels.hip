@@ -91,32 +91,83 @@ k_centroid_bounds(const float* __restrict__ prims, uint64_t n,
 
   float lo[3] = { INFINITY, INFINITY, INFINITY };
   float hi[3] = { -INFINITY, -INFINITY, -INFINITY };
-  // kCentroidUnroll primitives per thread per round, all loads issued first
-  for (uint64_t i0 = tid; i0 < n; i0 += stride * kCentroidUnroll) {
-    // unconditional loads (indexes past the end read the last primitive and
-    // are not used), so that all of them are in flight together
-    Box b[kCentroidUnroll];
+  if constexpr (KIND == kPrimTriangle) {
+    // Triangles are 48 bytes: a lane reading its own would make each of its
+    // three 16-byte loads touch three times the cache lines of a contiguous
+    // load.  The workgroup instead reads kCentroidUnroll x kBlock triangles
+    // as contiguous 16-byte pieces into LDS (the trip count is uniform
+    // across the workgroup), then each thread takes its triangles from there.
+    __shared__ float4 s_tri[kCentroidUnroll * kBlock * 3];
+    constexpr uint64_t kSpan = uint64_t(kCentroidUnroll) * kBlock;
+    for (uint64_t base = uint64_t(blockIdx.x) * kSpan; base < n;
+         base += uint64_t(gridDim.x) * kSpan) {
+      const uint64_t cnt = (n - base < kSpan) ? n - base : kSpan;
+      const float4* src = reinterpret_cast<const float4*>(prims + 12 * base);
+      float4 v[kCentroidUnroll * 3];
 #pragma unroll
-    for (int u = 0; u < kCentroidUnroll; u++) {
-      const uint64_t i = i0 + u * stride;
-      b[u] = load_prim_box(prims, KIND, i < n ? i : n - 1);
-    }
-#pragma unroll
-    for (int u = 0; u < kCentroidUnroll; u++) {
-      const uint64_t i = i0 + u * stride;
-      if (i >= n)
-        continue;
-      float c[3];
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        float s = b[u].lo[k] + b[u].hi[k];
-        c[k] = s * 0.5f;
-        lo[k] = std_min(lo[k], c[k]);
-        hi[k] = std_max(hi[k], c[k]);
+      for (int q = 0; q < kCentroidUnroll * 3; q++) {
+        const uint64_t j = uint64_t(q) * kBlock + threadIdx.x;
+        v[q] = j < 3 * cnt ? src[j] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
-      cx[i] = c[0];
-      cy[i] = c[1];
-      cz[i] = c[2];
+#pragma unroll
+      for (int q = 0; q < kCentroidUnroll * 3; q++)
+        s_tri[q * kBlock + threadIdx.x] = v[q];
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < kCentroidUnroll; u++) {
+        const uint64_t k = uint64_t(u) * kBlock + threadIdx.x;
+        if (k < cnt) {
+          const float4 a = s_tri[3 * k];     // p0.xyz, e0.x
+          const float4 b = s_tri[3 * k + 1]; // e0.yz, e1.xy
+          const float4 c = s_tri[3 * k + 2]; // e1.z, normal
+          const float p0[3] = { a.x, a.y, a.z };
+          const float e0[3] = { a.w, b.x, b.y };
+          const float e1[3] = { b.z, b.w, c.x };
+          const Box bx = tri_box(p0, e0, e1);
+          const uint64_t i = base + k;
+          float cc[3];
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            float s = bx.lo[q] + bx.hi[q];
+            cc[q] = s * 0.5f;
+            lo[q] = std_min(lo[q], cc[q]);
+            hi[q] = std_max(hi[q], cc[q]);
+          }
+          cx[i] = cc[0];
+          cy[i] = cc[1];
+          cz[i] = cc[2];
+        }
+      }
+      __syncthreads(); // s_tri is refilled next round
+    }
+  } else {
+    // kCentroidUnroll primitives per thread per round, all loads issued first
+    for (uint64_t i0 = tid; i0 < n; i0 += stride * kCentroidUnroll) {
+      // unconditional loads (indexes past the end read the last primitive and
+      // are not used), so that all of them are in flight together
+      Box b[kCentroidUnroll];
+#pragma unroll
+      for (int u = 0; u < kCentroidUnroll; u++) {
+        const uint64_t i = i0 + u * stride;
+        b[u] = load_prim_box(prims, KIND, i < n ? i : n - 1);
+      }
+#pragma unroll
+      for (int u = 0; u < kCentroidUnroll; u++) {
+        const uint64_t i = i0 + u * stride;
+        if (i >= n)
+          continue;
+        float c[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          float s = b[u].lo[k] + b[u].hi[k];
+          c[k] = s * 0.5f;
+          lo[k] = std_min(lo[k], c[k]);
+          hi[k] = std_max(hi[k], c[k]);
+        }
+        cx[i] = c[0];
+        cy[i] = c[1];
+        cz[i] = c[2];
+      }
     }
   }
   block_reduce_box(lo, hi, s_box);

@@ -734,9 +734,12 @@ lbvh_server_create(int device, const lbvh_server_opts* opts, lbvh_server** out)
     lbvh_ctx_set_autotune(c, 0);
     s->lanes.push_back(c);
     if (o.reserve_prims >= 2) {
+      // the power of two a build of reserve_prims would grow the lane to
+      uint64_t want = 2;
+      while (want < o.reserve_prims)
+        want <<= 1;
       const int key_bits = o.code_bits == 30 ? 30 : 60;
-      st = reserve_build(c, o.reserve_prims, int(o.code_bits),
-                         untuned_sort(key_bits, o.reserve_prims),
+      st = reserve_build(c, want, int(o.code_bits), untuned_sort(key_bits, want),
                          o.leaf_order == LBVH_LEAF_PRIMITIVE_ID);
       if (st != LBVH_OK) {
         for (auto* l : s->lanes)

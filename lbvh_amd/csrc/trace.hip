@@ -355,17 +355,21 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
   uint32_t ni = 0u;
   bool have = true;
   for (;;) {
+    // One path back to the loop head for both a pruned entry and an index out
+    // of range (only possible for a corrupt node array): with two, the
+    // compiler's structurizer nests loops and duplicates the leaf tests,
+    // measured 12% slower on the clustered 1M scene.
+    bool visit = true;
     if (!have) {
       if (sp == 0)
         break;
       const uint2 e = pop();
-      if (best_t < __uint_as_float(e.y))
-        continue;
+      visit = !(best_t < __uint_as_float(e.y));
       ni = e.x;
     }
     have = false;
-    if (ni + 1 >= nprims)
-      continue; // only possible for a corrupt node array
+    if (!visit || ni + 1 >= nprims)
+      continue;
     const float4* wp = reinterpret_cast<const float4*>(&nodes[ni]);
     const float4 w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
     if (stale) { // reload the top while the node is in flight

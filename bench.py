@@ -721,25 +721,44 @@ def bench_large(args):
 BUILD_BYTES_PER_PRIM_F64 = 48 + 56 + (8 + 8 * 16) + 36 + 104
 
 
-def _tri96(v):
-    """triangle<double> records {p0, e0 = a - b, e1 = c - a, normal} of FP64
-    vertex triples."""
+def _f64_trace_timing(ctx, d_boxes, tri96, args):
+    """traverser<double> on the 1M FP64 scene: the 1920x1080 primary rays
+    widened to FP64, reference intersector, device-resident inputs; steps =
+    timed traversals.  The tree is built with primitive-id leaves: with
+    reference leaves on unsorted input the fit and the traverser read
+    primitives[sorted position] (lbvh.h:1855, 1904-1905), so every box spans
+    random triangles of the whole scene and a ray visits most of the tree."""
     import numpy as np
-    v = np.asarray(v, dtype=np.float64).reshape(-1, 9)
-    a, b, c = v[:, 0:3], v[:, 3:6], v[:, 6:9]
-    e0 = a - b
-    e1 = c - a
-    return np.ascontiguousarray(np.concatenate([a, e0, e1, np.cross(e0, e1)], axis=1))
 
-
-def _boxes48(tri96):
-    """triangle_aabb_converter<double>: min / max of p0, p0 - e0, p0 + e1."""
-    import numpy as np
-    p0 = tri96[:, 0:3]
-    p1 = p0 - tri96[:, 3:6]
-    p2 = p0 + tri96[:, 6:9]
-    return np.ascontiguousarray(np.concatenate(
-        [np.minimum(np.minimum(p0, p1), p2), np.maximum(np.maximum(p0, p1), p2)], axis=1))
+    import lbvh_amd
+    from lbvh_amd import hip
+    from lbvh_amd import workload as W
+    rays = W.primary_rays(WIDTH, HEIGHT).astype(np.float64)
+    m = rays.shape[0]
+    n = tri96.shape[0]
+    nodes = hip.DeviceArray((n - 1,), lbvh_amd.NODE64_DTYPE)
+    ctx.build_aabbs_f64(d_boxes, lbvh_amd.LEAF_PRIMITIVE_ID, out=nodes)
+    d_tri = hip.DeviceArray.from_host(tri96)
+    d_rays = hip.DeviceArray.from_host(rays)
+    outs = (hip.DeviceArray(m, np.uint64), hip.DeviceArray(m, np.float64),
+            hip.DeviceArray((m, 2), np.float64))
+    for _ in range(max(args.warmup, 1)):
+        ctx.trace_f64(nodes, d_tri, d_rays, 0, out=outs)
+        ctx.synchronize()
+        log("bench: FP64 traversal warmup step done")
+    ctx.set_timing(2)
+    for _ in range(args.steps):
+        ctx.trace_f64(nodes, d_tri, d_rays, 0, out=outs)
+    ctx.synchronize()
+    tot, calls = ctx.phase_stats(2)
+    ctx.set_timing(0)
+    hits = int((outs[0].numpy() != np.uint64(2**64 - 1)).sum())
+    for a in outs + (d_tri, d_rays, nodes):
+        a.close()
+    ms = sum(tot) / calls
+    log("bench: FP64 traversal of %d rays %.3f ms" % (m, ms))
+    return {"rays": m, "hits": hits, "trace_ms": round(ms, 4),
+            "mrays_s": round(m / (ms * 1e-3) / 1e6, 2)}
 
 
 def bench_f64(args):
@@ -756,7 +775,7 @@ def bench_f64(args):
 
     log("bench: FP64 scenes (1,048,576 clustered triangles, seed 2; 16,777,216 uniform boxes)")
     v, _ = W.scene_clustered(2, N_PRIMS)
-    tri96 = _tri96(v)
+    tri96 = W.triangles_f64(v)
     del v
     rng = np.random.default_rng(1)
     lo = rng.random((1 << 24, 3))
@@ -764,7 +783,7 @@ def bench_f64(args):
     del lo
     ctx = lbvh_amd.Context(0)
     out = {}
-    for name, boxes in (("1m", _boxes48(tri96)), ("16m", big)):
+    for name, boxes in (("1m", W.triangle_boxes_f64(tri96)), ("16m", big)):
         n = boxes.shape[0]
         d = hip.DeviceArray.from_host(boxes)
         nodes = hip.DeviceArray((n - 1,), lbvh_amd.NODE64_DTYPE)
@@ -790,9 +809,11 @@ def bench_f64(args):
                      "phases_ms": {"encode (bounds+codes)": round(ph[0] * scale, 4),
                                    "sort": round(ph[1] * scale, 4),
                                    "tree+fit": round((ph[2] + ph[3]) * scale, 4)}}
+        log("bench: FP64 build of %d boxes %.3f ms" % (n, build_ms))
+        if name == "1m":
+            out["trace_1m"] = _f64_trace_timing(ctx, d, tri96, args)
         d.close()
         nodes.close()
-        log("bench: FP64 build of %d boxes %.3f ms" % (n, build_ms))
     ctx.close()
     del big
     one = out["1m"]
@@ -820,6 +841,7 @@ def bench_f64(args):
                      "bytes_per_prim": BUILD_BYTES_PER_PRIM_F64},
         "build_1m": one,
         "build_16m": out["16m"],
+        "trace_1m": out["trace_1m"],
     }
     if not args.no_cpu_baseline:
         import oracle

@@ -266,6 +266,34 @@ lbvh_status lbvh_trace_closest_device(lbvh_ctx* ctx, const lbvh_bvh* bvh,
                                       int mode, uint32_t* d_prim, float* d_t,
                                       float* d_uv);
 
+/* -- builder<double> traversal ----------------------------------------------
+ * Replaces lbvh::traverser<double, triangle<double>,
+ * triangle_intersection<double>>::operator()(ray, triangle_intersector)
+ * (lbvh.h:531-533, 1869-1942, 2055-2075) applied to a batch of rays, over the
+ * node<double> array lbvh_build_aabbs_f64 wrote for n triangles (n - 1
+ * records of 64 bytes).  tri96: n triangle<double> records {p0, e0, e1,
+ * normal} (12 doubles, lbvh.h:1960-1987); a leaf value v tests record v.
+ * rays64: ray<double> records {pos xyz, dir xyz, tmin, tmax} (8 doubles,
+ * lbvh.h:388-401).  All arithmetic is FP64 in the reference's order.
+ * Outputs per ray: prim u64 (UINT64_MAX = miss, lbvh.h:360-366), t (+inf on
+ * a miss), uv (2 doubles, 0 on a miss; may be NULL).  mode 0 = reference
+ * (first hit found wins ties), 1 = lowest leaf value on equal distance.  n in
+ * [2, 2^31 - 1].  The device variant reads device memory (nodes, triangles,
+ * rays and uv 16-byte aligned, prim and t 8-byte) and returns after queuing;
+ * the host variant copies in and out and returns when the outputs are
+ * written. */
+lbvh_status lbvh_trace_closest_f64(lbvh_ctx* ctx, const void* nodes64,
+                                   uint64_t n, const double* tri96,
+                                   const double* rays64, uint64_t nrays,
+                                   int mode, uint64_t* prim, double* t,
+                                   double* uv);
+lbvh_status lbvh_trace_closest_f64_device(lbvh_ctx* ctx, const void* d_nodes64,
+                                          uint64_t n, const double* d_tri96,
+                                          const double* d_rays64,
+                                          uint64_t nrays, int mode,
+                                          uint64_t* d_prim, double* d_t,
+                                          double* d_uv);
+
 /* -- scene batches -----------------------------------------------------------
  * Many independent triangle scenes built in one segmented pass (per-scene
  * bounds and codes, one radix sort keyed by (scene, code), then tree and fit

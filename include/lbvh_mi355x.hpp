@@ -25,6 +25,9 @@
 //     the FP64 boxes are built on the device (lbvh_build_aabbs_f64: u64
 //     codes, FP64 arithmetic, 64-byte node<double>), byte-identical to the
 //     reference's builder<double>.
+//   lbvh::gpu::trace_closest(bvh<double>, triangle<double>*, ...): the
+//     traverser<double> over a batch of rays on the device
+//     (lbvh_trace_closest_f64), equal to the host traverser bit for bit.
 #pragma once
 
 #include "lbvh_mi355x.h"
@@ -193,6 +196,7 @@ struct ray final
 };
 
 static_assert(sizeof(ray<float>) == 32, "ray<float> must match the engine's ray layout");
+static_assert(sizeof(ray<double>) == 64, "ray<double> must match the engine's FP64 ray layout");
 
 template<typename T, typename intersection_info>
 struct intersection_container final
@@ -591,6 +595,7 @@ private:
 };
 
 static_assert(sizeof(triangle<float>) == 48, "triangle<float> must match the engine's layout");
+static_assert(sizeof(triangle<double>) == 96, "triangle<double> must match the engine's layout");
 
 template<typename T>
 class triangle_aabb_converter final
@@ -698,6 +703,26 @@ private:
   std::unique_ptr<lbvh_bvh, del> h_;
   size_type n_ = 0;
 };
+
+// traverser<double, triangle<double>, triangle_intersection<double>> applied
+// to every ray of a batch on the device (lbvh_trace_closest_f64): the closest
+// hit over a builder<double> tree of `count` triangles, equal bit for bit to
+// the host traverser's.  prim = UINT64_MAX on a miss; uv may be null; mode
+// LBVH_ISECT_REFERENCE (first hit wins ties) or LBVH_ISECT_REFERENCE_LOWID.
+inline void
+trace_closest(const bvh<double>& b, const triangle<double>* tris, size_type count,
+              const ray<double>* rays, size_type nrays, std::uint64_t* prim, double* t,
+              vec2<double>* uv = nullptr, int mode = LBVH_ISECT_REFERENCE)
+{
+  auto& S = session::get();
+  if (b.size() + 1 != count)
+    throw std::invalid_argument("lbvh::gpu::trace_closest: the tree has count - 1 nodes");
+  S.check(lbvh_trace_closest_f64(S.ctx(), b.data(), count,
+                                 reinterpret_cast<const double*>(tris),
+                                 reinterpret_cast<const double*>(rays), nrays, mode, prim, t,
+                                 reinterpret_cast<double*>(uv)),
+          "lbvh_trace_closest_f64");
+}
 
 } // namespace gpu
 

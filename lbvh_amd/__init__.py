@@ -139,6 +139,9 @@ SIGNATURES = {
     "lbvh_build_aabbs": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), ctypes.POINTER(VP)]),
     "lbvh_build_aabbs_f64": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), VP]),
     "lbvh_build_aabbs_f64_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), VP]),
+    "lbvh_trace_closest_f64": (ctypes.c_int, [VP, VP, U64, VP, VP, U64, ctypes.c_int, VP, VP, VP]),
+    "lbvh_trace_closest_f64_device": (ctypes.c_int, [VP, VP, U64, VP, VP, U64, ctypes.c_int, VP,
+                                                     VP, VP]),
     "lbvh_build_aabbs_device": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts),
                                                ctypes.POINTER(VP)]),
     "lbvh_build_triangles_noalloc": (ctypes.c_int, [VP, VP, U64, ctypes.POINTER(BuildOpts), VP]),
@@ -709,6 +712,49 @@ class Context:
         self._check(lib().lbvh_build_aabbs_f64(self.handle, _hptr(a), n, ctypes.byref(opts),
                                                _hptr(nodes)))
         return nodes[:n - 1]
+
+    def trace_f64(self, nodes, tri, rays, mode=ISECT_REFERENCE, out=None, with_uv=True):
+        """traverser<double> closest hits over the node<double> array of an
+        FP64 build of the triangle<double> records `tri` ([n, 12] float64)
+        for ray<double> records `rays` ([m, 8] float64).  Host arrays ->
+        numpy (prim u64, t f64, uv f64 [m, 2]) after the work finished;
+        device arrays (DeviceArray or torch) -> device outputs, queued.
+        prim is 2^64 - 1 on a miss."""
+        if _is_device(rays):
+            if not (_is_device(nodes) and _is_device(tri)):
+                raise ValueError("device rays need device nodes and triangles")
+            n = _check_dev(tri, "triangles", (np.float64,), width=12)
+            m = _check_dev(rays, "rays", (np.float64,), width=8)
+            if _dev_nbytes(nodes) < 64 * max(n - 1, 0):
+                raise ValueError("nodes hold fewer than 64 (n - 1) bytes")
+            if out is None:
+                from .hip import DeviceArray
+                prim = DeviceArray(m, np.uint64)
+                t = DeviceArray(m, np.float64)
+                uv = DeviceArray((m, 2), np.float64) if with_uv else None
+            else:
+                prim, t, uv = out
+                _check_dev(prim, "prim", (np.uint64, np.int64), at_least=m)
+                _check_dev(t, "t", (np.float64,), at_least=m)
+                if uv is not None:
+                    _check_dev(uv, "uv", (np.float64,), at_least=2 * m)
+            self._check(lib().lbvh_trace_closest_f64_device(
+                self.handle, _dptr(nodes), n, _dptr(tri), _dptr(rays), m, mode, _dptr(prim),
+                _dptr(t), _dptr(uv) if uv is not None else None))
+            return prim, t, uv
+        tr = np.ascontiguousarray(tri, dtype=np.float64).reshape(-1, 12)
+        r = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 8)
+        nd = np.ascontiguousarray(nodes, dtype=NODE64_DTYPE)
+        n, m = tr.shape[0], r.shape[0]
+        if nd.shape[0] != n - 1:
+            raise ValueError("%d nodes for %d triangles, expected n - 1" % (nd.shape[0], n))
+        prim = np.empty(m, dtype=np.uint64)
+        t = np.empty(m, dtype=np.float64)
+        uv = np.empty((m, 2), dtype=np.float64) if with_uv else None
+        self._check(lib().lbvh_trace_closest_f64(self.handle, _hptr(nd), n, _hptr(tr), _hptr(r),
+                                                 m, mode, _hptr(prim), _hptr(t),
+                                                 _hptr(uv) if uv is not None else None))
+        return prim, t, uv
 
     def build_batch(self, tri, offsets, leaf_order=LEAF_REFERENCE, keep_codes=False,
                     sort_digit_bits=0, sort_tile=0, sort_strategy=SORT_AUTO):

@@ -25,7 +25,7 @@ ENGINE_SO = os.path.join(LIB, "liblbvh_mi355x.so")
 WORKLOAD_SO = os.path.join(LIB, "libworkload.so")
 DROPIN_BIN = os.path.join(LIB, "dropin_minimal")
 
-HIP_SOURCES = ["capi.cpp", "batch_api.cpp", "serve.cpp", "capi_f64.cpp", "capi_noalloc.cpp", "build_kernels.hip", "build_f64.hip", "radix_sort.hip", "trace.hip", "batch.hip"]
+HIP_SOURCES = ["capi.cpp", "batch_api.cpp", "serve.cpp", "capi_f64.cpp", "capi_noalloc.cpp", "build_kernels.hip", "build_f64.hip", "radix_sort.hip", "trace.hip", "trace_f64.hip", "batch.hip"]
 
 ARCH = os.environ.get("LBVH_OFFLOAD_ARCH", "gfx950")
 

@@ -180,9 +180,97 @@ f64_build(lbvh_ctx* c, const double* d_boxes, uint64_t n, const lbvh_build_opts*
   return queue_f64_build(c, d_boxes, n, opts, d_nodes, cfg);
 }
 
+// Validates a traversal of a node<double> array and queues it (phase kind 2
+// of the context's timing, as FP32 traversals).
+lbvh_status
+f64_trace(lbvh_ctx* c, const void* d_nodes, uint64_t n, const double* d_tris,
+          const double* d_rays, uint64_t nrays, int mode, uint64_t* d_prim, double* d_t,
+          double* d_uv)
+{
+  if (mode < 0 || mode > 1)
+    return fail(c, LBVH_E_INVALID, "unknown intersector mode (FP64 traversal: 0 or 1)");
+  if (n < 2)
+    return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
+  if (n > kMaxF64Prims)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 primitives per FP64 tree");
+  if (misaligned(d_nodes, 16) || misaligned(d_tris, 16) || misaligned(d_rays, 16) ||
+      misaligned(d_prim, 8) || misaligned(d_t, 8) || misaligned(d_uv, 16))
+    return fail(c, LBVH_E_INVALID,
+                "FP64 traversal: nodes, triangles, rays and uv 16-byte aligned, prim and t 8-byte");
+  if (nrays == 0)
+    return LBVH_OK;
+  begin_timed(c, 2);
+  record(c, 0);
+  HIP_TRY(c, launch_trace_f64(mode, d_nodes, d_tris, n, d_rays, nrays, d_prim, d_t, d_uv,
+                              c->stream));
+  record(c, 1, true);
+  return LBVH_OK;
+}
+
 } // namespace
 
 extern "C" {
+
+lbvh_status
+lbvh_trace_closest_f64_device(lbvh_ctx* c, const void* d_nodes, uint64_t n,
+                              const double* d_tri96, const double* d_rays64, uint64_t nrays,
+                              int mode, uint64_t* d_prim, double* d_t, double* d_uv)
+{
+  if (!c || !d_nodes || !d_tri96 || (nrays && (!d_rays64 || !d_prim || !d_t)))
+    return fail(c, LBVH_E_INVALID, "null argument");
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  return f64_trace(c, d_nodes, n, d_tri96, d_rays64, nrays, mode, d_prim, d_t, d_uv);
+}
+
+lbvh_status
+lbvh_trace_closest_f64(lbvh_ctx* c, const void* nodes, uint64_t n, const double* tri96,
+                       const double* rays64, uint64_t nrays, int mode, uint64_t* prim,
+                       double* t, double* uv)
+{
+  if (!c || !nodes || !tri96 || (nrays && (!rays64 || !prim || !t)))
+    return fail(c, LBVH_E_INVALID, "null argument");
+  if (n < 2)
+    return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
+  if (n > kMaxF64Prims)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 primitives per FP64 tree");
+  if (mode < 0 || mode > 1)
+    return fail(c, LBVH_E_INVALID, "unknown intersector mode (FP64 traversal: 0 or 1)");
+  if (nrays == 0)
+    return LBVH_OK;
+  lbvh_status st = set_device(c);
+  if (st != LBVH_OK)
+    return st;
+  // nodes, triangles and rays share the input buffer, each part 256-aligned
+  const uint64_t node_bytes = 64 * (n - 1), tri_bytes = 96 * n, ray_bytes = 64 * nrays;
+  const uint64_t tri_off = (node_bytes + 255) & ~uint64_t(255);
+  const uint64_t ray_off = tri_off + ((tri_bytes + 255) & ~uint64_t(255));
+  if ((st = ensure(c, c->io_in, ray_off + ray_bytes)) != LBVH_OK)
+    return st;
+  if ((st = ensure(c, c->io_out, 32 * nrays)) != LBVH_OK)
+    return st;
+  char* in = static_cast<char*>(c->io_in.p);
+  void* d_nodes = in;
+  double* d_tri = reinterpret_cast<double*>(in + tri_off);
+  double* d_rays = reinterpret_cast<double*>(in + ray_off);
+  uint64_t* d_prim = static_cast<uint64_t*>(c->io_out.p);
+  double* d_t = reinterpret_cast<double*>(d_prim + nrays);
+  double* d_uv = d_t + nrays;
+  HIP_TRY(c, hipMemcpyAsync(d_nodes, nodes, node_bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(d_tri, tri96, tri_bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(d_rays, rays64, ray_bytes, hipMemcpyHostToDevice, c->stream));
+  if ((st = f64_trace(c, d_nodes, n, d_tri, d_rays, nrays, mode, d_prim, d_t,
+                      uv ? d_uv : nullptr)) != LBVH_OK) {
+    (void)hipStreamSynchronize(c->stream);
+    return st;
+  }
+  HIP_TRY(c, hipMemcpyAsync(prim, d_prim, 8 * nrays, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(t, d_t, 8 * nrays, hipMemcpyDeviceToHost, c->stream));
+  if (uv)
+    HIP_TRY(c, hipMemcpyAsync(uv, d_uv, 16 * nrays, hipMemcpyDeviceToHost, c->stream));
+  return sync_and_check(c);
+}
 
 lbvh_status
 lbvh_build_aabbs_f64_device(lbvh_ctx* c, const double* d_boxes, uint64_t n,

@@ -206,6 +206,14 @@ uint64_t f64_tree_scratch_bytes(uint64_t n);
 hipError_t launch_f64_tree(const uint64_t* sorted, const uint32_t* ids, const double* boxes,
                            uint64_t n, void* nodes, void* scratch, hipStream_t s);
 
+// Closest-hit traversal of a node<double> array (n - 1 records of 64 B for
+// n primitives) over triangle<double> records (12 doubles) for ray<double>
+// records (8 doubles); mode 0 = reference, 1 = lowest id on equal distance.
+// prim: u64 (UINT64_MAX = miss), t: f64, uv: 2 f64 per ray or nullptr.
+hipError_t launch_trace_f64(int mode, const void* nodes, const double* tris, uint64_t n,
+                            const double* rays, uint64_t nrays, uint64_t* prim, double* t,
+                            double* uv, hipStream_t s);
+
 // Fill `words` 32-bit words with `value`.
 hipError_t launch_fill_u32(uint32_t* p, uint64_t words, uint32_t value, hipStream_t s);
 

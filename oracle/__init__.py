@@ -70,6 +70,10 @@ def orc():
         L.orc_fit_f64.argtypes = [VP, ctypes.c_uint64, F64P]
         L.orc_build_f64.argtypes = [F64P, ctypes.c_uint64, ctypes.c_int, VP, U64P, U32P]
         L.orc_build_f64.restype = ctypes.c_int
+        L.orc_trace_f64.argtypes = [VP, ctypes.c_uint64, F64P, F64P, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.c_int, U64P, F64P, F64P]
+        L.orc_brute_closest_f64.argtypes = [F64P, ctypes.c_uint64, F64P, ctypes.c_uint64,
+                                            ctypes.c_int, U64P, F64P]
         L.orc_fnv1a64.argtypes = [VP, ctypes.c_uint64]
         L.orc_fnv1a64.restype = ctypes.c_uint64
         _orc = L
@@ -99,6 +103,9 @@ def ref(fast=False):
         L.ref_build_f64_triangles.restype = ctypes.c_int
         L.ref_build_f64_aabbs.argtypes = [F64P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, VP]
         L.ref_build_f64_aabbs.restype = ctypes.c_int
+        L.ref_trace_f64.argtypes = [VP, ctypes.c_uint64, F64P, F64P, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.c_int, U64P, F64P, F64P]
+        L.ref_trace_f64.restype = ctypes.c_int
         L.ref_hardware_concurrency.restype = ctypes.c_int
         _ref[key] = L
     return _ref[key]
@@ -234,6 +241,36 @@ def build_f64(boxes48, leaf_order=LEAF_REFERENCE):
     return nodes, sc, ids
 
 
+def trace_f64(nodes64, tri96, rays64, mode=ISECT_REFERENCE, threads=None):
+    """traverser<double> over a node<double> array: triangle<double> records
+    [n, 12] and ray<double> records [m, 8].  Returns (prim u64, t f64,
+    uv f64[m, 2]); prim is 2^64 - 1 on a miss."""
+    tri96 = _f64(tri96, 12)
+    rays64 = _f64(rays64, 8)
+    nodes64 = np.ascontiguousarray(nodes64, dtype=NODE64_DTYPE)
+    m = rays64.shape[0]
+    prim = np.empty(m, dtype=np.uint64)
+    t = np.empty(m, dtype=np.float64)
+    uv = np.empty((m, 2), dtype=np.float64)
+    th = threads or min(8, os.cpu_count() or 1)
+    orc().orc_trace_f64(nodes64.ctypes.data, tri96.shape[0], _p(tri96, F64P), _p(rays64, F64P), m,
+                        mode, th, _p(prim, U64P), _p(t, F64P), _p(uv, F64P))
+    return prim, t, uv
+
+
+def brute_closest_f64(tri96, rays64, threads=None):
+    """FP64 closest hit over every triangle (lowest id on equal distance)."""
+    tri96 = _f64(tri96, 12)
+    rays64 = _f64(rays64, 8)
+    m = rays64.shape[0]
+    prim = np.empty(m, dtype=np.uint64)
+    t = np.empty(m, dtype=np.float64)
+    th = threads or min(8, os.cpu_count() or 1)
+    orc().orc_brute_closest_f64(_p(tri96, F64P), tri96.shape[0], _p(rays64, F64P), m, th,
+                                _p(prim, U64P), _p(t, F64P))
+    return prim, t
+
+
 def trace(nodes, tri, rays, mode=ISECT_REFERENCE, threads=None, counters=False):
     """Returns (prim u32, t f32, uv f32[m,2]) [, counters(popped, boxes, tests)]."""
     tri = _tri(tri)
@@ -312,6 +349,24 @@ def ref_build_f64(prims, kind="triangles", leaf_order=LEAF_REFERENCE, threads=No
     if fn(_p(prims, F64P), n, leaf_order, th, nodes.ctypes.data) != 0:
         raise ValueError("ref_build_f64 failed")
     return nodes
+
+
+def ref_trace_f64(nodes64, tri96, rays64, mode=0, threads=None):
+    """The compiled reference's traverser<double> (mode 0 as shipped, 1 with
+    lowest-id ties).  Returns (prim u64, t f64, uv f64[m, 2])."""
+    tri96 = _f64(tri96, 12)
+    rays64 = _f64(rays64, 8)
+    nodes64 = np.ascontiguousarray(nodes64, dtype=NODE64_DTYPE)
+    m = rays64.shape[0]
+    prim = np.empty(m, dtype=np.uint64)
+    t = np.empty(m, dtype=np.float64)
+    uv = np.empty((m, 2), dtype=np.float64)
+    th = threads or ref().ref_hardware_concurrency() or 1
+    rc = ref().ref_trace_f64(nodes64.ctypes.data, tri96.shape[0], _p(tri96, F64P), _p(rays64, F64P),
+                             m, mode, th, _p(prim, U64P), _p(t, F64P), _p(uv, F64P))
+    if rc != 0:
+        raise ValueError("ref_trace_f64 failed")
+    return prim, t, uv
 
 
 def ref_codes(tri, code_bits=30, threads=None):

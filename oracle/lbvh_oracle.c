@@ -967,6 +967,292 @@ orc_t_fp64(const float* tris, const float* rays, const uint32_t* prim,
   }
 }
 
+/* ---- traverser<double> (lbvh.h:1874-1942 with scalar_type = double) over
+ *      node<double> records, with triangle_intersector<double>
+ *      (lbvh.h:2055-2075), make_accel_ray (lbvh.h:991-1010) and the octant
+ *      box test (lbvh.h:1071-1091) in FP64.  A leaf value names the
+ *      triangle<double> record it tests (left_leaf_index, lbvh.h:1904-1905,
+ *      1915-1916): the sorted position for reference leaves, the primitive id
+ *      for primitive-id leaves.  Hits report that value; a miss is
+ *      UINT64_MAX (intersection_container<double>, lbvh.h:364-366). ------ */
+
+static inline double
+dm_min(double a, double b)
+{
+  return (a < b) ? a : b;
+}
+
+static inline double
+dm_max(double a, double b)
+{
+  return (a > b) ? a : b;
+}
+
+/* lbvh::math::dot (lbvh.h:621-630): ((0 + x x') + y y') + z z' */
+static inline double
+ddot3(const double* a, const double* b)
+{
+  double out = 0.0;
+  double px = a[0] * b[0];
+  out = out + px;
+  double py = a[1] * b[1];
+  out = out + py;
+  double pz = a[2] * b[2];
+  out = out + pz;
+  return out;
+}
+
+/* lbvh::math::cross (lbvh.h:609-616) */
+static inline void
+dcross3(const double* a, const double* b, double* o)
+{
+  double ay_bz = a[1] * b[2], az_by = a[2] * b[1];
+  double az_bx = a[2] * b[0], ax_bz = a[0] * b[2];
+  double ax_by = a[0] * b[1], ay_bx = a[1] * b[0];
+  o[0] = ay_bz - az_by;
+  o[1] = az_bx - ax_bz;
+  o[2] = ax_by - ay_bx;
+}
+
+/* triangle_intersector<double> (lbvh.h:2055-2075) on a triangle<double>
+ * record {p0, e0, e1, normal} and a ray<double> {pos, dir, tmin, tmax}. */
+static int
+isect_reference_f64(const double* tri, const double* ray, double* t_out, double* u_out,
+                    double* v_out)
+{
+  const double* p0 = tri;
+  const double* e0 = tri + 3;
+  const double* e1 = tri + 6;
+  const double* nrm = tri + 9;
+  const double* pos = ray;
+  const double* dir = ray + 3;
+  double c[3] = { p0[0] - pos[0], p0[1] - pos[1], p0[2] - pos[2] };
+  double r[3];
+  dcross3(dir, c, r);
+  double inv_det = 1.0 / ddot3(nrm, dir);
+  double u = ddot3(r, e1) * inv_det;
+  double v = ddot3(r, e0) * inv_det;
+  double w = (1.0 - u) - v;
+  if ((u >= 0.0) && (v >= 0.0) && (w >= 0.0)) {
+    double t = ddot3(nrm, c) * inv_det;
+    if ((t >= ray[6]) && (t <= ray[7])) {
+      *t_out = t;
+      *u_out = u;
+      *v_out = v;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+static inline int
+better64(int mode, double t, uint64_t id, double best_t, uint64_t best_id)
+{
+  if (mode == ORC_ISECT_REFERENCE)
+    return t < best_t;
+  return (t < best_t) || (t == best_t && id < best_id);
+}
+
+static void
+trace_one_f64(const orc_node64* nodes, const double* tris, const double* ray, int mode,
+              uint64_t* prim_out, double* t_out, double* uv_out)
+{
+  uint64_t st_node[ORC_STACK];
+  double st_t[ORC_STACK];
+  int sp = 0;
+  st_node[sp] = 0;
+  st_t[sp] = INFINITY;
+  sp++;
+  double rcp[3], inv_pos[3];
+  int oct[3], inv_oct[3];
+  for (int k = 0; k < 3; k++) {
+    rcp[k] = 1.0 / ray[3 + k];
+    double np = -ray[k];
+    inv_pos[k] = np * rcp[k];
+  }
+  oct[0] = (ray[3] > 0.0) ? 0 : 3;
+  oct[1] = (ray[4] > 0.0) ? 1 : 4;
+  oct[2] = (ray[5] > 0.0) ? 2 : 5;
+  inv_oct[0] = 3 - oct[0];
+  inv_oct[1] = 5 - oct[1];
+  inv_oct[2] = 7 - oct[2];
+  uint64_t best = UINT64_MAX;
+  double best_t = INFINITY, best_u = 0.0, best_v = 0.0;
+  while (sp > 0) {
+    sp--;
+    uint64_t ni = st_node[sp];
+    double etmin = st_t[sp];
+    if (best_t < etmin)
+      continue;
+    const orc_node64* nd = &nodes[ni];
+    double lt = 0.0, rt = 0.0;
+    int lh = 0, rh = 0;
+    for (int side = 0; side < 2; side++) {
+      uint64_t ch = side ? nd->right : nd->left;
+      if (ch & ORC_LEAF_BIT64) {
+        uint64_t p = ch & ~ORC_LEAF_BIT64;
+        double t, u, v;
+        if (isect_reference_f64(tris + 12 * p, ray, &t, &u, &v) &&
+            better64(mode, t, p, best_t, best)) {
+          best = p;
+          best_t = t;
+          best_u = u;
+          best_v = v;
+        }
+      } else {
+        const orc_node64* c = &nodes[ch];
+        const double b[6] = { c->min[0], c->min[1], c->min[2], c->max[0], c->max[1], c->max[2] };
+        double tn[3], tf[3];
+        for (int k = 0; k < 3; k++) {
+          double pn = b[oct[k]] * rcp[k];
+          tn[k] = pn + inv_pos[k];
+          double pf = b[inv_oct[k]] * rcp[k];
+          tf[k] = pf + inv_pos[k];
+        }
+        double tmin = dm_max(dm_max(tn[0], tn[1]), tn[2]);
+        double tmax = dm_min(dm_min(tf[0], tf[1]), tf[2]);
+        int h = tmax >= dm_max(0.0, tmin);
+        if (side) {
+          rh = h;
+          rt = tmin;
+        } else {
+          lh = h;
+          lt = tmin;
+        }
+      }
+    }
+    /* push far then near; a push onto a full stack is dropped */
+#define PUSH64(N, T)                                                           \
+  do {                                                                         \
+    if (sp < ORC_STACK) {                                                      \
+      st_node[sp] = (N);                                                       \
+      st_t[sp] = (T);                                                          \
+      sp++;                                                                    \
+    }                                                                          \
+  } while (0)
+    if (lh && rh) {
+      if (lt < rt) {
+        PUSH64(nd->right, rt);
+        PUSH64(nd->left, lt);
+      } else {
+        PUSH64(nd->left, lt);
+        PUSH64(nd->right, rt);
+      }
+    } else if (lh) {
+      PUSH64(nd->left, lt);
+    } else if (rh) {
+      PUSH64(nd->right, rt);
+    }
+#undef PUSH64
+  }
+  *prim_out = best;
+  *t_out = best_t;
+  if (uv_out) {
+    uv_out[0] = best != UINT64_MAX ? best_u : 0.0;
+    uv_out[1] = best != UINT64_MAX ? best_v : 0.0;
+  }
+}
+
+typedef struct
+{
+  const orc_node64* nodes;
+  const double* tris;
+  uint64_t ntris;
+  const double* rays;
+  uint64_t nrays;
+  int mode;
+  int brute;
+  uint64_t* prim;
+  double* t;
+  double* uv;
+  int tid, nth;
+} orc_job64;
+
+static void*
+job64_main(void* arg)
+{
+  orc_job64* j = (orc_job64*)arg;
+  for (uint64_t i = (uint64_t)j->tid; i < j->nrays; i += (uint64_t)j->nth) {
+    const double* ray = j->rays + 8 * i;
+    if (!j->brute) {
+      trace_one_f64(j->nodes, j->tris, ray, j->mode, &j->prim[i], &j->t[i],
+                    j->uv ? j->uv + 2 * i : NULL);
+      continue;
+    }
+    /* every triangle, lowest id on equal distance */
+    uint64_t best = UINT64_MAX;
+    double best_t = INFINITY;
+    for (uint64_t p = 0; p < j->ntris; p++) {
+      double t, u, v;
+      if (isect_reference_f64(j->tris + 12 * p, ray, &t, &u, &v) &&
+          better64(ORC_ISECT_REFERENCE_LOWID, t, p, best_t, best)) {
+        best = p;
+        best_t = t;
+      }
+    }
+    j->prim[i] = best;
+    j->t[i] = best_t;
+  }
+  return NULL;
+}
+
+static void
+run_jobs64(orc_job64* proto, int nthreads)
+{
+  if (nthreads < 1)
+    nthreads = 1;
+  if (nthreads > 256)
+    nthreads = 256;
+  orc_job64* jobs = (orc_job64*)calloc((size_t)nthreads, sizeof(orc_job64));
+  pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+  for (int i = 0; i < nthreads; i++) {
+    jobs[i] = *proto;
+    jobs[i].tid = i;
+    jobs[i].nth = nthreads;
+  }
+  for (int i = 1; i < nthreads; i++)
+    pthread_create(&th[i], NULL, job64_main, &jobs[i]);
+  job64_main(&jobs[0]);
+  for (int i = 1; i < nthreads; i++)
+    pthread_join(th[i], NULL);
+  free(jobs);
+  free(th);
+}
+
+void
+orc_trace_f64(const orc_node64* nodes, uint64_t n, const double* tri96, const double* rays64,
+              uint64_t nrays, int mode, int nthreads, uint64_t* prim, double* t, double* uv)
+{
+  orc_job64 j;
+  memset(&j, 0, sizeof(j));
+  j.nodes = nodes;
+  j.tris = tri96;
+  j.ntris = n;
+  j.rays = rays64;
+  j.nrays = nrays;
+  j.mode = mode;
+  j.prim = prim;
+  j.t = t;
+  j.uv = uv;
+  run_jobs64(&j, nthreads);
+}
+
+void
+orc_brute_closest_f64(const double* tri96, uint64_t n, const double* rays64, uint64_t nrays,
+                      int nthreads, uint64_t* prim, double* t)
+{
+  orc_job64 j;
+  memset(&j, 0, sizeof(j));
+  j.tris = tri96;
+  j.ntris = n;
+  j.rays = rays64;
+  j.nrays = nrays;
+  j.brute = 1;
+  j.prim = prim;
+  j.t = t;
+  run_jobs64(&j, nthreads);
+}
+
 /* FNV-1a 64-bit over raw bytes: fixture hashes of large arrays. */
 uint64_t
 orc_fnv1a64(const void* data, uint64_t len)

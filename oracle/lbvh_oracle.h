@@ -26,6 +26,8 @@
  *                          (builder<double> over triangle<double> and over
  *                          plain boxes); numpy min/max of centres; brute-force
  *                          radix tree; subtree min/max closed form
+ *   orc_trace_f64          the compiled lbvh.h traverser<double> (oracle/_ref);
+ *                          brute-force FP64 closest hit
  *   orc_trace (mode 2)     brute-force closest hit with the same test; FP64
  *                          plane-distance error bound.  No reference counterpart
  *                          exists for the watertight test: its hit set is pinned
@@ -106,6 +108,15 @@ void orc_fit_f64(orc_node64* nodes, uint64_t n, const double* boxes48);
 int orc_build_f64(const double* boxes48, uint64_t n, int leaf_order,
                   orc_node64* nodes, uint64_t* sorted_codes_out,
                   uint32_t* ids_out);
+/* traverser<double> with triangle_intersector<double> over triangle<double>
+ * records (12 doubles) and ray<double> records (8 doubles); modes
+ * ORC_ISECT_REFERENCE and ORC_ISECT_REFERENCE_LOWID; prim UINT64_MAX on a
+ * miss, uv 0. */
+void orc_trace_f64(const orc_node64* nodes, uint64_t n, const double* tri96,
+                   const double* rays64, uint64_t nrays, int mode, int nthreads,
+                   uint64_t* prim, double* t, double* uv);
+void orc_brute_closest_f64(const double* tri96, uint64_t n, const double* rays64,
+                           uint64_t nrays, int nthreads, uint64_t* prim, double* t);
 void orc_trace(const orc_node* nodes, uint64_t n, const float* tris,
                const float* rays, uint64_t nrays, int mode, int nthreads,
                uint32_t* prim, float* t, float* uv, uint64_t* counters);

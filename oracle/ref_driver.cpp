@@ -260,6 +260,26 @@ struct lowid_hit
   bool operator<(float d) const noexcept { return h.distance < d; }
 };
 
+// traverser<double> with equal-distance ties to the lowest id: the id rides
+// with the primitive, and the intersection orders by (distance, id) while
+// pruning compares the distance alone.
+struct lowid_tri64
+{
+  lbvh::triangle<double> t;
+  uint64_t id;
+};
+
+struct lowid_hit64
+{
+  lbvh::triangle_intersection<double> h;
+  uint64_t id = UINT64_MAX;
+  bool operator<(const lowid_hit64& o) const noexcept
+  {
+    return (h.distance < o.h.distance) || (h.distance == o.h.distance && id < o.id);
+  }
+  bool operator<(double d) const noexcept { return h.distance < d; }
+};
+
 // builder<double>: the converter of plain FP64 boxes (the generic
 // primitive/converter path with P = aabb<double>).
 struct box_cvt_f64
@@ -510,6 +530,71 @@ ref_build_f64_aabbs(const double* boxes48, uint64_t n, int leaf_order, int threa
   auto nodes = build_f64(prims.data(), n, leaf_order, threads > 0 ? unsigned(threads) : 1u,
                          box_cvt_f64());
   std::memcpy(nodes_out, static_cast<const void*>(nodes.data()), nodes.size() * 64);
+  return 0;
+}
+
+// traverser<double> + triangle_intersector<double> over a node<double> array
+// (64 B each), triangle<double> records (96 B) and ray<double> records
+// (8 doubles).  mode 0: as shipped (first hit wins ties); 1: lowest id on
+// equal distance.  uv is 0 on a miss.  Returns 0 on success.
+int
+ref_trace_f64(const void* nodes_in, uint64_t n, const double* tri96, const double* rays,
+              uint64_t nrays, int mode, int threads, uint64_t* prim, double* t, double* uv)
+{
+  if (n < 2)
+    return -1;
+  std::vector<node64_t> nv(n - 1);
+  std::memcpy(static_cast<void*>(nv.data()), nodes_in, (n - 1) * 64);
+  lbvh::bvh<double> bvh(std::move(nv));
+  std::vector<lbvh::triangle<double>> prims(n);
+  std::memcpy(static_cast<void*>(prims.data()), tri96, n * 96);
+  std::vector<lowid_tri64> lp;
+  if (mode == 1) {
+    lp.resize(n);
+    for (uint64_t i = 0; i < n; i++)
+      lp[i] = lowid_tri64{ prims[i], i };
+  }
+  const unsigned th = threads > 0 ? unsigned(threads) : 1u;
+  auto work = [&](unsigned tid) {
+    lbvh::triangle_intersector<double> isect;
+    for (uint64_t i = tid; i < nrays; i += th) {
+      const double* r = rays + 8 * i;
+      lbvh::ray<double> ray{ { r[0], r[1], r[2] }, { r[3], r[4], r[5] }, r[6], r[7] };
+      uint64_t p;
+      lbvh::triangle_intersection<double> h;
+      if (mode == 0) {
+        lbvh::traverser<double, lbvh::triangle<double>, lbvh::triangle_intersection<double>> tr(
+          bvh, prims.data());
+        auto hit = tr(ray, isect);
+        p = hit.primitive;
+        h = hit.info;
+      } else {
+        auto wrap = [&isect](const lowid_tri64& q, const lbvh::ray<double>& rr) -> lowid_hit64 {
+          lowid_hit64 x;
+          x.h = isect(q.t, rr);
+          x.id = (x.h.distance != std::numeric_limits<double>::infinity()) ? q.id : UINT64_MAX;
+          return x;
+        };
+        lbvh::traverser<double, lowid_tri64, lowid_hit64> tr(bvh, lp.data());
+        auto hit = tr(ray, wrap);
+        p = hit.primitive;
+        h = hit.info.h;
+      }
+      prim[i] = p;
+      t[i] = h.distance;
+      if (uv) {
+        const bool hit = p != UINT64_MAX;
+        uv[2 * i] = hit ? h.uv.x : 0.0;
+        uv[2 * i + 1] = hit ? h.uv.y : 0.0;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned i = 1; i < th; i++)
+    pool.emplace_back(work, i);
+  work(0);
+  for (auto& x : pool)
+    x.join();
   return 0;
 }
 

@@ -9,7 +9,8 @@
 //         then the 2 nodes of the 3-sphere tree, then u64 n, n FP64
 //         triangles (96 B each) and their builder<double> nodes (64 B each)
 //         with reference leaves, then with primitive-id leaves.
-// Exits non-zero if the device-tree closest hits differ from the host ones.
+// Exits non-zero if the device-tree closest hits differ from the host ones
+// (FP32 device_tree and the FP64 batch traversal alike).
 #include <lbvh_mi355x.hpp>
 
 #include <cstdio>
@@ -152,12 +153,28 @@ main(int argc, char** argv)
     bvh64, tris64.data());
   lbvh::triangle_intersector<double> isect64;
   std::size_t hits64 = 0;
+  std::vector<lbvh::ray<double>> rays64(256);
   for (std::size_t i = 0; i < 256; i++) {
-    lbvh::ray<double> r;
-    r.pos = { 0.5, 0.5, -1.0 };
-    r.dir = lbvh::math::normalize(
+    rays64[i].pos = { 0.5, 0.5, -1.0 };
+    rays64[i].dir = lbvh::math::normalize(
       lbvh::vec3<double>{ double(i % 16) / 15.0 - 0.5, double(i / 16) / 15.0 - 0.5, 1.0 });
+  }
+  // the same rays traced on the device in one batch
+  std::vector<std::uint64_t> gp(256);
+  std::vector<double> gt(256);
+  std::vector<lbvh::vec2<double>> guv(256);
+  lbvh::gpu::trace_closest(bvh64, tris64.data(), tris64.size(), rays64.data(), rays64.size(),
+                           gp.data(), gt.data(), guv.data());
+  for (std::size_t i = 0; i < 256; i++) {
+    const lbvh::ray<double>& r = rays64[i];
     auto h = tr64(r, isect64);
+    if (gp[i] != h.primitive || std::memcmp(&gt[i], &h.info.distance, 8) != 0 ||
+        (h && std::memcmp(&guv[i], &h.info.uv, 16) != 0)) {
+      std::fprintf(stderr, "builder<double> ray %zu: device hit %llu at %g, host %llu at %g\n", i,
+                   (unsigned long long)gp[i], gt[i], (unsigned long long)h.primitive,
+                   h.info.distance);
+      return 1;
+    }
     // brute force over every triangle, first hit wins ties as in the traverser
     lbvh::triangle_intersection<double> best;
     std::uint64_t bi = ~std::uint64_t(0);

@@ -71,6 +71,9 @@ def test_host_only_entry_points():
     assert lib.lbvh_batch_get_info(None, None) == 1
     assert lib.lbvh_server_submit(None, None, None, None) == 1
     assert lib.lbvh_server_get_stats(None, None) == 1
+    assert lib.lbvh_trace_closest_f64(None, None, 2, None, None, 0, 0, None, None, None) == 1
+    assert lib.lbvh_trace_closest_f64_device(None, None, 2, None, None, 0, 0, None, None,
+                                             None) == 1
     # server defaults: 288 GiB cap, 4 lanes, 64 in flight, 63-bit codes,
     # primitive-id leaves, reference intersector
     so = lbvh_amd.ServerOpts()

@@ -188,3 +188,141 @@ def test_f64_rejects_bad_arguments(ctx):
     boxes = _scene("uniform", 1000, 3)
     want, _, _ = oracle.build_f64(boxes, 0)
     assert ctx.build_aabbs_f64(boxes, 0).tobytes() == want.tobytes()
+
+
+# ---- traverser<double> ------------------------------------------------------------
+
+MISS64 = np.uint64(2**64 - 1)
+
+
+def _trace_scenes():
+    from lbvh_amd import workload as W
+    return {
+        "uniform": W.triangles_f64(W.uniform_vertices(21, 6000, 0.04)),
+        "clustered": W.triangles_f64(W.cluster_vertices(22, 6000, half_extent=0.02)),
+        # every triangle identical: all codes equal, equal distances everywhere
+        "all_equal": W.triangles_f64(np.tile(W.uniform_vertices(23, 1, 0.05), (300, 1))),
+    }
+
+
+def _rays_f64(tri, seed, m=4000):
+    """Primary rays of a 48x48 image plus rays from inside the scene's box in
+    random directions, some with a tmax that ends before the first hit."""
+    from lbvh_amd import workload as W
+    boxes = W.triangle_boxes_f64(tri)
+    lo, hi = boxes[:, :3].min(axis=0), boxes[:, 3:].max(axis=0)
+    inner = W.box_rays(seed, lo.astype(np.float32), hi.astype(np.float32), m).astype(np.float64)
+    inner[::3, 7] = 0.05
+    return np.ascontiguousarray(np.concatenate([W.primary_rays(48, 48).astype(np.float64), inner]))
+
+
+def W_boxes(tri):
+    from lbvh_amd import workload as W
+    return W.triangle_boxes_f64(tri)
+
+
+def _same_f64_hits(got, want, what):
+    p, t, uv = got
+    wp, wt, wuv = want
+    assert np.array_equal(p, wp), "%s: ids differ at %d rays" % (what, int((p != wp).sum()))
+    assert np.array_equal(t.view(np.uint64), wt.view(np.uint64)), "%s: t differs" % what
+    if uv is not None:
+        assert np.array_equal(uv.view(np.uint64), wuv.view(np.uint64)), "%s: uv differs" % what
+
+
+@pytest.mark.parametrize("scene", ["uniform", "clustered", "all_equal"])
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+@pytest.mark.parametrize("mode", [0, 1], ids=["first", "lowid"])
+def test_f64_trace_matches_oracle(ctx, scene, leaf, mode):
+    """The GPU's FP64 traversal of the GPU's FP64 tree: ids, t and uv bit for
+    bit the oracle's traverser<double> (pinned to the compiled reference's)."""
+    tri = _trace_scenes()[scene]
+    nodes = ctx.build_aabbs_f64(W_boxes(tri), leaf)
+    rays = _rays_f64(tri, 31)
+    got = ctx.trace_f64(nodes, tri, rays, mode)
+    want = oracle.trace_f64(nodes, tri, rays, mode)
+    _same_f64_hits(got, want, "%s leaf %d mode %d" % (scene, leaf, mode))
+    assert (got[0] != MISS64).sum() > 100
+    assert (got[0] == MISS64).sum() > 100
+    assert np.all(got[1][got[0] == MISS64] == np.inf)
+    assert np.all(got[2][got[0] == MISS64] == 0.0)
+
+
+def test_f64_trace_device_full_image(ctx):
+    """The 1,048,576-triangle clustered scene (seed 2) widened to FP64: GPU
+    build and traversal of the full 1920x1080 primary set with device-
+    resident nodes, triangles and rays, against the oracle in both modes."""
+    from lbvh_amd import hip
+    from lbvh_amd import workload as W
+    v, tri32 = W.scene_clustered(2, 1 << 20)
+    tri = W.triangles_f64(v)
+    rays = W.primary_rays(1920, 1080).astype(np.float64)
+    print("f64 trace: build 1M, trace 1920x1080", flush=True)
+    d_tri = hip.DeviceArray.from_host(tri)
+    d_boxes = hip.DeviceArray.from_host(W.triangle_boxes_f64(tri))
+    d_rays = hip.DeviceArray.from_host(rays)
+    nodes = ctx.build_aabbs_f64(d_boxes, 1)
+    for mode in (0, 1):
+        p, t, uv = ctx.trace_f64(nodes, d_tri, d_rays, mode)
+        ctx.synchronize()
+        got = (p.numpy(), t.numpy(), uv.numpy().reshape(-1, 2))
+        host_nodes = nodes.numpy()[: tri.shape[0] - 1]
+        print("f64 trace: oracle, mode %d" % mode, flush=True)
+        want = oracle.trace_f64(host_nodes, tri, rays, mode)
+        _same_f64_hits(got, want, "1M mode %d" % mode)
+        assert (got[0] != MISS64).sum() > 100000  # 148,745 of the primary rays hit
+        for a in (p, t, uv):
+            a.close()
+    for a in (d_tri, d_boxes, d_rays, nodes):
+        a.close()
+
+
+def test_f64_trace_without_uv_and_into_given_outputs(ctx):
+    from lbvh_amd import hip
+    tri = _trace_scenes()["uniform"]
+    nodes = ctx.build_aabbs_f64(W_boxes(tri), 1)
+    rays = _rays_f64(tri, 32)
+    want = oracle.trace_f64(nodes, tri, rays, 0)
+    p, t, uv = ctx.trace_f64(nodes, tri, rays, 0, with_uv=False)
+    assert uv is None
+    _same_f64_hits((p, t, None), want, "host without uv")
+    m = rays.shape[0]
+    d_nodes = hip.DeviceArray.from_host(nodes)
+    d_tri = hip.DeviceArray.from_host(tri)
+    d_rays = hip.DeviceArray.from_host(rays)
+    out = (hip.DeviceArray(m + 5, np.uint64), hip.DeviceArray(m + 5, np.float64),
+           hip.DeviceArray((m + 5, 2), np.float64))
+    # twice into the same outputs: the second call overwrites every ray
+    for _ in range(2):
+        ctx.trace_f64(d_nodes, d_tri, d_rays, 0, out=out)
+        ctx.synchronize()
+        _same_f64_hits((out[0].numpy()[:m], out[1].numpy()[:m], out[2].numpy()[:m]), want,
+                       "device into given outputs")
+    for a in out + (d_nodes, d_tri, d_rays):
+        a.close()
+
+
+def test_f64_trace_rejects_bad_arguments(ctx):
+    from lbvh_amd import hip
+    tri = _trace_scenes()["uniform"][:100]
+    nodes = ctx.build_aabbs_f64(W_boxes(tri), 1)
+    rays = _rays_f64(tri, 33, 100)
+    with pytest.raises(lbvh_amd.LbvhError) as e:
+        ctx.trace_f64(nodes, tri, rays, 2)  # no watertight FP64 intersector
+    assert e.value.status == 1
+    with pytest.raises(ValueError):
+        ctx.trace_f64(nodes[:50], tri, rays, 0)
+    # rays 8 bytes past a 16-byte boundary: refused before any launch
+    d_nodes = hip.DeviceArray.from_host(nodes)
+    d_tri = hip.DeviceArray.from_host(tri)
+    d_rays = hip.DeviceArray.from_host(np.concatenate([np.zeros(1), rays.reshape(-1)]))
+    with pytest.raises(lbvh_amd.LbvhError) as e:
+        ctx.trace_f64(d_nodes, d_tri, d_rays.rows(1, 1 + rays.size), 0)
+    assert e.value.status == 1
+    for a in (d_nodes, d_tri, d_rays):
+        a.close()
+    # the context stays usable; zero rays is a no-op
+    want = oracle.trace_f64(nodes, tri, rays, 1)
+    _same_f64_hits(ctx.trace_f64(nodes, tri, rays, 1), want, "after errors")
+    p, t, uv = ctx.trace_f64(nodes, tri, rays[:0], 1)
+    assert p.shape == (0,)

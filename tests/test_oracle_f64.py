@@ -177,3 +177,76 @@ def test_f64_oracle_equals_reference_large(leaf):
     want = oracle.ref_build_f64(tri, "triangles", leaf)
     got, _, _ = oracle.build_f64(oracle.triangle_aabb_f64(tri), leaf)
     assert got.tobytes() == want.tobytes()
+
+
+# ---- traverser<double> ------------------------------------------------------------
+
+def tri96_of(v):
+    from lbvh_amd import workload as W
+    return W.triangles_f64(v)
+
+
+def _f64_scenes():
+    from lbvh_amd import workload as W
+    out = {
+        "uniform": W.uniform_vertices(21, 6000, 0.01),
+        "clustered": W.cluster_vertices(22, 6000, half_extent=0.02),
+        # every triangle identical: all codes equal, ties everywhere
+        "all_equal": np.tile(W.uniform_vertices(23, 1, 0.05), (300, 1)),
+    }
+    return {k: tri96_of(v) for k, v in out.items()}
+
+
+def _rays64(w=48, h=48):
+    from lbvh_amd import workload as W
+    return W.primary_rays(w, h).astype(np.float64)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference driver not built")
+@pytest.mark.parametrize("scene", ["uniform", "clustered", "all_equal"])
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+@pytest.mark.parametrize("mode", [0, 1], ids=["first", "lowid"])
+def test_f64_trace_matches_compiled_reference(scene, leaf, mode):
+    tri = _f64_scenes()[scene]
+    nodes, _, _ = oracle.build_f64(oracle.triangle_aabb_f64(tri), leaf)
+    rays = _rays64()
+    p, t, uv = oracle.trace_f64(nodes, tri, rays, mode)
+    rp, rt, ruv = oracle.ref_trace_f64(nodes, tri, rays, mode)
+    assert np.array_equal(p, rp)
+    assert np.array_equal(t.view(np.uint64), rt.view(np.uint64))
+    assert np.array_equal(uv.view(np.uint64), ruv.view(np.uint64))
+    assert (p != np.uint64(2**64 - 1)).sum() > 0
+
+
+@pytest.mark.parametrize("scene", ["uniform", "clustered"])
+def test_f64_trace_equals_brute_force(scene):
+    """The closest hit does not depend on the tree: lowest-id ties against
+    every triangle tested in turn."""
+    tri = _f64_scenes()[scene]
+    nodes, _, _ = oracle.build_f64(oracle.triangle_aabb_f64(tri), 1)
+    rays = _rays64()
+    p, t, _ = oracle.trace_f64(nodes, tri, rays, 1)
+    bp, bt = oracle.brute_closest_f64(tri, rays)
+    assert np.array_equal(p, bp)
+    assert np.array_equal(t.view(np.uint64), bt.view(np.uint64))
+    assert (p != np.uint64(2**64 - 1)).sum() > 100
+
+
+@pytest.mark.parametrize("scene", ["uniform", "clustered"])
+def test_f64_trace_same_closest_hit_in_both_leaf_orders(scene):
+    """With reference leaves a leaf names its sorted position and both the
+    fit and the traverser read primitives[position] (lbvh.h:1855, 1904-1905):
+    every triangle is still a leaf exactly once under a box fitted from it,
+    so the closest hit (lowest id on ties) equals the primitive-id tree's and
+    brute force."""
+    tri = _f64_scenes()[scene]
+    rays = _rays64()
+    got = []
+    for leaf in (0, 1):
+        nodes, _, _ = oracle.build_f64(oracle.triangle_aabb_f64(tri), leaf)
+        values = np.concatenate([nodes["left"], nodes["right"]])
+        leaves = values[values >= BIT63] - BIT63
+        assert np.array_equal(np.sort(leaves), np.arange(len(tri), dtype=np.uint64))
+        got.append(oracle.trace_f64(nodes, tri, rays, 1))
+    assert np.array_equal(got[0][0], got[1][0])
+    assert np.array_equal(got[0][1].view(np.uint64), got[1][1].view(np.uint64))

@@ -29,8 +29,12 @@ constexpr uint32_t kLinkIndex = kLinkSibLeaf - 1;
 // Reproduces centroid_bounds_kernel + the serial union of per-thread boxes
 // (lbvh.h:1282-1293, 1539-1543): the union of (min + max) * 0.5 over every
 // primitive box (center_of, lbvh.h:1160-1167).  min/max are exact, so the
-// reduction order cannot change the result for finite inputs.  Centres are
-// also written out (SoA) for the encode pass.
+// reduction order cannot change the result for finite inputs (a NaN centre
+// never replaces the running value: std_min(acc, NaN) keeps acc).  Centres
+// are also written out (SoA) for the encode pass.  Each workgroup writes the
+// union of its centres as one partial box; the encode kernel reduces the
+// partials (no ticket, no drain: a workgroup exits as soon as its stores are
+// issued).
 
 __device__ __forceinline__ void
 wave_reduce_box(float lo[3], float hi[3])
@@ -78,11 +82,9 @@ __global__ __launch_bounds__(kBlock) void
 k_centroid_bounds(const float* __restrict__ prims, uint64_t n,
                   float* __restrict__ cx, float* __restrict__ cy,
                   float* __restrict__ cz, float* __restrict__ partials,
-                  uint32_t* __restrict__ ticket, float* __restrict__ bounds,
                   uint32_t* __restrict__ zero_ptr, uint32_t zero_words)
 {
   __shared__ float s_box[kBlock / kWave][6];
-  __shared__ int s_last;
   const uint64_t tid = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   const uint64_t stride = uint64_t(gridDim.x) * kBlock;
 
@@ -91,117 +93,41 @@ k_centroid_bounds(const float* __restrict__ prims, uint64_t n,
 
   float lo[3] = { INFINITY, INFINITY, INFINITY };
   float hi[3] = { -INFINITY, -INFINITY, -INFINITY };
-  if constexpr (KIND == kPrimTriangle) {
-    // Triangles are 48 bytes: a lane reading its own would make each of its
-    // three 16-byte loads touch three times the cache lines of a contiguous
-    // load.  The workgroup instead reads kCentroidUnroll x kBlock triangles
-    // as contiguous 16-byte pieces into LDS (the trip count is uniform
-    // across the workgroup), then each thread takes its triangles from there.
-    __shared__ float4 s_tri[kCentroidUnroll * kBlock * 3];
-    constexpr uint64_t kSpan = uint64_t(kCentroidUnroll) * kBlock;
-    for (uint64_t base = uint64_t(blockIdx.x) * kSpan; base < n;
-         base += uint64_t(gridDim.x) * kSpan) {
-      const uint64_t cnt = (n - base < kSpan) ? n - base : kSpan;
-      const float4* src = reinterpret_cast<const float4*>(prims + 12 * base);
-      float4 v[kCentroidUnroll * 3];
+  // kCentroidUnroll primitives per thread per round, all loads issued first
+  for (uint64_t i0 = tid; i0 < n; i0 += stride * kCentroidUnroll) {
+    // unconditional loads (indexes past the end read the last primitive and
+    // are not used), so that all of them are in flight together
+    Box b[kCentroidUnroll];
 #pragma unroll
-      for (int q = 0; q < kCentroidUnroll * 3; q++) {
-        const uint64_t j = uint64_t(q) * kBlock + threadIdx.x;
-        v[q] = j < 3 * cnt ? src[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-#pragma unroll
-      for (int q = 0; q < kCentroidUnroll * 3; q++)
-        s_tri[q * kBlock + threadIdx.x] = v[q];
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < kCentroidUnroll; u++) {
-        const uint64_t k = uint64_t(u) * kBlock + threadIdx.x;
-        if (k < cnt) {
-          const float4 a = s_tri[3 * k];     // p0.xyz, e0.x
-          const float4 b = s_tri[3 * k + 1]; // e0.yz, e1.xy
-          const float4 c = s_tri[3 * k + 2]; // e1.z, normal
-          const float p0[3] = { a.x, a.y, a.z };
-          const float e0[3] = { a.w, b.x, b.y };
-          const float e1[3] = { b.z, b.w, c.x };
-          const Box bx = tri_box(p0, e0, e1);
-          const uint64_t i = base + k;
-          float cc[3];
-#pragma unroll
-          for (int q = 0; q < 3; q++) {
-            float s = bx.lo[q] + bx.hi[q];
-            cc[q] = s * 0.5f;
-            lo[q] = std_min(lo[q], cc[q]);
-            hi[q] = std_max(hi[q], cc[q]);
-          }
-          cx[i] = cc[0];
-          cy[i] = cc[1];
-          cz[i] = cc[2];
-        }
-      }
-      __syncthreads(); // s_tri is refilled next round
+    for (int u = 0; u < kCentroidUnroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      b[u] = load_prim_box(prims, KIND, i < n ? i : n - 1);
     }
-  } else {
-    // kCentroidUnroll primitives per thread per round, all loads issued first
-    for (uint64_t i0 = tid; i0 < n; i0 += stride * kCentroidUnroll) {
-      // unconditional loads (indexes past the end read the last primitive and
-      // are not used), so that all of them are in flight together
-      Box b[kCentroidUnroll];
 #pragma unroll
-      for (int u = 0; u < kCentroidUnroll; u++) {
-        const uint64_t i = i0 + u * stride;
-        b[u] = load_prim_box(prims, KIND, i < n ? i : n - 1);
+    for (int u = 0; u < kCentroidUnroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      if (i >= n)
+        continue;
+      float c[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        float s = b[u].lo[k] + b[u].hi[k];
+        c[k] = s * 0.5f;
+        lo[k] = std_min(lo[k], c[k]);
+        hi[k] = std_max(hi[k], c[k]);
       }
-#pragma unroll
-      for (int u = 0; u < kCentroidUnroll; u++) {
-        const uint64_t i = i0 + u * stride;
-        if (i >= n)
-          continue;
-        float c[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          float s = b[u].lo[k] + b[u].hi[k];
-          c[k] = s * 0.5f;
-          lo[k] = std_min(lo[k], c[k]);
-          hi[k] = std_max(hi[k], c[k]);
-        }
-        cx[i] = c[0];
-        cy[i] = c[1];
-        cz[i] = c[2];
-      }
+      cx[i] = c[0];
+      cy[i] = c[1];
+      cz[i] = c[2];
     }
   }
   block_reduce_box(lo, hi, s_box);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      coherent_store_f32(&partials[6 * blockIdx.x + k], lo[k]);
-      coherent_store_f32(&partials[6 * blockIdx.x + 3 + k], hi[k]);
+      partials[6 * blockIdx.x + k] = lo[k];
+      partials[6 * blockIdx.x + 3 + k] = hi[k];
     }
-    vmem_drain();
-    uint32_t t = atomic_add_agent(ticket, 1u);
-    s_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!s_last)
-    return;
-  // Last workgroup: union of the per-workgroup boxes.
-  float glo[3] = { INFINITY, INFINITY, INFINITY };
-  float ghi[3] = { -INFINITY, -INFINITY, -INFINITY };
-  for (uint32_t g = threadIdx.x; g < gridDim.x; g += kBlock)
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      glo[k] = std_min(glo[k], coherent_load_f32(&partials[6 * g + k]));
-      ghi[k] = std_max(ghi[k], coherent_load_f32(&partials[6 * g + 3 + k]));
-    }
-  __syncthreads(); // s_box reuse
-  block_reduce_box(glo, ghi, s_box);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      bounds[k] = glo[k];
-      bounds[3 + k] = ghi[k];
-    }
-    *ticket = 0u;
   }
 }
 
@@ -271,25 +197,17 @@ template<typename KeyT>
 __global__ __launch_bounds__(kEncodeBlock) void
 k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
          const float* __restrict__ cz, uint64_t n,
-         const float* __restrict__ bounds, KeyT* __restrict__ codes,
+         const float* __restrict__ partials, uint32_t nparts, float* __restrict__ bounds_out,
+         KeyT* __restrict__ codes,
          uint32_t* __restrict__ hist, int digit_bits, int first_pass, int passes,
          uint32_t* __restrict__ counts, uint64_t chunk)
 {
   extern __shared__ uint32_t s_hist[];
+  __shared__ float s_pbox[kEncodeBlock / kWave][6];
   const int radix = 1 << digit_bits;
   const int words = radix * (passes - first_pass);
   for (int j = threadIdx.x; j < words; j += kEncodeBlock)
     s_hist[j] = 0u;
-  const float dom = sizeof(KeyT) == 4 ? 1024.0f : 1048576.0f;
-  float bmin[3], scale[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    bmin[k] = bounds[k];
-    float ext = bounds[3 + k] - bmin[k];
-    float den = ext + 1.0f;
-    scale[k] = dom / den;
-  }
-  __syncthreads();
   // grid-stride over [0, n), or this workgroup's chunk
   uint64_t first = uint64_t(blockIdx.x) * kEncodeBlock + threadIdx.x;
   uint64_t limit = n;
@@ -299,6 +217,52 @@ k_encode(const float* __restrict__ cx, const float* __restrict__ cy,
     first = b + threadIdx.x;
     limit = (b + chunk < n) ? b + chunk : n;
     stride = kEncodeBlock;
+  }
+  // the centroid bounds: union of the bounds kernel's per-workgroup boxes
+  // (finite or empty; every workgroup computes the same union)
+  float blo[3] = { INFINITY, INFINITY, INFINITY };
+  float bhi[3] = { -INFINITY, -INFINITY, -INFINITY };
+  for (uint32_t g = threadIdx.x; g < nparts; g += kEncodeBlock)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      blo[k] = std_min(blo[k], partials[6 * g + k]);
+      bhi[k] = std_max(bhi[k], partials[6 * g + 3 + k]);
+    }
+  wave_reduce_box(blo, bhi);
+  if (__lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      s_pbox[threadIdx.x / kWave][k] = blo[k];
+      s_pbox[threadIdx.x / kWave][3 + k] = bhi[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    blo[k] = s_pbox[0][k];
+    bhi[k] = s_pbox[0][3 + k];
+  }
+  for (int ww = 1; ww < kEncodeBlock / kWave; ww++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      blo[k] = std_min(blo[k], s_pbox[ww][k]);
+      bhi[k] = std_max(bhi[k], s_pbox[ww][3 + k]);
+    }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      bounds_out[k] = blo[k];
+      bounds_out[3 + k] = bhi[k];
+    }
+  }
+  const float dom = sizeof(KeyT) == 4 ? 1024.0f : 1048576.0f;
+  float bmin[3], scale[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    bmin[k] = blo[k];
+    float ext = bhi[k] - bmin[k];
+    float den = ext + 1.0f;
+    scale[k] = dom / den;
   }
   const KeyT mask = KeyT(radix - 1);
   for (uint64_t i0 = first; i0 < limit; i0 += stride * kEncodeUnroll) {
@@ -1436,22 +1400,22 @@ encode_groups(uint64_t n)
 
 hipError_t
 launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx, float* cy,
-                       float* cz, float* partials, uint32_t* ticket, float* bounds,
-                       uint32_t* zero_ptr, uint32_t zero_words, hipStream_t s)
+                       float* cz, float* partials, uint32_t* zero_ptr, uint32_t zero_words,
+                       hipStream_t s)
 {
   uint32_t g = centroid_partials_needed(n);
   if (kind == kPrimTriangle)
     hipLaunchKernelGGL(k_centroid_bounds<kPrimTriangle>, dim3(g), dim3(kBlock), 0, s, prims, n,
-                       cx, cy, cz, partials, ticket, bounds, zero_ptr, zero_words);
+                       cx, cy, cz, partials, zero_ptr, zero_words);
   else
     hipLaunchKernelGGL(k_centroid_bounds<kPrimAabb>, dim3(g), dim3(kBlock), 0, s, prims, n, cx,
-                       cy, cz, partials, ticket, bounds, zero_ptr, zero_words);
+                       cy, cz, partials, zero_ptr, zero_words);
   return hipGetLastError();
 }
 
 hipError_t
 launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, uint64_t n,
-              const float* bounds, void* codes, uint32_t* hist,
+              const float* partials, float* bounds, void* codes, uint32_t* hist,
               int digit_bits, int first_pass, int passes, uint32_t* counts, uint32_t groups,
               uint64_t chunk, hipStream_t s)
 {
@@ -1468,14 +1432,15 @@ launch_encode(int code_bits, const float* cx, const float* cy, const float* cz, 
     g = groups;
   }
   uint32_t* h = hist + size_t(first_pass) * (size_t(1) << digit_bits);
+  const uint32_t nparts = centroid_partials_needed(n);
   if (code_bits == 30)
     hipLaunchKernelGGL(k_encode<uint32_t>, dim3(g), dim3(kEncodeBlock), lds, s, cx, cy, cz, n,
-                       bounds, static_cast<uint32_t*>(codes), h, digit_bits, first_pass,
-                       passes, counts, chunk);
+                       partials, nparts, bounds, static_cast<uint32_t*>(codes), h, digit_bits,
+                       first_pass, passes, counts, chunk);
   else
     hipLaunchKernelGGL(k_encode<uint64_t>, dim3(g), dim3(kEncodeBlock), lds, s, cx, cy, cz, n,
-                       bounds, static_cast<uint64_t*>(codes), h, digit_bits, first_pass,
-                       passes, counts, chunk);
+                       partials, nparts, bounds, static_cast<uint64_t*>(codes), h, digit_bits,
+                       first_pass, passes, counts, chunk);
   return hipGetLastError();
 }
 

@@ -490,11 +490,11 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   record(c, 0);
   hipError_t e = launch_centroid_bounds(d_prims, kind, n, cent, cent + n, cent + 2 * n,
                                         static_cast<float*>(c->partials.p),
-                                        small + kSmallBoundsTicket, b->d_bounds,
                                         small + kSmallTickets, small_words - kSmallTickets,
                                         c->stream);
   if (e == hipSuccess)
-    e = launch_encode(opts.code_bits, cent, cent + n, cent + 2 * n, n, b->d_bounds,
+    e = launch_encode(opts.code_bits, cent, cent + n, cent + 2 * n, n,
+                      static_cast<const float*>(c->partials.p), b->d_bounds,
                       c->keys_a.p, ws.hist,
                       cfg.digit_bits, cfg.strategy == 2 ? passes - cfg.global_digits : 0, passes,
                       chunked ? ws.counts : nullptr, rts_g,

@@ -46,7 +46,7 @@ struct Layout
 {
   uint64_t tickets = 0; // tree tickets, then the small words (contiguous: cleared together)
   uint64_t small = 0;
-  uint32_t clear_words = 0; // tree tickets + the bounds ticket and error flag
+  uint32_t clear_words = 0; // tree tickets + the error flag (and a spare word)
   uint32_t small_words = 0;
   uint64_t cent = 0, keys_a = 0, keys_b = 0, ids_a = 0, ids_b = 0, partials = 0, tree = 0,
            counts = 0, counts_words = 0, parent = 0, flags = 0, other_end = 0, wide = 0,
@@ -215,17 +215,17 @@ lbvh_build_triangles_noalloc(lbvh_ctx* c, const float* d_tri, uint64_t n,
   ws.first_counts = true;
 
   // The previous call left its tickets at zero unless it failed part way:
-  // clear the tree tickets, the bounds ticket and the error flag.  The
-  // centroid-bounds kernel clears the histograms and sort tickets itself.
+  // clear the tree tickets and the error flag.  The centroid-bounds kernel
+  // clears the histograms and sort tickets itself.
   hipError_t e = launch_fill_u32(static_cast<uint32_t*>(at(L.tickets)), L.clear_words, 0u,
                                  c->stream);
   if (e == hipSuccess)
     e = launch_centroid_bounds(d_tri, 0, n, cent, cent + n, cent + 2 * n,
-                               static_cast<float*>(at(L.partials)), small + kSmallBoundsTicket,
-                               bounds, small + kSmallTickets, L.small_words - kSmallTickets,
-                               c->stream);
+                               static_cast<float*>(at(L.partials)), small + kSmallTickets,
+                               L.small_words - kSmallTickets, c->stream);
   if (e == hipSuccess)
-    e = launch_encode(int(opts.code_bits), cent, cent + n, cent + 2 * n, n, bounds,
+    e = launch_encode(int(opts.code_bits), cent, cent + n, cent + 2 * n, n,
+                      static_cast<const float*>(at(L.partials)), bounds,
                       at(L.keys_a), ws.hist, cfg.digit_bits,
                       cfg.strategy == 2 ? passes - cfg.global_digits : 0, passes, ws.counts,
                       rts_g, rts_chunk_keys(n, cfg), c->stream);

@@ -43,21 +43,25 @@ struct SortWorkspace
   bool first_counts = false;   // the first reduce-then-scan pass's counts are in place
 };
 
-// centroid of every primitive box (SoA) and the centroid bounds; also zeroes
-// `zero_words` 32-bit words at `zero_ptr` (histograms, tickets, error flag).
+// centroid of every primitive box (SoA) and one partial box of centroid
+// bounds per workgroup (centroid_partials_needed(n) boxes of 6 floats); also
+// zeroes `zero_words` 32-bit words at `zero_ptr` (histograms, tickets, error
+// flag).
 hipError_t launch_centroid_bounds(const float* prims, int kind, uint64_t n, float* cx,
-                                  float* cy, float* cz, float* partials, uint32_t* ticket,
-                                  float* bounds, uint32_t* zero_ptr, uint32_t zero_words,
-                                  hipStream_t s);
+                                  float* cy, float* cz, float* partials, uint32_t* zero_ptr,
+                                  uint32_t zero_words, hipStream_t s);
 uint32_t centroid_partials_needed(uint64_t n);
 
 // Morton codes (u32 for 30-bit, u64 for 63-bit) and the digit histograms of
 // passes first_pass .. passes - 1, added into hist[p << digit_bits ...] (zero
-// before).  counts != nullptr: `groups` workgroups encode chunks of `chunk`
-// keys and also write the chunk counts of pass first_pass (counts[d * groups
-// + g]), the reduce-then-scan layout.
+// before).  The centroid bounds are the union of launch_centroid_bounds'
+// partials for the same n; they are also written to `bounds` (6 floats).
+// counts != nullptr: `groups` workgroups encode chunks of `chunk` keys and
+// also write the chunk counts of pass first_pass (counts[d * groups + g]),
+// the reduce-then-scan layout.
 hipError_t launch_encode(int code_bits, const float* cx, const float* cy, const float* cz,
-                         uint64_t n, const float* bounds, void* codes, uint32_t* hist,
+                         uint64_t n, const float* partials, float* bounds, void* codes,
+                         uint32_t* hist,
                          int digit_bits, int first_pass, int passes, uint32_t* counts,
                          uint32_t groups, uint64_t chunk, hipStream_t s);
 uint32_t encode_groups(uint64_t n);

@@ -32,7 +32,7 @@ struct EventSet
 };
 
 // small workspace region layout (32-bit words)
-constexpr uint32_t kSmallBoundsTicket = 0; // self-resetting
+constexpr uint32_t kSmallSpare = 0;         // unused
 constexpr uint32_t kSmallErr = 1;          // cleared by the host after reading
 constexpr uint32_t kSmallTickets = 2;      // kMaxSortPasses words
 constexpr uint32_t kSmallTreeletCount = kSmallTickets + kMaxSortPasses;

@@ -30,6 +30,8 @@ static_assert(sizeof(Node64) == 64, "node<double> is 64 bytes");
 constexpr uint64_t kLeafBit64 = uint64_t(1) << 63;
 constexpr int kTraceF64Block = 256;
 constexpr int kStackF64 = 128;
+// Stack entries kept in LDS per thread (12 KB per 256-thread workgroup).
+constexpr int kStackF64Lds = 4;
 
 __device__ __forceinline__ double
 dm_min(double a, double b)
@@ -125,23 +127,75 @@ k_trace_f64(const Node64* __restrict__ nodes, const double* __restrict__ tris, u
     ipos[k] = np * rcp[k];
     pos_dir[k] = d[k] > 0.0;
   }
-  // the stack: (node index, tmin); node indices of FP64 trees stay below 2^31
-  uint32_t st_n[kStackF64];
-  double st_t[kStackF64];
+  // The stack: capacity 128 (pushes onto a full stack are dropped), entries
+  // (node index, tmin).  The first kStackF64Lds entries live in LDS, one
+  // column per thread, the rest in a private array; the top entry is kept in
+  // registers and reloaded only after a pop empties them (its reload is
+  // issued after the popped node's fetch, so the two latencies overlap).
+  __shared__ uint32_t s_node[kStackF64Lds * kTraceF64Block];
+  __shared__ double s_tmin[kStackF64Lds * kTraceF64Block];
+  uint32_t spill_n[kStackF64 - kStackF64Lds];
+  double spill_t[kStackF64 - kStackF64Lds];
+  auto stk_store = [&](int e, uint32_t node, double tm) {
+    if (e < kStackF64Lds) {
+      s_node[e * kTraceF64Block + threadIdx.x] = node;
+      s_tmin[e * kTraceF64Block + threadIdx.x] = tm;
+    } else {
+      spill_n[e - kStackF64Lds] = node;
+      spill_t[e - kStackF64Lds] = tm;
+    }
+  };
+  auto stk_load = [&](int e, uint32_t& node, double& tm) {
+    if (e < kStackF64Lds) {
+      node = s_node[e * kTraceF64Block + threadIdx.x];
+      tm = s_tmin[e * kTraceF64Block + threadIdx.x];
+    } else {
+      node = spill_n[e - kStackF64Lds];
+      tm = spill_t[e - kStackF64Lds];
+    }
+  };
+  uint32_t top_n = 0;
+  double top_t = 0.0;
   int sp = 0;
-  st_n[sp] = 0u;
-  st_t[sp] = INFINITY;
-  sp++;
+  bool stale = false; // the registers hold an entry already popped: reload before use
+  auto push = [&](uint32_t node, double tm) {
+    if (sp >= kStackF64)
+      return;
+    if (sp > 0 && !stale)
+      stk_store(sp - 1, top_n, top_t);
+    stale = false;
+    top_n = node;
+    top_t = tm;
+    sp++;
+  };
   uint64_t best = ~uint64_t(0);
   double best_t = INFINITY, best_u = 0.0, best_v = 0.0;
-  while (sp > 0) {
-    sp--;
-    const uint32_t ni = st_n[sp];
-    if (best_t < st_t[sp])
-      continue;
-    if (uint64_t(ni) + 1 >= n)
-      continue; // only possible for a corrupt node array
+  // The root is pushed and popped at once; a child that would be pushed and
+  // popped again straight away (the near one of two, or a single hit child)
+  // is taken directly, when its push would have found room, so the visit
+  // order and the drop-when-full rule are the reference's.
+  uint32_t ni = 0u;
+  bool have = true;
+  for (;;) {
+    bool visit = true;
+    if (!have) {
+      if (sp == 0)
+        break;
+      if (stale)
+        stk_load(sp - 1, top_n, top_t);
+      ni = top_n;
+      visit = !(best_t < top_t);
+      sp--;
+      stale = sp > 0;
+    }
+    have = false;
+    if (!visit || uint64_t(ni) + 1 >= n)
+      continue; // pruned, or (only for a corrupt node array) out of range
     const uint64_t lr[2] = { nodes[ni].left, nodes[ni].right };
+    if (stale) { // reload the top while the node is in flight
+      stk_load(sp - 1, top_n, top_t);
+      stale = false;
+    }
     bool hit2[2] = { false, false };
     double tn2[2] = { 0.0, 0.0 };
 #pragma unroll
@@ -183,26 +237,24 @@ k_trace_f64(const Node64* __restrict__ nodes, const double* __restrict__ tris, u
         tn2[side] = tmin;
       }
     }
-    // far child first, near child last; a push onto a full stack is dropped
-    auto push = [&](uint64_t node, double tm) {
-      if (sp < kStackF64) {
-        st_n[sp] = uint32_t(node);
-        st_t[sp] = tm;
-        sp++;
-      }
-    };
+    uint32_t next_n;
+    double next_t;
     if (hit2[0] && hit2[1]) {
-      if (tn2[0] < tn2[1]) {
-        push(lr[1], tn2[1]);
-        push(lr[0], tn2[0]);
-      } else {
-        push(lr[0], tn2[0]);
-        push(lr[1], tn2[1]);
-      }
-    } else if (hit2[0]) {
-      push(lr[0], tn2[0]);
-    } else if (hit2[1]) {
-      push(lr[1], tn2[1]);
+      const bool left_near = tn2[0] < tn2[1];
+      push(uint32_t(left_near ? lr[1] : lr[0]), left_near ? tn2[1] : tn2[0]);
+      next_n = uint32_t(left_near ? lr[0] : lr[1]);
+      next_t = left_near ? tn2[0] : tn2[1];
+    } else if (hit2[0] || hit2[1]) {
+      next_n = uint32_t(hit2[0] ? lr[0] : lr[1]);
+      next_t = hit2[0] ? tn2[0] : tn2[1];
+    } else {
+      continue;
+    }
+    // push-then-pop of next_n: its push needs room; its pop is then pruned
+    // when a closer hit is already known
+    if (sp < kStackF64 && !(best_t < next_t)) {
+      ni = next_n;
+      have = true;
     }
   }
   const bool hit = best != ~uint64_t(0);

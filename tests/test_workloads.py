@@ -150,3 +150,113 @@ def test_bench_fp64_distance_error_matches_oracle():
     e0 = bench.t_error_vs_fp64(tri, rays, p, np.where(hit, t_exact, t))
     assert e0["max_ulp"] <= 0.5 + 1e-6
     assert 2.5 <= e3["max_ulp"] <= 3.5 + 1e-6 and e3["mean_ulp"] >= 2.5
+
+
+def _py_traverse(boxes, left, right, tris, ray):
+    """traverser::operator() (lbvh.h:1874-1942) with triangle_intersector
+    (lbvh.h:2055-2075) and the octant box test (lbvh.h:991-1010, 1068-1094),
+    restated from the reference in Python with float32 arithmetic (every
+    operation rounds to float32, none fused).  Returns (prim, t, popped,
+    box tests, triangle tests): popped counts the entries not pruned."""
+    f = np.float32
+    o = [f(x) for x in ray[0:3]]
+    d = [f(x) for x in ray[3:6]]
+    rtmin, rtmax = f(ray[6]), f(ray[7])
+    rcp = [f(1) / d[k] for k in range(3)]
+    ipos = [(-o[k]) * rcp[k] for k in range(3)]
+    octs = [0 if d[0] > 0 else 3, 1 if d[1] > 0 else 4, 2 if d[2] > 0 else 5]
+    inv = [3 - octs[0], 5 - octs[1], 7 - octs[2]]
+    mx = lambda a, b: a if a > b else b
+    mn = lambda a, b: a if a < b else b
+
+    def dot(a, b):
+        out = f(0)
+        for k in range(3):
+            out = out + a[k] * b[k]
+        return out
+
+    def isect(i):
+        p0, e0, e1, nrm = tris[i, 0:3], tris[i, 3:6], tris[i, 6:9], tris[i, 9:12]
+        c = [p0[k] - o[k] for k in range(3)]
+        r = [d[1] * c[2] - d[2] * c[1], d[2] * c[0] - d[0] * c[2], d[0] * c[1] - d[1] * c[0]]
+        inv_det = f(1) / dot(nrm, d)
+        u = dot(r, e1) * inv_det
+        v = dot(r, e0) * inv_det
+        w = f(1) - u - v
+        if u >= 0 and v >= 0 and w >= 0:
+            t = dot(nrm, c) * inv_det
+            if rtmin <= t <= rtmax:
+                return t
+        return None
+
+    def box(i):
+        b = boxes[i]
+        tn = [b[octs[k]] * rcp[k] + ipos[k] for k in range(3)]
+        tf = [b[inv[k]] * rcp[k] + ipos[k] for k in range(3)]
+        tmin = mx(mx(tn[0], tn[1]), tn[2])
+        tmax = mn(mn(tf[0], tf[1]), tf[2])
+        return tmax >= mx(f(0), tmin), tmin
+
+    leaf = 0x80000000
+    stack = [(0, f(np.inf))]
+    best, best_t = 0xFFFFFFFF, f(np.inf)
+    popped = nbox = ntri = 0
+    while stack:
+        ni, etmin = stack.pop()
+        if best_t < etmin:
+            continue
+        popped += 1
+        hits = []
+        for ch in (int(left[ni]), int(right[ni])):
+            if ch & leaf:
+                ntri += 1
+                t = isect(ch & ~leaf)
+                if t is not None and t < best_t:
+                    best, best_t = ch & ~leaf, t
+                hits.append((False, None))
+            else:
+                nbox += 1
+                hits.append(box(ch))
+        (lh, lt), (rh, rt) = hits
+        entries = []
+        if lh and rh:
+            entries = [(int(right[ni]), rt), (int(left[ni]), lt)] if lt < rt else \
+                      [(int(left[ni]), lt), (int(right[ni]), rt)]
+        elif lh:
+            entries = [(int(left[ni]), lt)]
+        elif rh:
+            entries = [(int(right[ni]), rt)]
+        for e in entries:
+            if len(stack) < 128:
+                stack.append(e)
+    return best, best_t, popped, nbox, ntri
+
+
+def test_touched_byte_counts_match_an_independent_traversal():
+    """The traversal roofline's byte count rests on the oracle's per-ray
+    counters (popped nodes, box tests, triangle tests); a second restatement
+    of the reference traverser, written in Python, counts the same for every
+    ray of a sample of primary and bounce rays on the 1M benchmark tree, and
+    finds the same hits."""
+    import oracle
+    _, tri = W.scene_clustered(2, 1 << 20)
+    nodes, _, _, _ = oracle.build(tri, 63, oracle.LEAF_PRIMITIVE_ID)
+    rays = W.primary_rays(1920, 1080)
+    p, t, _ = oracle.trace(nodes, tri, rays, oracle.ISECT_REFERENCE)
+    bounce, _ = W.bounce_rays(5, tri, rays, p, t)
+    rng = np.random.default_rng(77)
+    hit = np.flatnonzero(p != 0xFFFFFFFF)
+    sample = np.concatenate([rays[rng.choice(rays.shape[0], 16, replace=False)],
+                             rays[rng.choice(hit, 16, replace=False)],
+                             bounce[rng.choice(bounce.shape[0], 32, replace=False)]])
+    boxes = nodes["box"]
+    left, right = nodes["left"], nodes["right"]
+    total = 0
+    for ray in sample:
+        op, ot, _, c = oracle.trace(nodes, tri, ray[None, :], oracle.ISECT_REFERENCE, threads=1,
+                                    counters=True)
+        bp, bt, popped, nbox, ntri = _py_traverse(boxes, left, right, tri, ray)
+        assert (int(op[0]), float(ot[0])) == (bp, float(bt))
+        assert (int(c[0]), int(c[1]), int(c[2])) == (popped, nbox, ntri)
+        total += popped + nbox + ntri
+    assert total > 64 * 50

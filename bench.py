@@ -1061,7 +1061,7 @@ def bench_single(args):
     tune_log = ctx.tune_log()
     trace_log = [r for r in tune_log if r["kind"] == "traversal" and r["log2_n"] in
                  (int(math.ceil(math.log2(n_primary))), int(math.ceil(math.log2(n_bounce))))
-                 and r["key_bits"] == 0]
+                 and r["mode"] == lbvh_amd.ISECT_REFERENCE]
     bvh.destroy()
     out_p = (hip.DeviceArray(n_primary, np.uint32), hip.DeviceArray(n_primary, np.float32),
              hip.DeviceArray((n_primary, 2), np.float32))

@@ -113,6 +113,8 @@ typedef struct lbvh_tune_record
   uint32_t trace_reorder;        /* traversals: rays put in key order first */
   uint32_t trace_waves_per_simd; /* traversals: waves sharing a SIMD (8: as
                                     many as the registers allow) */
+  uint32_t trace_mode;           /* traversals: the intersector mode tuned
+                                    (LBVH_ISECT_*); each mode is tuned apart */
 } lbvh_tune_record;
 
 /* -- status / options ---------------------------------------------------- */

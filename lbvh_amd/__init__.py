@@ -67,7 +67,8 @@ class TuneRecord(ctypes.Structure):
                 ("sort_tile", ctypes.c_uint32), ("sort_strategy", ctypes.c_uint32),
                 ("picked", ctypes.c_uint32), ("ms", ctypes.c_float),
                 ("scalar_bits", ctypes.c_uint32), ("kind", ctypes.c_uint32),
-                ("trace_reorder", ctypes.c_uint32), ("trace_waves_per_simd", ctypes.c_uint32)]
+                ("trace_reorder", ctypes.c_uint32), ("trace_waves_per_simd", ctypes.c_uint32),
+                ("trace_mode", ctypes.c_uint32)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -653,7 +654,8 @@ class Context:
                  "strategy": SORT_NAMES.get(r.sort_strategy, "-"),
                  "picked": bool(r.picked), "ms": round(r.ms, 4),
                  "scalar_bits": r.scalar_bits, "reorder": bool(r.trace_reorder),
-                 "waves_per_simd": r.trace_waves_per_simd} for r in buf[:n]]
+                 "waves_per_simd": r.trace_waves_per_simd, "mode": r.trace_mode}
+                for r in buf[:n]]
 
     def workspace_bytes(self):
         return int(lib().lbvh_ctx_workspace_bytes(self.handle))

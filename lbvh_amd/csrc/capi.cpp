@@ -962,6 +962,7 @@ lbvh_ctx_tune_log(const lbvh_ctx* c, lbvh_tune_record* out, uint32_t cap)
     out[i].kind = uint32_t(r.kind);
     out[i].trace_reorder = uint32_t(r.trace.reorder);
     out[i].trace_waves_per_simd = uint32_t(r.trace.waves_per_simd);
+    out[i].trace_mode = uint32_t(r.trace_mode);
   }
   return n;
 }
@@ -1125,6 +1126,7 @@ choose_trace(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, uint64_t nrays
     TuneRecord r{ 0, 0, ceil_log2_u64(nrays), SortConfig(), ms, false };
     r.kind = 1;
     r.trace = cand;
+    r.trace_mode = mode;
     c->tune_log.push_back(r);
     if (ms >= 0.0f && (best < 0.0f || ms < best)) {
       best = ms;

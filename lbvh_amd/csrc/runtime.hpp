@@ -51,6 +51,7 @@ struct TuneRecord
   int scalar_bits = 32; // 32: builder<float> builds, 64: builder<double> builds
   int kind = 0;         // 0: a build's sort, 1: a traversal
   TraceConfig trace{};  // kind 1: the traversal configuration
+  int trace_mode = 0;   // kind 1: the intersector mode
 };
 
 } // namespace lbvh_rt

@@ -521,3 +521,34 @@ def test_autotuner_times_candidates_and_keeps_results_exact():
     for x in (bvh, bvh2, bvh3, b):
         x.destroy()
     c.close()
+
+
+def test_traversal_tuner_records_each_mode_and_keeps_results_exact():
+    """The first traversal of a (mode, ray-count) bucket times every
+    traversal configuration (rays as given or in key order, x 8, 6 or 4
+    waves per SIMD) on its own rays; each mode is tuned apart, exactly one
+    configuration is kept per bucket, and the hits equal the oracle's on the
+    tuning call and on the calls that reuse the choice."""
+    c = lbvh_amd.Context(0)
+    tri = W.make_triangles(W.cluster_vertices(93, 100000))
+    bvh = c.build_triangles(tri, code_bits=63, leaf_order=1)
+    on, _, _, _ = oracle.build(tri, 63, 1)
+    rays = W.primary_rays(256, 256)
+    for mode in (0, 2):
+        want = oracle.trace(on, tri, rays, mode)
+        for _ in range(2):
+            p, t, uv = c.trace(bvh, rays, mode)
+            assert np.array_equal(p, want[0])
+            if mode == 0:
+                assert np.array_equal(t.view(np.uint32), want[1].view(np.uint32))
+                assert np.array_equal(uv.view(np.uint32), want[2].view(np.uint32))
+    log = [r for r in c.tune_log() if r["kind"] == "traversal"]
+    assert len(log) == 12
+    for mode in (0, 2):
+        mine = [r for r in log if r["mode"] == mode]
+        assert len(mine) == 6 and all(r["ms"] > 0 and r["log2_n"] == 16 for r in mine)
+        assert sum(r["picked"] for r in mine) == 1
+        assert sorted((r["reorder"], r["waves_per_simd"]) for r in mine) == \
+            sorted((ro, w) for ro in (False, True) for w in (8, 6, 4))
+    bvh.destroy()
+    c.close()

@@ -259,7 +259,8 @@ lbvh_status lbvh_build_aabbs_f64_device(lbvh_ctx* ctx,
  * triangle_intersection<float>>::operator()(ray, triangle_intersector)
  * (lbvh.h:531-533, 1869-1942) applied to a batch of rays.  Outputs per ray:
  * prim (UINT32_MAX = miss, lbvh.h:366), t (+inf on a miss), uv (2 floats,
- * 0 on a miss).  uv may be NULL. */
+ * 0 on a miss).  uv may be NULL.  Every traversal entry point (single tree,
+ * batch, FP64, server request) takes at most 2^31 - 1 rays per call. */
 lbvh_status lbvh_trace_closest(lbvh_ctx* ctx, const lbvh_bvh* bvh,
                                const float* rays32, uint64_t nrays, int mode,
                                uint32_t* prim, float* t, float* uv);

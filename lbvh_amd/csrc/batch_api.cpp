@@ -619,6 +619,8 @@ lbvh_batch_trace_closest_device(lbvh_ctx* c, const lbvh_batch* b, const float* d
     return fail(c, LBVH_E_INVALID, "null argument");
   if (mode < 0 || mode > 2)
     return fail(c, LBVH_E_INVALID, "unknown intersector mode");
+  if (nrays > kMaxRaysPerCall)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 rays per call");
   if (misaligned(d_rays, 16) || misaligned(d_ray_off, 8) ||
       trace_outputs_misaligned(d_prim, d_t, d_uv))
     return fail(c, LBVH_E_INVALID, "device rays must be 16-byte aligned, ray offsets and uv "

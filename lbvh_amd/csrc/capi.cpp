@@ -1154,6 +1154,8 @@ lbvh_trace_closest_device(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, u
     return fail(c, LBVH_E_INVALID, "device traversal needs a triangle tree");
   if (mode < 0 || mode > 2)
     return fail(c, LBVH_E_INVALID, "unknown intersector mode");
+  if (nrays > kMaxRaysPerCall)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 rays per call");
   if (misaligned(d_rays, 16) || trace_outputs_misaligned(d_prim, d_t, d_uv))
     return fail(c, LBVH_E_INVALID,
                 "device rays must be 16-byte aligned, prim and t 4-byte, uv 8-byte");
@@ -1176,6 +1178,8 @@ lbvh_trace_closest(lbvh_ctx* c, const lbvh_bvh* b, const float* rays, uint64_t n
     return fail(c, LBVH_E_INVALID, "null argument");
   if (nrays == 0)
     return LBVH_OK;
+  if (nrays > kMaxRaysPerCall)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 rays per call");
   lbvh_status st = set_device(c);
   if (st != LBVH_OK)
     return st;

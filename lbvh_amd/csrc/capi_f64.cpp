@@ -193,6 +193,8 @@ f64_trace(lbvh_ctx* c, const void* d_nodes, uint64_t n, const double* d_tris,
     return fail(c, LBVH_E_INVALID, "a tree needs at least 2 primitives");
   if (n > kMaxF64Prims)
     return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 primitives per FP64 tree");
+  if (nrays > kMaxRaysPerCall)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 rays per call");
   if (misaligned(d_nodes, 16) || misaligned(d_tris, 16) || misaligned(d_rays, 16) ||
       misaligned(d_prim, 8) || misaligned(d_t, 8) || misaligned(d_uv, 16))
     return fail(c, LBVH_E_INVALID,
@@ -239,6 +241,8 @@ lbvh_trace_closest_f64(lbvh_ctx* c, const void* nodes, uint64_t n, const double*
     return fail(c, LBVH_E_INVALID, "unknown intersector mode (FP64 traversal: 0 or 1)");
   if (nrays == 0)
     return LBVH_OK;
+  if (nrays > kMaxRaysPerCall)
+    return fail(c, LBVH_E_INVALID, "at most 2^31 - 1 rays per call");
   lbvh_status st = set_device(c);
   if (st != LBVH_OK)
     return st;

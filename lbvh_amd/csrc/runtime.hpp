@@ -125,6 +125,9 @@ trace_outputs_misaligned(const void* prim, const void* t, const void* uv)
 {
   return misaligned(prim, 4) || misaligned(t, 4) || misaligned(uv, 8);
 }
+// Rays per traversal call (and per coalesced server launch): a launch's
+// work-item count must fit the dispatch's 32-bit grid size.
+constexpr uint64_t kMaxRaysPerCall = (uint64_t(1) << 31) - 1;
 lbvh_status hip_fail(lbvh_ctx* c, hipError_t e, const char* what);
 lbvh_status ensure(lbvh_ctx* c, DevBuf& b, uint64_t bytes, bool zero_new = false);
 void dev_free(void* p);

@@ -636,9 +636,11 @@ worker_main(lbvh_server* s)
     for (auto& qs : to_start)
       start(s, qs.first, qs.second);
 
-    // every started request whose lane work finished joins one launch
+    // every started request whose lane work finished joins one launch, up to
+    // kMaxRaysPerCall rays in all (the rest join the next one)
     std::vector<Req*> ready;
     std::vector<Req*> failed;
+    uint64_t ready_rays = 0;
     for (Req* q : s->active) {
       if (q->st != St::Started)
         continue;
@@ -646,8 +648,11 @@ worker_main(lbvh_server* s)
         failed.push_back(q);
         continue;
       }
+      if (ready_rays + q->r.n_rays > kMaxRaysPerCall)
+        continue;
       if (hipEventQuery(q->ev_ready) == hipSuccess) {
         q->t_ready = us_since(s);
+        ready_rays += q->r.n_rays;
         ready.push_back(q);
       }
     }
@@ -801,6 +806,8 @@ lbvh_server_submit(lbvh_server* s, const lbvh_request* r, uint64_t* ticket, uint
   if (!s || !r || !ticket)
     return LBVH_E_INVALID;
   if (r->n_rays && (!r->rays32 || !r->prim || !r->t))
+    return LBVH_E_INVALID;
+  if (r->n_rays > kMaxRaysPerCall)
     return LBVH_E_INVALID;
   if (r->scene == 0 && (!r->tri48 || r->n_tri < 2 || r->n_tri >= (uint64_t(1) << 29)))
     return LBVH_E_INVALID;

@@ -195,25 +195,38 @@ def test_f64_rejects_bad_arguments(ctx):
 MISS64 = np.uint64(2**64 - 1)
 
 
+# scale, offset of the 1e6- and 1e-6-scale copies of the uniform scene
+SCALED = {"huge": (2.0**20, 3.0e6), "tiny": (2.0**-20, 0.0)}
+
+
 def _trace_scenes():
     from lbvh_amd import workload as W
-    return {
-        "uniform": W.triangles_f64(W.uniform_vertices(21, 6000, 0.04)),
+    u = W.uniform_vertices(21, 6000, 0.04).astype(np.float64)
+    out = {
+        "uniform": W.triangles_f64(u),
         "clustered": W.triangles_f64(W.cluster_vertices(22, 6000, half_extent=0.02)),
         # every triangle identical: all codes equal, equal distances everywhere
         "all_equal": W.triangles_f64(np.tile(W.uniform_vertices(23, 1, 0.05), (300, 1))),
     }
+    for k, (sc, off) in SCALED.items():
+        out[k] = W.triangles_f64(u * sc + off)
+    return out
 
 
-def _rays_f64(tri, seed, m=4000):
+def _rays_f64(tri, seed, m=4000, scene=None):
     """Primary rays of a 48x48 image plus rays from inside the scene's box in
     random directions, some with a tmax that ends before the first hit."""
     from lbvh_amd import workload as W
+    sc, off = SCALED.get(scene, (1.0, 0.0))
     boxes = W.triangle_boxes_f64(tri)
     lo, hi = boxes[:, :3].min(axis=0), boxes[:, 3:].max(axis=0)
     inner = W.box_rays(seed, lo.astype(np.float32), hi.astype(np.float32), m).astype(np.float64)
-    inner[::3, 7] = 0.05
-    return np.ascontiguousarray(np.concatenate([W.primary_rays(48, 48).astype(np.float64), inner]))
+    inner[:, 6] *= sc
+    inner[::3, 7] = 0.05 * sc
+    prim = W.primary_rays(48, 48).astype(np.float64)
+    prim[:, 0:3] = prim[:, 0:3] * sc + off
+    prim[:, 6] *= sc
+    return np.ascontiguousarray(np.concatenate([prim, inner]))
 
 
 def W_boxes(tri):
@@ -230,15 +243,16 @@ def _same_f64_hits(got, want, what):
         assert np.array_equal(uv.view(np.uint64), wuv.view(np.uint64)), "%s: uv differs" % what
 
 
-@pytest.mark.parametrize("scene", ["uniform", "clustered", "all_equal"])
+@pytest.mark.parametrize("scene", ["uniform", "clustered", "all_equal", "huge", "tiny"])
 @pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
 @pytest.mark.parametrize("mode", [0, 1], ids=["first", "lowid"])
 def test_f64_trace_matches_oracle(ctx, scene, leaf, mode):
     """The GPU's FP64 traversal of the GPU's FP64 tree: ids, t and uv bit for
-    bit the oracle's traverser<double> (pinned to the compiled reference's)."""
+    bit the oracle's traverser<double> (pinned to the compiled reference's),
+    also on 1e6- and 1e-6-scale copies of the uniform scene."""
     tri = _trace_scenes()[scene]
     nodes = ctx.build_aabbs_f64(W_boxes(tri), leaf)
-    rays = _rays_f64(tri, 31)
+    rays = _rays_f64(tri, 31, scene=scene)
     got = ctx.trace_f64(nodes, tri, rays, mode)
     want = oracle.trace_f64(nodes, tri, rays, mode)
     _same_f64_hits(got, want, "%s leaf %d mode %d" % (scene, leaf, mode))

@@ -194,22 +194,35 @@ def _f64_scenes():
         # every triangle identical: all codes equal, ties everywhere
         "all_equal": np.tile(W.uniform_vertices(23, 1, 0.05), (300, 1)),
     }
-    return {k: tri96_of(v) for k, v in out.items()}
+    scenes = {k: tri96_of(v) for k, v in out.items()}
+    # 1e6-scale and 1e-6-scale copies of the uniform scene (FP64 vertices
+    # scaled exactly by powers of two): the arithmetic must hold at both ends
+    u = np.asarray(out["uniform"], dtype=np.float64)
+    scenes["huge"] = tri96_of(u * 2.0**20 + 3.0e6)
+    scenes["tiny"] = tri96_of(u * 2.0**-20)
+    return scenes
 
 
-def _rays64(w=48, h=48):
+def _rays64(w=48, h=48, scene=None):
     from lbvh_amd import workload as W
-    return W.primary_rays(w, h).astype(np.float64)
+    r = W.primary_rays(w, h).astype(np.float64)
+    if scene == "huge":
+        r[:, 0:3] = r[:, 0:3] * 2.0**20 + 3.0e6
+        r[:, 6] *= 2.0**20
+    elif scene == "tiny":
+        r[:, 0:3] *= 2.0**-20
+        r[:, 6] *= 2.0**-20
+    return r
 
 
 @pytest.mark.skipif(not oracle.ref_available(), reason="reference driver not built")
-@pytest.mark.parametrize("scene", ["uniform", "clustered", "all_equal"])
+@pytest.mark.parametrize("scene", ["uniform", "clustered", "all_equal", "huge", "tiny"])
 @pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
 @pytest.mark.parametrize("mode", [0, 1], ids=["first", "lowid"])
 def test_f64_trace_matches_compiled_reference(scene, leaf, mode):
     tri = _f64_scenes()[scene]
     nodes, _, _ = oracle.build_f64(oracle.triangle_aabb_f64(tri), leaf)
-    rays = _rays64()
+    rays = _rays64(scene=scene)
     p, t, uv = oracle.trace_f64(nodes, tri, rays, mode)
     rp, rt, ruv = oracle.ref_trace_f64(nodes, tri, rays, mode)
     assert np.array_equal(p, rp)

@@ -340,3 +340,19 @@ def test_f64_trace_rejects_bad_arguments(ctx):
     _same_f64_hits(ctx.trace_f64(nodes, tri, rays, 1), want, "after errors")
     p, t, uv = ctx.trace_f64(nodes, tri, rays[:0], 1)
     assert p.shape == (0,)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 64])
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+def test_f64_trace_edge_rays_match_oracle(ctx, n, leaf):
+    """Tiny trees and degenerate rays (axis-aligned, in a triangle's plane,
+    NaN and infinite components, tmin > tmax, zero direction): the GPU's hits
+    equal the oracle's bit for bit, NaN results compared as bits."""
+    from lbvh_amd import workload as W
+    from test_oracle_f64 import _edge_rays64
+    tri = W.triangles_f64(W.uniform_vertices(70 + n, n, 0.2))
+    nodes = ctx.build_aabbs_f64(W_boxes(tri), leaf)
+    rays = _edge_rays64(tri)
+    for mode in (0, 1):
+        got = ctx.trace_f64(nodes, tri, rays, mode)
+        _same_f64_hits(got, oracle.trace_f64(nodes, tri, rays, mode), "n %d mode %d" % (n, mode))

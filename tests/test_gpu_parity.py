@@ -552,3 +552,24 @@ def test_traversal_tuner_records_each_mode_and_keeps_results_exact():
             sorted((ro, w) for ro in (False, True) for w in (8, 6, 4))
     bvh.destroy()
     c.close()
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 64])
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+def test_trace_edge_rays_match_oracle(ctx, n, leaf):
+    """Tiny trees and degenerate rays (axis-aligned, in a triangle's plane,
+    NaN and infinite components, tmin > tmax, zero direction) in every
+    intersector mode: ids, t and uv equal the oracle's bit for bit."""
+    from test_oracle import edge_rays32
+    tri = W.make_triangles(W.uniform_vertices(80 + n, n, 0.2))
+    bvh = ctx.build_triangles(tri, code_bits=30, leaf_order=leaf)
+    on, _, _, _ = oracle.build(tri, 30, leaf)
+    assert bvh.nodes().tobytes() == on.tobytes()
+    rays = edge_rays32(tri)
+    for mode in (0, 1, 2):
+        p, t, uv = ctx.trace(bvh, rays, mode)
+        op, ot, ouv = oracle.trace(on, tri, rays, mode)
+        assert np.array_equal(p, op), "mode %d ids" % mode
+        assert np.array_equal(t.view(np.uint32), ot.view(np.uint32)), "mode %d t" % mode
+        assert np.array_equal(uv.view(np.uint32), ouv.view(np.uint32)), "mode %d uv" % mode
+    bvh.destroy()

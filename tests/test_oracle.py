@@ -327,3 +327,49 @@ def test_uniform16m_hashes(hashes):
     nodes, bounds, sc, _ = oracle.build(tri, 30, 0)
     assert oracle.fnv1a64(nodes) == e["b30_l0_nodes"]
     assert oracle.fnv1a64(sc) == e["b30_l0_sorted"]
+
+
+def edge_rays32(tri):
+    """FP32 degenerate rays aimed at the scene: axis-aligned (infinite
+    reciprocals), in a triangle's plane, NaN and infinite components, an
+    empty [tmin, tmax] interval and a zero direction."""
+    rng = np.random.default_rng(62)
+    v = np.asarray(tri, dtype=np.float64)
+    lo = v[:, 0:3].min(axis=0) - 0.1
+    hi = v[:, 0:3].max(axis=0) + 0.1
+    out = []
+    for k in range(3):
+        for sgn in (1.0, -1.0):
+            for _ in range(8):
+                o = rng.uniform(lo, hi)
+                o[k] = lo[k] if sgn > 0 else hi[k]
+                d = np.zeros(3)
+                d[k] = sgn
+                out.append(np.concatenate([o, d, [0.0, np.inf]]))
+    p0, e1 = v[0, 0:3], v[0, 6:9]
+    out.append(np.concatenate([p0 - e1, e1, [0.0, np.inf]]))
+    c = (lo + hi) / 2
+    out.append(np.concatenate([c, [np.nan, 1.0, 0.0], [0.0, np.inf]]))
+    out.append(np.concatenate([[np.nan, 0.5, 0.5], [0.0, 0.0, 1.0], [0.0, np.inf]]))
+    out.append(np.concatenate([c - [0, 0, 2], [0.0, 0.0, np.inf], [0.0, np.inf]]))
+    out.append(np.concatenate([c - [0, 0, 2], [0.0, 0.0, 1.0], [5.0, 1.0]]))
+    out.append(np.concatenate([c, [0.0, 0.0, 0.0], [0.0, np.inf]]))
+    return np.ascontiguousarray(np.array(out, dtype=np.float32))
+
+
+@needs_ref
+@pytest.mark.parametrize("n", [2, 3, 5, 64])
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+def test_oracle_trace_edge_rays_equal_reference(n, leaf):
+    """Tiny trees and degenerate rays: the oracle's traversal equals the
+    compiled reference's bit for bit in both tie modes."""
+    tri = W.make_triangles(W.uniform_vertices(80 + n, n, 0.2))
+    nodes, _, _, _ = oracle.build(tri, 30, leaf)
+    rays = edge_rays32(tri)
+    for mode in (0, 1):
+        p, t, uv = oracle.trace(nodes, tri, rays, mode)
+        rp, rt, ruv = oracle.ref_trace(nodes, tri, rays, mode)
+        assert np.array_equal(p, rp)
+        assert np.array_equal(t.view(np.uint32), rt.view(np.uint32))
+        hit = p != 0xFFFFFFFF
+        assert np.array_equal(uv[hit].view(np.uint32), ruv[hit].view(np.uint32))

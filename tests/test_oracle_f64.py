@@ -263,3 +263,48 @@ def test_f64_trace_same_closest_hit_in_both_leaf_orders(scene):
         got.append(oracle.trace_f64(nodes, tri, rays, 1))
     assert np.array_equal(got[0][0], got[1][0])
     assert np.array_equal(got[0][1].view(np.uint64), got[1][1].view(np.uint64))
+
+
+def _edge_rays64(tri):
+    """Axis-aligned rays (zero direction components: infinite reciprocals),
+    rays along a triangle's plane, NaN and infinite components, an empty
+    [tmin, tmax] interval and a zero direction, aimed at the scene."""
+    rng = np.random.default_rng(61)
+    lo = tri[:, 0:3].min(axis=0) - 0.1
+    hi = tri[:, 0:3].max(axis=0) + 0.1
+    out = []
+    for k in range(3):
+        for sgn in (1.0, -1.0):
+            for _ in range(8):
+                o = rng.uniform(lo, hi)
+                o[k] = lo[k] if sgn > 0 else hi[k]
+                d = np.zeros(3)
+                d[k] = sgn
+                out.append(np.concatenate([o, d, [0.0, np.inf]]))
+    p0, e1 = tri[0, 0:3], tri[0, 6:9]
+    out.append(np.concatenate([p0 - e1, e1, [0.0, np.inf]]))       # in the plane of a triangle
+    c = (lo + hi) / 2
+    out.append(np.concatenate([c, [np.nan, 1.0, 0.0], [0.0, np.inf]]))
+    out.append(np.concatenate([[np.nan, 0.5, 0.5], [0.0, 0.0, 1.0], [0.0, np.inf]]))
+    out.append(np.concatenate([c - [0, 0, 2], [0.0, 0.0, np.inf], [0.0, np.inf]]))
+    out.append(np.concatenate([c - [0, 0, 2], [0.0, 0.0, 1.0], [5.0, 1.0]]))  # tmin > tmax
+    out.append(np.concatenate([c, [0.0, 0.0, 0.0], [0.0, np.inf]]))          # zero direction
+    return np.ascontiguousarray(np.array(out, dtype=np.float64))
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference driver not built")
+@pytest.mark.parametrize("n", [2, 3, 5, 64])
+@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
+def test_f64_trace_edge_rays_match_compiled_reference(n, leaf):
+    """Tiny trees and degenerate rays: the oracle's traverser<double> equals
+    the compiled reference's bit for bit (NaN results compared as bits)."""
+    from lbvh_amd import workload as W
+    tri = tri96_of(W.uniform_vertices(70 + n, n, 0.2))
+    nodes, _, _ = oracle.build_f64(oracle.triangle_aabb_f64(tri), leaf)
+    rays = _edge_rays64(tri)
+    for mode in (0, 1):
+        p, t, uv = oracle.trace_f64(nodes, tri, rays, mode)
+        rp, rt, ruv = oracle.ref_trace_f64(nodes, tri, rays, mode)
+        assert np.array_equal(p, rp)
+        assert np.array_equal(t.view(np.uint64), rt.view(np.uint64))
+        assert np.array_equal(uv.view(np.uint64), ruv.view(np.uint64))

@@ -197,9 +197,9 @@ lbvh_status lbvh_refit_triangles_device(lbvh_ctx* ctx, lbvh_bvh* bvh,
  * memory -- code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc: lbvh_build_triangles_device's build with no allocation
  * of any kind.  Every scratch buffer (centroids, keys, ids, sort counts, tree
- * segments and tickets, and the tree records the nodes are built beside) is
+ * segments and tickets, and the parent links the nodes are built beside) is
  * carved from one fixed device array that each device holds for the life of
- * the process (40 MiB, set up when a context is first created on the
+ * the process (24 MiB, set up when a context is first created on the
  * device).  Inputs: d_tri48, n device triangles (16-byte aligned), n in
  * [2, LBVH_NOALLOC_MAX_PRIMS].  Output: the (n - 1) lbvh::node<float> records
  * of the tree in d_nodes_out (device memory, 32-byte aligned, (n - 1) x 32

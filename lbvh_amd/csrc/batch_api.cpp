@@ -61,7 +61,6 @@ struct lbvh_batch
   uint32_t* d_flags = nullptr;
   uint32_t* d_other_end = nullptr;
   uint32_t* d_leafprim = nullptr;
-  void* d_wide = nullptr;
   int32_t* d_bounds_key = nullptr;
   // scratch
   float* d_cent = nullptr;
@@ -95,7 +94,7 @@ free_batch(lbvh_batch* b)
   b->g_build.reset();
   b->g_refit.reset();
   void* bufs[] = { b->d_off,       b->d_nodes,  b->d_parent,      b->d_flags,   b->d_other_end,
-                   b->d_leafprim,  b->d_wide,   b->d_bounds_key,  b->d_cent,    b->d_keys_a,
+                   b->d_leafprim,  b->d_bounds_key,  b->d_cent,    b->d_keys_a,
                    b->d_keys_b,    b->d_ids_a,  b->d_ids_b,       b->d_part_hist, b->d_small,
                    b->d_counts,    b->d_treelets, b->d_tree, b->d_tree_tickets };
   for (void* p : bufs)
@@ -151,7 +150,6 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
     { reinterpret_cast<void**>(&b->d_other_end), m * 4 },
     { reinterpret_cast<void**>(&b->d_leafprim),
       b->leaf_order == LBVH_LEAF_PRIMITIVE_ID ? n * 4 : 0 },
-    { &b->d_wide, m * 64 },
     { reinterpret_cast<void**>(&b->d_bounds_key), 24ull * b->nseg },
     { reinterpret_cast<void**>(&b->d_cent), 12 * n },
     { reinterpret_cast<void**>(&b->d_keys_a), 8 * n },
@@ -231,8 +229,8 @@ queue_build(lbvh_ctx* c, lbvh_batch* b, bool timed)
   const uint32_t* leaf_ids = b->leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr;
   if (e == hipSuccess)
     e = launch_tree(30, sorted, leaf_ids, n, b->d_off, b->nseg, b->d_prims, 0, b->d_nodes,
-                    b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end, b->d_wide,
-                    b->d_tree, b->d_tree_tickets, s);
+                    b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end, b->d_tree,
+                    b->d_tree_tickets, s);
   if (timed) {
     record(c, 3);
     record(c, 4, true);
@@ -252,7 +250,7 @@ queue_refit(lbvh_ctx* c, lbvh_batch* b)
   if (e == hipSuccess)
     e = launch_fit(b->d_prims, 0, b->n, b->d_off, b->nseg, b->d_nodes, b->d_parent,
                    b->d_other_end, b->d_leafprim, b->d_flags, b->d_treelets,
-                   small + kSmallTreeletCount, small + kSmallTreeletQueue, b->d_wide, s);
+                   small + kSmallTreeletCount, small + kSmallTreeletQueue, s);
   return e;
 }
 
@@ -630,7 +628,7 @@ lbvh_batch_trace_closest_device(lbvh_ctx* c, const lbvh_batch* b, const float* d
     return st;
   begin_timed(c, 2);
   record(c, 0);
-  HIP_TRY(c, launch_trace(mode, b->d_wide, b->d_prims, b->n, b->d_off, d_ray_off, b->nseg,
+  HIP_TRY(c, launch_trace(mode, b->d_nodes, b->d_prims, b->n, b->d_off, d_ray_off, b->nseg,
                           d_rays, nrays, d_prim, d_t, d_uv, c->stream));
   record(c, 1, true);
   return LBVH_OK;

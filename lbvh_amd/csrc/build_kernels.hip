@@ -430,21 +430,6 @@ store_node(Node* nd, const Box& b, uint32_t left, uint32_t right)
   q[1] = make_uint4(__float_as_uint(b.hi[1]), __float_as_uint(b.hi[2]), left, right);
 }
 
-// Its traversal record: both child boxes and both children, one full
-// 64-byte record.
-__device__ __forceinline__ void
-store_wide(WideNode* w, const Box& lb, const Box& rb, uint32_t left, uint32_t right)
-{
-  uint4* q = reinterpret_cast<uint4*>(w);
-  q[0] = make_uint4(__float_as_uint(lb.lo[0]), __float_as_uint(lb.lo[1]),
-                    __float_as_uint(lb.lo[2]), __float_as_uint(lb.hi[0]));
-  q[1] = make_uint4(__float_as_uint(lb.hi[1]), __float_as_uint(lb.hi[2]),
-                    __float_as_uint(rb.lo[0]), __float_as_uint(rb.lo[1]));
-  q[2] = make_uint4(__float_as_uint(rb.lo[2]), __float_as_uint(rb.hi[0]),
-                    __float_as_uint(rb.hi[1]), __float_as_uint(rb.hi[2]));
-  q[3] = make_uint4(left, right, 0u, 0u);
-}
-
 // Child value of a node: an internal index, or a leaf (bit 31) holding the
 // sorted position (reference leaves) or the primitive id there.
 __device__ __forceinline__ uint32_t
@@ -495,8 +480,7 @@ __global__ __launch_bounds__(kTreelet) void
 k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __restrict__ nodes_all,
                const uint32_t* __restrict__ other_end_all,
                const uint32_t* __restrict__ leafprim_all, const uint2* __restrict__ treelets,
-               const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ queue,
-               WideNode* __restrict__ wide_all)
+               const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ queue)
 {
   // leaf boxes at s_bt[m + e]; below m the segment tree, or (NaN) node
   // base + j's box at s_bt[1 + j]
@@ -521,7 +505,6 @@ k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* _
       Node* nodes = nodes_all + ts.nb;
       const uint32_t* other_end = other_end_all + ts.nb;
       const uint32_t* leafprim = leafprim_all ? leafprim_all + ts.P : nullptr;
-      WideNode* wide = wide_all + ts.nb;
       // the root is numbered lo when it is a right child (and for a
       // whole-tree treelet, whose root is node 0)
       const bool root_lo = (en.y & 0x80000000u) || (lo == 0 && hi == n - 1);
@@ -565,7 +548,6 @@ k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* _
           const Box lb = bt_query(s_bt, m, int(L - lo), int(g - lo));
           const Box rb = bt_query(s_bt, m, int(g + 1 - lo), int(R - lo));
           store_node(&nodes[k], box_union(lb, rb), lv, rv);
-          store_wide(&wide[k], lb, rb, lv, rv);
         }
       } else {
         // children before parents: node box of base + j at s_bt[1 + j]
@@ -582,7 +564,6 @@ k_fit_treelets(const float* __restrict__ prims_all, int kind, Scenes sc, Node* _
               const Box b = box_union(lb, rb);
               s_bt[1 + t] = b;
               store_node(&nodes[k], b, lv, rv);
-              store_wide(&wide[k], lb, rb, lv, rv);
               changed = 1;
             }
           }
@@ -602,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void
 k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __restrict__ nodes_all,
           const uint32_t* __restrict__ parent_all, const uint32_t* __restrict__ leafprim_all,
           const uint2* __restrict__ treelets, const uint32_t* __restrict__ count_ptr,
-          uint32_t* __restrict__ flags_all, WideNode* __restrict__ wide_all)
+          uint32_t* __restrict__ flags_all)
 {
   const uint32_t count = *count_ptr;
   for (uint32_t item = blockIdx.x * kBlock + threadIdx.x; item < count;
@@ -617,7 +598,6 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
     const uint32_t* parent = parent_all + ts.lb;
     const uint32_t* leafprim = leafprim_all ? leafprim_all + ts.P : nullptr;
     uint32_t* flags = flags_all + ts.nb;
-    WideNode* wide = wide_all + ts.nb;
     Box mine;
     int64_t cur;
     uint32_t cur_v;
@@ -666,7 +646,6 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
       const uint32_t rv = right ? cur_v : sib_v;
       const Box b = box_union(lb, rb);
       coherent_store_u32(&flags[p], 0u);
-      store_wide(&wide[p], lb, rb, lv, rv);
       if (p == 0) {
         store_node(&nodes[0], b, lv, rv);
         break;
@@ -697,7 +676,7 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
 // leaves (Apetrei, "Fast and Simple Agglomerative LBVH Construction", 2014):
 // the second child to reach a split completes that node -- its range, its
 // box union(left, right) as fit_boxes computes it (lbvh.h:1827-1866), its
-// number, its records and its children's parent links.
+// number, its node record and its children's parent links.
 //
 // k_tree runs this one level of workgroups after another, in LDS:
 //   level 0: a workgroup per kTreeLeaves sorted positions; its elements are
@@ -713,7 +692,7 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
 // are the nearest deltas below d[j + 1] on either side (searched in a min
 // tree over d); it lies in the workgroup when both exist.  Its children are
 // the elements j or j + 1 or the nodes numbered j and j + 1 in element
-// space, so boxes and records follow in passes, each node once both
+// space, so boxes follow in passes, each node once both
 // children are done (the reference's union order).  Nodes and elements
 // whose parent does not lie in the workgroup -- at most two per tree level,
 // so at most kTreeSegCap -- are listed in position order
@@ -722,7 +701,7 @@ k_fit_top(const float* __restrict__ prims_all, int kind, Scenes sc, Node* __rest
 // workgroups (and XCDs) through coherent stores, a vector-memory drain and
 // the ticket atomic; tickets reset themselves.
 //
-// Outputs: node and traversal records, parent links (kLink*), other_end,
+// Outputs: node records, parent links (kLink*), other_end,
 // leafprim, and the fit flags left at 0 (what a refit starts from).
 
 constexpr int kTreeLeaves = 512;                        // level-0 positions per workgroup
@@ -776,7 +755,6 @@ struct TreeScene
   int64_t P, n;
   const float* prims;
   Node* nodes;
-  WideNode* wide;
   uint32_t* parent;
   uint32_t* leafprim;
   uint32_t* flags;
@@ -786,7 +764,7 @@ struct TreeScene
 template<typename KeyT, bool SEG>
 __device__ __forceinline__ TreeScene
 tree_scene(const KeyT* __restrict__ keys, const Scenes& sc, int64_t x, const float* prims_all,
-           int kind, Node* nodes_all, WideNode* wide_all, uint32_t* parent_all,
+           int kind, Node* nodes_all, uint32_t* parent_all,
            uint32_t* leafprim_all, uint32_t* flags_all, uint32_t* other_end_all)
 {
   uint32_t seg = 0;
@@ -803,7 +781,6 @@ tree_scene(const KeyT* __restrict__ keys, const Scenes& sc, int64_t x, const flo
   t.n = n;
   t.prims = prims_all + int64_t(prim_stride(kind)) * P;
   t.nodes = nodes_all + nb;
-  t.wide = wide_all + nb;
   t.parent = parent_all + lb;
   t.leafprim = leafprim_all ? leafprim_all + P : nullptr;
   t.flags = flags_all + nb;
@@ -881,7 +858,7 @@ template<typename KeyT, typename CodeT, bool SEG, bool FIRST>
 __global__ __launch_bounds__(FIRST ? kTreeLeaves : kTreeUpThreads) void
 k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes sc,
        const float* __restrict__ prims_all, int kind, Node* __restrict__ nodes_all,
-       WideNode* __restrict__ wide_all, uint32_t* __restrict__ parent_all,
+       uint32_t* __restrict__ parent_all,
        uint32_t* __restrict__ leafprim_all, uint32_t* __restrict__ flags_all,
        uint32_t* __restrict__ other_end_all, TreeLevels lv, uint32_t* __restrict__ seg_count,
        TreeEntry* __restrict__ seg_entries, uint32_t* __restrict__ tickets)
@@ -1019,7 +996,7 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
     if (t < m) {
       const int64_t x = lo + t;
       const TreeScene ts = tree_scene<KeyT, SEG>(keys, sc, x, prims_all, kind, nodes_all,
-                                                 wide_all, parent_all, leafprim_all, flags_all,
+                                                 parent_all, leafprim_all, flags_all,
                                                  other_end_all);
       const int64_t lx = x - ts.P;
       if (lx < ts.n - 1)
@@ -1128,8 +1105,8 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
         break;
     }
     // ---- records of the completed nodes ------------------------------------------
-    // children's parent links by split (consecutive positions), node and
-    // traversal records by node number (consecutive records)
+    // children's parent links by split (consecutive positions), node records
+    // by node number (consecutive records)
     auto node_scene = [&](int j, int64_t& P, int64_t& n, uint32_t& seg) {
       P = 0;
       n = N;
@@ -1193,7 +1170,6 @@ k_tree(const KeyT* __restrict__ keys, const uint32_t* __restrict__ ids, Scenes s
       const int64_t lg = g - P;
       const uint32_t lvv = lleaf ? s_el_val[j] : uint32_t(lg);
       const uint32_t rvv = rleaf ? s_el_val[j + 1] : uint32_t(lg + 1);
-      store_wide(&wide_all[nb + k], lb, rb, lvv, rvv);
       store_node(&nodes_all[nb + k], bx, lvv, rvv);
       other_end_all[nb + k] = uint32_t((k == L - P) ? R - P : L - P);
     }
@@ -1458,7 +1434,7 @@ hipError_t
 launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off, uint32_t nseg,
            void* nodes, const uint32_t* parent, const uint32_t* other_end,
            const uint32_t* leafprim, uint32_t* flags, const void* treelets,
-           uint32_t* treelet_count, uint32_t* queue, void* wide, hipStream_t s)
+           uint32_t* treelet_count, uint32_t* queue, hipStream_t s)
 {
   const Scenes sc = make_scenes(prim_off, nseg, n);
   const uint2* tl = static_cast<const uint2*>(treelets);
@@ -1466,15 +1442,13 @@ launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off, u
   // scene); the loops are persistent, so any count is covered
   const uint32_t ga = grid_for(2 * n + uint64_t(sc.nseg) * kTreelet, kTreelet, 768);
   hipLaunchKernelGGL(k_fit_treelets, dim3(ga), dim3(kTreelet), 0, s, prims, kind, sc,
-                     static_cast<Node*>(nodes), other_end, leafprim, tl, treelet_count, queue,
-                     static_cast<WideNode*>(wide));
+                     static_cast<Node*>(nodes), other_end, leafprim, tl, treelet_count, queue);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
   const uint32_t gb = grid_for(4 * n / kTreelet + 64, kBlock, 1024);
   hipLaunchKernelGGL(k_fit_top, dim3(gb), dim3(kBlock), 0, s, prims, kind, sc,
-                     static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, flags,
-                     static_cast<WideNode*>(wide));
+                     static_cast<Node*>(nodes), parent, leafprim, tl, treelet_count, flags);
   return hipGetLastError();
 }
 
@@ -1543,7 +1517,7 @@ template<typename KeyT, typename CodeT, bool SEG>
 static void
 tree_launch(const void* sorted, const uint32_t* ids, const Scenes& sc, const float* prims,
             int kind, void* nodes, uint32_t* parent, uint32_t* leafprim, uint32_t* flags,
-            uint32_t* other_end, void* wide, void* scratch, uint32_t* tk, hipStream_t s)
+            uint32_t* other_end, void* scratch, uint32_t* tk, hipStream_t s)
 {
   const uint64_t n = uint64_t(sc.n);
   TreeLevels lv = tree_levels(n);
@@ -1554,20 +1528,20 @@ tree_launch(const void* sorted, const uint32_t* ids, const Scenes& sc, const flo
   TreeEntry* entries = reinterpret_cast<TreeEntry*>(static_cast<char*>(scratch) + head);
   hipLaunchKernelGGL((k_tree<KeyT, CodeT, SEG, true>), dim3(lv.nblocks[0]), dim3(kTreeLeaves), 0,
                      s, static_cast<const KeyT*>(sorted), ids, sc, prims, kind,
-                     static_cast<Node*>(nodes), static_cast<WideNode*>(wide), parent, leafprim,
-                     flags, other_end, lv, counts, entries, tk);
+                     static_cast<Node*>(nodes), parent, leafprim, flags, other_end, lv, counts,
+                     entries, tk);
   if (lv.levels > 1)
     hipLaunchKernelGGL((k_tree<KeyT, CodeT, SEG, false>), dim3(lv.nblocks[1]), dim3(kTreeUpThreads),
                        0, s, static_cast<const KeyT*>(sorted), ids, sc, prims, kind,
-                       static_cast<Node*>(nodes), static_cast<WideNode*>(wide), parent, leafprim,
-                       flags, other_end, lv, counts, entries, tk);
+                       static_cast<Node*>(nodes), parent, leafprim, flags, other_end, lv, counts,
+                       entries, tk);
 }
 
 hipError_t
 launch_tree(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
             const uint64_t* prim_off, uint32_t nseg, const float* prims, int kind, void* nodes,
             uint32_t* parent, uint32_t* leafprim, uint32_t* flags, uint32_t* other_end,
-            void* wide, void* scratch, uint32_t* tickets, hipStream_t s)
+            void* scratch, uint32_t* tickets, hipStream_t s)
 {
   const Scenes sc = make_scenes(prim_off, nseg, n);
   if (tree_levels(n).nblocks[tree_levels(n).levels - 1] != 1)
@@ -1576,13 +1550,13 @@ launch_tree(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
     if (code_bits != 30)
       return hipErrorInvalidValue;
     tree_launch<uint64_t, uint32_t, true>(sorted, ids, sc, prims, kind, nodes, parent, leafprim,
-                                          flags, other_end, wide, scratch, tickets, s);
+                                          flags, other_end, scratch, tickets, s);
   } else if (code_bits == 30) {
     tree_launch<uint32_t, uint32_t, false>(sorted, ids, sc, prims, kind, nodes, parent, leafprim,
-                                           flags, other_end, wide, scratch, tickets, s);
+                                           flags, other_end, scratch, tickets, s);
   } else {
     tree_launch<uint64_t, uint64_t, false>(sorted, ids, sc, prims, kind, nodes, parent, leafprim,
-                                           flags, other_end, wide, scratch, tickets, s);
+                                           flags, other_end, scratch, tickets, s);
   }
   return hipGetLastError();
 }

@@ -249,7 +249,6 @@ alloc_bvh(lbvh_ctx* c, lbvh_bvh* b, bool keep_codes)
     { reinterpret_cast<void**>(&b->d_parent), (2 * n - 1) * 4 },
     { reinterpret_cast<void**>(&b->d_flags), m * 4 },
     { reinterpret_cast<void**>(&b->d_other_end), m * 4 },
-    { &b->d_wide, m * 64 },
     { reinterpret_cast<void**>(&b->d_bounds), 32 },
     { &b->d_codes, keep_codes ? n * cw : 0 },
     { reinterpret_cast<void**>(&b->d_ids), keep_codes ? n * 4 : 0 },
@@ -514,7 +513,7 @@ build_common(lbvh_ctx* c, const float* d_prims, int kind, uint64_t n,
   const uint32_t* leaf_ids = opts.leaf_order == LBVH_LEAF_PRIMITIVE_ID ? sorted_ids : nullptr;
   if (e == hipSuccess)
     e = launch_tree(opts.code_bits, sorted, leaf_ids, n, nullptr, 1, d_prims, kind, b->d_nodes,
-                    b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end, b->d_wide, c->tree.p,
+                    b->d_parent, b->d_leafprim, b->d_flags, b->d_other_end, c->tree.p,
                     static_cast<uint32_t*>(c->tree_tickets.p), c->stream);
   record(c, 3);
   record(c, 4, true);
@@ -861,8 +860,7 @@ queue_refit(lbvh_ctx* c, lbvh_bvh* b)
     c->treelets_of = b->id;
   }
   HIP_TRY(c, launch_fit(b->d_prims, 0, b->n, nullptr, 1, b->d_nodes, b->d_parent, b->d_other_end,
-                        b->d_leafprim, b->d_flags, c->treelets.p, tw, tw + 1, b->d_wide,
-                        c->stream));
+                        b->d_leafprim, b->d_flags, c->treelets.p, tw, tw + 1, c->stream));
   record(c, 1, true);
   return LBVH_OK;
 }
@@ -1079,7 +1077,7 @@ queue_trace(lbvh_ctx* c, const lbvh_bvh* b, const float* d_rays, uint64_t nrays,
     begin_timed(c, 2);
     record(c, 0);
   }
-  HIP_TRY(c, launch_trace(mode, b->d_wide, b->d_prims, b->n, nullptr, nullptr, 1, d_rays, nrays,
+  HIP_TRY(c, launch_trace(mode, b->d_nodes, b->d_prims, b->n, nullptr, nullptr, 1, d_rays, nrays,
                           d_prim, d_t, d_uv, c->stream, order, tc.waves_per_simd));
   record(c, 1, true);
   return LBVH_OK;

@@ -2,7 +2,7 @@
 //
 // The same kernels as lbvh_build_triangles_device, with every scratch buffer
 // (centroids, key and id ping-pong buffers, sort counts, tree segments and
-// tickets, and the tree records the node array is built beside) carved out
+// tickets, and the parent links the node array is built beside) carved out
 // of one file-scope device array of fixed size instead of the context's
 // grow-only workspace and a tree handle.  The caller passes the triangles and
 // the node array; nothing is allocated per call.  Each device holds its own
@@ -23,8 +23,8 @@ namespace {
 
 // Sized for LBVH_NOALLOC_MAX_PRIMS primitives with either code width and any
 // sort configuration this path accepts (the largest layout, 63-bit codes with
-// 10-bit digits and 2,048-key tiles, is 34.4 MiB).
-constexpr uint64_t kScratchBytes = uint64_t(40) << 20;
+// 10-bit digits and 2,048-key tiles, is 18.4 MiB).
+constexpr uint64_t kScratchBytes = uint64_t(24) << 20;
 constexpr int kMaxDevices = 64;
 
 // A host-side name for the device array: its device address is obtained once
@@ -49,7 +49,7 @@ struct Layout
   uint32_t clear_words = 0; // tree tickets + the error flag (and a spare word)
   uint32_t small_words = 0;
   uint64_t cent = 0, keys_a = 0, keys_b = 0, ids_a = 0, ids_b = 0, partials = 0, tree = 0,
-           counts = 0, counts_words = 0, parent = 0, flags = 0, other_end = 0, wide = 0,
+           counts = 0, counts_words = 0, parent = 0, flags = 0, other_end = 0,
            leafprim = 0, bounds = 0, total = 0;
 };
 
@@ -90,7 +90,6 @@ layout_for(uint64_t n, int code_bits, const SortConfig& cfg)
   L.parent = take(4 * (2 * n - 1));
   L.flags = take(4 * (n - 1));
   L.other_end = take(4 * (n - 1));
-  L.wide = take(64 * (n - 1));
   L.leafprim = take(4 * n);
   L.bounds = take(32);
   L.total = o;
@@ -241,7 +240,7 @@ lbvh_build_triangles_noalloc(lbvh_ctx* c, const float* d_tri, uint64_t n,
                     static_cast<uint32_t*>(at(L.parent)),
                     with_ids ? static_cast<uint32_t*>(at(L.leafprim)) : nullptr,
                     static_cast<uint32_t*>(at(L.flags)), static_cast<uint32_t*>(at(L.other_end)),
-                    at(L.wide), at(L.tree), static_cast<uint32_t*>(at(L.tickets)), c->stream);
+                    at(L.tree), static_cast<uint32_t*>(at(L.tickets)), c->stream);
   if (e != hipSuccess)
     return hip_fail(c, e, "allocation-free build launch");
   return LBVH_OK;

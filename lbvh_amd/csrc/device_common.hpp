@@ -24,19 +24,6 @@ struct alignas(32) Node
 };
 static_assert(sizeof(Node) == 32, "node<float> is 32 bytes");
 
-// Traversal layout of an internal node: the boxes of both children beside
-// their indices, so one 64-byte load serves a whole traversal step.  The box
-// of a leaf child is not stored (leaves are intersected directly).
-struct alignas(64) WideNode
-{
-  float lbox[6];
-  float rbox[6];
-  uint32_t left;
-  uint32_t right;
-  uint32_t pad[2];
-};
-static_assert(sizeof(WideNode) == 64, "WideNode is 64 bytes");
-
 struct Box
 {
   float lo[3];

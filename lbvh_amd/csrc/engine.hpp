@@ -101,13 +101,13 @@ uint64_t rts_chunk_keys(uint64_t n, const SortConfig& cfg);
 hipError_t launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* prim_off,
                       uint32_t nseg, void* nodes, const uint32_t* parent,
                       const uint32_t* other_end, const uint32_t* leafprim, uint32_t* flags,
-                      const void* treelets, uint32_t* treelet_count, uint32_t* queue, void* wide,
+                      const void* treelets, uint32_t* treelet_count, uint32_t* queue,
                       hipStream_t s);
 
 // Rebuilds the treelet list of a built tree (for a refit); *treelet_count
 // must be zero.
 // Bottom-up tree construction and fit in one kernel (levels of workgroups
-// merging in LDS).  Writes node and traversal records, parent links,
+// merging in LDS).  Writes the node records, parent links,
 // other_end, leafprim (ids != nullptr and primitive-id leaves) and leaves
 // the fit flags at 0.  `scratch` holds tree_scratch_bytes(n) bytes;
 // `tickets` holds tree_ticket_words(n) words that were zero when first used
@@ -115,21 +115,20 @@ hipError_t launch_fit(const float* prims, int kind, uint64_t n, const uint64_t* 
 hipError_t launch_tree(int code_bits, const void* sorted, const uint32_t* ids, uint64_t n,
                        const uint64_t* prim_off, uint32_t nseg, const float* prims, int kind,
                        void* nodes, uint32_t* parent, uint32_t* leafprim, uint32_t* flags,
-                       uint32_t* other_end, void* wide, void* scratch, uint32_t* tickets,
-                       hipStream_t s);
+                       uint32_t* other_end, void* scratch, uint32_t* tickets, hipStream_t s);
 uint64_t tree_scratch_bytes(uint64_t n);
 uint64_t tree_ticket_words(uint64_t n);
 hipError_t launch_enum_treelets(uint64_t n, const uint64_t* prim_off, uint32_t nseg,
                                 const uint32_t* parent, const uint32_t* other_end,
                                 void* treelets, uint32_t* treelet_count, hipStream_t s);
 
-// Closest-hit traversal over the 64-byte traversal records (wide).  For a
+// Closest-hit traversal over the node records (lbvh::node<float>).  For a
 // batch (prim_off != nullptr) ray r belongs to the scene s with ray_off[s] <=
 // r < ray_off[s + 1] and hits report scene-local primitive ids.  order (one
 // tree only, may be null): a permutation of the rays; the launch takes them
 // in that order and writes each hit to the ray's own slot.  waves_per_simd
 // (0 or 8: as many as the registers allow) caps the waves sharing a SIMD.
-hipError_t launch_trace(int mode, const void* wide, const float* tris, uint64_t n,
+hipError_t launch_trace(int mode, const void* nodes, const float* tris, uint64_t n,
                         const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg,
                         const float* rays, uint64_t nrays, uint32_t* prim, float* t, float* uv,
                         hipStream_t s, const uint32_t* order = nullptr, int waves_per_simd = 0);
@@ -157,7 +156,7 @@ hipError_t launch_ray_keys(const float* rays, uint64_t nrays, const float* bound
 // count) and read from / written to the job's own buffers.
 struct TraceJob
 {
-  const void* nodes; // 64-byte traversal records
+  const void* nodes; // node records
   const float* tris;
   const float* rays;
   uint32_t* prim;

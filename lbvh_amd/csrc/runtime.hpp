@@ -82,8 +82,8 @@ struct lbvh_ctx
   lbvh_rt::DevBuf treelet_words;
 };
 
-// A built tree: node array, parent links, fit flags, centroid bounds,
-// traversal records, and the primitives (owned copy or borrowed pointer).
+// A built tree: node array, parent links, fit flags, centroid bounds, and
+// the primitives (owned copy or borrowed pointer).
 struct lbvh_bvh
 {
   int device = 0;
@@ -101,7 +101,6 @@ struct lbvh_bvh
   uint32_t* d_ids = nullptr;
   uint32_t* d_leafprim = nullptr; // primitive of each leaf position (primitive-id leaves)
   uint32_t* d_other_end = nullptr; // other end of each internal node's leaf range
-  void* d_wide = nullptr;          // traversal records (64 B per internal node)
   void* block = nullptr;           // the one allocation holding every buffer above
   uint64_t bytes = 0;
   lbvh_eng::SortConfig sort;       // the sort configuration the build used

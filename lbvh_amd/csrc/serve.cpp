@@ -196,9 +196,9 @@ us_since(const lbvh_server* s)
 uint64_t
 tree_bytes_estimate(uint64_t n, uint32_t leaf_order)
 {
-  // nodes 32, traversal records 64, parent links 8, flags 4, other end 4,
-  // centroid bounds, leaf primitives 4 (primitive-id leaves)
-  return n * (32 + 64 + 8 + 4 + 4) + (leaf_order == LBVH_LEAF_PRIMITIVE_ID ? 4 * n : 0) + 4096;
+  // nodes 32, parent links 8, flags 4, other end 4, centroid bounds, leaf
+  // primitives 4 (primitive-id leaves)
+  return n * (32 + 8 + 4 + 4) + (leaf_order == LBVH_LEAF_PRIMITIVE_ID ? 4 * n : 0) + 4096;
 }
 
 uint64_t
@@ -505,7 +505,7 @@ launch_traversal(lbvh_server* s, std::vector<Req*>& ready)
     Req* q = ready[k];
     Scene* sc = q->sc;
     const uint64_t nr = q->r.n_rays;
-    jobs[k].nodes = sc->bvh->d_wide;
+    jobs[k].nodes = sc->bvh->d_nodes;
     jobs[k].tris = sc->d_tri;
     if (on_device(q)) {
       jobs[k].rays = q->r.rays32;

@@ -8,6 +8,13 @@
 // tmax >= max(0, tmin), lbvh.h:1023-1026) and are pushed far first, near
 // last; the stack holds 128 entries and drops pushes when full
 // (traversal_stack<float,128>, lbvh.h:1746-1789).
+//
+// It walks the reference's node array itself.  A node is visited with its
+// two children already known (they came with its own record, loaded when its
+// box was tested), so one round of independent loads serves a visit: the
+// 32-byte record of an internal child (its box and its children) and the
+// 48-byte triangle of a leaf child are fetched together.  A pushed node
+// carries its children on the stack.
 #include "device_common.hpp"
 #include "engine.hpp"
 
@@ -19,9 +26,9 @@ constexpr int kTraceBlock = 256;
 // Waves per SIMD the registers allow (the occupancy knob's "no limit").
 constexpr int kTraceMaxWavesPerSimd = 8;
 constexpr int kStack = 128;
-// Stack entries below this depth live in LDS (16 KB per workgroup keeps
-// eight workgroups per CU); deeper ones in the private array.
-constexpr int kStackLds = 8;
+// Stack entries below this depth live in LDS (12 bytes each: 18 KB per
+// workgroup keeps eight workgroups per CU); deeper ones in the private array.
+constexpr int kStackLds = 6;
 
 struct Ray
 {
@@ -208,7 +215,7 @@ job_of(const TraceJob* __restrict__ jobs, uint32_t njobs, uint64_t i)
 // puts rays which visit the same nodes into the same wave.
 template<int MODE, int LAYOUT>
 __global__ __launch_bounds__(kTraceBlock) void
-k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
+k_trace(const Node* __restrict__ nodes, const float* __restrict__ tris, uint32_t nprims,
         const uint64_t* __restrict__ prim_off, const uint64_t* __restrict__ ray_off,
         uint32_t nseg, const TraceJob* __restrict__ jobs, const float* __restrict__ rays,
         const uint32_t* __restrict__ order, uint64_t nrays, uint32_t* __restrict__ prim_out,
@@ -270,7 +277,7 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
     tris += 12 * P;
   } else if constexpr (LAYOUT == kLayoutJobs) {
     const TraceJob& jb = jobs[sel];
-    nodes = static_cast<const WideNode*>(jb.nodes);
+    nodes = static_cast<const Node*>(jb.nodes);
     tris = jb.tris;
     nprims = jb.nprims;
     rays = jb.rays;
@@ -306,40 +313,50 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
   }
 
   // The traversal stack: capacity 128 with pushes dropped when full, as the
-  // reference's.  Entries: (node index, tmin bits).  Entries 0 .. sp - 2 are
-  // in LDS (the first kStackLds, one column per thread) and a private array
-  // (the rest, the only thing in private memory); the top one is kept in
-  // registers: a pop hands it over at once and leaves the register stale
-  // (stale: the top is still entry sp - 1).  The reload is issued after the
-  // popped node's fetch, so the two latencies overlap; a push onto a stale
-  // top needs no reload at all.
-  __shared__ uint2 s_stk[kStackLds * kTraceBlock];
-  uint2 spill[kStack - kStackLds];
-  auto stk_store = [&](uint32_t e, uint2 v) {
-    if (e < uint32_t(kStackLds))
-      s_stk[e * kTraceBlock + threadIdx.x] = v;
-    else
+  // reference's.  An entry stands for a pushed node and holds what its own
+  // node record gave when its box was tested: its two children and the box's
+  // entry distance (the reference's tmin), so popping it needs no further
+  // load.  Entries 0 .. sp - 2 are in LDS (the first kStackLds, one column
+  // per thread and field) and a private array (the rest, the only thing in
+  // private memory); the top one is kept in registers: a pop hands it over at
+  // once and leaves the register stale (stale: the top is still entry
+  // sp - 1).  The reload is issued after the children's fetches, so the
+  // latencies overlap; a push onto a stale top needs no reload at all.
+  __shared__ uint32_t s_stk[3 * kStackLds * kTraceBlock];
+  uint3 spill[kStack - kStackLds];
+  auto stk_store = [&](uint32_t e, uint3 v) {
+    if (e < uint32_t(kStackLds)) {
+      uint32_t* q = &s_stk[e * kTraceBlock + threadIdx.x];
+      q[0] = v.x;
+      q[kStackLds * kTraceBlock] = v.y;
+      q[2 * kStackLds * kTraceBlock] = v.z;
+    } else {
       spill[e - kStackLds] = v;
+    }
   };
-  auto stk_load = [&](uint32_t e) -> uint2 {
-    return e < uint32_t(kStackLds) ? s_stk[e * kTraceBlock + threadIdx.x] : spill[e - kStackLds];
+  auto stk_load = [&](uint32_t e) -> uint3 {
+    if (e < uint32_t(kStackLds)) {
+      const uint32_t* q = &s_stk[e * kTraceBlock + threadIdx.x];
+      return make_uint3(q[0], q[kStackLds * kTraceBlock], q[2 * kStackLds * kTraceBlock]);
+    }
+    return spill[e - kStackLds];
   };
-  uint2 top = make_uint2(0u, 0u);
+  uint3 top = make_uint3(0u, 0u, 0u);
   uint32_t sp = 0;
   bool stale = false;
-  auto push = [&](uint32_t node, float tn) {
+  auto push = [&](uint32_t l, uint32_t rr, float tn) {
     if (sp >= kStack)
       return;
     if (sp > 0 && !stale)
       stk_store(sp - 1, top);
     stale = false;
-    top = make_uint2(node, __float_as_uint(tn));
+    top = make_uint3(l, rr, __float_as_uint(tn));
     sp++;
   };
   auto pop = [&]() {
     if (stale)
       top = stk_load(sp - 1);
-    const uint2 e = top;
+    const uint3 e = top;
     sp--;
     stale = sp > 0;
     return e;
@@ -348,47 +365,85 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
   uint32_t best = 0xffffffffu;
   float best_t = INFINITY, best_u = 0.0f, best_v = 0.0f;
 
-  // The reference pushes the root and pops it at once; a node it would push
-  // and pop again straight away (the near child, or a single hit child) is
-  // taken directly here, but only when that push would have found room, so
-  // the visit order and the drop-when-full rule are unchanged.
-  uint32_t ni = 0u;
-  bool have = true;
+  // The reference pushes the root and pops it at once: its record gives the
+  // first two children.  A node it would push and pop again straight away
+  // (the near child, or a single hit child) is taken directly here, but only
+  // when that push would have found room, so the visit order and the
+  // drop-when-full rule are unchanged.
+  const float* nodes_f = reinterpret_cast<const float*>(nodes);
+  uint32_t cl = 0u, cr = 0u;
+  bool have = false;
+  if (nprims >= 2u) {
+    const float4 q1 = reinterpret_cast<const float4*>(nodes_f)[1];
+    cl = __float_as_uint(q1.z);
+    cr = __float_as_uint(q1.w);
+    have = true;
+  }
   for (;;) {
-    // One path back to the loop head for both a pruned entry and an index out
-    // of range (only possible for a corrupt node array): with two, the
-    // compiler's structurizer nests loops and duplicates the leaf tests,
-    // measured 12% slower on the clustered 1M scene.
+    // A pruned entry goes back to the loop head by the same path as a node
+    // whose children both miss: with a separate path, the compiler's
+    // structurizer nests loops and duplicates the child tests (measured 12%
+    // slower on the clustered 1M scene).
     bool visit = true;
     if (!have) {
       if (sp == 0)
         break;
-      const uint2 e = pop();
-      visit = !(best_t < __uint_as_float(e.y));
-      ni = e.x;
+      const uint3 e = pop();
+      visit = !(best_t < __uint_as_float(e.z));
+      cl = e.x;
+      cr = e.y;
     }
     have = false;
-    if (!visit || ni + 1 >= nprims)
+    if (!visit)
       continue;
-    const float4* wp = reinterpret_cast<const float4*>(&nodes[ni]);
-    const float4 w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
-    if (stale) { // reload the top while the node is in flight
+    // Both children's data at once: a leaf's triangle, an internal node's
+    // record (its box and children).  A child out of range (only possible
+    // for a corrupt node array) is skipped.
+    const uint32_t ch[2] = { cl, cr };
+    bool ok[2];
+    float4 q[2][3];
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const uint32_t c = ch[side];
+      const bool leaf = (c & kLeafBit) != 0u;
+      ok[side] = leaf ? ((c & ~kLeafBit) < nprims) : (c + 1 < nprims);
+      q[side][0] = q[side][1] = q[side][2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (ok[side]) {
+        const float4* a = reinterpret_cast<const float4*>(
+          leaf ? tris + 12 * uint64_t(c & ~kLeafBit) : nodes_f + 8 * uint64_t(c));
+        q[side][0] = a[0];
+        q[side][1] = a[1];
+        if (leaf)
+          q[side][2] = a[2];
+      }
+    }
+    if (stale) { // reload the top while the children are in flight
       top = stk_load(sp - 1);
       stale = false;
     }
-    const uint32_t lr[2] = { __float_as_uint(w3.x), __float_as_uint(w3.y) };
-    const float bx[2][6] = { { w0.x, w0.y, w0.z, w0.w, w1.x, w1.y },
-                             { w1.z, w1.w, w2.x, w2.y, w2.z, w2.w } };
     bool hit2[2] = { false, false };
     float tn2[2] = { 0.0f, 0.0f };
+    uint32_t gl[2] = { 0u, 0u }, gr[2] = { 0u, 0u };
 #pragma unroll
     for (int side = 0; side < 2; side++) {
-      const uint32_t ch = lr[side];
-      if ((ch & kLeafBit) ? ((ch & ~kLeafBit) >= nprims) : (ch + 1 >= nprims))
-        continue; // out-of-range child: only possible for a corrupt node array
-      if (ch & kLeafBit) {
-        const uint32_t p = ch & ~kLeafBit;
-        const Tri tr = load_tri(tris, p);
+      if (!ok[side])
+        continue;
+      const float4 a = q[side][0], b = q[side][1], c = q[side][2];
+      if (ch[side] & kLeafBit) {
+        const uint32_t p = ch[side] & ~kLeafBit;
+        Tri tr;
+        tr.p0[0] = a.x;
+        tr.p0[1] = a.y;
+        tr.p0[2] = a.z;
+        tr.e0[0] = a.w;
+        tr.e0[1] = b.x;
+        tr.e0[2] = b.y;
+        tr.e1[0] = b.z;
+        tr.e1[1] = b.w;
+        tr.e1[2] = c.x;
+        tr.nrm[0] = c.y;
+        tr.nrm[1] = c.z;
+        tr.nrm[2] = c.w;
         float t, u, v;
         bool hit = (MODE == 2) ? isect_watertight(tr, r, t, u, v) : isect_reference(tr, r, t, u, v);
         if (hit && better<MODE>(t, p, best_t, best)) {
@@ -398,8 +453,10 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
           best_v = v;
         }
       } else {
-        const float* lo = &bx[side][0];
-        const float* hi = &bx[side][3];
+        const float lo[3] = { a.x, a.y, a.z };
+        const float hi[3] = { a.w, b.x, b.y };
+        gl[side] = __float_as_uint(b.z);
+        gr[side] = __float_as_uint(b.w);
         float tn[3], tf[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -417,23 +474,27 @@ k_trace(const WideNode* __restrict__ nodes, const float* __restrict__ tris, uint
       }
     }
     const bool lh = hit2[0], rh = hit2[1];
-    uint32_t next_n;
+    uint32_t next_l, next_r;
     float next_t;
     if (lh && rh) {
       const bool left_near = tn2[0] < tn2[1];
-      push(left_near ? lr[1] : lr[0], left_near ? tn2[1] : tn2[0]);
-      next_n = left_near ? lr[0] : lr[1];
-      next_t = left_near ? tn2[0] : tn2[1];
+      const int far = left_near ? 1 : 0;
+      push(gl[far], gr[far], tn2[far]);
+      next_l = gl[1 - far];
+      next_r = gr[1 - far];
+      next_t = tn2[1 - far];
     } else if (lh || rh) {
-      next_n = lh ? lr[0] : lr[1];
+      next_l = lh ? gl[0] : gl[1];
+      next_r = lh ? gr[0] : gr[1];
       next_t = lh ? tn2[0] : tn2[1];
     } else {
       continue;
     }
-    // push-then-pop of next_n: its push needs room; its pop is then pruned
-    // when a closer hit is already known
+    // push-then-pop of the near child: its push needs room; its pop is then
+    // pruned when a closer hit is already known
     if (sp < kStack && !(best_t < next_t)) {
-      ni = next_n;
+      cl = next_l;
+      cr = next_r;
       have = true;
     }
   }
@@ -520,7 +581,7 @@ using namespace lbvh_dev;
 
 template<int MODE>
 static void
-trace_mode(uint32_t g, size_t lds, hipStream_t s, const WideNode* nd, const float* tris,
+trace_mode(uint32_t g, size_t lds, hipStream_t s, const Node* nd, const float* tris,
            uint64_t n, const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg,
            const TraceJob* jobs, const float* rays, const uint32_t* order, uint64_t nrays,
            uint32_t* prim, float* t, float* uv)
@@ -548,7 +609,7 @@ occupancy_lds(int waves)
   if (waves <= 0 || waves >= kTraceMaxWavesPerSimd)
     return 0;
   constexpr size_t kLdsPerCu = 160 * 1024;
-  constexpr size_t kOwn = sizeof(uint2) * kStackLds * kTraceBlock + 4096; // stack + small
+  constexpr size_t kOwn = 3 * sizeof(uint32_t) * kStackLds * kTraceBlock + 4096; // stack + small
   const size_t per_wg = (kLdsPerCu / size_t(waves)) & ~size_t(511);
   return per_wg > kOwn ? per_wg - kOwn : 0;
 }
@@ -562,7 +623,7 @@ trace_dispatch(int mode, const void* nodes, const float* tris, uint64_t n, const
   if (nrays == 0)
     return hipSuccess;
   const uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
-  const WideNode* nd = static_cast<const WideNode*>(nodes);
+  const Node* nd = static_cast<const Node*>(nodes);
   const size_t lds = occupancy_lds(waves);
 #define LBVH_TM(M)                                                                                \
   trace_mode<M>(g, lds, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, order, nrays, prim,   \

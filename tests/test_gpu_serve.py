@@ -118,7 +118,7 @@ def test_server_memory_cap_evicts_idle_scenes(ctx):
     cap = 40 << 20
     srv = lbvh_amd.Server(0, memory_cap_bytes=cap, lanes=2)
     ids = []
-    for k in range(6):
+    for k in range(10):  # about 5 MB each (triangles + tree): more than the cap holds
         t, sid = srv.submit(_rays(k, 500), 0, _scene(300 + k, 50000))
         srv.wait(t)
         ids.append(sid)
@@ -130,7 +130,7 @@ def test_server_memory_cap_evicts_idle_scenes(ctx):
         srv.submit(_rays(9, 10), ids[0])
     assert e.value.status == 1
     # the last scene is resident and still answers correctly
-    ref = ctx.build_triangles(_scene(305, 50000), code_bits=63, leaf_order=1)
+    ref = ctx.build_triangles(_scene(309, 50000), code_bits=63, leaf_order=1)
     r = _rays(10, 3000)
     p, tt, _, _ = srv.wait(srv.submit(r, ids[-1])[0])
     ep, et, _ = ctx.trace(ref, r, 0)

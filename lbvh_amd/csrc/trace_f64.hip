@@ -30,7 +30,7 @@ static_assert(sizeof(Node64) == 64, "node<double> is 64 bytes");
 constexpr uint64_t kLeafBit64 = uint64_t(1) << 63;
 constexpr int kTraceF64Block = 256;
 constexpr int kStackF64 = 128;
-// Stack entries kept in LDS per thread (12 KB per 256-thread workgroup).
+// Stack entries kept in LDS per thread (16 KB per 256-thread workgroup).
 constexpr int kStackF64Lds = 4;
 
 __device__ __forceinline__ double
@@ -71,12 +71,11 @@ dcross3(const double* a, const double* b, double* o)
 }
 
 // triangle_intersector<double> on a triangle<double> record {p0, e0, e1,
-// normal} (12 doubles, 16-byte aligned).
+// normal} (12 doubles) already loaded as six double2.
 __device__ __forceinline__ bool
-isect_f64(const double* __restrict__ tri, const double o[3], const double d[3], double tmin,
-          double tmax, double& t_out, double& u_out, double& v_out)
+isect_f64(const double2 (&q)[6], const double o[3], const double d[3], double tmin, double tmax,
+          double& t_out, double& u_out, double& v_out)
 {
-  const double2* q = reinterpret_cast<const double2*>(tri);
   const double2 a = q[0], b = q[1], c = q[2], e = q[3], f = q[4], g = q[5];
   const double p0[3] = { a.x, a.y, b.x };
   const double e0[3] = { b.y, c.x, c.y };
@@ -127,86 +126,127 @@ k_trace_f64(const Node64* __restrict__ nodes, const double* __restrict__ tris, u
     ipos[k] = np * rcp[k];
     pos_dir[k] = d[k] > 0.0;
   }
-  // The stack: capacity 128 (pushes onto a full stack are dropped), entries
-  // (node index, tmin).  The first kStackF64Lds entries live in LDS, one
-  // column per thread, the rest in a private array; the top entry is kept in
-  // registers and reloaded only after a pop empties them (its reload is
-  // issued after the popped node's fetch, so the two latencies overlap).
-  __shared__ uint32_t s_node[kStackF64Lds * kTraceF64Block];
+  // The stack: capacity 128 (pushes onto a full stack are dropped).  An
+  // entry stands for a pushed node and holds what its own record gave when
+  // its box was tested: its two children (packed to 32 bits, bit 31 marking a
+  // leaf, as every index is below 2^31) and the box's entry distance (the
+  // reference's tmin), so popping it needs no further load.  The first
+  // kStackF64Lds entries live in LDS, one column per thread, the rest in a
+  // private array; the top entry is kept in registers and reloaded only after
+  // a pop empties them (its reload is issued after the children's fetches, so
+  // the latencies overlap).
+  __shared__ uint2 s_lr[kStackF64Lds * kTraceF64Block];
   __shared__ double s_tmin[kStackF64Lds * kTraceF64Block];
-  uint32_t spill_n[kStackF64 - kStackF64Lds];
+  uint2 spill_lr[kStackF64 - kStackF64Lds];
   double spill_t[kStackF64 - kStackF64Lds];
-  auto stk_store = [&](int e, uint32_t node, double tm) {
+  auto stk_store = [&](int e, uint2 lr, double tm) {
     if (e < kStackF64Lds) {
-      s_node[e * kTraceF64Block + threadIdx.x] = node;
+      s_lr[e * kTraceF64Block + threadIdx.x] = lr;
       s_tmin[e * kTraceF64Block + threadIdx.x] = tm;
     } else {
-      spill_n[e - kStackF64Lds] = node;
+      spill_lr[e - kStackF64Lds] = lr;
       spill_t[e - kStackF64Lds] = tm;
     }
   };
-  auto stk_load = [&](int e, uint32_t& node, double& tm) {
+  auto stk_load = [&](int e, uint2& lr, double& tm) {
     if (e < kStackF64Lds) {
-      node = s_node[e * kTraceF64Block + threadIdx.x];
+      lr = s_lr[e * kTraceF64Block + threadIdx.x];
       tm = s_tmin[e * kTraceF64Block + threadIdx.x];
     } else {
-      node = spill_n[e - kStackF64Lds];
+      lr = spill_lr[e - kStackF64Lds];
       tm = spill_t[e - kStackF64Lds];
     }
   };
-  uint32_t top_n = 0;
+  auto pack = [](uint64_t v) -> uint32_t {
+    return (v & kLeafBit64) ? (uint32_t(v) | 0x80000000u) : uint32_t(v);
+  };
+  uint2 top_lr = make_uint2(0u, 0u);
   double top_t = 0.0;
   int sp = 0;
   bool stale = false; // the registers hold an entry already popped: reload before use
-  auto push = [&](uint32_t node, double tm) {
+  auto push = [&](uint2 lr, double tm) {
     if (sp >= kStackF64)
       return;
     if (sp > 0 && !stale)
-      stk_store(sp - 1, top_n, top_t);
+      stk_store(sp - 1, top_lr, top_t);
     stale = false;
-    top_n = node;
+    top_lr = lr;
     top_t = tm;
     sp++;
   };
   uint64_t best = ~uint64_t(0);
   double best_t = INFINITY, best_u = 0.0, best_v = 0.0;
-  // The root is pushed and popped at once; a child that would be pushed and
-  // popped again straight away (the near one of two, or a single hit child)
-  // is taken directly, when its push would have found room, so the visit
-  // order and the drop-when-full rule are the reference's.
-  uint32_t ni = 0u;
-  bool have = true;
+  // The root is pushed and popped at once: its record gives the first two
+  // children.  A child that would be pushed and popped again straight away
+  // (the near one of two, or a single hit child) is taken directly, when its
+  // push would have found room, so the visit order and the drop-when-full
+  // rule are the reference's.
+  uint2 cur = make_uint2(0u, 0u);
+  bool have = false;
+  if (n >= 2) {
+    const ulonglong2 lr0 = *reinterpret_cast<const ulonglong2*>(&nodes[0].left);
+    cur = make_uint2(pack(lr0.x), pack(lr0.y));
+    have = true;
+  }
   for (;;) {
+    // a pruned entry returns to the loop head by the same path as a node
+    // whose children both miss (a second path nests loops in the compiler's
+    // structurizer)
     bool visit = true;
     if (!have) {
       if (sp == 0)
         break;
       if (stale)
-        stk_load(sp - 1, top_n, top_t);
-      ni = top_n;
+        stk_load(sp - 1, top_lr, top_t);
+      cur = top_lr;
       visit = !(best_t < top_t);
       sp--;
       stale = sp > 0;
     }
     have = false;
-    if (!visit || uint64_t(ni) + 1 >= n)
-      continue; // pruned, or (only for a corrupt node array) out of range
-    const uint64_t lr[2] = { nodes[ni].left, nodes[ni].right };
-    if (stale) { // reload the top while the node is in flight
-      stk_load(sp - 1, top_n, top_t);
+    if (!visit)
+      continue;
+    // both children's data at once: a leaf's triangle<double> record, an
+    // internal node's record; a child out of range (only possible for a
+    // corrupt node array) is skipped
+    const uint32_t ch[2] = { cur.x, cur.y };
+    bool ok[2];
+    double2 q[2][6];
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const bool leaf = (ch[side] & 0x80000000u) != 0u;
+      const uint64_t idx = ch[side] & 0x7fffffffu;
+      ok[side] = leaf ? idx < n : idx + 1 < n;
+#pragma unroll
+      for (int k = 0; k < 6; k++)
+        q[side][k] = make_double2(0.0, 0.0);
+      if (ok[side]) {
+        const double2* a = reinterpret_cast<const double2*>(leaf ? tris + 12 * idx
+                                                                 : &nodes[idx].bmin[0]);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          q[side][k] = a[k];
+        if (leaf) {
+          q[side][4] = a[4];
+          q[side][5] = a[5];
+        }
+      }
+    }
+    if (stale) { // reload the top while the children are in flight
+      stk_load(sp - 1, top_lr, top_t);
       stale = false;
     }
     bool hit2[2] = { false, false };
     double tn2[2] = { 0.0, 0.0 };
+    uint2 g2[2] = { make_uint2(0u, 0u), make_uint2(0u, 0u) };
 #pragma unroll
     for (int side = 0; side < 2; side++) {
-      const uint64_t ch = lr[side];
-      const uint64_t idx = ch & ~kLeafBit64;
-      if ((ch & kLeafBit64) ? idx >= n : idx + 1 >= n)
-        continue; // out-of-range child: only possible for a corrupt node array
-      if (ch & kLeafBit64) {
+      if (!ok[side])
+        continue;
+      if (ch[side] & 0x80000000u) {
+        const uint64_t idx = ch[side] & 0x7fffffffu;
         double t, u, v;
-        if (isect_f64(tris + 12 * idx, o, d, rtmin, rtmax, t, u, v)) {
+        if (isect_f64(q[side], o, d, rtmin, rtmax, t, u, v)) {
           const bool take = (MODE == 0) ? (t < best_t)
                                         : ((t < best_t) || (t == best_t && idx < best));
           if (take) {
@@ -217,10 +257,11 @@ k_trace_f64(const Node64* __restrict__ nodes, const double* __restrict__ tris, u
           }
         }
       } else {
-        const double2* bp = reinterpret_cast<const double2*>(&nodes[idx]);
-        const double2 b0 = bp[0], b1 = bp[1], b2 = bp[2];
+        const double2 b0 = q[side][0], b1 = q[side][1], b2 = q[side][2], b3 = q[side][3];
         const double lo[3] = { b0.x, b0.y, b1.x };
         const double hi[3] = { b1.y, b2.x, b2.y };
+        g2[side] = make_uint2(pack(uint64_t(__double_as_longlong(b3.x))),
+                              pack(uint64_t(__double_as_longlong(b3.y))));
         double tn[3], tf[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -237,23 +278,23 @@ k_trace_f64(const Node64* __restrict__ nodes, const double* __restrict__ tris, u
         tn2[side] = tmin;
       }
     }
-    uint32_t next_n;
+    uint2 next;
     double next_t;
     if (hit2[0] && hit2[1]) {
       const bool left_near = tn2[0] < tn2[1];
-      push(uint32_t(left_near ? lr[1] : lr[0]), left_near ? tn2[1] : tn2[0]);
-      next_n = uint32_t(left_near ? lr[0] : lr[1]);
+      push(left_near ? g2[1] : g2[0], left_near ? tn2[1] : tn2[0]);
+      next = left_near ? g2[0] : g2[1];
       next_t = left_near ? tn2[0] : tn2[1];
     } else if (hit2[0] || hit2[1]) {
-      next_n = uint32_t(hit2[0] ? lr[0] : lr[1]);
+      next = hit2[0] ? g2[0] : g2[1];
       next_t = hit2[0] ? tn2[0] : tn2[1];
     } else {
       continue;
     }
-    // push-then-pop of next_n: its push needs room; its pop is then pruned
-    // when a closer hit is already known
+    // push-then-pop of the near child: its push needs room; its pop is then
+    // pruned when a closer hit is already known
     if (sp < kStackF64 && !(best_t < next_t)) {
-      ni = next_n;
+      cur = next;
       have = true;
     }
   }

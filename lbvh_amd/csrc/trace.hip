@@ -22,12 +22,15 @@ namespace lbvh_dev {
 
 using lbvh_eng::TraceJob;
 
-constexpr int kTraceBlock = 256;
-// Waves per SIMD the registers allow (the occupancy knob's "no limit").
+// Two waves per workgroup: measured faster than four (the clustered 1M
+// primary launch 0.68 -> 0.65 ms) and the same as one.
+constexpr int kTraceBlock = 128;
+// The occupancy knob's "no limit" (the registers then allow what they allow).
 constexpr int kTraceMaxWavesPerSimd = 8;
 constexpr int kStack = 128;
-// Stack entries below this depth live in LDS (12 bytes each: 18 KB per
-// workgroup keeps eight workgroups per CU); deeper ones in the private array.
+// Stack entries below this depth live in LDS (12 bytes each: 9 KB per
+// workgroup, so the registers, not LDS, limit the waves per CU); deeper ones
+// in the private array.
 constexpr int kStackLds = 6;
 
 struct Ray
@@ -67,27 +70,6 @@ struct Tri
 {
   float p0[3], e0[3], e1[3], nrm[3];
 };
-
-__device__ __forceinline__ Tri
-load_tri(const float* __restrict__ tris, uint32_t i)
-{
-  const float4* p = reinterpret_cast<const float4*>(tris + 12 * uint64_t(i));
-  float4 a = p[0], b = p[1], c = p[2];
-  Tri t;
-  t.p0[0] = a.x;
-  t.p0[1] = a.y;
-  t.p0[2] = a.z;
-  t.e0[0] = a.w;
-  t.e0[1] = b.x;
-  t.e0[2] = b.y;
-  t.e1[0] = b.z;
-  t.e1[1] = b.w;
-  t.e1[2] = c.x;
-  t.nrm[0] = c.y;
-  t.nrm[1] = c.z;
-  t.nrm[2] = c.w;
-  return t;
-}
 
 // triangle_intersector (lbvh.h:2055-2075): two-sided, not watertight.
 __device__ __forceinline__ bool
@@ -579,39 +561,55 @@ namespace lbvh_eng {
 
 using namespace lbvh_dev;
 
-template<int MODE>
-static void
-trace_mode(uint32_t g, size_t lds, hipStream_t s, const Node* nd, const float* tris,
-           uint64_t n, const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg,
-           const TraceJob* jobs, const float* rays, const uint32_t* order, uint64_t nrays,
-           uint32_t* prim, float* t, float* uv)
-{
-  if (jobs)
-    hipLaunchKernelGGL((k_trace<MODE, kLayoutJobs>), dim3(g), dim3(kTraceBlock), lds, s,
-                       nullptr, nullptr, 0u, nullptr, nullptr, nseg, jobs, nullptr, nullptr,
-                       nrays, nullptr, nullptr, nullptr);
-  else if (prim_off)
-    hipLaunchKernelGGL((k_trace<MODE, kLayoutBatch>), dim3(g), dim3(kTraceBlock), lds, s, nd,
-                       tris, uint32_t(n), prim_off, ray_off, nseg, nullptr, rays, nullptr, nrays,
-                       prim, t, uv);
-  else
-    hipLaunchKernelGGL((k_trace<MODE, kLayoutSingle>), dim3(g), dim3(kTraceBlock), lds, s, nd,
-                       tris, uint32_t(n), nullptr, nullptr, 1u, nullptr, rays, order, nrays, prim,
-                       t, uv);
-}
-
-// LDS a workgroup reserves beyond its own so that at most `waves` waves
-// share a SIMD (a workgroup puts one wave on each of the CU's 4 SIMDs, so
-// waves per SIMD = workgroups per CU); 0 = no limit beyond the registers'.
+// LDS a workgroup reserves beyond its own static `own` bytes so that at most
+// `waves` waves share a SIMD: a workgroup puts its kTraceBlock / 64 waves on
+// as many of the CU's 4 SIMDs, so that is waves * 4 / (kTraceBlock / 64)
+// workgroups per CU, and one more must not fit.  0 = no limit beyond the
+// registers'.
 static size_t
-occupancy_lds(int waves)
+occupancy_lds(int waves, size_t own)
 {
   if (waves <= 0 || waves >= kTraceMaxWavesPerSimd)
     return 0;
   constexpr size_t kLdsPerCu = 160 * 1024;
-  constexpr size_t kOwn = 3 * sizeof(uint32_t) * kStackLds * kTraceBlock + 4096; // stack + small
-  const size_t per_wg = (kLdsPerCu / size_t(waves)) & ~size_t(511);
-  return per_wg > kOwn ? per_wg - kOwn : 0;
+  const size_t wgs = size_t(waves) * 4 / size_t(kTraceBlock / kWave);
+  const size_t per_wg = (kLdsPerCu / wgs) & ~size_t(511);
+  return per_wg > own ? per_wg - own : 0;
+}
+
+// Static LDS of a kernel, from its attributes (0 if they cannot be read).
+template<typename K>
+static size_t
+static_lds(K kernel)
+{
+  hipFuncAttributes a{};
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kernel)) == hipSuccess
+           ? a.sharedSizeBytes
+           : 0;
+}
+
+template<int MODE>
+static void
+trace_mode(uint32_t g, int waves, hipStream_t s, const Node* nd, const float* tris,
+           uint64_t n, const uint64_t* prim_off, const uint64_t* ray_off, uint32_t nseg,
+           const TraceJob* jobs, const float* rays, const uint32_t* order, uint64_t nrays,
+           uint32_t* prim, float* t, float* uv)
+{
+  static const size_t own_jobs = static_lds(k_trace<MODE, kLayoutJobs>);
+  static const size_t own_batch = static_lds(k_trace<MODE, kLayoutBatch>);
+  static const size_t own_single = static_lds(k_trace<MODE, kLayoutSingle>);
+  if (jobs)
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutJobs>), dim3(g), dim3(kTraceBlock),
+                       occupancy_lds(waves, own_jobs), s, nullptr, nullptr, 0u, nullptr, nullptr,
+                       nseg, jobs, nullptr, nullptr, nrays, nullptr, nullptr, nullptr);
+  else if (prim_off)
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutBatch>), dim3(g), dim3(kTraceBlock),
+                       occupancy_lds(waves, own_batch), s, nd, tris, uint32_t(n), prim_off,
+                       ray_off, nseg, nullptr, rays, nullptr, nrays, prim, t, uv);
+  else
+    hipLaunchKernelGGL((k_trace<MODE, kLayoutSingle>), dim3(g), dim3(kTraceBlock),
+                       occupancy_lds(waves, own_single), s, nd, tris, uint32_t(n), nullptr,
+                       nullptr, 1u, nullptr, rays, order, nrays, prim, t, uv);
 }
 
 static hipError_t
@@ -624,10 +622,9 @@ trace_dispatch(int mode, const void* nodes, const float* tris, uint64_t n, const
     return hipSuccess;
   const uint32_t g = uint32_t((nrays + kTraceBlock - 1) / kTraceBlock);
   const Node* nd = static_cast<const Node*>(nodes);
-  const size_t lds = occupancy_lds(waves);
 #define LBVH_TM(M)                                                                                \
-  trace_mode<M>(g, lds, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, order, nrays, prim,   \
-                t, uv)
+  trace_mode<M>(g, waves, s, nd, tris, n, prim_off, ray_off, nseg, jobs, rays, order, nrays,       \
+                prim, t, uv)
   switch (mode) {
     case 0:
       LBVH_TM(0);

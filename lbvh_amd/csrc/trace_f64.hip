@@ -28,9 +28,11 @@ struct alignas(64) Node64
 static_assert(sizeof(Node64) == 64, "node<double> is 64 bytes");
 
 constexpr uint64_t kLeafBit64 = uint64_t(1) << 63;
-constexpr int kTraceF64Block = 256;
+// Two waves per workgroup: the 1M primary launch 1.10 ms with four, 1.00 ms
+// with two (one timed the same).
+constexpr int kTraceF64Block = 128;
 constexpr int kStackF64 = 128;
-// Stack entries kept in LDS per thread (16 KB per 256-thread workgroup).
+// Stack entries kept in LDS per thread (8 KB per 128-thread workgroup).
 constexpr int kStackF64Lds = 4;
 
 __device__ __forceinline__ double

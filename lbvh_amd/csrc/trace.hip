@@ -577,15 +577,18 @@ occupancy_lds(int waves, size_t own)
   return per_wg > own ? per_wg - own : 0;
 }
 
-// Static LDS of a kernel, from its attributes (0 if they cannot be read).
+// Static LDS of a kernel, from its attributes (0 if they cannot be read; the
+// failed query's error is cleared, so that it is not reported by the launch).
 template<typename K>
 static size_t
 static_lds(K kernel)
 {
   hipFuncAttributes a{};
-  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kernel)) == hipSuccess
-           ? a.sharedSizeBytes
-           : 0;
+  if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kernel)) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return a.sharedSizeBytes;
 }
 
 template<int MODE>

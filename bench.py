@@ -57,9 +57,9 @@ RAY_COMPULSORY_BYTES = 48  # 32-byte ray in, 16-byte hit out
 # HBM bytes per k_trace dispatch on this workload, averaged over the primary
 # and bounce dispatches: rocprofv3 --pmc FETCH_SIZE (x 2.0 KiB) + WRITE_SIZE
 # (x 1.0 KiB), scaled by a 512 MiB copy kernel
-# (profiles/clustered1m_63bit_hbm_traffic.md: primary 514.9 + 46.4 MB, bounce
-# 907.2 + 16.9 MB; mean 711.1 MB read + 31.7 MB written).
-TRACE_HBM_TRAFFIC_PER_LAUNCH = 742.7e6
+# (profiles/clustered1m_63bit_hbm_traffic.md: primary 719.1 + 47.6 MB, bounce
+# 903.2 + 16.9 MB; mean 811.2 MB read + 32.3 MB written).
+TRACE_HBM_TRAFFIC_PER_LAUNCH = 843.4e6
 # Offered load of the serving comparison in the default line (requests/s):
 # above the plain path's capacity, so both paths run saturated.
 HEADLINE_SERVING_RATE = 4000.0

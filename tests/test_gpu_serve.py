@@ -22,15 +22,14 @@ def _rays(seed, m):
     return r[rng.integers(0, r.shape[0], m)]
 
 
-def _plan():
+def _plan(sizes=(3000, 50000, 200000, 1000, 120000, 7000), steps=40, max_rays=9000):
     """(kind, scene index, triangles or None, rays) in arrival order."""
-    sizes = [3000, 50000, 200000, 1000, 120000, 7000]
     tris = [_scene(500 + k, n) for k, n in enumerate(sizes)]
     rng = np.random.default_rng(7)
     plan = []
     made = []
-    for step in range(40):
-        ray_n = int(rng.integers(1, 9000))
+    for step in range(steps):
+        ray_n = int(rng.integers(1, max_rays))
         rays = _rays(step, ray_n)
         if len(made) < len(sizes) and (step % 4 == 0 or not made):
             k = len(made)
@@ -46,15 +45,15 @@ def _plan():
     return plan
 
 
-def _plain_results(ctx, plan):
+def _plain_results(ctx, plan, code_bits=63, leaf_order=1, mode=lbvh_amd.ISECT_REFERENCE):
     trees = {}
     out = []
     for kind, k, tri, rays in plan:
         if kind == "new":
-            trees[k] = ctx.build_triangles(tri, code_bits=63, leaf_order=1)
+            trees[k] = ctx.build_triangles(tri, code_bits=code_bits, leaf_order=leaf_order)
         elif kind == "refit":
             ctx.refit_triangles(trees[k], tri)
-        out.append(ctx.trace(trees[k], rays, lbvh_amd.ISECT_REFERENCE))
+        out.append(ctx.trace(trees[k], rays, mode))
     for b in trees.values():
         b.destroy()
     return out
@@ -87,6 +86,42 @@ def test_server_results_equal_plain_path_in_order(ctx):
     assert st["completed"] == len(plan) and st["failed"] == 0
     assert st["resident_scenes"] == 6
     assert st["launches"] <= len(plan)
+    srv.close()
+
+
+@pytest.mark.parametrize("code_bits,leaf_order,mode", [(30, 1, 2), (63, 1, 1), (30, 0, 0)],
+                         ids=["b30-pid-watertight", "b63-pid-lowid", "b30-ref-reference"])
+def test_server_options_equal_plain_path(ctx, code_bits, leaf_order, mode):
+    """A server built with other code widths, leaf orders and intersector
+    modes gives, request by request, the plain path's hits with the same
+    options (reference leaves on unsorted triangles make traversal visit most
+    of each tree, so that plan is smaller)."""
+    print("server options bits=%d leaf=%d mode=%d" % (code_bits, leaf_order, mode), flush=True)
+    if leaf_order == 0:
+        plan = _plan(sizes=(3000, 20000, 1000, 9000), steps=16, max_rays=2000)
+    else:
+        plan = _plan(steps=24)
+    expect = _plain_results(ctx, plan, code_bits, leaf_order, mode)
+    srv = lbvh_amd.Server(0, code_bits=code_bits, leaf_order=leaf_order, isect_mode=mode)
+    ids = {}
+    tickets = []
+    for kind, k, tri, rays in plan:
+        if kind == "new":
+            t, sid = srv.submit(rays, 0, tri)
+            ids[k] = sid
+        elif kind == "refit":
+            t, _ = srv.submit(rays, ids[k], tri)
+        else:
+            t, _ = srv.submit(rays, ids[k])
+        tickets.append(t)
+    for i, t in enumerate(tickets):
+        p, tt, uv, _ = srv.wait(t)
+        ep, et, euv = expect[i]
+        assert np.array_equal(p, ep), "request %d (%s) ids" % (i, plan[i][0])
+        assert np.array_equal(tt.view(np.uint32), et.view(np.uint32)), "request %d t" % i
+        assert np.array_equal(uv.view(np.uint32), euv.view(np.uint32)), "request %d uv" % i
+    st = srv.stats()
+    assert st["completed"] == len(plan) and st["failed"] == 0
     srv.close()
 
 

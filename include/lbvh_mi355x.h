@@ -281,10 +281,12 @@ lbvh_status lbvh_trace_closest_device(lbvh_ctx* ctx, const lbvh_bvh* bvh,
  * Outputs per ray: prim u64 (UINT64_MAX = miss, lbvh.h:360-366), t (+inf on
  * a miss), uv (2 doubles, 0 on a miss; may be NULL).  mode 0 = reference
  * (first hit found wins ties), 1 = lowest leaf value on equal distance.  n in
- * [2, 2^31 - 1].  The device variant reads device memory (nodes, triangles,
- * rays and uv 16-byte aligned, prim and t 8-byte) and returns after queuing;
- * the host variant copies in and out and returns when the outputs are
- * written. */
+ * [2, 2^31 - 1].  nodes64 must be the node array of a tree over the n
+ * records; a ray stops loading nodes after n visits (a tree never needs
+ * more), so a corrupt array ends instead of looping.  The device variant
+ * reads device memory (nodes, triangles, rays and uv 16-byte aligned, prim
+ * and t 8-byte) and returns after queuing; the host variant copies in and out
+ * and returns when the outputs are written. */
 lbvh_status lbvh_trace_closest_f64(lbvh_ctx* ctx, const void* nodes64,
                                    uint64_t n, const double* tri96,
                                    const double* rays64, uint64_t nrays,

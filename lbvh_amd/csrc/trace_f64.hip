@@ -185,6 +185,10 @@ k_trace_f64(const Node64* __restrict__ nodes, const double* __restrict__ tris, u
   // rule are the reference's.
   uint2 cur = make_uint2(0u, 0u);
   bool have = false;
+  // A tree's traversal visits each of its n - 1 internal nodes at most once;
+  // a caller's node array that is not a tree (a child cycle) stops loading
+  // after n visits and drains its stack, instead of never ending.
+  uint32_t visits = 0;
   if (n >= 2) {
     const ulonglong2 lr0 = *reinterpret_cast<const ulonglong2*>(&nodes[0].left);
     cur = make_uint2(pack(lr0.x), pack(lr0.y));
@@ -206,7 +210,7 @@ k_trace_f64(const Node64* __restrict__ nodes, const double* __restrict__ tris, u
       stale = sp > 0;
     }
     have = false;
-    if (!visit)
+    if (!visit || ++visits > n)
       continue;
     // both children's data at once: a leaf's triangle<double> record, an
     // internal node's record; a child out of range (only possible for a

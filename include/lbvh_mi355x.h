@@ -334,6 +334,11 @@ typedef struct lbvh_batch_info
   uint64_t device_bytes;  /* device memory held by the handle */
   const void* d_nodes;    /* device lbvh::node<float>[n_nodes] */
   const void* d_prims;    /* device triangles */
+  uint32_t sort_global_passes; /* radix passes over the whole array per rebuild
+                                  (hybrid: 1 when the keys' global digits hold
+                                  one digit of code bits and each scene is
+                                  sorted by it in one scene-segmented pass) */
+  uint32_t reserved;
 } lbvh_batch_info;
 
 /* prim_off: host array of n_scenes + 1 offsets. */

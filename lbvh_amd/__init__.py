@@ -77,7 +77,8 @@ class BatchInfo(ctypes.Structure):
                 ("sort_digit_bits", ctypes.c_uint32), ("sort_tile", ctypes.c_uint32),
                 ("sort_passes", ctypes.c_uint32), ("sort_strategy", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64), ("d_nodes", ctypes.c_void_p),
-                ("d_prims", ctypes.c_void_p)]
+                ("d_prims", ctypes.c_void_p), ("sort_global_passes", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class ServerOpts(ctypes.Structure):

@@ -195,13 +195,15 @@ expand10_seg(uint32_t v)
 // first_pass .. passes - 1, one row per workgroup in part_hist.  counts !=
 // nullptr: workgroup g encodes the chunk [g * chunk, (g + 1) * chunk) and
 // also writes its pass-first_pass counts to counts[d * gridDim.x + g] (the
-// reduce-then-scan sort's first chunk counts).
+// reduce-then-scan sort's first chunk counts).  chunks != nullptr: workgroup
+// g encodes [chunks[2g], chunks[2g + 1]) and writes counts[g * radix + d]
+// (chunk-major, the scene-segmented pass), no part_hist row.
 __global__ __launch_bounds__(kSegBlock) void
 k_scene_encode(const float* __restrict__ cx, const float* __restrict__ cy,
                const float* __restrict__ cz, uint64_t n, const uint64_t* __restrict__ off,
                uint32_t nseg, const int32_t* __restrict__ bkeys, uint64_t* __restrict__ keys,
                uint32_t* __restrict__ part_hist, int digit_bits, int first_pass, int passes,
-               uint32_t* __restrict__ counts, uint64_t chunk)
+               uint32_t* __restrict__ counts, uint64_t chunk, const uint32_t* __restrict__ chunks)
 {
   extern __shared__ uint32_t s_hist[];
   __shared__ uint32_t s_first;
@@ -210,7 +212,10 @@ k_scene_encode(const float* __restrict__ cx, const float* __restrict__ cy,
   for (int j = threadIdx.x; j < words; j += kSegBlock)
     s_hist[j] = 0u;
   BlockRange r;
-  if (counts) {
+  if (chunks) {
+    r.begin = chunks[2 * blockIdx.x];
+    r.end = chunks[2 * blockIdx.x + 1];
+  } else if (counts) {
     r.begin = uint64_t(blockIdx.x) * chunk;
     if (r.begin > n)
       r.begin = n;
@@ -267,6 +272,11 @@ k_scene_encode(const float* __restrict__ cx, const float* __restrict__ cy,
     }
   }
   __syncthreads();
+  if (chunks) {
+    for (int d = threadIdx.x; d < radix; d += kSegBlock)
+      counts[size_t(blockIdx.x) * radix + d] = s_hist[d];
+    return;
+  }
   uint32_t* out = part_hist + size_t(blockIdx.x) * words;
   for (int j = threadIdx.x; j < words; j += kSegBlock)
     out[j] = s_hist[j];
@@ -337,7 +347,7 @@ launch_scene_encode(const float* cx, const float* cy, const float* cz, uint64_t 
                     const uint64_t* prim_off, uint32_t nseg, const int32_t* bounds_key,
                     uint64_t* keys, uint32_t* part_hist, uint32_t* hist, int digit_bits,
                     int first_pass, int passes, uint32_t* counts, uint32_t groups,
-                    uint64_t chunk, hipStream_t s)
+                    uint64_t chunk, const uint32_t* chunks, hipStream_t s)
 {
   if (first_pass < 0 || first_pass >= passes)
     return hipErrorInvalidValue;
@@ -345,6 +355,14 @@ launch_scene_encode(const float* cx, const float* cy, const float* cz, uint64_t 
   if (words > uint32_t(kSegHistMaxWords))
     return hipErrorInvalidValue;
   uint32_t g = scene_encode_groups(n);
+  if (chunks) {
+    if (!counts || groups == 0 || passes != first_pass + 1)
+      return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scene_encode, dim3(groups), dim3(kSegBlock), size_t(words) * 4, s, cx,
+                       cy, cz, n, prim_off, nseg, bounds_key, keys, nullptr, digit_bits,
+                       first_pass, passes, counts, 0, chunks);
+    return hipGetLastError();
+  }
   if (counts) {
     if (groups == 0 || chunk == 0 || uint64_t(groups) * chunk < n)
       return hipErrorInvalidValue;
@@ -352,7 +370,7 @@ launch_scene_encode(const float* cx, const float* cy, const float* cz, uint64_t 
   }
   hipLaunchKernelGGL(k_scene_encode, dim3(g), dim3(kSegBlock), size_t(words) * 4, s, cx, cy, cz,
                      n, prim_off, nseg, bounds_key, keys, part_hist, digit_bits, first_pass,
-                     passes, counts, chunk);
+                     passes, counts, chunk, nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;

@@ -73,6 +73,10 @@ struct lbvh_batch
   uint32_t small_words = 0;
   uint32_t* d_counts = nullptr;
   uint64_t counts_words = 0;
+  // scene-aligned sort chunks: [2 * seg_groups] begin/end, then [nseg + 1]
+  // first chunk of each scene
+  uint32_t* d_seg = nullptr;
+  uint32_t seg_groups = 0;
   void* d_treelets = nullptr;
   void* d_tree = nullptr;           // tree kernel segments
   uint32_t* d_tree_tickets = nullptr; // tree kernel tickets (zero at rest)
@@ -96,7 +100,7 @@ free_batch(lbvh_batch* b)
   void* bufs[] = { b->d_off,       b->d_nodes,  b->d_parent,      b->d_flags,   b->d_other_end,
                    b->d_leafprim,  b->d_bounds_key,  b->d_cent,    b->d_keys_a,
                    b->d_keys_b,    b->d_ids_a,  b->d_ids_b,       b->d_part_hist, b->d_small,
-                   b->d_counts,    b->d_treelets, b->d_tree, b->d_tree_tickets };
+                   b->d_counts,    b->d_treelets, b->d_tree, b->d_tree_tickets, b->d_seg };
   for (void* p : bufs)
     dev_free(p);
   if (b->owns_prims)
@@ -114,6 +118,27 @@ ceil_log2(uint32_t x)
   return b;
 }
 
+// Chunks of at most seg_chunk_keys(n) keys that each lie in one scene, in
+// key order: begin/end pairs, then the first chunk of every scene (nseg + 1).
+std::vector<uint32_t>
+seg_table(const std::vector<uint64_t>& off, uint64_t n, uint32_t* groups)
+{
+  const uint64_t ck = seg_chunk_keys(n);
+  const size_t nseg = off.size() - 1;
+  std::vector<uint32_t> pairs, first(nseg + 1);
+  for (size_t s = 0; s < nseg; s++) {
+    first[s] = uint32_t(pairs.size() / 2);
+    for (uint64_t b = off[s]; b < off[s + 1]; b += ck) {
+      pairs.push_back(uint32_t(b));
+      pairs.push_back(uint32_t(std::min(off[s + 1], b + ck)));
+    }
+  }
+  first[nseg] = uint32_t(pairs.size() / 2);
+  *groups = first[nseg];
+  pairs.insert(pairs.end(), first.begin(), first.end());
+  return pairs;
+}
+
 // Scratch is sized for every sort configuration in `cfgs` (the tuner's
 // candidates, or the one configuration in use).
 lbvh_status
@@ -125,7 +150,17 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
   uint64_t hist_rows = scene_encode_groups(n);
   b->small_words = 0;
   b->counts_words = 0;
+  uint32_t seg_groups = 0;
+  const std::vector<uint32_t> seg = seg_table(b->h_off, n, &seg_groups);
+  bool any_seg = false;
   for (const SortConfig& cfg : cfgs) {
+    if (seg_pass_applies(b->key_bits, cfg)) {
+      any_seg = true;
+      const uint64_t cw =
+        (uint64_t(seg_groups) << cfg.digit_bits) + 2 + 2 * (n / kHybridTileCap + 1);
+      if (cw > b->counts_words)
+        b->counts_words = cw;
+    }
     // reduce-then-scan and hybrid: one encode workgroup per sort chunk
     hist_rows = std::max<uint64_t>(hist_rows, rts_groups(n, cfg, nullptr));
     const uint32_t radix = 1u << cfg.digit_bits;
@@ -162,6 +197,7 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
     { &b->d_treelets, 8 * n },
     { &b->d_tree, tree_scratch_bytes(n) },
     { reinterpret_cast<void**>(&b->d_tree_tickets), 4 * tree_ticket_words(n) },
+    { reinterpret_cast<void**>(&b->d_seg), any_seg ? 4 * seg.size() : 0 },
   };
   for (auto& r : req) {
     if (!r.bytes)
@@ -176,6 +212,12 @@ alloc_batch(lbvh_ctx* c, lbvh_batch* b, const std::vector<SortConfig>& cfgs)
   hipError_t e = hipMemset(b->d_tree_tickets, 0, 4 * tree_ticket_words(n));
   if (e != hipSuccess)
     return hip_fail(c, e, "clearing the tree tickets");
+  if (any_seg) {
+    e = hipMemcpy(b->d_seg, seg.data(), 4 * seg.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+      return hip_fail(c, e, "the scene chunk table");
+    b->seg_groups = seg_groups;
+  }
   return LBVH_OK;
 }
 
@@ -199,6 +241,15 @@ queue_build(lbvh_ctx* c, lbvh_batch* b, bool timed)
   const bool chunked = b->cfg.strategy == 1 || b->cfg.strategy == 2;
   ws.first_counts = chunked;
   const int first_pass = b->cfg.strategy == 2 ? b->passes - b->cfg.global_digits : 0;
+  // the hybrid of one code digit: one pass over scene-aligned chunks
+  const bool seg = b->d_seg && seg_pass_applies(b->key_bits, b->cfg);
+  if (seg) {
+    ws.seg_chunks = b->d_seg;
+    ws.seg_first = b->d_seg + 2 * size_t(b->seg_groups);
+    ws.seg_off = b->d_off;
+    ws.seg_groups = b->seg_groups;
+    ws.nseg = b->nseg;
+  }
   if (timed)
     record(c, 0);
   hipError_t e = launch_fill_u32(small + kSmallTickets, b->small_words - kSmallTickets, 0u, s);
@@ -209,11 +260,12 @@ queue_build(lbvh_ctx* c, lbvh_batch* b, bool timed)
                             b->d_cent + 2 * n, b->d_bounds_key, s);
   if (e == hipSuccess)
     e = launch_scene_encode(b->d_cent, b->d_cent + n, b->d_cent + 2 * n, n, b->d_off, b->nseg,
-                            b->d_bounds_key, b->d_keys_a, b->d_part_hist, ws.hist,
-                            b->cfg.digit_bits, first_pass, b->passes,
-                            chunked ? b->d_counts : nullptr,
-                            chunked ? rts_groups(n, b->cfg, nullptr) : 0u,
-                            chunked ? rts_chunk_keys(n, b->cfg) : 0, s);
+                            b->d_bounds_key, b->d_keys_a, seg ? nullptr : b->d_part_hist,
+                            seg ? nullptr : ws.hist, b->cfg.digit_bits, first_pass,
+                            seg ? first_pass + 1 : b->passes, chunked ? b->d_counts : nullptr,
+                            seg ? b->seg_groups : chunked ? rts_groups(n, b->cfg, nullptr) : 0u,
+                            chunked && !seg ? rts_chunk_keys(n, b->cfg) : 0,
+                            seg ? b->d_seg : nullptr, s);
   if (timed)
     record(c, 1);
   bool in_b = false;
@@ -550,6 +602,11 @@ lbvh_batch_get_info(const lbvh_batch* b, lbvh_batch_info* out)
   out->device_bytes = b->bytes;
   out->d_nodes = b->d_nodes;
   out->d_prims = b->d_prims;
+  out->sort_global_passes = uint32_t(b->cfg.strategy == 2
+                                       ? (b->d_seg && seg_pass_applies(b->key_bits, b->cfg)
+                                            ? 1
+                                            : b->cfg.global_digits)
+                                       : b->passes);
   return LBVH_OK;
 }
 

@@ -41,7 +41,22 @@ struct SortWorkspace
   uint32_t* counts = nullptr;  // reduce-then-scan chunk counts [radix * groups]
   uint64_t counts_words = 0;
   bool first_counts = false;   // the first reduce-then-scan pass's counts are in place
+  // Scene batches (keys (scene << 30 | code), grouped by scene in ascending
+  // order): chunks that each lie in one scene.  When set and the hybrid's
+  // global digits hold at most one digit of code bits, one pass over these
+  // chunks (counts chunk-major, [groups * radix]) replaces the global passes.
+  const uint32_t* seg_chunks = nullptr; // [2 * seg_groups]: begin, end of each chunk
+  const uint32_t* seg_first = nullptr;  // [nseg + 1]: first chunk of each scene
+  const uint64_t* seg_off = nullptr;    // [nseg + 1]: scene offsets
+  uint32_t seg_groups = 0;
+  uint32_t nseg = 0;
 };
+
+// True when the hybrid sort of batch keys (code bits below bit 30) runs as
+// one scene-segmented pass plus the LDS sort (see SortWorkspace::seg_chunks).
+bool seg_pass_applies(int key_bits, const SortConfig& cfg);
+// Keys per scene-aligned chunk of that pass (a multiple of every sort tile).
+uint64_t seg_chunk_keys(uint64_t n);
 
 // centroid of every primitive box (SoA) and one partial box of centroid
 // bounds per workgroup (centroid_partials_needed(n) boxes of 6 floats); also
@@ -184,11 +199,15 @@ hipError_t launch_scene_bounds(const float* prims, int kind, uint64_t n, const u
 // summed into hist[p << digit_bits ...], which must be zero).  counts !=
 // nullptr: `groups` workgroups encode chunks of `chunk` keys and write the
 // first pass's chunk counts (counts[d * groups + g]) as launch_encode does.
+// chunks != nullptr as well: workgroup g encodes [chunks[2g], chunks[2g + 1])
+// and writes its counts chunk-major (counts[g * radix + d]); part_hist and
+// hist are then unused (may be nullptr) and passes must be first_pass + 1.
 hipError_t launch_scene_encode(const float* cx, const float* cy, const float* cz, uint64_t n,
                                const uint64_t* prim_off, uint32_t nseg, const int32_t* bounds_key,
                                uint64_t* keys, uint32_t* part_hist, uint32_t* hist,
                                int digit_bits, int first_pass, int passes, uint32_t* counts,
-                               uint32_t groups, uint64_t chunk, hipStream_t s);
+                               uint32_t groups, uint64_t chunk, const uint32_t* chunks,
+                               hipStream_t s);
 uint32_t scene_encode_groups(uint64_t n);
 
 // builder<double>: centroid bounds of FP64 boxes (6 doubles each, 16-byte

@@ -269,11 +269,30 @@ k_onesweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
 
 constexpr int kUpsweepUnroll = 8;
 
-// counts[d * groups + g] = number of keys of chunk g whose digit is d.
+// Key range [*begin, *end) of chunk g: fixed chunks of chunk_keys keys, or
+// the scene-aligned chunk table (begin, end pairs) when chunks != nullptr.
+__device__ __forceinline__ void
+chunk_range(uint32_t g, uint32_t n, uint64_t chunk_keys, const uint32_t* __restrict__ chunks,
+            uint32_t* begin, uint32_t* end)
+{
+  if (chunks) {
+    *begin = chunks[2 * g];
+    *end = chunks[2 * g + 1];
+    return;
+  }
+  const uint64_t b64 = uint64_t(g) * chunk_keys;
+  const uint64_t e64 = b64 + chunk_keys;
+  *begin = b64 < n ? uint32_t(b64) : n;
+  *end = e64 < n ? uint32_t(e64) : n;
+}
+
+// Number of keys of chunk g whose digit is d: counts[d * groups + g], or
+// counts[g * RADIX + d] (chunk-major) with a chunk table.
 template<typename KeyT, int RBITS>
 __global__ __launch_bounds__(kSortBlock) void
 k_rts_upsweep(const KeyT* __restrict__ keys, uint32_t n, int shift, uint32_t chunk_keys,
-              uint32_t groups, uint32_t* __restrict__ counts)
+              uint32_t groups, uint32_t* __restrict__ counts,
+              const uint32_t* __restrict__ chunks)
 {
   constexpr int RADIX = 1 << RBITS;
   __shared__ uint32_t s_h[kSortWaves][RADIX];
@@ -283,9 +302,8 @@ k_rts_upsweep(const KeyT* __restrict__ keys, uint32_t n, int shift, uint32_t chu
     (&s_h[0][0])[j] = 0u;
   __syncthreads();
   const uint32_t g = blockIdx.x;
-  const uint64_t begin64 = uint64_t(g) * chunk_keys;
-  const uint32_t begin = begin64 < n ? uint32_t(begin64) : n;
-  const uint32_t end = (uint64_t(begin) + chunk_keys < n) ? begin + chunk_keys : n;
+  uint32_t begin, end;
+  chunk_range(g, n, chunk_keys, chunks, &begin, &end);
   const KeyT dmask = KeyT(RADIX - 1);
   const uint64_t lt = lanemask_lt();
   for (uint32_t base = begin; base < end; base += kSortBlock * kUpsweepUnroll) {
@@ -317,7 +335,7 @@ k_rts_upsweep(const KeyT* __restrict__ keys, uint32_t n, int shift, uint32_t chu
 #pragma unroll
     for (int ww = 0; ww < kSortWaves; ww++)
       c += s_h[ww][d];
-    counts[size_t(d) * groups + g] = c;
+    counts[chunks ? size_t(g) * RADIX + d : size_t(d) * groups + g] = c;
   }
 }
 
@@ -356,13 +374,60 @@ k_rts_scan(uint32_t* __restrict__ counts, uint32_t groups, const uint32_t* __res
   (void)RADIX;
 }
 
-// Ranks and scatters chunk g, tile by tile, from its per-digit offsets.
+// The scene-segmented pass: one workgroup per scene s, whose chunks are
+// [first[s], first[s + 1]) of the chunk-major counts.  counts[g][d] <-
+// off[s] + (keys of scene s with a digit below d) + (keys of digit d in the
+// scene's chunks before g): the destinations of a stable sort of each scene
+// by the digit, in place.
+template<int RBITS>
+__global__ __launch_bounds__(kSortBlock) void
+k_seg_scan(uint32_t* __restrict__ counts, const uint32_t* __restrict__ first,
+           const uint64_t* __restrict__ off, uint32_t* __restrict__ zero_word)
+{
+  constexpr int RADIX = 1 << RBITS;
+  constexpr int DPT = RADIX / kSortBlock; // digits d = tid * DPT + dd
+  static_assert(DPT * kSortBlock == RADIX, "digits per thread");
+  __shared__ uint32_t s_tmp[kSortWaves];
+  const uint32_t sc = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (zero_word && sc == 0 && tid == 0)
+    *zero_word = 0u; // a counter of a later kernel of the same sort
+  const uint32_t g0 = first[sc];
+  const uint32_t g1 = first[sc + 1];
+  uint32_t tot[DPT];
+  uint32_t mine = 0;
+#pragma unroll
+  for (int dd = 0; dd < DPT; dd++) {
+    const size_t d = size_t(tid) * DPT + dd;
+    uint32_t acc = 0;
+    for (uint32_t g = g0; g < g1; g++) {
+      const uint32_t c = counts[size_t(g) * RADIX + d];
+      counts[size_t(g) * RADIX + d] = acc;
+      acc += c;
+    }
+    tot[dd] = acc;
+    mine += acc;
+  }
+  uint32_t all;
+  uint32_t run = block_exclusive_scan(mine, s_tmp, &all) + uint32_t(off[sc]);
+#pragma unroll
+  for (int dd = 0; dd < DPT; dd++) {
+    const size_t d = size_t(tid) * DPT + dd;
+    for (uint32_t g = g0; g < g1; g++)
+      counts[size_t(g) * RADIX + d] += run;
+    run += tot[dd];
+  }
+}
+
+// Ranks and scatters chunk g, tile by tile, from its per-digit offsets
+// (offsets[d * groups + g], or chunk-major offsets[g * RADIX + d] with a
+// chunk table).
 template<typename KeyT, bool VALS, int RBITS, int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void
 k_rts_downsweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
                 const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout, uint32_t n,
                 int shift, uint32_t chunk_tiles, uint32_t groups,
-                const uint32_t* __restrict__ offsets)
+                const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ chunks)
 {
   constexpr int RADIX = 1 << RBITS;
   constexpr int TILE = kSortBlock * ITEMS;
@@ -381,15 +446,16 @@ k_rts_downsweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
   const uint32_t g = blockIdx.x;
   uint32_t running[DPT];
 #pragma unroll
-  for (int dd = 0; dd < DPT; dd++)
-    running[dd] = offsets[size_t(tid * DPT + dd) * groups + g];
+  for (int dd = 0; dd < DPT; dd++) {
+    const size_t d = size_t(tid) * DPT + dd;
+    running[dd] = offsets[chunks ? size_t(g) * RADIX + d : d * groups + g];
+  }
   const uint64_t lt = lanemask_lt();
+  uint32_t cbegin, cend;
+  chunk_range(g, n, uint64_t(chunk_tiles) * TILE, chunks, &cbegin, &cend);
 
-  for (uint32_t tt = 0; tt < chunk_tiles; tt++) {
-    const uint64_t tile_begin64 = (uint64_t(g) * chunk_tiles + tt) * TILE;
-    if (tile_begin64 >= n)
-      break; // uniform across the workgroup
-    const uint32_t tile_begin = uint32_t(tile_begin64);
+  for (uint64_t tb = cbegin; tb < cend; tb += TILE) {
+    const uint32_t tile_begin = uint32_t(tb);
     for (int j = tid; j < kSortWaves * RADIX; j += kSortBlock)
       (&s_wcnt[0][0])[j] = 0u;
     __syncthreads();
@@ -400,7 +466,7 @@ k_rts_downsweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
       uint32_t idx = base + uint32_t(j) * kWave;
-      bool valid = idx < n;
+      bool valid = idx < cend;
       key[j] = valid ? kin[idx] : KeyT(0);
       if (VALS)
         val[j] = valid ? (vin ? vin[idx] : idx) : 0u;
@@ -408,7 +474,7 @@ k_rts_downsweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
       const uint32_t idx = base + uint32_t(j) * kWave;
-      const bool valid = idx < n;
+      const bool valid = idx < cend;
       const uint32_t d = uint32_t((key[j] >> shift) & dmask);
       uint64_t peers = ballot(valid);
 #pragma unroll
@@ -458,7 +524,7 @@ k_rts_downsweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
       const uint32_t idx = base + uint32_t(j) * kWave;
-      if (idx < n) {
+      if (idx < cend) {
         const uint32_t d = uint32_t((key[j] >> shift) & dmask);
         const uint32_t pos = s_bstart[d] + s_wcnt[w][d] + rank[j];
         s_keys[pos] = key[j];
@@ -467,7 +533,8 @@ k_rts_downsweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
       }
     }
     __syncthreads();
-    const uint32_t tile_n = (n - tile_begin < uint32_t(TILE)) ? (n - tile_begin) : uint32_t(TILE);
+    const uint32_t tile_n =
+      (cend - tile_begin < uint32_t(TILE)) ? (cend - tile_begin) : uint32_t(TILE);
     for (uint32_t i = tid; i < tile_n; i += kSortBlock) {
       const KeyT k = s_keys[i];
       const uint32_t d = uint32_t((k >> shift) & dmask);
@@ -1220,6 +1287,28 @@ rts_groups(uint64_t n, const SortConfig& cfg, uint32_t* chunk_tiles)
   return uint32_t((tiles + ct - 1) / ct);
 }
 
+bool
+seg_pass_applies(int key_bits, const SortConfig& cfg)
+{
+  if (cfg.strategy != 2 || key_bits < 30)
+    return false;
+  const int passes = sort_passes(key_bits, cfg.digit_bits);
+  const int hi_shift = (passes - cfg.global_digits) * cfg.digit_bits;
+  return cfg.global_digits >= 1 && hi_shift < 30 && hi_shift + cfg.digit_bits >= 30;
+}
+
+uint64_t
+seg_chunk_keys(uint64_t n)
+{
+  // about 2,048 chunks over the whole array (as the fixed chunks), at least
+  // 16,384 keys, in whole tiles of either size
+  constexpr uint64_t kUnit = uint64_t(kSortBlock) * 16;
+  uint64_t c = (n + 2047) / 2048;
+  if (c < 16384)
+    c = 16384;
+  return (c + kUnit - 1) / kUnit * kUnit;
+}
+
 uint64_t
 rts_chunk_keys(uint64_t n, const SortConfig& cfg)
 {
@@ -1249,12 +1338,12 @@ run_rts(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb
     const int shift = p * RBITS;
     if (p > 0 || !ws.first_counts)
       hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(groups), dim3(kSortBlock), 0, s, src,
-                         uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts);
+                         uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts, nullptr);
     hipLaunchKernelGGL((k_rts_scan<RBITS>), dim3(RADIX), dim3(kSortBlock), 0, s, ws.counts,
                        groups, ws.hist + size_t(p) * RADIX, nullptr);
     hipLaunchKernelGGL((k_rts_downsweep<KeyT, VALS, RBITS, ITEMS>), dim3(groups),
                        dim3(kSortBlock), 0, s, src, dst, vsrc, vdst, uint32_t(n), shift,
-                       chunk_tiles, groups, ws.counts);
+                       chunk_tiles, groups, ws.counts, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
       return e;
@@ -1269,6 +1358,24 @@ run_rts(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t* vb
   }
   *in_b = cur_b;
   return hipSuccess;
+}
+
+// The hybrid's LDS stage: keys src (grouped into buckets of equal
+// key >> hi_shift, in bucket order) sorted into dst; *in_b = dst_is_b.
+template<typename KeyT, bool VALS>
+static hipError_t
+bucket_sorts(int key_bits, uint64_t n, KeyT* src, KeyT* dst, uint32_t* vsrc, uint32_t* vdst,
+             int hi_shift, uint32_t* big, uint32_t* big_count, uint64_t big_cap, bool dst_is_b,
+             bool* in_b, hipStream_t s)
+{
+  const uint32_t tiles = uint32_t((n + kHybridTileKeys - 1) / kHybridTileKeys);
+  hipLaunchKernelGGL((k_tile_sort<KeyT, VALS>), dim3(tiles), dim3(kTileThreads), 0, s, src, dst,
+                     vsrc, vdst, uint32_t(n), hi_shift, key_bits, big, big_count,
+                     uint32_t(big_cap));
+  hipLaunchKernelGGL((k_big_sort<KeyT, VALS>), dim3(kBigBlocks), dim3(kTileThreads), 0, s, src,
+                     dst, vsrc, vdst, key_bits, big, big_count, uint32_t(big_cap));
+  *in_b = dst_is_b;
+  return hipGetLastError();
 }
 
 template<typename KeyT, bool VALS, int RBITS, int ITEMS>
@@ -1287,23 +1394,48 @@ run_hybrid(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t*
   const uint64_t big_cap = n / kHybridTileCap + 1; // ranges longer than the cap are disjoint
   if (uint64_t(groups) * RADIX + 2 + 2 * big_cap > ws.counts_words)
     return hipErrorInvalidValue;
-  uint32_t* big_count = ws.counts + size_t(groups) * RADIX;
-  uint32_t* big = big_count + 2;
   KeyT* src = ka;
   KeyT* dst = kb;
   uint32_t* vsrc = nullptr;
   uint32_t* vdst = vb;
+  const int hi_shift = (passes - gd) * RBITS;
+  if (ws.seg_chunks && seg_pass_applies(key_bits, cfg)) {
+    // keys grouped by scene (bits 30 and up) in ascending order, and the
+    // global digits hold one digit of code bits: sort each scene by that
+    // digit in one pass over scene-aligned chunks
+    const uint32_t sg = ws.seg_groups;
+    if (uint64_t(sg) * RADIX + 2 + 2 * big_cap > ws.counts_words)
+      return hipErrorInvalidValue;
+    uint32_t* big_count = ws.counts + size_t(sg) * RADIX;
+    uint32_t* big = big_count + 2;
+    if (!ws.first_counts)
+      hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(sg), dim3(kSortBlock), 0, s, src,
+                         uint32_t(n), hi_shift, 0u, sg, ws.counts, ws.seg_chunks);
+    hipLaunchKernelGGL((k_seg_scan<RBITS>), dim3(ws.nseg), dim3(kSortBlock), 0, s, ws.counts,
+                       ws.seg_first, ws.seg_off, big_count);
+    hipLaunchKernelGGL((k_rts_downsweep<KeyT, VALS, RBITS, ITEMS>), dim3(sg), dim3(kSortBlock),
+                       0, s, src, dst, vsrc, vdst, uint32_t(n), hi_shift, 0u, sg, ws.counts,
+                       ws.seg_chunks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
+    return bucket_sorts<KeyT, VALS>(key_bits, n, dst, src, VALS ? vdst : nullptr,
+                                    VALS ? (vdst == vb ? va : vb) : nullptr, hi_shift, big,
+                                    big_count, big_cap, src == kb, in_b, s);
+  }
+  uint32_t* big_count = ws.counts + size_t(groups) * RADIX;
+  uint32_t* big = big_count + 2;
   for (int p = passes - gd; p < passes; p++) {
     const int shift = p * RBITS;
     if (p > passes - gd || !ws.first_counts)
       hipLaunchKernelGGL((k_rts_upsweep<KeyT, RBITS>), dim3(groups), dim3(kSortBlock), 0, s, src,
-                         uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts);
+                         uint32_t(n), shift, chunk_tiles * TILE, groups, ws.counts, nullptr);
     hipLaunchKernelGGL((k_rts_scan<RBITS>), dim3(RADIX), dim3(kSortBlock), 0, s, ws.counts,
                        groups, ws.hist + size_t(p) * RADIX,
                        p == passes - gd ? big_count : nullptr);
     hipLaunchKernelGGL((k_rts_downsweep<KeyT, VALS, RBITS, ITEMS>), dim3(groups),
                        dim3(kSortBlock), 0, s, src, dst, vsrc, vdst, uint32_t(n), shift,
-                       chunk_tiles, groups, ws.counts);
+                       chunk_tiles, groups, ws.counts, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
       return e;
@@ -1315,14 +1447,8 @@ run_hybrid(int key_bits, uint64_t n, KeyT* ka, KeyT* kb, uint32_t* va, uint32_t*
       vdst = (vdst == vb) ? va : vb;
     }
   }
-  const uint32_t tiles = uint32_t((n + kHybridTileKeys - 1) / kHybridTileKeys);
-  hipLaunchKernelGGL((k_tile_sort<KeyT, VALS>), dim3(tiles), dim3(kTileThreads), 0, s, src, dst,
-                     vsrc, vdst, uint32_t(n), (passes - gd) * RBITS, key_bits, big, big_count,
-                     uint32_t(big_cap));
-  hipLaunchKernelGGL((k_big_sort<KeyT, VALS>), dim3(kBigBlocks), dim3(kTileThreads), 0, s, src,
-                     dst, vsrc, vdst, key_bits, big, big_count, uint32_t(big_cap));
-  *in_b = dst == kb;
-  return hipGetLastError();
+  return bucket_sorts<KeyT, VALS>(key_bits, n, src, dst, vsrc, vdst, hi_shift, big, big_count,
+                                  big_cap, dst == kb, in_b, s);
 }
 
 uint64_t

@@ -94,7 +94,7 @@ def test_struct_layouts_match_the_header():
     """The ctypes mirrors have the C structs' sizes (x86-64 ABI)."""
     assert ctypes.sizeof(lbvh_amd.BuildOpts) == 32
     assert ctypes.sizeof(lbvh_amd.BvhInfo) == 64
-    assert ctypes.sizeof(lbvh_amd.BatchInfo) == 64
+    assert ctypes.sizeof(lbvh_amd.BatchInfo) == 72
     assert ctypes.sizeof(lbvh_amd.TuneRecord) == 52
     assert ctypes.sizeof(lbvh_amd.ServerOpts) == 48
     assert ctypes.sizeof(lbvh_amd.Request) == 72

@@ -221,30 +221,33 @@ def test_batch_sort_configurations_agree(ctx, mixed):
 
 
 @pytest.mark.parametrize("tile", [2048, 4096])
-@pytest.mark.parametrize("leaf", [0, 1], ids=["ref", "pid"])
-def test_batch_segmented_hybrid_equals_oracle(ctx, tile, leaf):
+@pytest.mark.parametrize("leaf,keep", [(0, False), (0, True), (1, True)],
+                         ids=["ref-keys-only", "ref-keep-codes", "pid"])
+def test_batch_segmented_hybrid_equals_oracle(ctx, tile, leaf, keep):
     """The hybrid sort with 10-bit digits on 49 scenes (36-bit keys, global
     digits = the scene bits and code bits 20-29) runs as one pass that sorts
     each scene by code bits 20-29 over chunks of 16,384 keys that never cross
     a scene (scenes of 1 to 7 chunks, chunk-sized and one-past scenes, and
     many tiny ones), then the LDS bucket sort.  Every scene's codes, ids and
     nodes equal the oracle's; clustered scenes bring long runs of equal
-    codes (ties in input order)."""
+    codes (ties in input order).  Reference leaves without keep_codes sort
+    keys alone (no id payload)."""
     sizes = [100000, 2, 3, 16384, 16385, 64, 40000] + [5] * 40 + [5000, 32768]
     tri, off = W.batched_scenes(seed=31, sizes=sizes)
     for s in (0, 4, 6):
         m = sizes[s]
         tri[int(off[s]):int(off[s + 1])] = W.make_triangles(W.cluster_vertices(31000 + s, m))
-    b = ctx.build_batch(tri, off, leaf_order=leaf, keep_codes=True, sort_strategy=3,
+    b = ctx.build_batch(tri, off, leaf_order=leaf, keep_codes=keep, sort_strategy=3,
                         sort_digit_bits=10, sort_tile=tile)
     inf = b.info()
     assert inf.sort_strategy == 3 and inf.sort_digit_bits == 10
     assert inf.sort_global_passes == 1
     for s in range(len(sizes)):
         on, ob, osc, oids = oracle.build(_scene(tri, off, s), 30, leaf)
-        codes, ids, bounds = b.codes(s)
-        assert np.array_equal(codes, osc), "scene %d codes" % s
-        assert np.array_equal(ids, oids), "scene %d ids" % s
+        if keep:
+            codes, ids, bounds = b.codes(s)
+            assert np.array_equal(codes, osc), "scene %d codes" % s
+            assert np.array_equal(ids, oids), "scene %d ids" % s
         assert b.nodes(s).tobytes() == on.tobytes(), "scene %d nodes" % s
     b.rebuild(graph=True)  # the same sort replayed from a graph
     ctx.synchronize()

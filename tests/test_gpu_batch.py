@@ -257,6 +257,23 @@ def test_batch_segmented_hybrid_equals_oracle(ctx, tile, leaf, keep):
     b.destroy()
 
 
+def test_batch_segmented_hybrid_one_scene(ctx):
+    """A one-scene batch (30-bit keys, no scene bits) of 50,000 clustered
+    triangles: the hybrid's one global digit is code bits 20-29, sorted by
+    the segmented pass over the scene's four chunks; equal to the oracle."""
+    tri, off = W.batched_scenes(seed=32, sizes=[50000])
+    tri[:] = W.make_triangles(W.cluster_vertices(32000, 50000))
+    b = ctx.build_batch(tri, off, leaf_order=1, keep_codes=True, sort_strategy=3,
+                        sort_digit_bits=10, sort_tile=2048)
+    assert b.info().sort_global_passes == 1
+    on, ob, osc, oids = oracle.build(tri, 30, 1)
+    codes, ids, bounds = b.codes(0)
+    assert np.array_equal(codes, osc)
+    assert np.array_equal(ids, oids)
+    assert b.nodes(0).tobytes() == on.tobytes()
+    b.destroy()
+
+
 def test_batch_rejects_bad_arguments(ctx):
     tri, off = W.batched_scenes(seed=3, sizes=[4, 5, 6])
     with pytest.raises(lbvh_amd.LbvhError) as e:

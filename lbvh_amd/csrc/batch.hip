@@ -13,8 +13,8 @@
 namespace lbvh_dev {
 
 constexpr int kSegBlock = 256;
-constexpr int kSegMaxBlocks = 2048;
-constexpr int kSegUnroll = 4;
+constexpr int kSegMaxBlocks = 4096;
+constexpr int kSegUnroll = 8;
 constexpr int kSegHistMaxWords = 8192;
 
 // Order-preserving map of floats to signed integers (x < y as floats iff

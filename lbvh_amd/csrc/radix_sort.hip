@@ -1300,10 +1300,11 @@ seg_pass_applies(int key_bits, const SortConfig& cfg)
 uint64_t
 seg_chunk_keys(uint64_t n)
 {
-  // about 2,048 chunks over the whole array (as the fixed chunks), at least
-  // 16,384 keys, in whole tiles of either size
+  // 16,384 keys (measured best on 125M keys in 4,096 scenes against 8,192,
+  // 32,768 and 61,440), or about 8,192 chunks over larger arrays, in whole
+  // tiles of either size
   constexpr uint64_t kUnit = uint64_t(kSortBlock) * 16;
-  uint64_t c = (n + 2047) / 2048;
+  uint64_t c = (n + 8191) / 8192;
   if (c < 16384)
     c = 16384;
   return (c + kUnit - 1) / kUnit * kUnit;
